@@ -1,0 +1,196 @@
+/*
+ * pcom_gpu.h - C ABI of the MI355X time-stepping library for the PCOM
+ * pressure-coordinate B-grid ocean model (ouyuyuan/pcom v2.2).
+ *
+ * The library replaces the Fortran module procedures that main.f90 calls
+ * once per baroclinic step (main.f90:158-238):
+ *
+ *   den_prho  (mod_den.f90:120-153)   int_bnd   (mod_int.f90:649-691)
+ *   den_alpha (mod_den.f90:33-55)     int_pgra  (mod_int.f90:64-88)
+ *   int_readyc(mod_int.f90:90-151)    int_trop  (mod_int.f90:153-296)
+ *   int_clin  (mod_int.f90:298-401)   int_ssh   (mod_int.f90:602-624)
+ *   int_ts    (mod_int.f90:442-507)
+ *
+ * and the start-up chain that fixes every static field those routines read
+ * (main.f90:252-704: set_cf, calc_hpos, set_hgrid, set_vgrid, set_grid,
+ * calc_cf, Coriolis, inistat).  The reference passes non-interoperable
+ * derived types and module globals (mod_type.f90:138-205, mod_op.f90:13-19);
+ * here the caller hands over plain arrays once, the device owns the state,
+ * and fields are copied in and out on request.
+ *
+ * Layout at the boundary: every 2-D/3-D field is FP64, Fortran
+ * column-major, WITH the reference's one-cell halo: (ni, nj[, nk|nkp]) with
+ * ni = glo_ni + 2, nj = glo_nj + 2 (mod_param.f90:313-342 with one rank).
+ * Raw inputs (pcom_inputs) are the global arrays of initial.nc/forcing.nc
+ * (glo_ni, glo_nj[, nk|12]) without halo, as mod_io reads them.
+ *
+ * Errors: every call returns 0 or a negative PCOM_ERR_* code and never
+ * aborts; pcom_last_error() returns the message of the last failure on the
+ * calling thread.  The reference stops the program instead (mod_op.f90:337,
+ * mod_io.f90:540-550).
+ *
+ * Threading: one host thread per context; calls are stream-ordered on the
+ * context's own HIP stream; pcom_get_field() synchronises.
+ */
+#ifndef PCOM_GPU_H
+#define PCOM_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PCOM_OK = 0,
+  PCOM_ERR_ARG = -1,      /* bad argument, size or field id               */
+  PCOM_ERR_HIP = -2,      /* HIP runtime error (message says which call)  */
+  PCOM_ERR_NAN = -3,      /* non-finite value found by pcom_check_finite  */
+  PCOM_ERR_STATE = -4,    /* call not valid in the context's state        */
+  PCOM_ERR_NOGPU = -5     /* no HIP device visible                        */
+};
+
+/* Arithmetic of the 3-D baroclinic fields. */
+enum {
+  PCOM_FP64 = 0,   /* everything FP64, bit-identical to the FP64 oracle   */
+  PCOM_MIXED = 1   /* 3-D stencils and EOS in FP32; barotropic subcycle,
+                      time filter and global sums in FP64                  */
+};
+
+/* Raw inputs, as read from initial.nc, forcing.nc and vmix.nc
+ * (main.f90:263-345, mod_mympi.f90:493-560).  Column-major, no halo. */
+typedef struct pcom_inputs {
+  int glo_ni, glo_nj, nk;
+  const double *lon;   /* [glo_ni]  degrees east, grid g4 (initial.nc lon) */
+  const double *lat;   /* [glo_nj]  degrees north, grid g4                 */
+  const double *z;     /* [nk]      depth of layer centres, m              */
+  const int    *itn;   /* [glo_ni*glo_nj] number of wet levels (0 = land)  */
+  const double *tmn;   /* [nk]      layer-mean T for the vertical grid     */
+  const double *smn;   /* [nk]      layer-mean S                           */
+  const double *pt;    /* [glo_ni*glo_nj*nk] initial potential temperature */
+  const double *sa;    /* [glo_ni*glo_nj*nk] initial salinity              */
+  const double *vmix;  /* [glo_ni*glo_nj*nk] kh in cm^2/s at interfaces
+                          2..nk+1, or NULL = fill value everywhere         */
+  const double *taux, *tauy, *bct, *bcs, *pa, *fw; /* [glo_ni*glo_nj*12]   */
+} pcom_inputs;
+
+typedef struct pcom_opts {
+  int dtbt;        /* barotropic step, s (namelist time_ctrl dtbt)         */
+  int dtbc;        /* baroclinic step, s (namelist time_ctrl dtbc)         */
+  int year, month, day, hour;   /* bdate                                   */
+  int precision;   /* PCOM_FP64 or PCOM_MIXED                              */
+  int restart_run; /* namelist restart_run: 1 skips the AB3 bootstrap      */
+  int device;      /* HIP device ordinal                                   */
+} pcom_opts;
+
+typedef struct pcom_ctx pcom_ctx;
+
+/* Field ids for pcom_get_field / pcom_set_field.  Shapes are the halo'd
+ * reference shapes; (3D) = (ni,nj,nk), (3Dp) = (ni,nj,nk+1), (2D) = (ni,nj). */
+enum {
+  PCOM_F_TC = 0, PCOM_F_SC, PCOM_F_TP, PCOM_F_SP,            /* 3D, g1 */
+  PCOM_F_UC, PCOM_F_VC, PCOM_F_UP, PCOM_F_VP,                /* 3D, g3 */
+  PCOM_F_AUC, PCOM_F_AVC, PCOM_F_AUP, PCOM_F_AVP,
+  PCOM_F_AUPP, PCOM_F_AVPP,                                  /* 3D, g3 */
+  PCOM_F_FX, PCOM_F_FY, PCOM_F_AM,                           /* 3D, g3 */
+  PCOM_F_ALPHA, PCOM_F_ADP,                                  /* 3D, g1 */
+  PCOM_F_WM,                                                 /* 3Dp, g1 */
+  PCOM_F_PRHO_T, PCOM_F_PRHO_S,                              /* 3D, g1 */
+  PCOM_F_ACT, PCOM_F_ACS, PCOM_F_ACR, PCOM_F_ACU, PCOM_F_ACV,
+  PCOM_F_ACW,                                                /* 3D     */
+  PCOM_F_CHC, PCOM_F_CHP, PCOM_F_MCH, PCOM_F_MCH2,           /* 2D, g1 */
+  PCOM_F_ACCH, PCOM_F_ACSSH,                                 /* 2D, g1 */
+  PCOM_F_BUC, PCOM_F_BVC, PCOM_F_BUP, PCOM_F_BVP,
+  PCOM_F_BUT, PCOM_F_BVT,                                    /* 2D, g3 */
+  PCOM_F_PAX, PCOM_F_PAY,                                    /* 2D, g3 */
+  PCOM_F_BPHI_XN, PCOM_F_BPHI_XS, PCOM_F_BPHI_YE, PCOM_F_BPHI_YW,
+  PCOM_F_BGRA_XN, PCOM_F_BGRA_XS, PCOM_F_BGRA_YE, PCOM_F_BGRA_YW,
+  PCOM_F_BND_TAUX, PCOM_F_BND_TAUY, PCOM_F_BND_T, PCOM_F_BND_S,
+  PCOM_F_BND_PA, PCOM_F_BND_FW,                              /* 2D     */
+  PCOM_F_SSH,        /* 2D: sea surface height of the last int_ssh call */
+  PCOM_F_GRAPHIHX, PCOM_F_GRAPHIHY, PCOM_F_COR,              /* 2D, g3 */
+  PCOM_F_PHIH_T, PCOM_F_PRH_T, PCOM_F_PHIH_U, PCOM_F_PRH_U,  /* 2D     */
+  PCOM_F_KM, PCOM_F_KH,                                      /* 3Dp    */
+  PCOM_F_COUNT
+};
+
+/* Counters of the time-mean accumulators (eqts%n, equv%n, eqw%n, eqch%n,
+ * acssh%n in mod_type.f90) and of the time control (tctr%i). */
+typedef struct pcom_counters {
+  long long step;          /* baroclinic steps taken since create (tctr%i) */
+  int n_ts, n_uv, n_w, n_ch, n_ssh;
+  int year, month, day, hour, minute, second;   /* tctr%ct */
+} pcom_counters;
+
+/* Create a context: runs the reference's start-up chain on the host
+ * (main.f90:104-122, 312-704), uploads the state and builds the step.   */
+int pcom_create(const pcom_inputs *in, const pcom_opts *opts, pcom_ctx **out);
+
+/* Advance nsteps baroclinic steps: main.f90:158-227 without output,
+ * including the daily int_bnd + den_prho on day change.                */
+int pcom_step(pcom_ctx *ctx, int nsteps);
+
+/* Dimensions of the halo'd arrays. */
+int pcom_get_dims(const pcom_ctx *ctx, int *ni, int *nj, int *nk);
+
+/* Number of doubles of field id (0 for an unknown id). */
+size_t pcom_field_size(const pcom_ctx *ctx, int id);
+
+/* Copy a field to / from host memory (FP64, halo'd, column-major).  In
+ * PCOM_MIXED, 3-D fields are converted from / to their FP32 device copy. */
+int pcom_get_field(pcom_ctx *ctx, int id, double *host, size_t n);
+int pcom_set_field(pcom_ctx *ctx, int id, const double *host, size_t n);
+
+int pcom_get_counters(pcom_ctx *ctx, pcom_counters *c);
+
+/* Horizontal metric of stagger g (1..4) from the start-up chain
+ * (main.f90:408-497): which = 0 rh = cos(lat), 1 tn = tan(lat),
+ * 2 dx%x(1), 3 dx%x(2) (radians); halo'd 2-D, column-major.             */
+int pcom_get_metric(const pcom_ctx *ctx, int g, int which, double *out, size_t n);
+
+/* Area/volume-weighted global sums over sea points, accumulated in FP64 on
+ * the device: out[0] = sum(rh*dx1*dx2*dpr*msk*T), out[1] = same for S,
+ * out[2] = sum(rh*dx1*dx2*msk*chc) (column mass).  New: the reference has
+ * no global reduction besides the SSH mean (mod_mympi.f90:308-343).      */
+int pcom_global_sums(pcom_ctx *ctx, double out[3]);
+
+/* Returns PCOM_ERR_NAN if any prognostic field holds a non-finite value. */
+int pcom_check_finite(pcom_ctx *ctx);
+
+/* Wait for all work queued on the context's stream. */
+int pcom_sync(pcom_ctx *ctx);
+
+/* Average device duration (ms) of each phase over the steps since the last
+ * reset, measured with HIP events on the context's stream; phases are
+ * 0 = whole step, 1 = 3-D baroclinic kernels, 2 = barotropic subcycle.  */
+int pcom_phase_times(pcom_ctx *ctx, int enable, double out_ms[3], int *nsteps);
+
+int pcom_destroy(pcom_ctx *ctx);
+
+const char *pcom_last_error(void);
+
+/* Single-operator entry points on the context's grid, for parity tests.
+ * Stagger ids 1..4 are the reference's g1j..g4j (mod_arrays.f90:148-166);
+ * in/out are halo'd host arrays of nd3 levels.                          */
+
+/* op_ter for 2-D/3-D reals (mod_op.f90:361-427); dft is the prefill.   */
+int pcom_op_ter(pcom_ctx *ctx, int ga, int gb, int nd3, double dft,
+                const double *in, double *out);
+/* op_px1 / op_px2 (mod_op.f90:496-527, 597-652, 689-713).              */
+int pcom_op_px1(pcom_ctx *ctx, int ga, int gb, int nd3,
+                const double *in, double *out);
+int pcom_op_px2(pcom_ctx *ctx, int ga, int gb, int nd3,
+                const double *in, double *out);
+/* op_div (mod_op.f90:813-848): (va on ga, vb on gb) -> gc.             */
+int pcom_op_div(pcom_ctx *ctx, int ga, int gb, int gc, int nd3,
+                const double *va, const double *vb, double *out);
+/* op_lap (mod_op.f90:904-925) on 2-D fields.                           */
+int pcom_op_lap(pcom_ctx *ctx, int ga, int gb, const double *in, double *out);
+/* den_rho (mod_den.f90:57-118) on n points: t degC, s psu, p Pa.       */
+int pcom_den_rho(pcom_ctx *ctx, size_t n, const double *t, const double *s,
+                 const double *p, double *rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCOM_GPU_H */

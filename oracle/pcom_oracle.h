@@ -1,0 +1,81 @@
+/*
+ * pcom_oracle.h - CPU FP64 restatement of the PCOM time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the MI355X library is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links it.
+ *
+ * The restatement follows the reference routine by routine on full halo'd
+ * arrays (ni = glo_ni+2, nj = glo_nj+2), reproducing its write ranges,
+ * prefills, one-rank halo swap, loop orders and the float32 rounding of its
+ * default-REAL literals.  Citations are to ouyuyuan/pcom v2.2 files.
+ */
+#ifndef PCOM_ORACLE_H
+#define PCOM_ORACLE_H
+
+#include <stddef.h>
+#include "../include/pcom_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+orc *orc_create(const pcom_inputs *in, const pcom_opts *opts);
+void orc_destroy(orc *o);
+/* main.f90:158-227 for n steps (no output). */
+void orc_step(orc *o, int n);
+/* Same field ids and halo'd layout as pcom_get_field. Returns 0 or -1. */
+int orc_get(const orc *o, int id, double *out, size_t n);
+int orc_set(orc *o, int id, const double *in, size_t n);
+size_t orc_field_size(const orc *o, int id);
+void orc_dims(const orc *o, int *ni, int *nj, int *nk);
+void orc_counters(const orc *o, pcom_counters *c);
+
+/* Individual routines (state in o), for stage-wise parity tests. */
+void orc_den_prho(orc *o);
+void orc_den_alpha_main(orc *o);       /* main.f90:205                  */
+void orc_int_pgra(orc *o);
+void orc_int_readyc(orc *o);
+void orc_int_trop(orc *o);
+void orc_int_clin(orc *o);
+void orc_int_ssh(orc *o);
+void orc_int_ts(orc *o);
+void orc_int_bnd(orc *o);
+/* Advance the calendar as main.f90:225-226 does after int_ts. */
+void orc_advance_time(orc *o);
+/* Set tctr%i (the step index the AB3 bootstrap reads, mod_int.f90:122). */
+void orc_set_step_index(orc *o, long long i);
+
+/* Single operators on the context's grid (halo'd arrays, nd3 levels). */
+void orc_op_ter(orc *o, int ga, int gb, int nd3, double dft,
+                const double *in, double *out);
+void orc_op_px1(orc *o, int ga, int gb, int nd3, const double *in, double *out);
+void orc_op_px2(orc *o, int ga, int gb, int nd3, const double *in, double *out);
+void orc_op_div(orc *o, int ga, int gb, int gc, int nd3,
+                const double *va, const double *vb, double *out);
+void orc_op_lap(orc *o, int ga, int gb, const double *in, double *out);
+/* Horizontal metric arrays: which = 0 rh, 1 tn, 2 dx1, 3 dx2. */
+int orc_metric(const orc *o, int g, int which, double *out, size_t n);
+/* Vertical grid: which = 0 z, 1 pr, 2 dz, 3 dpr of gi1 (nk+1) or gi2 (nk). */
+int orc_vgrid(const orc *o, int gi, int which, double *out, size_t n);
+/* Integer grid: which = 0 g12 msk (3D), 1 g32 msk (3D), 2 g12 lev, 3 g32 lev. */
+int orc_igrid(const orc *o, int which, int *out, size_t n);
+
+/* mod_den.f90:57-118 */
+double orc_den_rho(double t, double s, double p0);
+void orc_den_rho_n(size_t n, const double *t, const double *s,
+                   const double *p, double *rho);
+/* mod_con.f90:16-41 constants as the active build evaluates them:
+ * names[i] and values[i] for i < count. */
+int orc_constants(const char ***names, const double **values);
+/* mod_int.f90:509-600 on one column (for brute-force tests). */
+void orc_convect_column(int nk, int nlev, const double *dpr,
+                        const double *prho_t, const double *prho_s,
+                        double *t, double *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,100 @@
+"""ctypes mirror of include/pcom_gpu.h (the C ABI of the MI355X library).
+
+The structs and field ids here must match the header exactly; tests check
+that every symbol the header declares is exported by the built library.
+"""
+import ctypes as C
+
+PCOM_OK = 0
+PCOM_ERR_ARG = -1
+PCOM_ERR_HIP = -2
+PCOM_ERR_NAN = -3
+PCOM_ERR_STATE = -4
+PCOM_ERR_NOGPU = -5
+
+PCOM_FP64 = 0
+PCOM_MIXED = 1
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+
+class PcomInputs(C.Structure):
+    _fields_ = [
+        ("glo_ni", C.c_int), ("glo_nj", C.c_int), ("nk", C.c_int),
+        ("lon", _DP), ("lat", _DP), ("z", _DP), ("itn", _IP),
+        ("tmn", _DP), ("smn", _DP), ("pt", _DP), ("sa", _DP), ("vmix", _DP),
+        ("taux", _DP), ("tauy", _DP), ("bct", _DP), ("bcs", _DP),
+        ("pa", _DP), ("fw", _DP),
+    ]
+
+
+class PcomOpts(C.Structure):
+    _fields_ = [
+        ("dtbt", C.c_int), ("dtbc", C.c_int),
+        ("year", C.c_int), ("month", C.c_int), ("day", C.c_int), ("hour", C.c_int),
+        ("precision", C.c_int), ("restart_run", C.c_int), ("device", C.c_int),
+    ]
+
+
+class PcomCounters(C.Structure):
+    _fields_ = [
+        ("step", C.c_longlong),
+        ("n_ts", C.c_int), ("n_uv", C.c_int), ("n_w", C.c_int),
+        ("n_ch", C.c_int), ("n_ssh", C.c_int),
+        ("year", C.c_int), ("month", C.c_int), ("day", C.c_int),
+        ("hour", C.c_int), ("minute", C.c_int), ("second", C.c_int),
+    ]
+
+
+# Field ids, in header order.
+FIELD_NAMES = [
+    "tc", "sc", "tp", "sp",
+    "uc", "vc", "up", "vp",
+    "auc", "avc", "aup", "avp", "aupp", "avpp",
+    "fx", "fy", "am",
+    "alpha", "adp",
+    "wm",
+    "prho_t", "prho_s",
+    "act", "acs", "acr", "acu", "acv", "acw",
+    "chc", "chp", "mch", "mch2",
+    "acch", "acssh",
+    "buc", "bvc", "bup", "bvp", "but", "bvt",
+    "pax", "pay",
+    "bphi_xn", "bphi_xs", "bphi_ye", "bphi_yw",
+    "bgra_xn", "bgra_xs", "bgra_ye", "bgra_yw",
+    "bnd_taux", "bnd_tauy", "bnd_t", "bnd_s", "bnd_pa", "bnd_fw",
+    "ssh",
+    "graphihx", "graphihy", "cor",
+    "phih_t", "prh_t", "phih_u", "prh_u",
+    "km", "kh",
+]
+FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
+FIELD_COUNT = len(FIELD_NAMES)
+
+# Fields with nk+1 levels and fields with nk levels; the rest are 2-D.
+LEVELS_NKP = {"wm", "km", "kh"}
+LEVELS_NK = {
+    "tc", "sc", "tp", "sp", "uc", "vc", "up", "vp", "auc", "avc", "aup", "avp",
+    "aupp", "avpp", "fx", "fy", "am", "alpha", "adp", "prho_t", "prho_s",
+    "act", "acs", "acr", "acu", "acv", "acw",
+}
+
+
+def field_levels(name, nk):
+    if name in LEVELS_NKP:
+        return nk + 1
+    if name in LEVELS_NK:
+        return nk
+    return 1
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array, or NULL for None."""
+    if a is None:
+        return _DP()
+    return a.ctypes.data_as(_DP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_IP)

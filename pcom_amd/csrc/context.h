@@ -1,0 +1,79 @@
+// context.h - the device-resident state of one PCOM time stepper.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "pcom_dev.h"
+#include "setup.h"
+
+namespace pcom {
+
+// 3-D fields are stored in R (double, or float in the mixed form); 2-D
+// fields (free surface, barotropic mode, forcing) are always double.
+struct Fields3 {
+  void *T[2], *S[2];        // potential temperature / salinity, ping-pong:
+                            // T[cur] is both tc and tp (int_ts ends with
+                            // tp = tc, mod_int.f90:496-497)
+  void *U, *V;              // uc == up, vc == vp (int_clin ends with up = uc)
+  void *AU[3], *AV[3];      // AB3 advection ring: auc = AU[ab], aup =
+                            // AU[ab-1], aupp = AU[ab-2]
+  void *fx, *fy, *am, *alpha, *adp, *wm, *prho1, *prho2;
+  void *act, *acs, *acr, *acu, *acv, *acw;
+  void *km, *kh;
+};
+
+struct Fields2 {
+  double *chc, *chp, *mch, *mch2, *acch, *acssh, *ssh;
+  double *buc, *bvc, *bup, *bvp, *but, *bvt, *pax, *pay;
+  double *bphi[4], *bgra[4];   // xn xs ye yw
+  double *bnd[6];              // taux tauy t s pa fw
+  double *frc[6];              // (ni,nj,12)
+  double *graphihx, *graphihy;
+  double *tu, *tv, *upb1, *upb2, *vz1, *vz2;   // subcycle scratch
+  double *red;                                  // reduction scratch
+  int *flag;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = PCOM_FP64;
+  int rst = 0;
+  int dtbt = 60, dtbc = 3600;
+  int ni = 0, nj = 0, nk = 0, nkp = 0;
+  size_t n2 = 0, n3 = 0, n3p = 0;
+  HostSetup hs;
+  Calendar ct, pt;
+  long long step_i = 0;
+  int n_ts = 0, n_uv = 0, n_w = 0, n_ch = 0, n_ssh = 0;
+  int cur = 0, ab = 0;
+  Grid<double> gd{};
+  Grid<float> gf{};
+  Fields3 f3{};
+  Fields2 f2{};
+  std::vector<void*> allocs;
+  int trop_blocks = 0;
+  // phase timing (HIP events on `stream`)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double t_ms[3] = {0, 0, 0};
+  int t_steps = 0;
+  size_t rsize() const { return precision == PCOM_FP64 ? sizeof(double) : sizeof(float); }
+};
+
+// kernels.hip
+std::string launch_step(Ctx& c);                 // one baroclinic step
+std::string launch_global_sums(Ctx& c, double out[3]);
+std::string launch_check_finite(Ctx& c, int* bad);
+std::string launch_trop_blocks(Ctx& c);          // size the subcycle grid
+// single-operator entry points on device buffers (FP64)
+std::string launch_op_ter(Ctx& c, int ga, int gb, int nd3, double dft, const double* in, double* out);
+std::string launch_op_px(Ctx& c, int dir, int ga, int gb, int nd3, const double* in, double* out);
+std::string launch_op_div(Ctx& c, int ga, int gb, int gc, int nd3, const double* va, const double* vb, double* out);
+std::string launch_op_lap(Ctx& c, int ga, int gb, const double* in, double* out);
+std::string launch_den_rho(hipStream_t st, size_t n, const double* t, const double* s, const double* p, double* rho);
+
+}  // namespace pcom

@@ -1,0 +1,1242 @@
+// kernels.hip - CDNA4 kernels of one PCOM baroclinic step (main.f90:158-227).
+//
+// Each kernel evaluates one or more reference routines per grid point with
+// the pointwise operators of pcom_dev.h, so intermediates the reference
+// materialises as whole arrays (and halo-swaps) stay in registers / cache.
+// 3-D kernels are templated on R (double; float in the mixed form); the
+// barotropic subcycle is FP64 always.
+//
+// Thread mapping: point kernels use one thread per (i, j, k) with i fastest
+// (coalesced along the Fortran-contiguous dimension); column kernels use
+// one thread per (i, j) and walk k in the reference's summation order so
+// that every column sum is accumulated exactly as Fortran's sum() does.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "context.h"
+
+namespace cg = cooperative_groups;
+
+namespace pcom {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline std::string hip_err(hipError_t e, const char* what) {
+  if (e == hipSuccess) return "";
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// decode a point index into 1-based (i, j, k)
+template <typename G>
+__device__ __forceinline__ void ijk(const G& g, long p, int& i, int& j, int& k) {
+  i = (int)(p % g.ni) + 1;
+  long q = p / g.ni;
+  j = (int)(q % g.nj) + 1;
+  k = (int)(q / g.nj) + 1;
+}
+
+// ---------------------------------------------------------------------------
+// daily forcing: int_bnd (mod_int.f90:678-689)
+struct Six {
+  const double* a[6];
+  double* b[6];
+};
+
+__global__ void k_bnd(long n2, Six s, int m1, int m2, double f) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n2) return;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) {
+    double a1 = s.a[v][p + n2 * (m1 - 1)], a2 = s.a[v][p + n2 * (m2 - 1)];
+    s.b[v][p] = (a2 - a1) * f + a1;
+  }
+}
+
+// atmospheric pressure gradient (mod_int.f90:100-103): gra_r2d_b(pa, g1)
+// then ter to g3.  bnd%pa changes only with int_bnd, so this runs daily.
+__global__ void k_pa(Grid<double> g, const double* __restrict__ pa, double* pax, double* pay) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  auto PA = [&](int m, int jj) -> double { return pa[g.ix(m, jj)]; };
+  auto WX = [&](int m, int jj) -> double { return grax<1, 2>(g, PA, m, jj); };
+  auto WY = [&](int m, int jj) -> double { return gray<1, 4>(g, PA, m, jj); };
+  pax[p] = ter_ns<2>(g, WX, i, j, 0.0);
+  pay[p] = ter_ew<4>(g, WY, i, j);
+}
+
+// den_prho (mod_den.f90:120-153)
+template <typename R>
+__global__ void k_prho(Grid<R> g, const R* __restrict__ tc, const R* __restrict__ sc, const double* __restrict__ chc,
+                       R* prho1, R* prho2) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  if (g.mskt(i, j, k) > 0) {
+    R t = tc[p], s = sc[p];
+    R pr = (R)chc[g.ix(i, j)] * g.pr2[k - 1];
+    R ra = den_rho<R>(t + (R)DELTAT, s, pr);
+    R rb = den_rho<R>(t - (R)DELTAT, s, pr);
+    prho1[p] = (ra - rb) / ((R)2 * (R)DELTAT);
+    ra = den_rho<R>(t, s + (R)DELTAS, pr);
+    rb = den_rho<R>(t, s - (R)DELTAS, pr);
+    prho2[p] = (ra - rb) / ((R)2 * (R)DELTAS);
+  }
+}
+
+// den_alpha (mod_den.f90:33-55): alpha = 1/rho where mask == 1; land is
+// left as it is (1/1000.0 from mod_arrays.f90:298).
+template <typename R>
+__global__ void k_alpha(Grid<R> g, const R* __restrict__ t, const R* __restrict__ s, const double* __restrict__ ch,
+                        R* alpha) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  if (g.mskt(i, j, k) == 1) alpha[p] = (R)1.0 / den_rho<R>(t[p], s[p], (R)ch[g.ix(i, j)] * g.pr2[k - 1]);
+}
+
+// int_tobot (mod_op.f90:927-944): adp(k) = t(k)/2 + sum(t(k+1:nk)),
+// t = alpha*dpr*msk; the suffix sum is accumulated in increasing k as
+// Fortran's sum() does, so FP64 results are the reference's bits.
+template <typename R>
+__global__ void k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const int nk = g.nk;
+  auto tt = [&](int k) -> R { return alpha[p + g.n2 * (k - 1)] * g.dpr2[k - 1] * (R)g.mskt(i, j, k); };
+  if constexpr (sizeof(R) == sizeof(double)) {
+    for (int k = 1; k <= nk - 1; ++k) {
+      R s = 0;
+      for (int kk = k + 1; kk <= nk; ++kk) s += tt(kk);
+      adp[p + g.n2 * (k - 1)] = tt(k) * (R)0.5 + s;
+    }
+    adp[p + g.n2 * (nk - 1)] = tt(nk) * (R)0.5;
+  } else {
+    // mixed form: one pass from the bottom, accumulated in FP64
+    double s = 0;
+    for (int k = nk; k >= 1; --k) {
+      double t = (double)tt(k);
+      adp[p + g.n2 * (k - 1)] = (R)(t * 0.5 + s);
+      s += t;
+    }
+  }
+}
+
+// int_pgra's face integrals (mod_int.f90:72-86 with op_vint_ns / op_vint_ew,
+// mod_op.f90:946-981): bphi = vint(ter(adp) + pr*ter(alpha)),
+// bgraphi = vint(gra(adp)), each with the local mask and the mask cshift'ed
+// by -1 (row 1 wraps to row nj; the E/W wrap is overwritten by the swap).
+template <typename R>
+__global__ void k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict__ alpha, double* b0, double* b1,
+                       double* b2, double* b3, double* c0, double* c1, double* c2, double* c3) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  int ic = g.canon(i);
+  int js = (j == 1) ? g.nj : j - 1;
+  double sxn = 0, sxs = 0, sye = 0, syw = 0, gxn = 0, gxs = 0, gye = 0, gyw = 0;
+  for (int k = 1; k <= g.nk; ++k) {
+    const R* ad = adp + g.n2 * (k - 1);
+    const R* al = alpha + g.n2 * (k - 1);
+    auto AD = [&](int m, int jj) -> R { return ad[g.ix(m, jj)]; };
+    auto AL = [&](int m, int jj) -> R { return al[g.ix(m, jj)]; };
+    R pr = g.pr2[k - 1], dpr = g.dpr2[k - 1];
+    R wk2 = ter_ew<1>(g, AD, ic, j) + pr * ter_ew<1>(g, AL, ic, j);
+    R wk4 = ter_ns<1>(g, AD, ic, j, (R)0) + pr * ter_ns<1>(g, AL, ic, j, (R)0);
+    R wx = grax<1, 2>(g, AD, ic, j);
+    R wy = gray<1, 4>(g, AD, ic, j);
+    R mu = (R)g.msku(ic, j, k), mus = (R)g.msku(ic, js, k), muw = (R)g.msku(ic - 1, j, k);
+    R v2 = wk2 * dpr, v4 = wk4 * dpr, vx = wx * dpr, vy = wy * dpr;
+    sxn += (double)(v2 * mu);
+    sxs += (double)(v2 * mus);
+    sye += (double)(v4 * mu);
+    syw += (double)(v4 * muw);
+    gxn += (double)(vx * mu);
+    gxs += (double)(vx * mus);
+    gye += (double)(vy * mu);
+    gyw += (double)(vy * muw);
+  }
+  double ph = g.prh_u[g.ix(ic, j)], phs = g.prh_u[g.ix(ic, js)], phw = g.prh_u[g.ix(ic - 1, j)];
+  b0[p] = sxn / ph;
+  b1[p] = sxs / phs;
+  b2[p] = sye / ph;
+  b3[p] = syw / phw;
+  c0[p] = gxn / ph;
+  c1[p] = gxs / phs;
+  c2[p] = gye / ph;
+  c3[p] = gyw / phw;
+}
+
+// upwelling (mod_int.f90:626-647): wm from the continuity equation;
+// with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).
+template <typename R>
+__global__ void k_upw(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const double* __restrict__ chp,
+                      R* wm, R* acw, const R* __restrict__ alpha, const double* __restrict__ mch2) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  auto CHP = [&](int m, int jj) -> R { return (R)chp[g.ix(m, jj)]; };
+  auto SQ = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, CHP, m, jj, (R)0)); };
+  auto wk3 = [&](int k) -> R {
+    const R* u = uc + g.n2 * (k - 1);
+    const R* v = vc + g.n2 * (k - 1);
+    auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * u[g.ix(m, jj)]; };
+    auto VB = [&](int m, int jj) -> R { return SQ(m, jj) * v[g.ix(m, jj)]; };
+    return divg<3, 3, 1>(g, VA, VB, i, j);
+  };
+  R s = 0;
+  for (int k = 1; k <= g.nk; ++k) s += wk3(k) * g.dpr2[k - 1] * (R)g.mskt(i, j, k);
+  R wk2 = -(s / g.prh_t_r[p]);
+  R w = wm[p];  // wm(1) is never written by upwelling
+  if (acw) acw[p] = acw[p] + w * alpha[p] / ((R)GRAV * (R)mch2[p]);
+  R prev = wk3(1);
+  for (int k = 2; k <= g.nk; ++k) {
+    long q = p + g.n2 * (k - 1);
+    w = w + g.dpr2[k - 2] * (wk2 + prev);
+    w = w * (R)g.mskt(i, j, k);
+    wm[q] = w;
+    if (acw) acw[q] = acw[q] + w * alpha[q] / ((R)GRAV * (R)mch2[p]);
+    if (k < g.nk) prev = wk3(k);
+  }
+}
+
+// sqrt(ter(chc, g1 -> g3, dft = 1)), land = 1 (mod_int.f90:107-109)
+template <typename R>
+__device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* chc, int m, int jj) {
+  if (g.lev_u[g.ix(m, jj)] == 0) return (R)1.0;
+  auto CH = [&](int a, int b) -> R { return (R)chc[g.ix(a, b)]; };
+  return sqrt(ter_di<1>(g, CH, m, jj, (R)1.0));
+}
+
+// op_adv (mod_op.f90:224-278) for uc and vc at one point.
+template <typename R>
+__global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ wm,
+                      const double* __restrict__ chc, R* auc, R* avc) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  const long ko = g.n2 * (k - 1);
+  const R* u = uc + ko;
+  const R* v = vc + ko;
+  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, chc, m, jj); };
+  auto X = [&](int m, int jj) -> R { return u[g.ix(m, jj)] / SCH(m, jj); };
+  auto Y = [&](int m, int jj) -> R {
+    R y = v[g.ix(m, jj)] / SCH(m, jj);
+    return y * g.RH(3, m, jj);
+  };
+  auto UU = [&](int m, int jj) -> R { return ter_ew<3>(g, X, m, jj); };          // g3 -> g4
+  auto VV = [&](int m, int jj) -> R { return ter_ns<3>(g, Y, m, jj, (R)0); };    // g3 -> g2
+  // w = ter(wm/chc, g1 -> g3) on level kk of nkp
+  auto W = [&](int kk) -> R {
+    const R* w = wm + g.n2 * (kk - 1);
+    auto WW = [&](int m, int jj) -> R { return w[g.ix(m, jj)] / (R)chc[g.ix(m, jj)]; };
+    return ter_di<1>(g, WW, i, j, (R)0);
+  };
+  const R w_k = W(k), w_k1 = W(k + 1);
+  const R rh3 = g.RH(3, i, j);
+  const R dpr = g.dpr2[k - 1];
+  const R d1 = p1<4, 3>(g, UU, i, j);
+  const R d2 = p2<2, 3>(g, VV, i, j);
+  const R d3 = (w_k - w_k1) / dpr;
+  const R mu = (R)g.msku(i, j, k);
+  R out[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const R* var = c == 0 ? uc : vc;
+    const R* vk = var + ko;
+    auto VK = [&](int m, int jj) -> R { return vk[g.ix(m, jj)]; };
+    auto M1 = [&](int m, int jj) -> R { return ter_ew<3>(g, VK, m, jj) * UU(m, jj); };
+    auto M2 = [&](int m, int jj) -> R { return ter_ns<3>(g, VK, m, jj, (R)0) * VV(m, jj); };
+    R ans = (p1<4, 3>(g, M1, i, j) + p2<2, 3>(g, M2, i, j)) / ((R)ARAD * rh3);
+    // vertical flux through the U-cell faces, zero unless both layers wet
+    auto WB = [&](int kk) -> R {
+      if (kk < 2 || kk > g.nk) return (R)0;
+      R a = (var[g.ix(i, j, kk - 1)] + var[g.ix(i, j, kk)]) * (R)0.5;
+      a = a * (kk == k ? w_k : w_k1);
+      return a * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk);
+    };
+    ans = ans + (WB(k) - WB(k + 1)) / dpr;
+    ans = ans - (R)0.5 * vk[g.ix(i, j)] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
+    out[c] = ans * mu;
+  }
+  auc[p] = out[0];
+  avc[p] = out[1];
+}
+
+// fri_r3d (mod_op.f90:103-167) at one point: horizontal viscosity,
+// metric terms, vertical viscosity with wind stress and bottom drag.
+// v / vcc are callables (unweighted velocity and the "vc" argument).
+template <typename R, typename FV, typename FC>
+__device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v, const R* vp, FC vcc, R tau,
+                                       R am, R sch, int i, int j, int k) {
+  auto GX = [&](int m, int jj) -> R { return grax<3, 4>(g, v, m, jj); };
+  auto GY = [&](int m, int jj) -> R { return gray<3, 2>(g, v, m, jj); };
+  auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
+  auto VA = [&](int m, int jj) -> R { return ter_ns<1>(g, CH, m, jj, (R)0) * GX(m, jj); };
+  auto VB = [&](int m, int jj) -> R { return ter_ew<1>(g, CH, m, jj) * GY(m, jj); };
+  R wka = divg<4, 2, 3>(g, VA, VB, i, j);
+  long q = g.ix(i, j);
+  R ans = am / sch * wka;
+  ans = ans + am * g.cv1[q] * vp[g.ix(i, j, k)];
+  int ic = g.canon(i);
+  R dx = (R)ARAD * g.RH(4, ic, j) * (g.DX1(4, ic, j) + g.DX1(4, ic + 1, j));
+  R pxc = (vcc(ic + 1, j, k) - vcc(ic - 1, j, k)) / dx;
+  ans = ans + am * g.cv2[q] * sch * pxc;
+  const int lev = g.lev_u[q];
+  auto WKP = [&](int kk) -> R {
+    if (kk == 1) return (R)GRAV * tau / sch;
+    if (lev > 0 && kk == lev + 1) {
+      R vv = v(i, j, lev), cc = vcc(i, j, lev);
+      R mag = sqrt(vv * vv + cc * cc);
+      return vv * (R)GRAV * (R)CDBOT * mag / sch;
+    }
+    if (kk > g.nk) return (R)0;
+    R pz = (vp[g.ix(i, j, kk - 1)] - vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk) /
+           g.dz1[kk - 1];
+    return (R)GRAV * (R)RHO0 * g.km[g.ix(i, j, kk)] * pz / (sch * sch);
+  };
+  R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
+  return ans * (R)g.msku(i, j, k) + vert;
+}
+
+// op_fri (mod_op.f90:169-184): fx from (u, uc, -v, taux), fy from
+// (v, vc, u, tauy) with u = uc/sch, v = vc/sch.
+template <typename R>
+__global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ am,
+                      const double* __restrict__ chc, const double* __restrict__ taux,
+                      const double* __restrict__ tauy, R* fx, R* fy) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, chc, m, jj); };
+  // 3-D callables (m, jj, kk)
+  auto U3 = [&](int m, int jj, int kk) -> R { return uc[g.ix(m, jj, kk)] / SCH(m, jj); };
+  auto V3 = [&](int m, int jj, int kk) -> R { return vc[g.ix(m, jj, kk)] / SCH(m, jj); };
+  auto MV3 = [&](int m, int jj, int kk) -> R { return -V3(m, jj, kk); };
+  const long q = g.ix(i, j);
+  const R sch = SCH(i, j);
+  const R a = am[p];
+  // the 2-D (m, jj) views at level k that the horizontal operators take,
+  // plus a 3-D view for the bottom drag level
+  struct View {
+    const Grid<R>& g;
+    int k;
+    bool isu;
+    const R* uc;
+    const R* vc;
+    const double* chc;
+    __device__ R operator()(int m, int jj) const {
+      R s = sch_ready<R>(g, chc, m, jj);
+      return (isu ? uc : vc)[g.ix(m, jj, k)] / s;
+    }
+    __device__ R operator()(int m, int jj, int kk) const {
+      R s = sch_ready<R>(g, chc, m, jj);
+      return (isu ? uc : vc)[g.ix(m, jj, kk)] / s;
+    }
+  };
+  View vu{g, k, true, uc, vc, chc};
+  View vv{g, k, false, uc, vc, chc};
+  (void)U3;
+  (void)V3;
+  fx[p] = fri_point<R>(g, chc, vu, uc, MV3, (R)taux[q], a, sch, i, j, k);
+  fy[p] = fri_point<R>(g, chc, vv, vc, vu, (R)tauy[q], a, sch, i, j, k);
+}
+
+// column integral of the baroclinic tendency (mod_int.f90:136-149)
+template <typename R>
+__global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__ avc, const R* __restrict__ fx,
+                     const R* __restrict__ fy, const double* __restrict__ chc, const double* __restrict__ pax,
+                     const double* __restrict__ pay, double* ut, double* vt) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const R sch = sch_ready<R>(g, chc, i, j);
+  const R px = (R)pax[p], py = (R)pay[p];
+  R su = 0, sv = 0;
+  for (int k = 1; k <= g.nk; ++k) {
+    long q = p + g.n2 * (k - 1);
+    R x = -auc[q] - sch * (R)A0 * px + fx[q];
+    R y = -avc[q] - sch * (R)A0 * py + fy[q];
+    R mu = (R)g.msku(i, j, k);
+    su = su + x * g.dpr2[k - 1] * mu;
+    sv = sv + y * g.dpr2[k - 1] * mu;
+  }
+  ut[p] = (double)su / g.prh_u[p];
+  vt[p] = (double)sv / g.prh_u[p];
+}
+
+// ---------------------------------------------------------------------------
+// int_trop (mod_int.f90:153-296): the barotropic predictor-corrector
+// subcycle in FP64, one cooperative launch for all 2*dtbc/dtbt substeps.
+// Per substep: A) predictor ch on T points, B) momentum on U points,
+// C) corrector ch + time-filter accumulators; grid barriers between.
+template <typename R>
+struct TropArgs {
+  Grid<double> g;
+  const R* am;  // level 1 of am (g32)
+  double *chc, *chp, *mch, *mch2, *acch;
+  double *buc, *bvc, *bup, *bvp;
+  const double *but, *bvt;
+  const double *bphi[4], *bgra[4];
+  const double *graphihx, *graphihy, *cor;
+  double *tu, *tv, *upb1, *upb2, *vz1, *vz2;
+  int nstep;
+  double bt;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_trop(TropArgs<R> a) {
+  cg::grid_group grid = cg::this_grid();
+  const Grid<double>& g = a.g;
+  const long n2 = g.n2;
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long p0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double bt = a.bt;
+  const int nstep = a.nstep;
+
+  for (long p = p0; p < n2; p += stride) {
+    a.upb1[p] = a.buc[p];
+    a.upb2[p] = a.bvc[p];
+    a.mch[p] = a.chc[p];
+    a.mch2[p] = a.chc[p];
+  }
+
+  auto CHP = [&](int m, int jj) -> double { return a.chp[g.ix(m, jj)]; };
+  auto CHC = [&](int m, int jj) -> double { return a.chc[g.ix(m, jj)]; };
+  auto schp = [&](int m, int jj) -> double {
+    return g.lev_u[g.ix(m, jj)] > 0 ? sqrt(ter_di<1>(g, CHP, m, jj, 0.0)) : 1.0;
+  };
+  auto schc = [&](int m, int jj) -> double {
+    return g.lev_u[g.ix(m, jj)] > 0 ? sqrt(ter_di<1>(g, CHC, m, jj, 0.0)) : 1.0;
+  };
+
+  for (int nt = 1; nt <= nstep; ++nt) {
+    double* pu = (nt & 1) ? a.bup : a.tu;  // up, vp of this substep
+    double* pv = (nt & 1) ? a.bvp : a.tv;
+    double* qu = (nt & 1) ? a.tu : a.bup;  // uc, vc written by it
+    double* qv = (nt & 1) ? a.tv : a.bvp;
+
+    // A) predictor: chc = chp - gamma_b*dtbt*div(prh*sch*ub_p)/prh (sea)
+    for (long p = p0; p < n2; p += stride) {
+      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+      auto S1 = [&](int m, int jj) -> double {
+        long q = g.ix(m, jj);
+        return g.prh_u[q] * schp(m, jj) * pu[q];
+      };
+      auto S2 = [&](int m, int jj) -> double {
+        long q = g.ix(m, jj);
+        return g.prh_u[q] * schp(m, jj) * pv[q];
+      };
+      double wk = divg<3, 3, 1>(g, S1, S2, i, j);
+      wk = a.chp[p] - wk * bt * GAMMA_B / g.prh_t[p];
+      if (g.lev_t[p] > 0) a.chc[p] = wk;
+    }
+    grid.sync();
+
+    // B) momentum: viscosity, pressure gradient, Coriolis adjustment
+    for (long p = p0; p < n2; p += stride) {
+      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+      int ic = g.canon(i);
+      const double su = g.lev_u[p] >= 1 ? 1.0 : 0.0;
+      auto PU = [&](int m, int jj) -> double { return pu[g.ix(m, jj)]; };
+      auto PV = [&](int m, int jj) -> double { return pv[g.ix(m, jj)]; };
+      double l1 = lap<3, 3>(g, PU, i, j);
+      double l2 = lap<3, 3>(g, PV, i, j);
+      double am1 = (double)a.am[p];
+      double vs1 = su * am1 * l1;
+      double vs2 = su * am1 * l2;
+      if (nt == 1) {
+        a.vz1[p] = vs1;
+        a.vz2[p] = vs2;
+      }
+      const double z1 = a.vz1[p], z2 = a.vz2[p];
+      auto GX = [&](int m, int jj) -> double { return grax<1, 2>(g, CHC, m, jj); };
+      auto GY = [&](int m, int jj) -> double { return gray<1, 4>(g, CHC, m, jj); };
+      auto C2 = [&](int m, int jj) -> double { return ter_ew<1>(g, CHC, m, jj); };
+      auto C4 = [&](int m, int jj) -> double { return ter_ns<1>(g, CHC, m, jj, 0.0); };
+      double gi1 = 0.0, ci1 = 0.0;  // row nj is never written (local array)
+      if (j <= g.njm) {
+        gi1 = 0.5 * (GX(ic, j) * a.bphi[0][g.ix(ic, j)] + GX(ic, j + 1) * a.bphi[1][g.ix(ic, j + 1)]);
+        ci1 = 0.5 * (C2(ic, j) * a.bgra[0][g.ix(ic, j)] + C2(ic, j + 1) * a.bgra[1][g.ix(ic, j + 1)]);
+      }
+      double gi2 = 0.5 * (GY(ic, j) * a.bphi[2][g.ix(ic, j)] + GY(ic + 1, j) * a.bphi[3][g.ix(ic + 1, j)]);
+      double ci2 = 0.5 * (C4(ic, j) * a.bgra[2][g.ix(ic, j)] + C4(ic + 1, j) * a.bgra[3][g.ix(ic + 1, j)]);
+      const double sch = schc(i, j);
+      const double cor = a.cor[p];
+      double pg1 = (gi1 + ci1 + a.graphihx[p]) * su * sch;
+      double pg2 = (gi2 + ci2 + a.graphihy[p]) * su * sch;
+      const double up = pu[p], vp = pv[p];
+      double pu1 = su * (vs1 - z1 + a.but[p] - pg1 + 0.5 * cor * vp);
+      double pu2 = su * (vs2 - z2 + a.bvt[p] - pg2 - 0.5 * cor * up);
+      double w1 = pu1 * bt + up;
+      double w2 = pu2 * bt + vp;
+      double x = 0.5 * cor * bt;
+      double uc = (w1 + 0.5 * cor * bt * w2) / (1 + x * x);
+      double vc = (w2 - 0.5 * cor * bt * w1) / (1 + x * x);
+      uc = 0.5 * (uc + up);
+      vc = 0.5 * (vc + vp);
+      pu1 = su * (vs1 - z1 - pg1 + a.but[p] + cor * vc);
+      pu2 = su * (vs2 - z2 - pg2 + a.bvt[p] - cor * uc);
+      qu[p] = pu1 * bt + up;
+      qv[p] = pu2 * bt + vp;
+    }
+    grid.sync();
+
+    // C) corrector and the time-filter accumulators
+    for (long p = p0; p < n2; p += stride) {
+      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+      auto Q1 = [&](int m, int jj) -> double {
+        long q = g.ix(m, jj);
+        return g.prh_u[q] * schc(m, jj) * qu[q];
+      };
+      auto Q2 = [&](int m, int jj) -> double {
+        long q = g.ix(m, jj);
+        return g.prh_u[q] * schc(m, jj) * qv[q];
+      };
+      double wk = divg<3, 3, 1>(g, Q1, Q2, i, j);
+      double mt = g.lev_t[p] >= 1 ? 1.0 : 0.0;
+      double c = a.chp[p] - mt * wk * bt / g.prh_t[p];
+      if (nt <= nstep / 2) a.mch[p] = a.mch[p] + c;
+      a.chp[p] = c;
+      a.upb1[p] = a.upb1[p] + qu[p];
+      a.upb2[p] = a.upb2[p] + qv[p];
+      a.mch2[p] = a.mch2[p] + c;
+    }
+    grid.sync();
+  }
+
+  for (long p = p0; p < n2; p += stride) {
+    double u = a.upb1[p] / (double)(nstep + 1);
+    double v = a.upb2[p] / (double)(nstep + 1);
+    a.buc[p] = u;
+    a.bvc[p] = v;
+    a.bup[p] = u;
+    a.bvp[p] = v;
+    a.mch[p] = a.mch[p] / (double)(nstep / 2 + 1);
+    double m2 = a.mch2[p] / (double)(nstep + 1);
+    a.mch2[p] = m2;
+    a.chc[p] = m2;
+    a.chp[p] = m2;
+    a.acch[p] = a.acch[p] + m2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// int_clin (mod_int.f90:314-368): baroclinic momentum at one point,
+// updated in place (uc == up; every read of uc/vc here is pointwise).
+template <typename R>
+struct ClinArgs {
+  Grid<R> g;
+  R *uc, *vc;
+  const R *auc, *aup, *aupp, *avc, *avp, *avpp, *fx, *fy, *adp, *alpha;
+  const double *mch, *pax, *pay;
+  double bc;
+};
+
+template <typename R>
+__global__ void k_clin(ClinArgs<R> a) {
+  const Grid<R>& g = a.g;
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  const R c_tc = (R)(23.0f / 12.0f), c_tp = (R)(-(16.0f / 12.0f)), c_tpp = (R)(5.0f / 12.0f);
+  const long q = g.ix(i, j);
+  const long ko = g.n2 * (k - 1);
+  auto MCH = [&](int m, int jj) -> R { return (R)a.mch[g.ix(m, jj)]; };
+  R sch = (R)1.0;
+  if (g.lev_u[q] > 0) sch = sqrt(ter_di<1>(g, MCH, i, j, (R)1.0));
+  const R* ad = a.adp + ko;
+  const R* al = a.alpha + ko;
+  const R pr = g.pr2[k - 1];
+  auto WK = [&](int m, int jj) -> R {
+    long r = g.ix(m, jj);
+    return (R)g.phih_t[r] + (R)a.mch[r] * ad[r];
+  };
+  auto AL = [&](int m, int jj) -> R { return al[g.ix(m, jj)]; };
+  auto PG1 = [&](int m, int jj) -> R {
+    return grax<1, 2>(g, WK, m, jj) + ter_ew<1>(g, AL, m, jj) * pr * grax<1, 2>(g, MCH, m, jj);
+  };
+  auto PG2 = [&](int m, int jj) -> R {
+    return gray<1, 4>(g, WK, m, jj) + ter_ns<1>(g, AL, m, jj, (R)0) * pr * gray<1, 4>(g, MCH, m, jj);
+  };
+  R wk = ter_ns<2>(g, PG1, i, j, (R)0);
+  R pu1 = -((wk + (R)A0 * (R)a.pax[q]) * sch);
+  wk = ter_ew<4>(g, PG2, i, j);
+  R pu2 = -((wk + (R)A0 * (R)a.pay[q]) * sch);
+  pu1 = pu1 - (c_tc * a.auc[p] + c_tp * a.aup[p] + c_tpp * a.aupp[p]);
+  pu2 = pu2 - (c_tc * a.avc[p] + c_tp * a.avp[p] + c_tpp * a.avpp[p]);
+  pu1 = pu1 + a.fx[p];
+  pu2 = pu2 + a.fy[p];
+  const R cor = g.cor[q];
+  const R u0 = a.uc[p], v0 = a.vc[p];
+  pu1 = pu1 + cor * v0;
+  pu2 = pu2 - cor * u0;
+  const R x1 = pu1, x2 = pu2;
+  const R c = (R)0.5 * cor * (R)a.bc;
+  pu1 = x1 + c * x2;
+  pu2 = x2 - c * x1;
+  const R d = (R)1 + c * c;
+  const R mu = (R)g.msku(i, j, k);
+  pu1 = pu1 * mu / d;
+  pu2 = pu2 * mu / d;
+  a.uc[p] = u0 + pu1 * (R)a.bc;
+  a.vc[p] = v0 + pu2 * (R)a.bc;
+}
+
+// barotropic / baroclinic mode coupling (mod_int.f90:372-380)
+template <typename R>
+__global__ void k_clin_cpl(Grid<R> g, R* uc, R* vc, const double* __restrict__ buc, const double* __restrict__ bvc) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  R s1 = 0, s2 = 0;
+  for (int k = 1; k <= g.nk; ++k) {
+    long q = p + g.n2 * (k - 1);
+    R mu = (R)g.msku(i, j, k);
+    s1 += uc[q] * g.dpr2[k - 1] * mu;
+    s2 += vc[q] * g.dpr2[k - 1] * mu;
+  }
+  R m1, m2;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    m1 = s1 / g.prh_u_r[p];
+    m2 = s2 / g.prh_u_r[p];
+  } else {
+    m1 = (R)((double)s1 / g.prh_u[p]);
+    m2 = (R)((double)s2 / g.prh_u[p]);
+  }
+  const R bu = (R)buc[p], bv = (R)bvc[p];
+  for (int k = 1; k <= g.nk; ++k) {
+    if (g.msku(i, j, k) > 0) {
+      long q = p + g.n2 * (k - 1);
+      uc[q] = uc[q] - m1 + bu;
+      vc[q] = vc[q] - m2 + bv;
+    }
+  }
+}
+
+// unweighted velocity, its accumulator and smag (mod_int.f90:384-440)
+template <typename R>
+__global__ void k_smag(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc,
+                       const double* __restrict__ mch2, R* acu, R* acv, R* am, int dtbc) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  const long ko = g.n2 * (k - 1);
+  auto M2 = [&](int m, int jj) -> R { return (R)mch2[g.ix(m, jj)]; };
+  auto S2 = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, M2, m, jj, (R)1.0)); };
+  auto U1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
+  auto U2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
+  acu[p] = acu[p] + U1(i, j);
+  acv[p] = acv[p] + U2(i, j);
+  if (j >= 2 && j <= g.nj - 1) {
+    int ic = g.canon(i);
+    R dx = (R)ARAD * (g.RH(4, ic, j) * g.DX1(4, ic, j) + g.RH(4, ic + 1, j) * g.DX1(4, ic + 1, j));
+    R dy = (R)ARAD * (g.DX2(2, ic, j) + g.DX2(2, ic, j + 1));
+    R tn = g.tn[3][g.ix(ic, j)];
+    R dt = (R)4 * (U1(ic + 1, j) - U1(ic - 1, j)) / dx - (R)4 * (U2(ic, j + 1) - U2(ic, j - 1)) / dy +
+           U2(ic, j) * tn / (R)ARAD;
+    R ds = (R)4 * (U2(ic + 1, j) - U2(ic - 1, j)) / dx + (R)4 * (U1(ic, j + 1) - U1(ic, j - 1)) / dy -
+           U1(ic, j) * tn / (R)ARAD;
+    R cc = (R)0.5 * (R)SMAG_C / (R)PI_F * ((R)0.5 * dx + (R)0.5 * dy);
+    R c = cc * cc;
+    R wk = c * sqrt(dt * dt + ds * ds);
+    R amax = (dx * dx + dy * dy) / (R)(4 * 8 * dtbc);
+    am[p] = (amax < wk) ? amax : wk;
+  }
+}
+
+// int_ssh (mod_int.f90:602-624): alpha from (tp, sp, mch), column SSH
+template <typename R>
+__global__ void k_ssh(Grid<R> g, const R* __restrict__ tp, const R* __restrict__ sp, const double* __restrict__ mch,
+                      R* alpha, double* ssh, double* acssh) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const R m = (R)mch[p];
+  R s = 0;
+  for (int k = 1; k <= g.nk; ++k) {
+    long q = p + g.n2 * (k - 1);
+    int mk = g.mskt(i, j, k);
+    R al = alpha[q];
+    if (mk == 1) {
+      al = (R)1.0 / den_rho<R>(tp[q], sp[q], m * g.pr2[k - 1]);
+      alpha[q] = al;
+    }
+    s += al * g.dpr2[k - 1] * m * (R)mk;
+  }
+  double h;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    h = s / GRAV;
+  } else {
+    h = (double)s / GRAV;
+  }
+  h = g.phih_t[p] / GRAV + h;
+  ssh[p] = h;
+  acssh[p] = acssh[p] + h;
+}
+
+// op_adv_ts (mod_op.f90:280-325) + op_dif (186-222) + restoring + forward
+// step + freezing clamp (mod_int.f90:464-491).  Reads the old T/S (tc ==
+// tp), writes the new T/S into the other buffer of the pair.
+template <typename R>
+struct TsArgs {
+  Grid<R> g;
+  const R *t0, *s0, *wm, *uc, *vc;
+  const double *chc, *mch, *bndt, *bnds;
+  R *t1, *s1;
+  double bc;
+};
+
+template <typename R, typename FV>
+__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* varp, const R* wm, const double* chc,
+                                          const R* uc, const R* vc, int i, int j, int k) {
+  const long ko = g.n2 * (k - 1);
+  auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
+  auto SCH = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, CH, m, jj, (R)1.0)); };
+  auto UM1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * SCH(m, jj); };
+  auto UM2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
+  auto X = [&](int m, int jj) -> R { return ter_ns<3>(g, UM1, m, jj, (R)0) * p1<1, 2>(g, var, m, jj); };
+  auto Y = [&](int m, int jj) -> R { return ter_ew<3>(g, UM2, m, jj) * p2<1, 4>(g, var, m, jj); };
+  const R rh1 = g.RH(1, i, j);
+  R ans = ter_ew<2>(g, X, i, j) / ((R)ARAD * rh1);
+  ans = ans + ter_ns<4>(g, Y, i, j, (R)0) / ((R)ARAD * rh1);
+  auto DX3 = [&](int kk) -> R {
+    if (kk < 2 || kk > g.nk) return (R)0;
+    return varp[g.ix(i, j, kk - 1)] - varp[g.ix(i, j, kk)];
+  };
+  R wka = (wm[g.ix(i, j, k)] * DX3(k) + wm[g.ix(i, j, k + 1)] * DX3(k + 1)) * (R)0.5;
+  wka = wka / g.dpr2[k - 1];
+  return ans + wka;
+}
+
+template <typename R, typename FV>
+__device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const R* varp, const double* ch, int i, int j,
+                                       int k) {
+  auto MX = [&](int m, int jj) -> R {
+    int mc = g.canon(m);
+    R d = (var(mc + 1, jj) - var(mc, jj)) * (R)g.mskt(mc + 1, jj, k) * (R)g.mskt(mc, jj, k);
+    d = d / g.DX1(2, mc, jj);
+    return d / ((R)ARAD * g.RH(2, mc, jj));
+  };
+  auto MY = [&](int m, int jj) -> R {
+    int mc = g.canon(m);
+    R d = (jj > g.njm) ? (R)0 : (var(mc, jj + 1) - var(mc, jj)) * (R)g.mskt(mc, jj + 1, k) * (R)g.mskt(mc, jj, k);
+    d = d / g.DX2(4, mc, jj);
+    return d / (R)ARAD;
+  };
+  auto CH = [&](int m, int jj) -> R { return (R)ch[g.ix(m, jj)]; };
+  auto VA = [&](int m, int jj) -> R { return ter_ew<1>(g, CH, m, jj) * MX(m, jj); };
+  auto VB = [&](int m, int jj) -> R { return ter_ns<1>(g, CH, m, jj, (R)0) * MY(m, jj); };
+  R ans = (R)AH_C * divg<2, 4, 1>(g, VA, VB, i, j);
+  auto WKP = [&](int kk) -> R {
+    R v = (R)0;
+    if (kk >= 2 && kk <= g.nk)
+      v = (varp[g.ix(i, j, kk - 1)] - varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) * (R)g.mskt(i, j, kk) /
+          g.dz1[kk - 1];
+    return (R)GRAV * (R)RHO0 * g.kh[g.ix(i, j, kk)] * v;
+  };
+  return ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
+}
+
+template <typename R>
+__global__ void k_ts(TsArgs<R> a) {
+  const Grid<R>& g = a.g;
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * g.nk) return;
+  int i, j, k;
+  ijk(g, p, i, j, k);
+  const long ko = g.n2 * (k - 1);
+  const long q = g.ix(i, j);
+  auto TV = [&](int m, int jj) -> R { return a.t0[ko + g.ix(m, jj)]; };
+  auto SV = [&](int m, int jj) -> R { return a.s0[ko + g.ix(m, jj)]; };
+  R pt1 = -adv_ts_point<R>(g, TV, a.t0, a.wm, a.chc, a.uc, a.vc, i, j, k);
+  R pt2 = -adv_ts_point<R>(g, SV, a.s0, a.wm, a.chc, a.uc, a.vc, i, j, k);
+  pt1 = pt1 + dif_point<R>(g, TV, a.t0, a.mch, i, j, k);
+  pt2 = pt2 + dif_point<R>(g, SV, a.s0, a.mch, i, j, k);
+  const R mk = (R)g.mskt(i, j, k);
+  const R m = (R)a.mch[q];
+  if (k == 1) {
+    pt1 = pt1 + (R)GAMMA_T * ((R)a.bndt[q] - a.t0[p]) * mk / g.dpr2[0];
+    pt2 = pt2 + (R)GAMMA_S * ((R)a.bnds[q] - a.s0[p]) * mk * m;
+  }
+  R tn = a.t0[p] + pt1 * (R)a.bc * mk / m;
+  R sn = a.s0[p] + pt2 * (R)a.bc * mk / m;
+  if (g.mskt(i, j, k) > 0 && tn < (R)TICE) tn = (R)TICE;
+  a.t1[p] = tn;
+  a.s1[p] = sn;
+}
+
+// convect / adjust / mix (mod_int.f90:509-600) on one column, in place.
+// rhoa(k) = prho(k).(t(k), s(k)) and rhob(k) = prho(k+1).(t(k), s(k)) are
+// kept equal to those expressions of the current column by mix(), and
+// every entry the search reads is one mix() has kept current, so they are
+// evaluated on demand instead of stored.
+template <typename R>
+struct Column {
+  R* t;
+  R* s;
+  const R* p1;
+  const R* p2;
+  const R* dpr;
+  long st;
+  __device__ R rhoa(int k) const { return p1[st * (k - 1)] * t[st * (k - 1)] + p2[st * (k - 1)] * s[st * (k - 1)]; }
+  __device__ R rhob(int k) const { return p1[st * k] * t[st * (k - 1)] + p2[st * k] * s[st * (k - 1)]; }
+  __device__ void mix(int na, int nb) {
+    R sumt = 0, sumd = 0, sums = 0;
+    for (int k = na; k <= nb; ++k) sumt += t[st * (k - 1)] * dpr[k - 1];
+    for (int k = na; k <= nb; ++k) sumd += dpr[k - 1];
+    R tv = sumt / sumd;
+    for (int k = na; k <= nb; ++k) sums += s[st * (k - 1)] * dpr[k - 1];
+    R sv = sums / sumd;
+    for (int k = na; k <= nb; ++k) {
+      t[st * (k - 1)] = tv;
+      s[st * (k - 1)] = sv;
+    }
+  }
+};
+
+template <typename R>
+__global__ void k_conv(Grid<R> g, R* tc, R* sc, const R* __restrict__ prho1, const R* __restrict__ prho2,
+                       const double* __restrict__ chc, R* act, R* acs, R* acr) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const int nlev = g.lev_t[p];
+  if (nlev > 1) {
+    Column<R> c{tc + p, sc + p, prho1 + p, prho2 + p, g.dpr2, g.n2};
+    int kcnt = 1;
+    while (kcnt < nlev) {
+      if (c.rhob(kcnt) > c.rhoa(kcnt + 1)) {
+        int na = kcnt, nb = kcnt + 1;
+        c.mix(na, nb);
+        while (nb < nlev) {
+          if (c.rhob(nb) > c.rhoa(nb + 1)) {
+            nb = nb + 1;
+            c.mix(na, nb);
+          } else {
+            if (na <= 1 || c.rhob(na - 1) <= c.rhoa(na)) break;
+            na = na - 1;
+            c.mix(na, nb);
+          }
+        }
+        while (na > 1) {
+          if (c.rhob(na - 1) <= c.rhoa(na)) break;
+          na = na - 1;
+          c.mix(na, nb);
+        }
+        kcnt = nb;
+      } else {
+        kcnt = kcnt + 1;
+      }
+    }
+  }
+  const R ch = (R)chc[p];
+  for (int k = 1; k <= g.nk; ++k) {
+    long q = p + g.n2 * (k - 1);
+    R t = tc[q], s = sc[q];
+    act[q] = act[q] + t;
+    acs[q] = acs[q] + s;
+    if (g.mskt(i, j, k) == 1) {
+      R wk = (R)1.0 / den_rho<R>(t, s, ch * g.pr2[k - 1]);
+      acr[q] = acr[q] + (R)1.0 / wk;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: FP64 global sums (deterministic two-level reduction) and a
+// non-finite scan.
+template <typename R>
+__global__ void k_sums_partial(Grid<R> g, const R* __restrict__ t, const R* __restrict__ s,
+                               const double* __restrict__ chc, const double* __restrict__ w2, double* part) {
+  __shared__ double sh[3][kBlock];
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0, b = 0, c = 0;
+  if (p < g.n2) {
+    int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+    if (i >= 2 && i <= g.nim && j >= 2 && j <= g.njm) {
+      double area = w2[p];
+      for (int k = 1; k <= g.nk; ++k) {
+        if (g.mskt(i, j, k)) {
+          long q = p + g.n2 * (k - 1);
+          double v = area * (double)g.dpr2[k - 1];
+          a += v * (double)t[q];
+          b += v * (double)s[q];
+        }
+      }
+      if (g.lev_t[p] > 0) c += area * chc[p];
+    }
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  sh[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int v = 0; v < 3; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int v = 0; v < 3; ++v) part[3 * blockIdx.x + v] = sh[v][0];
+}
+
+__global__ void k_sums_final(const double* part, int nb, double* out) {
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += part[3 * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+template <typename R>
+__global__ void k_finite(const R* a, long n, int* flag) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && !isfinite((double)a[p])) atomicOr(flag, 1);
+}
+
+__global__ void k_finite_d(const double* a, long n, int* flag) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && !isfinite(a[p])) atomicOr(flag, 1);
+}
+
+__global__ void k_cell_area(Grid<double> g, double* w) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  w[p] = g.rh[1][p] * g.dx1[1][p] * g.dx2[1][p] * ARAD * ARAD;
+}
+
+template <typename R>
+inline R* P(void* v) {
+  return static_cast<R*>(v);
+}
+
+template <typename R>
+const Grid<R>& grid_of(const Ctx& c);
+template <>
+const Grid<double>& grid_of<double>(const Ctx& c) {
+  return c.gd;
+}
+template <>
+const Grid<float>& grid_of<float>(const Ctx& c) {
+  return c.gf;
+}
+
+#define LAUNCH(kern, n, ...)                                                        \
+  do {                                                                              \
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
+  } while (0)
+
+template <typename R>
+std::string step_t(Ctx& c) {
+  const Grid<R>& g = grid_of<R>(c);
+  Fields3& f = c.f3;
+  Fields2& h = c.f2;
+  const size_t n2 = c.n2, n3 = c.n3;
+  R* T = P<R>(f.T[c.cur]);
+  R* S = P<R>(f.S[c.cur]);
+  R* U = P<R>(f.U);
+  R* V = P<R>(f.V);
+  hipError_t e;
+
+  c.step_i += 1;
+  if (c.timing) (void)hipEventRecord(c.ev[0], c.stream);
+  // daily forcing (main.f90:164-180)
+  if (c.step_i == 1 || c.ct.d != c.pt.d) {
+    int m1, m2;
+    double fw;
+    bnd_weights(c.ct, &m1, &m2, &fw);
+    Six s6;
+    for (int v = 0; v < 6; ++v) {
+      s6.a[v] = h.frc[v];
+      s6.b[v] = h.bnd[v];
+    }
+    LAUNCH(k_bnd, n2, (long)n2, s6, m1, m2, fw);
+    LAUNCH(k_pa, n2, c.gd, h.bnd[4], h.pax, h.pay);
+    LAUNCH(k_prho<R>, n3, g, T, S, h.chc, P<R>(f.prho1), P<R>(f.prho2));
+  }
+  // den_alpha, int_pgra
+  LAUNCH(k_alpha<R>, n3, g, T, S, h.chc, P<R>(f.alpha));
+  LAUNCH(k_adp<R>, n2, g, P<R>(f.alpha), P<R>(f.adp));
+  LAUNCH(k_bphi<R>, n2, g, P<R>(f.adp), P<R>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
+         h.bgra[1], h.bgra[2], h.bgra[3]);
+  // int_readyc
+  LAUNCH(k_upw<R>, n2, g, U, V, h.chp, P<R>(f.wm), (R*)nullptr, (const R*)nullptr, (const double*)nullptr);
+  c.ab = (c.ab + 1) % 3;
+  R* au = P<R>(f.AU[c.ab]);
+  R* av = P<R>(f.AV[c.ab]);
+  R* aup = P<R>(f.AU[(c.ab + 2) % 3]);
+  R* avp = P<R>(f.AV[(c.ab + 2) % 3]);
+  R* aupp = P<R>(f.AU[(c.ab + 1) % 3]);
+  R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
+  LAUNCH(k_adv<R>, n3, g, U, V, P<R>(f.wm), h.chc, au, av);
+  if (c.rst != 1) {  // AB3 bootstrap (mod_int.f90:122-132)
+    size_t bytes = n3 * sizeof(R);
+    if (c.step_i == 1) {
+      if ((e = hipMemcpyAsync(aup, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      if ((e = hipMemcpyAsync(aupp, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      if ((e = hipMemcpyAsync(avp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      if ((e = hipMemcpyAsync(avpp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+    } else if (c.step_i == 2) {
+      if ((e = hipMemcpyAsync(aupp, aup, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      if ((e = hipMemcpyAsync(avpp, avp, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+    }
+  }
+  LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
+  LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.chc, h.pax, h.pay, h.but, h.bvt);
+  if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
+  // int_trop
+  {
+    TropArgs<R> ta;
+    ta.g = c.gd;
+    ta.am = P<R>(f.am);
+    ta.chc = h.chc; ta.chp = h.chp; ta.mch = h.mch; ta.mch2 = h.mch2; ta.acch = h.acch;
+    ta.buc = h.buc; ta.bvc = h.bvc; ta.bup = h.bup; ta.bvp = h.bvp;
+    ta.but = h.but; ta.bvt = h.bvt;
+    for (int q = 0; q < 4; ++q) { ta.bphi[q] = h.bphi[q]; ta.bgra[q] = h.bgra[q]; }
+    ta.graphihx = h.graphihx; ta.graphihy = h.graphihy; ta.cor = c.gd.cor;
+    ta.tu = h.tu; ta.tv = h.tv; ta.upb1 = h.upb1; ta.upb2 = h.upb2; ta.vz1 = h.vz1; ta.vz2 = h.vz2;
+    ta.nstep = c.dtbc / c.dtbt * 2;
+    ta.bt = (double)c.dtbt;
+    void* args[] = {&ta};
+    e = hipLaunchCooperativeKernel((const void*)k_trop<R>, dim3(c.trop_blocks), dim3(kBlock), args, 0, c.stream);
+    if (e != hipSuccess) return hip_err(e, "k_trop (cooperative launch)");
+    c.n_ch += 1;
+  }
+  if (c.timing) (void)hipEventRecord(c.ev[2], c.stream);
+  // int_clin
+  {
+    ClinArgs<R> ca;
+    ca.g = g; ca.uc = U; ca.vc = V;
+    ca.auc = au; ca.aup = aup; ca.aupp = aupp; ca.avc = av; ca.avp = avp; ca.avpp = avpp;
+    ca.fx = P<R>(f.fx); ca.fy = P<R>(f.fy); ca.adp = P<R>(f.adp); ca.alpha = P<R>(f.alpha);
+    ca.mch = h.mch; ca.pax = h.pax; ca.pay = h.pay; ca.bc = (double)c.dtbc;
+    LAUNCH(k_clin<R>, n3, ca);
+  }
+  LAUNCH(k_clin_cpl<R>, n2, g, U, V, h.buc, h.bvc);
+  LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
+  c.n_uv += 1;
+  // int_ssh
+  LAUNCH(k_ssh<R>, n2, g, T, S, h.mch, P<R>(f.alpha), h.ssh, h.acssh);
+  c.n_ssh += 1;
+  // int_ts
+  LAUNCH(k_upw<R>, n2, g, U, V, h.chp, P<R>(f.wm), P<R>(f.acw), (const R*)P<R>(f.alpha), (const double*)h.mch2);
+  c.n_w += 1;
+  {
+    TsArgs<R> a;
+    a.g = g; a.t0 = T; a.s0 = S; a.wm = P<R>(f.wm); a.uc = U; a.vc = V;
+    a.chc = h.chc; a.mch = h.mch; a.bndt = h.bnd[2]; a.bnds = h.bnd[3];
+    a.t1 = P<R>(f.T[1 - c.cur]); a.s1 = P<R>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
+    LAUNCH(k_ts<R>, n3, a);
+  }
+  c.cur = 1 - c.cur;
+  LAUNCH(k_conv<R>, n2, g, P<R>(f.T[c.cur]), P<R>(f.S[c.cur]), P<R>(f.prho1), P<R>(f.prho2), h.chc, P<R>(f.act),
+         P<R>(f.acs), P<R>(f.acr));
+  c.n_ts += 1;
+  if (c.timing) {
+    (void)hipEventRecord(c.ev[3], c.stream);
+    (void)hipEventSynchronize(c.ev[3]);
+    float a = 0, b = 0, d = 0;
+    (void)hipEventElapsedTime(&a, c.ev[0], c.ev[3]);
+    (void)hipEventElapsedTime(&b, c.ev[1], c.ev[2]);
+    (void)hipEventElapsedTime(&d, c.ev[0], c.ev[1]);
+    float d2 = 0;
+    (void)hipEventElapsedTime(&d2, c.ev[2], c.ev[3]);
+    c.t_ms[0] += a;
+    c.t_ms[1] += d + d2;
+    c.t_ms[2] += b;
+    c.t_steps += 1;
+  }
+  // calendar (main.f90:225-226)
+  c.pt = c.ct;
+  c.ct = time_plus(c.ct, c.dtbc);
+  return "";
+}
+
+// ---- single operators (FP64) for parity tests ------------------------------
+template <int GA, int GB>
+__global__ void k_op_ter(Grid<double> g, int nd3, double dft, const double* in, double* out) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * nd3) return;
+  int i = (int)(p % g.ni) + 1;
+  long r = p / g.ni;
+  int j = (int)(r % g.nj) + 1, k = (int)(r / g.nj) + 1;
+  const double* v = in + g.n2 * (k - 1);
+  auto F = [&](int m, int jj) -> double { return v[g.ix(m, jj)]; };
+  double o;
+  if constexpr (gew(GA) == GB) o = ter_ew<GA>(g, F, i, j);
+  else if constexpr (gns(GA) == GB) o = ter_ns<GA>(g, F, i, j, dft);
+  else o = ter_di<GA>(g, F, i, j, dft);
+  out[p] = o;
+}
+
+template <int DIR, int GA, int GB>
+__global__ void k_op_px(Grid<double> g, int nd3, const double* in, double* out) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * nd3) return;
+  int i = (int)(p % g.ni) + 1;
+  long r = p / g.ni;
+  int j = (int)(r % g.nj) + 1, k = (int)(r / g.nj) + 1;
+  const double* v = in + g.n2 * (k - 1);
+  auto F = [&](int m, int jj) -> double { return v[g.ix(m, jj)]; };
+  out[p] = DIR == 1 ? p1<GA, GB>(g, F, i, j) : p2<GA, GB>(g, F, i, j);
+}
+
+template <int GA, int GB, int GC>
+__global__ void k_op_div(Grid<double> g, int nd3, const double* va, const double* vb, double* out) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2 * nd3) return;
+  int i = (int)(p % g.ni) + 1;
+  long r = p / g.ni;
+  int j = (int)(r % g.nj) + 1, k = (int)(r / g.nj) + 1;
+  const double* a = va + g.n2 * (k - 1);
+  const double* b = vb + g.n2 * (k - 1);
+  auto FA = [&](int m, int jj) -> double { return a[g.ix(m, jj)]; };
+  auto FB = [&](int m, int jj) -> double { return b[g.ix(m, jj)]; };
+  out[p] = divg<GA, GB, GC>(g, FA, FB, i, j);
+}
+
+__global__ void k_op_lap33(Grid<double> g, const double* in, double* out) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  auto F = [&](int m, int jj) -> double { return in[g.ix(m, jj)]; };
+  out[p] = lap<3, 3>(g, F, i, j);
+}
+
+__global__ void k_den_rho(long n, const double* t, const double* s, const double* pr, double* rho) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) rho[p] = den_rho<double>(t[p], s[p], pr[p]);
+}
+
+}  // namespace
+
+std::string launch_step(Ctx& c) {
+  if (c.precision == PCOM_FP64) return step_t<double>(c);
+  return step_t<float>(c);
+}
+
+std::string launch_trop_blocks(Ctx& c) {
+  int per_cu = 0;
+  hipError_t e;
+  if (c.precision == PCOM_FP64)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop<double>, kBlock, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop<float>, kBlock, 0);
+  if (e != hipSuccess) return hip_err(e, "occupancy of k_trop");
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, c.device)) != hipSuccess) return hip_err(e, "device properties");
+  long resident = (long)per_cu * prop.multiProcessorCount;
+  long need = (long)nblocks(c.n2);
+  c.trop_blocks = (int)(need < resident ? need : resident);
+  if (c.trop_blocks < 1) return "k_trop cannot be made resident";
+  return "";
+}
+
+std::string launch_global_sums(Ctx& c, double out[3]) {
+  const size_t n2 = c.n2;
+  unsigned nb = nblocks(n2);
+  double* area = c.f2.red;              // n2
+  double* part = c.f2.red + n2;         // 3 * nb
+  double* res = part + 3 * (size_t)nb;  // 3
+  LAUNCH(k_cell_area, n2, c.gd, area);
+  if (c.precision == PCOM_FP64)
+    LAUNCH(k_sums_partial<double>, n2, c.gd, P<double>(c.f3.T[c.cur]), P<double>(c.f3.S[c.cur]), c.f2.chc, area, part);
+  else
+    LAUNCH(k_sums_partial<float>, n2, c.gf, P<float>(c.f3.T[c.cur]), P<float>(c.f3.S[c.cur]), c.f2.chc, area, part);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, c.stream, part, (int)nb, res);
+  hipError_t e = hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (e != hipSuccess) return hip_err(e, "copy sums");
+  e = hipStreamSynchronize(c.stream);
+  return hip_err(e, "sync sums");
+}
+
+std::string launch_check_finite(Ctx& c, int* bad) {
+  hipError_t e = hipMemsetAsync(c.f2.flag, 0, sizeof(int), c.stream);
+  if (e != hipSuccess) return hip_err(e, "memset flag");
+  void* f3s[] = {c.f3.T[c.cur], c.f3.S[c.cur], c.f3.U, c.f3.V};
+  for (void* a : f3s) {
+    if (c.precision == PCOM_FP64) LAUNCH(k_finite<double>, c.n3, P<double>(a), (long)c.n3, c.f2.flag);
+    else LAUNCH(k_finite<float>, c.n3, P<float>(a), (long)c.n3, c.f2.flag);
+  }
+  double* f2s[] = {c.f2.chc, c.f2.buc, c.f2.bvc};
+  for (double* a : f2s) LAUNCH(k_finite_d, c.n2, a, (long)c.n2, c.f2.flag);
+  e = hipMemcpyAsync(bad, c.f2.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  if (e != hipSuccess) return hip_err(e, "copy flag");
+  return hip_err(hipStreamSynchronize(c.stream), "sync flag");
+}
+
+#define TER_CASE(A, B)                                                              \
+  case A * 10 + B:                                                                  \
+    LAUNCH((k_op_ter<A, B>), n, c.gd, nd3, dft, in, out);                           \
+    return "";
+
+std::string launch_op_ter(Ctx& c, int ga, int gb, int nd3, double dft, const double* in, double* out) {
+  size_t n = c.n2 * (size_t)nd3;
+  switch (ga * 10 + gb) {
+    TER_CASE(1, 2) TER_CASE(1, 3) TER_CASE(1, 4)
+    TER_CASE(2, 1) TER_CASE(2, 3) TER_CASE(2, 4)
+    TER_CASE(3, 1) TER_CASE(3, 2) TER_CASE(3, 4)
+    TER_CASE(4, 1) TER_CASE(4, 2) TER_CASE(4, 3)
+    default:
+      return "op_ter: staggers must differ and lie in 1..4";
+  }
+}
+
+#define PX_CASE(A, B)                                                                       \
+  case A * 10 + B:                                                                          \
+    if (dir == 1) LAUNCH((k_op_px<1, A, B>), n, c.gd, nd3, in, out);                        \
+    else LAUNCH((k_op_px<2, A, B>), n, c.gd, nd3, in, out);                                 \
+    return "";
+
+std::string launch_op_px(Ctx& c, int dir, int ga, int gb, int nd3, const double* in, double* out) {
+  size_t n = c.n2 * (size_t)nd3;
+  switch (ga * 10 + gb) {
+    PX_CASE(1, 1) PX_CASE(1, 2) PX_CASE(1, 3) PX_CASE(1, 4)
+    PX_CASE(2, 1) PX_CASE(2, 2) PX_CASE(2, 3) PX_CASE(2, 4)
+    PX_CASE(3, 1) PX_CASE(3, 2) PX_CASE(3, 3) PX_CASE(3, 4)
+    PX_CASE(4, 1) PX_CASE(4, 2) PX_CASE(4, 3) PX_CASE(4, 4)
+    default:
+      return "op_px: staggers must lie in 1..4";
+  }
+}
+
+std::string launch_op_div(Ctx& c, int ga, int gb, int gc, int nd3, const double* va, const double* vb, double* out) {
+  size_t n = c.n2 * (size_t)nd3;
+  int code = ga * 100 + gb * 10 + gc;
+  switch (code) {
+    case 331: LAUNCH((k_op_div<3, 3, 1>), n, c.gd, nd3, va, vb, out); return "";
+    case 423: LAUNCH((k_op_div<4, 2, 3>), n, c.gd, nd3, va, vb, out); return "";
+    case 241: LAUNCH((k_op_div<2, 4, 1>), n, c.gd, nd3, va, vb, out); return "";
+    default:
+      return "op_div: supported (ga,gb,gc) are the reference's (3,3,1), (4,2,3), (2,4,1)";
+  }
+}
+
+std::string launch_op_lap(Ctx& c, int ga, int gb, const double* in, double* out) {
+  if (ga != 3 || gb != 3) return "op_lap: the reference applies it on g3 -> g3 only (mod_int.f90:198-199)";
+  LAUNCH(k_op_lap33, c.n2, c.gd, in, out);
+  return "";
+}
+
+std::string launch_den_rho(hipStream_t st, size_t n, const double* t, const double* s, const double* p, double* rho) {
+  hipLaunchKernelGGL(k_den_rho, dim3(nblocks(n)), dim3(kBlock), 0, st, (long)n, t, s, p, rho);
+  return hip_err(hipGetLastError(), "k_den_rho");
+}
+
+}  // namespace pcom

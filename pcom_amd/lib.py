@@ -1,0 +1,195 @@
+"""Loader and Python mirror of the C ABI in include/pcom_gpu.h.
+
+``Model`` mirrors the reference's driver loop for the time-stepping path
+(main.f90:158-238): create a context from raw inputs, ``step(n)``, read or
+write fields.  The HIP library is required; there is no CPU fallback, and
+a missing or unloadable library raises at import of ``lib()``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpcom_gpu.so")
+_LIB = None
+
+# every symbol include/pcom_gpu.h declares
+EXPORTS = [
+    "pcom_create", "pcom_step", "pcom_get_dims", "pcom_field_size", "pcom_get_field",
+    "pcom_set_field", "pcom_get_counters", "pcom_global_sums", "pcom_check_finite", "pcom_sync",
+    "pcom_phase_times", "pcom_destroy", "pcom_last_error", "pcom_op_ter", "pcom_op_px1",
+    "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
+]
+
+
+class PcomError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pcom error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpcom_gpu.so (built by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C pcom_amd/csrc` "
+                               "or __graft_entry__.build(); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
+        L.pcom_create.argtypes = [C.POINTER(abi.PcomInputs), C.POINTER(abi.PcomOpts), C.POINTER(vp)]
+        L.pcom_step.argtypes = [vp, C.c_int]
+        L.pcom_get_dims.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.pcom_field_size.argtypes = [vp, C.c_int]
+        L.pcom_field_size.restype = sz
+        L.pcom_get_field.argtypes = [vp, C.c_int, dp, sz]
+        L.pcom_set_field.argtypes = [vp, C.c_int, dp, sz]
+        L.pcom_get_counters.argtypes = [vp, C.POINTER(abi.PcomCounters)]
+        L.pcom_global_sums.argtypes = [vp, dp]
+        L.pcom_check_finite.argtypes = [vp]
+        L.pcom_sync.argtypes = [vp]
+        L.pcom_phase_times.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_int)]
+        L.pcom_destroy.argtypes = [vp]
+        L.pcom_last_error.restype = C.c_char_p
+        L.pcom_op_ter.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp]
+        L.pcom_op_px1.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.pcom_op_px2.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.pcom_op_div.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+        L.pcom_op_lap.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+        L.pcom_den_rho.argtypes = [vp, sz, dp, dp, dp, dp]
+        L.pcom_get_metric.argtypes = [vp, C.c_int, C.c_int, dp, sz]
+        for name in EXPORTS:
+            if name not in ("pcom_field_size", "pcom_last_error"):
+                getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != abi.PCOM_OK:
+        raise PcomError(rc, lib().pcom_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def den_rho(t, s, p):
+    """den_rho (mod_den.f90:57-118) evaluated by the HIP kernel."""
+    t, s, p = np.broadcast_arrays(np.asarray(t, np.float64), np.asarray(s, np.float64),
+                                  np.asarray(p, np.float64))
+    t, s, p = (np.ascontiguousarray(x) for x in (t, s, p))
+    out = np.empty_like(t)
+    _check(lib().pcom_den_rho(None, t.size, _dp(t), _dp(s), _dp(p), _dp(out)))
+    return out
+
+
+class Model:
+    """One device-resident PCOM time stepper (a pcom_ctx)."""
+
+    def __init__(self, cfg, precision=abi.PCOM_FP64, device=0, restart_run=0):
+        self.cfg = cfg
+        self.precision = precision
+        self._inp = cfg.c_inputs()
+        self._opt = cfg.c_opts(precision=precision, device=device, restart_run=restart_run)
+        h = C.c_void_p()
+        _check(lib().pcom_create(C.byref(self._inp), C.byref(self._opt), C.byref(h)))
+        self.h = h
+        ni, nj, nk = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().pcom_get_dims(self.h, C.byref(ni), C.byref(nj), C.byref(nk)))
+        self.ni, self.nj, self.nk = ni.value, nj.value, nk.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _LIB is not None:
+            _LIB.pcom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n=1):
+        _check(lib().pcom_step(self.h, int(n)))
+
+    def sync(self):
+        _check(lib().pcom_sync(self.h))
+
+    def shape(self, name):
+        nd3 = abi.field_levels(name, self.nk)
+        return (self.ni, self.nj) if nd3 == 1 else (self.ni, self.nj, nd3)
+
+    def get(self, name):
+        out = np.empty(self.shape(name), dtype=np.float64, order="F")
+        _check(lib().pcom_get_field(self.h, abi.FIELD[name], _dp(out), out.size))
+        return out
+
+    def set(self, name, arr):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        if a.shape != self.shape(name):
+            raise ValueError(f"{name}: shape {a.shape} != {self.shape(name)}")
+        _check(lib().pcom_set_field(self.h, abi.FIELD[name], _dp(a), a.size))
+
+    def counters(self):
+        c = abi.PcomCounters()
+        _check(lib().pcom_get_counters(self.h, C.byref(c)))
+        return c
+
+    def global_sums(self):
+        out = np.zeros(3)
+        _check(lib().pcom_global_sums(self.h, _dp(out)))
+        return out
+
+    def check_finite(self):
+        rc = lib().pcom_check_finite(self.h)
+        return rc == abi.PCOM_OK
+
+    def phase_times(self, enable=True):
+        out = np.zeros(3)
+        n = C.c_int()
+        _check(lib().pcom_phase_times(self.h, int(enable), _dp(out), C.byref(n)))
+        return out, n.value
+
+    def metric(self, g, which):
+        out = np.empty((self.ni, self.nj), order="F")
+        _check(lib().pcom_get_metric(self.h, g, which, _dp(out), out.size))
+        return out
+
+    def _out(self, nd3):
+        return np.zeros((self.ni, self.nj, nd3) if nd3 > 1 else (self.ni, self.nj), order="F")
+
+    def op_ter(self, ga, gb, var, dft=0.0):
+        var = np.asfortranarray(var, np.float64)
+        nd3 = 1 if var.ndim == 2 else var.shape[2]
+        out = self._out(nd3)
+        _check(lib().pcom_op_ter(self.h, ga, gb, nd3, dft, _dp(var), _dp(out)))
+        return out
+
+    def op_px1(self, ga, gb, var):
+        var = np.asfortranarray(var, np.float64)
+        nd3 = 1 if var.ndim == 2 else var.shape[2]
+        out = self._out(nd3)
+        _check(lib().pcom_op_px1(self.h, ga, gb, nd3, _dp(var), _dp(out)))
+        return out
+
+    def op_px2(self, ga, gb, var):
+        var = np.asfortranarray(var, np.float64)
+        nd3 = 1 if var.ndim == 2 else var.shape[2]
+        out = self._out(nd3)
+        _check(lib().pcom_op_px2(self.h, ga, gb, nd3, _dp(var), _dp(out)))
+        return out
+
+    def op_div(self, ga, gb, gc, va, vb):
+        va = np.asfortranarray(va, np.float64)
+        vb = np.asfortranarray(vb, np.float64)
+        nd3 = 1 if va.ndim == 2 else va.shape[2]
+        out = self._out(nd3)
+        _check(lib().pcom_op_div(self.h, ga, gb, gc, nd3, _dp(va), _dp(vb), _dp(out)))
+        return out
+
+    def op_lap(self, ga, gb, var):
+        var = np.asfortranarray(var, np.float64)
+        out = self._out(1)
+        _check(lib().pcom_op_lap(self.h, ga, gb, _dp(var), _dp(out)))
+        return out

@@ -1,0 +1,139 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle.
+
+The FP64 form must reproduce the oracle bit for bit (compared as values:
++0.0 == -0.0) on every field, since both evaluate the reference's
+statements with the same IEEE operations in the same order.
+"""
+import numpy as np
+import pytest
+
+from pcom_amd import abi, configs
+from pcom_amd.lib import Model, den_rho as gpu_den_rho
+
+from oracle.oracle import Oracle, den_rho as orc_den_rho
+
+pytestmark = pytest.mark.gpu
+
+# fields that a step leaves defined (alpha / wm as int_ssh / int_ts leave them)
+STEP_FIELDS = [
+    "tc", "sc", "uc", "vc", "auc", "avc", "aup", "avp", "aupp", "avpp", "fx", "fy", "am",
+    "alpha", "adp", "wm", "prho_t", "prho_s", "act", "acs", "acr", "acu", "acv", "acw",
+    "chc", "chp", "mch", "mch2", "acch", "acssh", "buc", "bvc", "bup", "bvp", "but", "bvt",
+    "pax", "pay", "bphi_xn", "bphi_xs", "bphi_ye", "bphi_yw", "bgra_xn", "bgra_xs", "bgra_ye",
+    "bgra_yw", "bnd_taux", "bnd_tauy", "bnd_t", "bnd_s", "bnd_pa", "bnd_fw", "ssh",
+]
+STATIC_FIELDS = ["graphihx", "graphihy", "cor", "phih_t", "prh_t", "phih_u", "prh_u", "km", "kh"]
+
+
+def _diff(name, a, b):
+    """Describe a mismatch (value equality, NaN-aware)."""
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if same.all():
+        return None
+    idx = np.argwhere(~same)
+    first = tuple(int(x) + 1 for x in idx[0])  # 1-based like the reference
+    d = np.abs(a - b)
+    d = d[np.isfinite(d)]
+    return (f"{name}: {int((~same).sum())} of {a.size} differ, first at {first} "
+            f"gpu={a[tuple(idx[0])]!r} oracle={b[tuple(idx[0])]!r}, max|d|={d.max() if d.size else 'nan'}")
+
+
+def _compare(gpu, orc, names):
+    bad = []
+    for n in names:
+        r = _diff(n, gpu.get(n), orc.get(n))
+        if r:
+            bad.append(r)
+    return bad
+
+
+@pytest.fixture(scope="module")
+def small_cfg():
+    return configs.small()
+
+
+def test_den_rho_bitwise(gpu_lib):
+    rng = np.random.default_rng(0)
+    t = rng.uniform(-2, 32, 4096)
+    s = rng.uniform(0, 40, 4096)
+    p = rng.uniform(0, 6e7, 4096)
+    t[:3], s[:3], p[:3] = [3.0, 0.0, 25.0], [35.5, 0.0, 35.0], [300e5, 0.0, 0.0]
+    np.testing.assert_array_equal(gpu_den_rho(t, s, p), orc_den_rho(t, s, p))
+
+
+def test_static_fields(gpu_lib, small_cfg):
+    m = Model(small_cfg)
+    o = Oracle(small_cfg)
+    bad = _compare(m, o, STATIC_FIELDS)
+    for g in range(1, 5):
+        for w in range(4):
+            r = _diff(f"metric g{g} #{w}", m.metric(g, w), o.metric(g, w))
+            if r:
+                bad.append(r)
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("ga,gb", [(a, b) for a in range(1, 5) for b in range(1, 5) if a != b])
+def test_op_ter(gpu_lib, small_cfg, ga, gb):
+    m = Model(small_cfg)
+    o = Oracle(small_cfg)
+    rng = np.random.default_rng(10 * ga + gb)
+    v = rng.standard_normal((m.ni, m.nj, 3))
+    # inputs of the reference's operators are halo-swapped (periodic)
+    v[0] = v[-2]
+    v[-1] = v[1]
+    for dft in (0.0, 1.0):
+        np.testing.assert_array_equal(m.op_ter(ga, gb, v, dft), o.op_ter(ga, gb, v, dft))
+
+
+@pytest.mark.parametrize("ga,gb", [(a, b) for a in range(1, 5) for b in range(1, 5)])
+def test_op_px(gpu_lib, small_cfg, ga, gb):
+    m = Model(small_cfg)
+    o = Oracle(small_cfg)
+    rng = np.random.default_rng(100 + 10 * ga + gb)
+    v = rng.standard_normal((m.ni, m.nj, 2))
+    v[0] = v[-2]
+    v[-1] = v[1]
+    np.testing.assert_array_equal(m.op_px1(ga, gb, v), o.op_px1(ga, gb, v))
+    np.testing.assert_array_equal(m.op_px2(ga, gb, v), o.op_px2(ga, gb, v))
+
+
+@pytest.mark.parametrize("ga,gb,gc", [(3, 3, 1), (4, 2, 3), (2, 4, 1)])
+def test_op_div_lap(gpu_lib, small_cfg, ga, gb, gc):
+    m = Model(small_cfg)
+    o = Oracle(small_cfg)
+    rng = np.random.default_rng(7)
+    va = rng.standard_normal((m.ni, m.nj, 2))
+    vb = rng.standard_normal((m.ni, m.nj, 2))
+    for v in (va, vb):
+        v[0] = v[-2]
+        v[-1] = v[1]
+    np.testing.assert_array_equal(m.op_div(ga, gb, gc, va, vb), o.op_div(ga, gb, gc, va, vb))
+    np.testing.assert_array_equal(m.op_lap(3, 3, va[:, :, 0]), o.op_lap(3, 3, va[:, :, 0]))
+
+
+@pytest.mark.parametrize("cfgname,nsteps", [("small", 1), ("small", 3), ("box", 2)])
+def test_step_bitwise(gpu_lib, cfgname, nsteps):
+    cfg = configs.CONFIGS[cfgname]()
+    m = Model(cfg)
+    o = Oracle(cfg)
+    for s in range(nsteps):
+        m.step(1)
+        o.step(1)
+        bad = _compare(m, o, STEP_FIELDS)
+        assert not bad, f"after step {s + 1}:\n" + "\n".join(bad)
+    cm, co = m.counters(), o.counters()
+    for f in ("step", "n_ts", "n_uv", "n_w", "n_ch", "n_ssh", "year", "month", "day", "hour"):
+        assert getattr(cm, f) == getattr(co, f), f
+
+
+def test_box_many_steps_bitwise(gpu_lib):
+    """Config 1 (box, 60x40x10): 30 steps crossing a day boundary."""
+    cfg = configs.box()
+    m = Model(cfg)
+    o = Oracle(cfg)
+    m.step(30)
+    o.step(30)
+    bad = _compare(m, o, STEP_FIELDS)
+    assert not bad, "\n".join(bad)
+    assert m.check_finite()
