@@ -81,6 +81,8 @@ typedef struct pcom_opts {
   int precision;   /* PCOM_FP64 or PCOM_MIXED                              */
   int restart_run; /* namelist restart_run: 1 skips the AB3 bootstrap      */
   int device;      /* HIP device ordinal                                   */
+  int trop_tile_i, trop_tile_j; /* barotropic subcycle tile (0 = 16, 16)   */
+  int trop_substeps;            /* substeps between tile exchanges (0 = 2) */
 } pcom_opts;
 
 typedef struct pcom_ctx pcom_ctx;

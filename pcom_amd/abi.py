@@ -34,6 +34,7 @@ class PcomOpts(C.Structure):
         ("dtbt", C.c_int), ("dtbc", C.c_int),
         ("year", C.c_int), ("month", C.c_int), ("day", C.c_int), ("hour", C.c_int),
         ("precision", C.c_int), ("restart_run", C.c_int), ("device", C.c_int),
+        ("trop_tile_i", C.c_int), ("trop_tile_j", C.c_int), ("trop_substeps", C.c_int),
     ]
 
 

@@ -79,8 +79,9 @@ class Config:
         self._keep = keep
         return s
 
-    def c_opts(self, precision=abi.PCOM_FP64, device=0, restart_run=0):
+    def c_opts(self, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0)):
         o = abi.PcomOpts()
+        o.trop_tile_i, o.trop_tile_j, o.trop_substeps = trop_tile
         o.dtbt, o.dtbc = self.dtbt, self.dtbc
         o.year, o.month, o.day, o.hour = self.start
         o.precision = precision

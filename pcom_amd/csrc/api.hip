@@ -52,8 +52,20 @@ int upload(Ctx& c, void* dst, const double* src, size_t n, bool as_float) {
   return PCOM_OK;
 }
 
+// a barotropic tile that timed out waiting for its neighbours leaves the
+// state undefined: report it at the next synchronising call
+int check_async_error(Ctx& c) {
+  if (!c.trop_error) return PCOM_OK;
+  int err = 0;
+  HIPCHK(hipStreamSynchronize(c.stream), "sync");
+  HIPCHK(hipMemcpy(&err, c.trop_error, sizeof(int), hipMemcpyDeviceToHost), "read subcycle status");
+  if (err) return fail(PCOM_ERR_STATE, "barotropic subcycle: a tile exchange wait exceeded its bound; state invalid");
+  return PCOM_OK;
+}
+
 int download(Ctx& c, double* dst, const void* src, size_t n, bool as_float) {
   HIPCHK(hipStreamSynchronize(c.stream), "sync before download");
+  if (int rc = check_async_error(c)) return rc;
   if (!as_float) {
     HIPCHK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost), "download");
   } else {
@@ -221,6 +233,9 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if (opts->dtbt <= 0 || opts->dtbc <= 0 || opts->dtbc % opts->dtbt != 0 || 86400 % opts->dtbt != 0 ||
       86400 % opts->dtbc != 0)
     return fail(PCOM_ERR_ARG, "dtbt and dtbc must divide 86400 and dtbt must divide dtbc (mod_param.f90:228-235)");
+  if (opts->trop_tile_i < 0 || opts->trop_tile_j < 0 || opts->trop_substeps < 0 || opts->trop_tile_i > 256 ||
+      opts->trop_tile_j > 256 || opts->trop_substeps > 16)
+    return fail(PCOM_ERR_ARG, "barotropic tiling out of range (tile <= 256, substeps per exchange <= 16)");
   if (opts->precision != PCOM_FP64 && opts->precision != PCOM_MIXED)
     return fail(PCOM_ERR_ARG, "precision must be PCOM_FP64 or PCOM_MIXED");
   if (opts->month < 1 || opts->month > 12 || opts->day < 1 || opts->day > 31 || opts->year < 1)
@@ -240,6 +255,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
     return fail(PCOM_ERR_ARG, "pcom_create: " + err);
   }
   HostSetup& hs = c->hs;
+  if (hs.nk > 64) return fail(PCOM_ERR_ARG, "pcom_create: at most 64 levels are supported");
   c->ni = hs.ni; c->nj = hs.nj; c->nk = hs.nk; c->nkp = hs.nkp;
   c->n2 = hs.n2; c->n3 = hs.n3; c->n3p = hs.n3p;
   c->ct.y = opts->year; c->ct.m = opts->month; c->ct.d = opts->day; c->ct.h = opts->hour;
@@ -304,7 +320,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   for (double** a : ones)
     if ((rc = make2(*c, a, n2, nullptr, 1.0))) return rc;
   double** zeros[] = {&h.acch, &h.acssh, &h.ssh, &h.buc, &h.bvc, &h.bup, &h.bvp, &h.but, &h.bvt, &h.pax, &h.pay,
-                      &h.tu, &h.tv, &h.upb1, &h.upb2, &h.vz1, &h.vz2};
+                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1]};
   for (double** a : zeros)
     if ((rc = make2(*c, a, n2, nullptr))) return rc;
   for (int q = 0; q < 4; ++q) {
@@ -324,8 +340,21 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
     if ((rc = dev_alloc(*c, &p, 1, sizeof(int)))) return rc;
     h.flag = (int*)p;
   }
-  std::string te = launch_trop_blocks(*c);
+  c->trop_tw = opts->trop_tile_i;
+  c->trop_th = opts->trop_tile_j;
+  c->trop_s = opts->trop_substeps;
+  std::string te = trop_configure(*c);
   if (!te.empty()) return fail(PCOM_ERR_HIP, te);
+  {
+    void* p;
+    size_t nflag = (size_t)c->trop_blocks * 32 + 32;
+    if ((rc = dev_alloc(*c, &p, nflag, sizeof(unsigned long long)))) return rc;
+    HIPCHK(hipMemset(p, 0, nflag * sizeof(unsigned long long)), "zero tile flags");
+    c->trop_flags = (unsigned long long*)p;
+    if ((rc = dev_alloc(*c, &p, 1, sizeof(int)))) return rc;
+    HIPCHK(hipMemset(p, 0, sizeof(int)), "zero trop error");
+    c->trop_error = (int*)p;
+  }
   HIPCHK(hipDeviceSynchronize(), "create sync");
   return PCOM_OK;
 }
@@ -391,7 +420,7 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
 int pcom_sync(pcom_ctx* ctx) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_sync: NULL context");
   HIPCHK(hipStreamSynchronize(C(ctx)->stream), "hipStreamSynchronize");
-  return PCOM_OK;
+  return check_async_error(*C(ctx));
 }
 
 int pcom_get_dims(const pcom_ctx* ctx, int* ni, int* nj, int* nk) {
@@ -467,6 +496,7 @@ int pcom_check_finite(pcom_ctx* ctx) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_check_finite: NULL context");
   Ctx* c = C(ctx);
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  if (int rc = check_async_error(*c)) return rc;
   int bad = 0;
   std::string e = launch_check_finite(*c, &bad);
   if (!e.empty()) return fail(PCOM_ERR_HIP, e);

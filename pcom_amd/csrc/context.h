@@ -32,7 +32,9 @@ struct Fields2 {
   double *bnd[6];              // taux tauy t s pa fw
   double *frc[6];              // (ni,nj,12)
   double *graphihx, *graphihy;
-  double *tu, *tv, *upb1, *upb2, *vz1, *vz2;   // subcycle scratch
+  double *vz1, *vz2, *am1;                      // subcycle statics
+  double *schr, *sqp, *schs, *schm;             // sqrt(ter(ch)) factors on g3
+  double *xc[2], *xu[2], *xv[2];                // subcycle tile exchange
   double *red;                                  // reduction scratch
   int *flag;
 };
@@ -55,7 +57,12 @@ struct Ctx {
   Fields3 f3{};
   Fields2 f2{};
   std::vector<void*> allocs;
-  int trop_blocks = 0;
+  int trop_tw = 0, trop_th = 0, trop_s = 0;   // barotropic tile and substeps/exchange
+  int trop_tiles_x = 0, trop_blocks = 0;
+  size_t trop_lds = 0;
+  unsigned long long* trop_flags = nullptr;  // per-tile exchange flags (zeroed at create)
+  int* trop_error = nullptr;                 // set by a timed-out neighbour wait
+  unsigned long long trop_epoch = 0;
   // phase timing (HIP events on `stream`)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -68,7 +75,8 @@ struct Ctx {
 std::string launch_step(Ctx& c);                 // one baroclinic step
 std::string launch_global_sums(Ctx& c, double out[3]);
 std::string launch_check_finite(Ctx& c, int* bad);
-std::string launch_trop_blocks(Ctx& c);          // size the subcycle grid
+std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
+std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
 // single-operator entry points on device buffers (FP64)
 std::string launch_op_ter(Ctx& c, int ga, int gb, int nd3, double dft, const double* in, double* out);
 std::string launch_op_px(Ctx& c, int dir, int ga, int gb, int nd3, const double* in, double* out);

@@ -25,6 +25,8 @@ namespace pcom {
 namespace {
 
 constexpr int kBlock = 256;
+// column kernels: kCW columns x kKL k-lanes per workgroup; levels <= kNKMAX
+constexpr int kCW = 64, kKL = 4, kNKMAX = 64;
 
 inline std::string hip_err(hipError_t e, const char* what) {
   if (e == hipSuccess) return "";
@@ -72,6 +74,33 @@ __global__ void k_pa(Grid<double> g, const double* __restrict__ pa, double* pax,
   pay[p] = ter_ew<4>(g, WY, i, j);
 }
 
+// sqrt of ch interpolated to the U points, the factor every 3-D operator of
+// the step divides or multiplies by.  Computed once per use in FP64:
+//  before int_readyc: schr = sqrt(ter(chc,g1->g3,dft=1)), 1 on land
+//                     (mod_int.f90:107-109); sqp = sqrt(ter(chp)) (upwelling,
+//                     mod_int.f90:636-637);
+//  after int_trop:    schs = sqrt(ter(chc, dft=1)) (op_adv_ts, mod_op.f90:
+//                     294-295; equal to int_clin's sqrt(ter(mch2, dft=1)) since
+//                     chc = mch2 there, mod_int.f90:284, 385-386); sqp again;
+//                     schm = sqrt(ter(mch, dft=1)), 1 on land (mod_int.f90:318-320).
+__global__ void k_sch(Grid<double> g, const double* __restrict__ chc, const double* __restrict__ chp,
+                      const double* __restrict__ mch, double* schr, double* sqp, double* schs, double* schm) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  auto CHC = [&](int m, int jj) -> double { return chc[g.ix(m, jj)]; };
+  auto CHP = [&](int m, int jj) -> double { return chp[g.ix(m, jj)]; };
+  const bool land = g.lev_u[p] == 0;
+  double c1 = sqrt(ter_di<1>(g, CHC, i, j, 1.0));
+  if (schr) schr[p] = land ? 1.0 : c1;
+  if (schs) schs[p] = c1;
+  sqp[p] = sqrt(ter_di<1>(g, CHP, i, j, 0.0));
+  if (schm) {
+    auto MCH = [&](int m, int jj) -> double { return mch[g.ix(m, jj)]; };
+    schm[p] = land ? 1.0 : sqrt(ter_di<1>(g, MCH, i, j, 1.0));
+  }
+}
+
 // den_prho (mod_den.f90:120-153)
 template <typename R>
 __global__ void k_prho(Grid<R> g, const R* __restrict__ tc, const R* __restrict__ sc, const double* __restrict__ chc,
@@ -108,26 +137,26 @@ __global__ void k_alpha(Grid<R> g, const R* __restrict__ t, const R* __restrict_
 // t = alpha*dpr*msk; the suffix sum is accumulated in increasing k as
 // Fortran's sum() does, so FP64 results are the reference's bits.
 template <typename R>
-__global__ void k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
+  __shared__ R t[kNKMAX][kCW];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  const long p = (long)blockIdx.x * kCW + tx;
+  if (p < g.n2) {
+    const int lev = g.lev_t[p];
+    for (int k = 1 + kl; k <= nk; k += kKL)
+      t[k - 1][tx] = alpha[p + g.n2 * (k - 1)] * g.dpr2[k - 1] * (R)(lev >= k ? 1 : 0);
+  }
+  __syncthreads();
   if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  const int nk = g.nk;
-  auto tt = [&](int k) -> R { return alpha[p + g.n2 * (k - 1)] * g.dpr2[k - 1] * (R)g.mskt(i, j, k); };
-  if constexpr (sizeof(R) == sizeof(double)) {
-    for (int k = 1; k <= nk - 1; ++k) {
+  for (int k = 1 + kl; k <= nk; k += kKL) {
+    if constexpr (sizeof(R) == sizeof(double)) {
       R s = 0;
-      for (int kk = k + 1; kk <= nk; ++kk) s += tt(kk);
-      adp[p + g.n2 * (k - 1)] = tt(k) * (R)0.5 + s;
-    }
-    adp[p + g.n2 * (nk - 1)] = tt(nk) * (R)0.5;
-  } else {
-    // mixed form: one pass from the bottom, accumulated in FP64
-    double s = 0;
-    for (int k = nk; k >= 1; --k) {
-      double t = (double)tt(k);
-      adp[p + g.n2 * (k - 1)] = (R)(t * 0.5 + s);
-      s += t;
+      for (int kk = k + 1; kk <= nk; ++kk) s += t[kk - 1][tx];
+      adp[p + g.n2 * (k - 1)] = t[k - 1][tx] * (R)0.5 + s;
+    } else {
+      double s = 0;
+      for (int kk = k + 1; kk <= nk; ++kk) s += (double)t[kk - 1][tx];
+      adp[p + g.n2 * (k - 1)] = (R)((double)t[k - 1][tx] * 0.5 + s);
     }
   }
 }
@@ -180,48 +209,59 @@ __global__ void k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict
 // upwelling (mod_int.f90:626-647): wm from the continuity equation;
 // with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).
 template <typename R>
-__global__ void k_upw(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const double* __restrict__ chp,
-                      R* wm, R* acw, const R* __restrict__ alpha, const double* __restrict__ mch2) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  auto CHP = [&](int m, int jj) -> R { return (R)chp[g.ix(m, jj)]; };
-  auto SQ = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, CHP, m, jj, (R)0)); };
-  auto wk3 = [&](int k) -> R {
-    const R* u = uc + g.n2 * (k - 1);
-    const R* v = vc + g.n2 * (k - 1);
-    auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * u[g.ix(m, jj)]; };
-    auto VB = [&](int m, int jj) -> R { return SQ(m, jj) * v[g.ix(m, jj)]; };
-    return divg<3, 3, 1>(g, VA, VB, i, j);
-  };
-  R s = 0;
-  for (int k = 1; k <= g.nk; ++k) s += wk3(k) * g.dpr2[k - 1] * (R)g.mskt(i, j, k);
-  R wk2 = -(s / g.prh_t_r[p]);
-  R w = wm[p];  // wm(1) is never written by upwelling
-  if (acw) acw[p] = acw[p] + w * alpha[p] / ((R)GRAV * (R)mch2[p]);
-  R prev = wk3(1);
-  for (int k = 2; k <= g.nk; ++k) {
+__global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc,
+                                                   const double* __restrict__ sqp, R* wm, R* acw,
+                                                   const R* __restrict__ alpha, const double* __restrict__ mch2) {
+  __shared__ R w3[kNKMAX][kCW];
+  __shared__ R wv[kNKMAX][kCW];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  const long p = (long)blockIdx.x * kCW + tx;
+  const bool ok = p < g.n2;
+  const int i = ok ? (int)(p % g.ni) + 1 : 1, j = ok ? (int)(p / g.ni) + 1 : 1;
+  auto SQ = [&](int m, int jj) -> R { return (R)sqp[g.ix(m, jj)]; };
+  if (ok) {
+    for (int k = 1 + kl; k <= nk; k += kKL) {
+      const R* u = uc + g.n2 * (k - 1);
+      const R* v = vc + g.n2 * (k - 1);
+      auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * u[g.ix(m, jj)]; };
+      auto VB = [&](int m, int jj) -> R { return SQ(m, jj) * v[g.ix(m, jj)]; };
+      w3[k - 1][tx] = divg<3, 3, 1>(g, VA, VB, i, j);
+    }
+  }
+  __syncthreads();
+  if (ok && kl == 0) {
+    const int lev = g.lev_t[p];
+    R s = 0;
+    for (int k = 1; k <= nk; ++k) s += w3[k - 1][tx] * g.dpr2[k - 1] * (R)(lev >= k ? 1 : 0);
+    R wk2 = -(s / g.prh_t_r[p]);
+    R w = wm[p];  // wm(1) is never written by upwelling
+    wv[0][tx] = w;
+    for (int k = 2; k <= nk; ++k) {
+      w = w + g.dpr2[k - 2] * (wk2 + w3[k - 2][tx]);
+      w = w * (R)(lev >= k ? 1 : 0);
+      wv[k - 1][tx] = w;
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+  for (int k = 1 + kl; k <= nk; k += kKL) {
     long q = p + g.n2 * (k - 1);
-    w = w + g.dpr2[k - 2] * (wk2 + prev);
-    w = w * (R)g.mskt(i, j, k);
-    wm[q] = w;
+    R w = wv[k - 1][tx];
+    if (k >= 2) wm[q] = w;
     if (acw) acw[q] = acw[q] + w * alpha[q] / ((R)GRAV * (R)mch2[p]);
-    if (k < g.nk) prev = wk3(k);
   }
 }
 
-// sqrt(ter(chc, g1 -> g3, dft = 1)), land = 1 (mod_int.f90:107-109)
+// int_readyc's sch (mod_int.f90:107-109), precomputed by k_sch
 template <typename R>
-__device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* chc, int m, int jj) {
-  if (g.lev_u[g.ix(m, jj)] == 0) return (R)1.0;
-  auto CH = [&](int a, int b) -> R { return (R)chc[g.ix(a, b)]; };
-  return sqrt(ter_di<1>(g, CH, m, jj, (R)1.0));
+__device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* schr, int m, int jj) {
+  return (R)schr[g.ix(m, jj)];
 }
 
 // op_adv (mod_op.f90:224-278) for uc and vc at one point.
 template <typename R>
 __global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ wm,
-                      const double* __restrict__ chc, R* auc, R* avc) {
+                      const double* __restrict__ chc, const double* __restrict__ schr, R* auc, R* avc) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
@@ -229,7 +269,7 @@ __global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
   const long ko = g.n2 * (k - 1);
   const R* u = uc + ko;
   const R* v = vc + ko;
-  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, chc, m, jj); };
+  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
   auto X = [&](int m, int jj) -> R { return u[g.ix(m, jj)] / SCH(m, jj); };
   auto Y = [&](int m, int jj) -> R {
     R y = v[g.ix(m, jj)] / SCH(m, jj);
@@ -314,13 +354,13 @@ __device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v
 // (v, vc, u, tauy) with u = uc/sch, v = vc/sch.
 template <typename R>
 __global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ am,
-                      const double* __restrict__ chc, const double* __restrict__ taux,
-                      const double* __restrict__ tauy, R* fx, R* fy) {
+                      const double* __restrict__ chc, const double* __restrict__ schr,
+                      const double* __restrict__ taux, const double* __restrict__ tauy, R* fx, R* fy) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
-  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, chc, m, jj); };
+  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
   // 3-D callables (m, jj, kk)
   auto U3 = [&](int m, int jj, int kk) -> R { return uc[g.ix(m, jj, kk)] / SCH(m, jj); };
   auto V3 = [&](int m, int jj, int kk) -> R { return vc[g.ix(m, jj, kk)] / SCH(m, jj); };
@@ -336,18 +376,18 @@ __global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
     bool isu;
     const R* uc;
     const R* vc;
-    const double* chc;
+    const double* schr;
     __device__ R operator()(int m, int jj) const {
-      R s = sch_ready<R>(g, chc, m, jj);
+      R s = sch_ready<R>(g, schr, m, jj);
       return (isu ? uc : vc)[g.ix(m, jj, k)] / s;
     }
     __device__ R operator()(int m, int jj, int kk) const {
-      R s = sch_ready<R>(g, chc, m, jj);
+      R s = sch_ready<R>(g, schr, m, jj);
       return (isu ? uc : vc)[g.ix(m, jj, kk)] / s;
     }
   };
-  View vu{g, k, true, uc, vc, chc};
-  View vv{g, k, false, uc, vc, chc};
+  View vu{g, k, true, uc, vc, schr};
+  View vv{g, k, false, uc, vc, schr};
   (void)U3;
   (void)V3;
   fx[p] = fri_point<R>(g, chc, vu, uc, MV3, (R)taux[q], a, sch, i, j, k);
@@ -357,12 +397,12 @@ __global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
 // column integral of the baroclinic tendency (mod_int.f90:136-149)
 template <typename R>
 __global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__ avc, const R* __restrict__ fx,
-                     const R* __restrict__ fy, const double* __restrict__ chc, const double* __restrict__ pax,
+                     const R* __restrict__ fy, const double* __restrict__ schr, const double* __restrict__ pax,
                      const double* __restrict__ pay, double* ut, double* vt) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  const R sch = sch_ready<R>(g, chc, i, j);
+  const R sch = sch_ready<R>(g, schr, i, j);
   const R px = (R)pax[p], py = (R)pay[p];
   R su = 0, sv = 0;
   for (int k = 1; k <= g.nk; ++k) {
@@ -378,162 +418,6 @@ __global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// int_trop (mod_int.f90:153-296): the barotropic predictor-corrector
-// subcycle in FP64, one cooperative launch for all 2*dtbc/dtbt substeps.
-// Per substep: A) predictor ch on T points, B) momentum on U points,
-// C) corrector ch + time-filter accumulators; grid barriers between.
-template <typename R>
-struct TropArgs {
-  Grid<double> g;
-  const R* am;  // level 1 of am (g32)
-  double *chc, *chp, *mch, *mch2, *acch;
-  double *buc, *bvc, *bup, *bvp;
-  const double *but, *bvt;
-  const double *bphi[4], *bgra[4];
-  const double *graphihx, *graphihy, *cor;
-  double *tu, *tv, *upb1, *upb2, *vz1, *vz2;
-  int nstep;
-  double bt;
-};
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_trop(TropArgs<R> a) {
-  cg::grid_group grid = cg::this_grid();
-  const Grid<double>& g = a.g;
-  const long n2 = g.n2;
-  const long stride = (long)gridDim.x * blockDim.x;
-  const long p0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double bt = a.bt;
-  const int nstep = a.nstep;
-
-  for (long p = p0; p < n2; p += stride) {
-    a.upb1[p] = a.buc[p];
-    a.upb2[p] = a.bvc[p];
-    a.mch[p] = a.chc[p];
-    a.mch2[p] = a.chc[p];
-  }
-
-  auto CHP = [&](int m, int jj) -> double { return a.chp[g.ix(m, jj)]; };
-  auto CHC = [&](int m, int jj) -> double { return a.chc[g.ix(m, jj)]; };
-  auto schp = [&](int m, int jj) -> double {
-    return g.lev_u[g.ix(m, jj)] > 0 ? sqrt(ter_di<1>(g, CHP, m, jj, 0.0)) : 1.0;
-  };
-  auto schc = [&](int m, int jj) -> double {
-    return g.lev_u[g.ix(m, jj)] > 0 ? sqrt(ter_di<1>(g, CHC, m, jj, 0.0)) : 1.0;
-  };
-
-  for (int nt = 1; nt <= nstep; ++nt) {
-    double* pu = (nt & 1) ? a.bup : a.tu;  // up, vp of this substep
-    double* pv = (nt & 1) ? a.bvp : a.tv;
-    double* qu = (nt & 1) ? a.tu : a.bup;  // uc, vc written by it
-    double* qv = (nt & 1) ? a.tv : a.bvp;
-
-    // A) predictor: chc = chp - gamma_b*dtbt*div(prh*sch*ub_p)/prh (sea)
-    for (long p = p0; p < n2; p += stride) {
-      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-      auto S1 = [&](int m, int jj) -> double {
-        long q = g.ix(m, jj);
-        return g.prh_u[q] * schp(m, jj) * pu[q];
-      };
-      auto S2 = [&](int m, int jj) -> double {
-        long q = g.ix(m, jj);
-        return g.prh_u[q] * schp(m, jj) * pv[q];
-      };
-      double wk = divg<3, 3, 1>(g, S1, S2, i, j);
-      wk = a.chp[p] - wk * bt * GAMMA_B / g.prh_t[p];
-      if (g.lev_t[p] > 0) a.chc[p] = wk;
-    }
-    grid.sync();
-
-    // B) momentum: viscosity, pressure gradient, Coriolis adjustment
-    for (long p = p0; p < n2; p += stride) {
-      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-      int ic = g.canon(i);
-      const double su = g.lev_u[p] >= 1 ? 1.0 : 0.0;
-      auto PU = [&](int m, int jj) -> double { return pu[g.ix(m, jj)]; };
-      auto PV = [&](int m, int jj) -> double { return pv[g.ix(m, jj)]; };
-      double l1 = lap<3, 3>(g, PU, i, j);
-      double l2 = lap<3, 3>(g, PV, i, j);
-      double am1 = (double)a.am[p];
-      double vs1 = su * am1 * l1;
-      double vs2 = su * am1 * l2;
-      if (nt == 1) {
-        a.vz1[p] = vs1;
-        a.vz2[p] = vs2;
-      }
-      const double z1 = a.vz1[p], z2 = a.vz2[p];
-      auto GX = [&](int m, int jj) -> double { return grax<1, 2>(g, CHC, m, jj); };
-      auto GY = [&](int m, int jj) -> double { return gray<1, 4>(g, CHC, m, jj); };
-      auto C2 = [&](int m, int jj) -> double { return ter_ew<1>(g, CHC, m, jj); };
-      auto C4 = [&](int m, int jj) -> double { return ter_ns<1>(g, CHC, m, jj, 0.0); };
-      double gi1 = 0.0, ci1 = 0.0;  // row nj is never written (local array)
-      if (j <= g.njm) {
-        gi1 = 0.5 * (GX(ic, j) * a.bphi[0][g.ix(ic, j)] + GX(ic, j + 1) * a.bphi[1][g.ix(ic, j + 1)]);
-        ci1 = 0.5 * (C2(ic, j) * a.bgra[0][g.ix(ic, j)] + C2(ic, j + 1) * a.bgra[1][g.ix(ic, j + 1)]);
-      }
-      double gi2 = 0.5 * (GY(ic, j) * a.bphi[2][g.ix(ic, j)] + GY(ic + 1, j) * a.bphi[3][g.ix(ic + 1, j)]);
-      double ci2 = 0.5 * (C4(ic, j) * a.bgra[2][g.ix(ic, j)] + C4(ic + 1, j) * a.bgra[3][g.ix(ic + 1, j)]);
-      const double sch = schc(i, j);
-      const double cor = a.cor[p];
-      double pg1 = (gi1 + ci1 + a.graphihx[p]) * su * sch;
-      double pg2 = (gi2 + ci2 + a.graphihy[p]) * su * sch;
-      const double up = pu[p], vp = pv[p];
-      double pu1 = su * (vs1 - z1 + a.but[p] - pg1 + 0.5 * cor * vp);
-      double pu2 = su * (vs2 - z2 + a.bvt[p] - pg2 - 0.5 * cor * up);
-      double w1 = pu1 * bt + up;
-      double w2 = pu2 * bt + vp;
-      double x = 0.5 * cor * bt;
-      double uc = (w1 + 0.5 * cor * bt * w2) / (1 + x * x);
-      double vc = (w2 - 0.5 * cor * bt * w1) / (1 + x * x);
-      uc = 0.5 * (uc + up);
-      vc = 0.5 * (vc + vp);
-      pu1 = su * (vs1 - z1 - pg1 + a.but[p] + cor * vc);
-      pu2 = su * (vs2 - z2 - pg2 + a.bvt[p] - cor * uc);
-      qu[p] = pu1 * bt + up;
-      qv[p] = pu2 * bt + vp;
-    }
-    grid.sync();
-
-    // C) corrector and the time-filter accumulators
-    for (long p = p0; p < n2; p += stride) {
-      int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-      auto Q1 = [&](int m, int jj) -> double {
-        long q = g.ix(m, jj);
-        return g.prh_u[q] * schc(m, jj) * qu[q];
-      };
-      auto Q2 = [&](int m, int jj) -> double {
-        long q = g.ix(m, jj);
-        return g.prh_u[q] * schc(m, jj) * qv[q];
-      };
-      double wk = divg<3, 3, 1>(g, Q1, Q2, i, j);
-      double mt = g.lev_t[p] >= 1 ? 1.0 : 0.0;
-      double c = a.chp[p] - mt * wk * bt / g.prh_t[p];
-      if (nt <= nstep / 2) a.mch[p] = a.mch[p] + c;
-      a.chp[p] = c;
-      a.upb1[p] = a.upb1[p] + qu[p];
-      a.upb2[p] = a.upb2[p] + qv[p];
-      a.mch2[p] = a.mch2[p] + c;
-    }
-    grid.sync();
-  }
-
-  for (long p = p0; p < n2; p += stride) {
-    double u = a.upb1[p] / (double)(nstep + 1);
-    double v = a.upb2[p] / (double)(nstep + 1);
-    a.buc[p] = u;
-    a.bvc[p] = v;
-    a.bup[p] = u;
-    a.bvp[p] = v;
-    a.mch[p] = a.mch[p] / (double)(nstep / 2 + 1);
-    double m2 = a.mch2[p] / (double)(nstep + 1);
-    a.mch2[p] = m2;
-    a.chc[p] = m2;
-    a.chp[p] = m2;
-    a.acch[p] = a.acch[p] + m2;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // int_clin (mod_int.f90:314-368): baroclinic momentum at one point,
 // updated in place (uc == up; every read of uc/vc here is pointwise).
 template <typename R>
@@ -541,7 +425,7 @@ struct ClinArgs {
   Grid<R> g;
   R *uc, *vc;
   const R *auc, *aup, *aupp, *avc, *avp, *avpp, *fx, *fy, *adp, *alpha;
-  const double *mch, *pax, *pay;
+  const double *mch, *schm, *pax, *pay;
   double bc;
 };
 
@@ -556,8 +440,7 @@ __global__ void k_clin(ClinArgs<R> a) {
   const long q = g.ix(i, j);
   const long ko = g.n2 * (k - 1);
   auto MCH = [&](int m, int jj) -> R { return (R)a.mch[g.ix(m, jj)]; };
-  R sch = (R)1.0;
-  if (g.lev_u[q] > 0) sch = sqrt(ter_di<1>(g, MCH, i, j, (R)1.0));
+  const R sch = (R)a.schm[q];
   const R* ad = a.adp + ko;
   const R* al = a.alpha + ko;
   const R pr = g.pr2[k - 1];
@@ -630,14 +513,14 @@ __global__ void k_clin_cpl(Grid<R> g, R* uc, R* vc, const double* __restrict__ b
 // unweighted velocity, its accumulator and smag (mod_int.f90:384-440)
 template <typename R>
 __global__ void k_smag(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc,
-                       const double* __restrict__ mch2, R* acu, R* acv, R* am, int dtbc) {
+                       const double* __restrict__ mch2, const double* __restrict__ schs, R* acu, R* acv, R* am,
+                       int dtbc) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   const long ko = g.n2 * (k - 1);
-  auto M2 = [&](int m, int jj) -> R { return (R)mch2[g.ix(m, jj)]; };
-  auto S2 = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, M2, m, jj, (R)1.0)); };
+  auto S2 = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
   auto U1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
   auto U2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
   acu[p] = acu[p] + U1(i, j);
@@ -661,23 +544,29 @@ __global__ void k_smag(Grid<R> g, const R* __restrict__ uc, const R* __restrict_
 
 // int_ssh (mod_int.f90:602-624): alpha from (tp, sp, mch), column SSH
 template <typename R>
-__global__ void k_ssh(Grid<R> g, const R* __restrict__ tp, const R* __restrict__ sp, const double* __restrict__ mch,
-                      R* alpha, double* ssh, double* acssh) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  const R m = (R)mch[p];
-  R s = 0;
-  for (int k = 1; k <= g.nk; ++k) {
-    long q = p + g.n2 * (k - 1);
-    int mk = g.mskt(i, j, k);
-    R al = alpha[q];
-    if (mk == 1) {
-      al = (R)1.0 / den_rho<R>(tp[q], sp[q], m * g.pr2[k - 1]);
-      alpha[q] = al;
+__global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const R* __restrict__ tp, const R* __restrict__ sp,
+                                                   const double* __restrict__ mch, R* alpha, double* ssh,
+                                                   double* acssh) {
+  __shared__ R term[kNKMAX][kCW];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  const long p = (long)blockIdx.x * kCW + tx;
+  if (p < g.n2) {
+    const R m = (R)mch[p];
+    const int lev = g.lev_t[p];
+    for (int k = 1 + kl; k <= nk; k += kKL) {
+      long q = p + g.n2 * (k - 1);
+      R al = alpha[q];
+      if (lev >= k) {
+        al = (R)1.0 / den_rho<R>(tp[q], sp[q], m * g.pr2[k - 1]);
+        alpha[q] = al;
+      }
+      term[k - 1][tx] = al * g.dpr2[k - 1] * m * (R)(lev >= k ? 1 : 0);
     }
-    s += al * g.dpr2[k - 1] * m * (R)mk;
   }
+  __syncthreads();
+  if (p >= g.n2 || kl != 0) return;
+  R s = 0;
+  for (int k = 1; k <= nk; ++k) s += term[k - 1][tx];
   double h;
   if constexpr (sizeof(R) == sizeof(double)) {
     h = s / GRAV;
@@ -696,17 +585,16 @@ template <typename R>
 struct TsArgs {
   Grid<R> g;
   const R *t0, *s0, *wm, *uc, *vc;
-  const double *chc, *mch, *bndt, *bnds;
+  const double *schs, *mch, *bndt, *bnds;
   R *t1, *s1;
   double bc;
 };
 
 template <typename R, typename FV>
-__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* varp, const R* wm, const double* chc,
+__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* varp, const R* wm, const double* schs,
                                           const R* uc, const R* vc, int i, int j, int k) {
   const long ko = g.n2 * (k - 1);
-  auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
-  auto SCH = [&](int m, int jj) -> R { return sqrt(ter_di<1>(g, CH, m, jj, (R)1.0)); };
+  auto SCH = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
   auto UM1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * SCH(m, jj); };
   auto UM2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
   auto X = [&](int m, int jj) -> R { return ter_ns<3>(g, UM1, m, jj, (R)0) * p1<1, 2>(g, var, m, jj); };
@@ -763,8 +651,8 @@ __global__ void k_ts(TsArgs<R> a) {
   const long q = g.ix(i, j);
   auto TV = [&](int m, int jj) -> R { return a.t0[ko + g.ix(m, jj)]; };
   auto SV = [&](int m, int jj) -> R { return a.s0[ko + g.ix(m, jj)]; };
-  R pt1 = -adv_ts_point<R>(g, TV, a.t0, a.wm, a.chc, a.uc, a.vc, i, j, k);
-  R pt2 = -adv_ts_point<R>(g, SV, a.s0, a.wm, a.chc, a.uc, a.vc, i, j, k);
+  R pt1 = -adv_ts_point<R>(g, TV, a.t0, a.wm, a.schs, a.uc, a.vc, i, j, k);
+  R pt2 = -adv_ts_point<R>(g, SV, a.s0, a.wm, a.schs, a.uc, a.vc, i, j, k);
   pt1 = pt1 + dif_point<R>(g, TV, a.t0, a.mch, i, j, k);
   pt2 = pt2 + dif_point<R>(g, SV, a.s0, a.mch, i, j, k);
   const R mk = (R)g.mskt(i, j, k);
@@ -810,13 +698,14 @@ struct Column {
 };
 
 template <typename R>
-__global__ void k_conv(Grid<R> g, R* tc, R* sc, const R* __restrict__ prho1, const R* __restrict__ prho2,
-                       const double* __restrict__ chc, R* act, R* acs, R* acr) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  const int nlev = g.lev_t[p];
-  if (nlev > 1) {
+__global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<R> g, R* tc, R* sc, const R* __restrict__ prho1,
+                                                    const R* __restrict__ prho2, const double* __restrict__ chc,
+                                                    R* act, R* acs, R* acr) {
+  const int tx = threadIdx.x, kl = threadIdx.y;
+  const long p = (long)blockIdx.x * kCW + tx;
+  const bool ok = p < g.n2;
+  const int nlev = ok ? g.lev_t[p] : 0;
+  if (ok && kl == 0 && nlev > 1) {
     Column<R> c{tc + p, sc + p, prho1 + p, prho2 + p, g.dpr2, g.n2};
     int kcnt = 1;
     while (kcnt < nlev) {
@@ -844,13 +733,15 @@ __global__ void k_conv(Grid<R> g, R* tc, R* sc, const R* __restrict__ prho1, con
       }
     }
   }
+  __syncthreads();  // the column's new T/S are visible to the other k-lanes
+  if (!ok) return;
   const R ch = (R)chc[p];
-  for (int k = 1; k <= g.nk; ++k) {
+  for (int k = 1 + kl; k <= g.nk; k += kKL) {
     long q = p + g.n2 * (k - 1);
     R t = tc[q], s = sc[q];
     act[q] = act[q] + t;
     acs[q] = acs[q] + s;
-    if (g.mskt(i, j, k) == 1) {
+    if (nlev >= k) {
       R wk = (R)1.0 / den_rho<R>(t, s, ch * g.pr2[k - 1]);
       acr[q] = acr[q] + (R)1.0 / wk;
     }
@@ -935,6 +826,13 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
   return c.gf;
 }
 
+#define LAUNCH_COL(kern, ...)                                                       \
+  do {                                                                              \
+    hipLaunchKernelGGL(kern, dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), 0, c.stream, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
+  } while (0)
+
 #define LAUNCH(kern, n, ...)                                                        \
   do {                                                                              \
     hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, __VA_ARGS__); \
@@ -972,11 +870,12 @@ std::string step_t(Ctx& c) {
   }
   // den_alpha, int_pgra
   LAUNCH(k_alpha<R>, n3, g, T, S, h.chc, P<R>(f.alpha));
-  LAUNCH(k_adp<R>, n2, g, P<R>(f.alpha), P<R>(f.adp));
+  LAUNCH_COL(k_adp<R>, g, P<R>(f.alpha), P<R>(f.adp));
   LAUNCH(k_bphi<R>, n2, g, P<R>(f.adp), P<R>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
          h.bgra[1], h.bgra[2], h.bgra[3]);
   // int_readyc
-  LAUNCH(k_upw<R>, n2, g, U, V, h.chp, P<R>(f.wm), (R*)nullptr, (const R*)nullptr, (const double*)nullptr);
+  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
+  LAUNCH_COL(k_upw<R>, g, U, V, h.sqp, P<R>(f.wm), (R*)nullptr, (const R*)nullptr, (const double*)nullptr);
   c.ab = (c.ab + 1) % 3;
   R* au = P<R>(f.AU[c.ab]);
   R* av = P<R>(f.AV[c.ab]);
@@ -984,7 +883,7 @@ std::string step_t(Ctx& c) {
   R* avp = P<R>(f.AV[(c.ab + 2) % 3]);
   R* aupp = P<R>(f.AU[(c.ab + 1) % 3]);
   R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
-  LAUNCH(k_adv<R>, n3, g, U, V, P<R>(f.wm), h.chc, au, av);
+  LAUNCH(k_adv<R>, n3, g, U, V, P<R>(f.wm), h.chc, h.schr, au, av);
   if (c.rst != 1) {  // AB3 bootstrap (mod_int.f90:122-132)
     size_t bytes = n3 * sizeof(R);
     if (c.step_i == 1) {
@@ -997,55 +896,44 @@ std::string step_t(Ctx& c) {
       if ((e = hipMemcpyAsync(avpp, avp, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
     }
   }
-  LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
-  LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.chc, h.pax, h.pay, h.but, h.bvt);
+  LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
+  LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
   if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
   // int_trop
   {
-    TropArgs<R> ta;
-    ta.g = c.gd;
-    ta.am = P<R>(f.am);
-    ta.chc = h.chc; ta.chp = h.chp; ta.mch = h.mch; ta.mch2 = h.mch2; ta.acch = h.acch;
-    ta.buc = h.buc; ta.bvc = h.bvc; ta.bup = h.bup; ta.bvp = h.bvp;
-    ta.but = h.but; ta.bvt = h.bvt;
-    for (int q = 0; q < 4; ++q) { ta.bphi[q] = h.bphi[q]; ta.bgra[q] = h.bgra[q]; }
-    ta.graphihx = h.graphihx; ta.graphihy = h.graphihy; ta.cor = c.gd.cor;
-    ta.tu = h.tu; ta.tv = h.tv; ta.upb1 = h.upb1; ta.upb2 = h.upb2; ta.vz1 = h.vz1; ta.vz2 = h.vz2;
-    ta.nstep = c.dtbc / c.dtbt * 2;
-    ta.bt = (double)c.dtbt;
-    void* args[] = {&ta};
-    e = hipLaunchCooperativeKernel((const void*)k_trop<R>, dim3(c.trop_blocks), dim3(kBlock), args, 0, c.stream);
-    if (e != hipSuccess) return hip_err(e, "k_trop (cooperative launch)");
+    std::string te = launch_trop(c);
+    if (!te.empty()) return te;
     c.n_ch += 1;
   }
   if (c.timing) (void)hipEventRecord(c.ev[2], c.stream);
+  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, (double*)nullptr, h.sqp, h.schs, h.schm);
   // int_clin
   {
     ClinArgs<R> ca;
     ca.g = g; ca.uc = U; ca.vc = V;
     ca.auc = au; ca.aup = aup; ca.aupp = aupp; ca.avc = av; ca.avp = avp; ca.avpp = avpp;
     ca.fx = P<R>(f.fx); ca.fy = P<R>(f.fy); ca.adp = P<R>(f.adp); ca.alpha = P<R>(f.alpha);
-    ca.mch = h.mch; ca.pax = h.pax; ca.pay = h.pay; ca.bc = (double)c.dtbc;
+    ca.mch = h.mch; ca.schm = h.schm; ca.pax = h.pax; ca.pay = h.pay; ca.bc = (double)c.dtbc;
     LAUNCH(k_clin<R>, n3, ca);
   }
   LAUNCH(k_clin_cpl<R>, n2, g, U, V, h.buc, h.bvc);
-  LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
+  LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, h.schs, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
   c.n_uv += 1;
   // int_ssh
-  LAUNCH(k_ssh<R>, n2, g, T, S, h.mch, P<R>(f.alpha), h.ssh, h.acssh);
+  LAUNCH_COL(k_ssh<R>, g, T, S, h.mch, P<R>(f.alpha), h.ssh, h.acssh);
   c.n_ssh += 1;
   // int_ts
-  LAUNCH(k_upw<R>, n2, g, U, V, h.chp, P<R>(f.wm), P<R>(f.acw), (const R*)P<R>(f.alpha), (const double*)h.mch2);
+  LAUNCH_COL(k_upw<R>, g, U, V, h.sqp, P<R>(f.wm), P<R>(f.acw), (const R*)P<R>(f.alpha), (const double*)h.mch2);
   c.n_w += 1;
   {
     TsArgs<R> a;
     a.g = g; a.t0 = T; a.s0 = S; a.wm = P<R>(f.wm); a.uc = U; a.vc = V;
-    a.chc = h.chc; a.mch = h.mch; a.bndt = h.bnd[2]; a.bnds = h.bnd[3];
+    a.schs = h.schs; a.mch = h.mch; a.bndt = h.bnd[2]; a.bnds = h.bnd[3];
     a.t1 = P<R>(f.T[1 - c.cur]); a.s1 = P<R>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
     LAUNCH(k_ts<R>, n3, a);
   }
   c.cur = 1 - c.cur;
-  LAUNCH(k_conv<R>, n2, g, P<R>(f.T[c.cur]), P<R>(f.S[c.cur]), P<R>(f.prho1), P<R>(f.prho2), h.chc, P<R>(f.act),
+  LAUNCH_COL(k_conv<R>, g, P<R>(f.T[c.cur]), P<R>(f.S[c.cur]), P<R>(f.prho1), P<R>(f.prho2), h.chc, P<R>(f.act),
          P<R>(f.acs), P<R>(f.acr));
   c.n_ts += 1;
   if (c.timing) {
@@ -1129,23 +1017,6 @@ __global__ void k_den_rho(long n, const double* t, const double* s, const double
 std::string launch_step(Ctx& c) {
   if (c.precision == PCOM_FP64) return step_t<double>(c);
   return step_t<float>(c);
-}
-
-std::string launch_trop_blocks(Ctx& c) {
-  int per_cu = 0;
-  hipError_t e;
-  if (c.precision == PCOM_FP64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop<double>, kBlock, 0);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop<float>, kBlock, 0);
-  if (e != hipSuccess) return hip_err(e, "occupancy of k_trop");
-  hipDeviceProp_t prop;
-  if ((e = hipGetDeviceProperties(&prop, c.device)) != hipSuccess) return hip_err(e, "device properties");
-  long resident = (long)per_cu * prop.multiProcessorCount;
-  long need = (long)nblocks(c.n2);
-  c.trop_blocks = (int)(need < resident ? need : resident);
-  if (c.trop_blocks < 1) return "k_trop cannot be made resident";
-  return "";
 }
 
 std::string launch_global_sums(Ctx& c, double out[3]) {

@@ -53,6 +53,7 @@ __host__ __device__ constexpr int gdi(int g) { return g == 1 ? 3 : g == 2 ? 4 : 
 // arrays exist in both types.
 template <typename R>
 struct Grid {
+  using real = R;
   int ni, nj, nk, nkp, nim, njm;
   long n2;
   const R *rh[5], *tn[5], *dx1[5], *dx2[5];  // g1j..g4j (index 1..4)
@@ -108,23 +109,23 @@ __device__ __host__ inline R den_rho(R t, R s, R p0) {
 // (periodic) input on its own stagger at column i in [1, ni], row j.
 
 // ter_r3d/ter_r2d, hga -> hga%ew (mod_op.f90:376-378): all rows written.
-template <int GA, typename R, typename F>
-__device__ __forceinline__ R ter_ew(const Grid<R>& g, F f, int i, int j) {
+template <int GA, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R ter_ew(const G& g, F f, int i, int j) {
   constexpr int oi = goi(GA);
   int ic = g.canon(i);
   return (R)0.5 * (f(ic - oi, j) + f(ic - oi + 1, j));
 }
 // hga -> hga%ns (mod_op.f90:379-381): rows 1+oj .. njm+oj, else dft.
-template <int GA, typename R, typename F>
-__device__ __forceinline__ R ter_ns(const Grid<R>& g, F f, int i, int j, R dft) {
+template <int GA, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R ter_ns(const G& g, F f, int i, int j, R dft) {
   constexpr int oj = goj(GA);
   if (j < 1 + oj || j > g.njm + oj) return dft;
   int ic = g.canon(i);
   return (R)0.5 * (f(ic, j - oj) + f(ic, j - oj + 1));
 }
 // hga -> hga%di (mod_op.f90:382-385)
-template <int GA, typename R, typename F>
-__device__ __forceinline__ R ter_di(const Grid<R>& g, F f, int i, int j, R dft) {
+template <int GA, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R ter_di(const G& g, F f, int i, int j, R dft) {
   constexpr int oi = goi(GA), oj = goj(GA);
   if (j < 1 + oj || j > g.njm + oj) return dft;
   int ic = g.canon(i);
@@ -135,16 +136,16 @@ __device__ __forceinline__ R ter_di(const Grid<R>& g, F f, int i, int j, R dft) 
 // p1_r3d/p1_r2d when gb%ew == ga (no pre-interpolation), mod_op.f90:
 // 496-527: written on all rows, i in 1+ga%i .. nim+ga%i; divided by
 // gb%dx%x(1) everywhere.  `tf` is the source on stagger GA.
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p1_direct(const Grid<R>& g, F tf, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p1_direct(const G& g, F tf, int i, int j) {
   static_assert(gew(GB) == GA, "p1_direct needs gb%ew == ga");
   constexpr int oi = goi(GA);
   int ic = g.canon(i);
   return (tf(ic - oi + 1, j) - tf(ic - oi, j)) / g.DX1(GB, ic, j);
 }
 // p1 with pre-interpolation ter(var, ga -> gb%ew) (mod_op.f90:510-511).
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p1_interp(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p1_interp(const G& g, F f, int i, int j) {
   constexpr int GT = gew(GB);
   static_assert(GT != GA, "use p1_direct");
   constexpr int oi = goi(GA);
@@ -158,16 +159,16 @@ __device__ __forceinline__ R p1_interp(const Grid<R>& g, F f, int i, int j) {
 }
 // p2_r3d/p2_r2d when gb%ns == ga (mod_op.f90:622-652): rows
 // 1+ga%j .. njm+ga%j, zero elsewhere, divided by gb%dx%x(2).
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p2_direct(const Grid<R>& g, F tf, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p2_direct(const G& g, F tf, int i, int j) {
   static_assert(gns(GB) == GA, "p2_direct needs gb%ns == ga");
   constexpr int oj = goj(GA);
   int ic = g.canon(i);
   R d = (j < 1 + oj || j > g.njm + oj) ? (R)0 : (tf(ic, j - oj + 1) - tf(ic, j - oj));
   return d / g.DX2(GB, ic, j);
 }
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p2_interp(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p2_interp(const G& g, F f, int i, int j) {
   constexpr int GT = gns(GB);
   static_assert(GT != GA, "use p2_direct");
   constexpr int oj = goj(GA);
@@ -181,20 +182,20 @@ __device__ __forceinline__ R p2_interp(const Grid<R>& g, F f, int i, int j) {
   return d / g.DX2(GB, ic, j);
 }
 
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p1(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p1(const G& g, F f, int i, int j) {
   if constexpr (gew(GB) == GA) return p1_direct<GA, GB>(g, f, i, j);
   else return p1_interp<GA, GB>(g, f, i, j);
 }
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R p2(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R p2(const G& g, F f, int i, int j) {
   if constexpr (gns(GB) == GA) return p2_direct<GA, GB>(g, f, i, j);
   else return p2_interp<GA, GB>(g, f, i, j);
 }
 
 // div_r3d / div_r2d (mod_op.f90:813-848): (va on GA, vb on GB) -> GC.
-template <int GA, int GB, int GC, typename R, typename FA, typename FB>
-__device__ __forceinline__ R divg(const Grid<R>& g, FA fa, FB fb, int i, int j) {
+template <int GA, int GB, int GC, typename G, typename FA, typename FB, typename R = typename G::real>
+__device__ __forceinline__ R divg(const G& g, FA fa, FB fb, int i, int j) {
   auto vr = [&](int m, int jj) -> R { return fb(m, jj) * g.RH(GB, m, jj); };
   R a1 = p1<GA, GC>(g, fa, i, j);
   R b1 = p2<GB, GC>(g, vr, i, j);
@@ -203,19 +204,19 @@ __device__ __forceinline__ R divg(const Grid<R>& g, FA fa, FB fb, int i, int j) 
 }
 
 // gra x and y parts (mod_op.f90:850-902): p1/(a*gb%rh), p2/a.
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R grax(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R grax(const G& g, F f, int i, int j) {
   int ic = g.canon(i);
   return p1<GA, GB>(g, f, i, j) / ((R)ARAD * g.RH(GB, ic, j));
 }
-template <int GA, int GC, typename R, typename F>
-__device__ __forceinline__ R gray(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GC, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R gray(const G& g, F f, int i, int j) {
   return p2<GA, GC>(g, f, i, j) / (R)ARAD;
 }
 
 // op_lap (mod_op.f90:904-925) for 2-D var on GA, result on GB.
-template <int GA, int GB, typename R, typename F>
-__device__ __forceinline__ R lap(const Grid<R>& g, F f, int i, int j) {
+template <int GA, int GB, typename G, typename F, typename R = typename G::real>
+__device__ __forceinline__ R lap(const G& g, F f, int i, int j) {
   constexpr int GE = gew(GB), GN = gns(GB);
   auto wa = [&](int m, int jj) -> R {
     int mc = g.canon(m);

@@ -90,11 +90,12 @@ def den_rho(t, s, p):
 class Model:
     """One device-resident PCOM time stepper (a pcom_ctx)."""
 
-    def __init__(self, cfg, precision=abi.PCOM_FP64, device=0, restart_run=0):
+    def __init__(self, cfg, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0)):
         self.cfg = cfg
         self.precision = precision
         self._inp = cfg.c_inputs()
-        self._opt = cfg.c_opts(precision=precision, device=device, restart_run=restart_run)
+        self._opt = cfg.c_opts(precision=precision, device=device, restart_run=restart_run,
+                               trop_tile=trop_tile)
         h = C.c_void_p()
         _check(lib().pcom_create(C.byref(self._inp), C.byref(self._opt), C.byref(h)))
         self.h = h

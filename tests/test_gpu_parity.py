@@ -127,6 +127,20 @@ def test_step_bitwise(gpu_lib, cfgname, nsteps):
         assert getattr(cm, f) == getattr(co, f), f
 
 
+@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (8, 8, 4), (24, 12, 2), (7, 9, 5), (8, 6, 5)])
+def test_subcycle_tiling_is_exact(gpu_lib, tile):
+    """The LDS-tiled barotropic subcycle gives the reference's bits for any
+    tile shape and any number of substeps between exchanges."""
+    for cfgname in ("small", "box"):
+        cfg = configs.CONFIGS[cfgname]()
+        m = Model(cfg, trop_tile=tile)
+        o = Oracle(cfg)
+        m.step(2)
+        o.step(2)
+        bad = _compare(m, o, ["chc", "chp", "mch", "mch2", "acch", "buc", "bvc", "bup", "bvp", "tc", "uc"])
+        assert not bad, f"{cfgname} tile {tile}:\n" + "\n".join(bad)
+
+
 def test_box_many_steps_bitwise(gpu_lib):
     """Config 1 (box, 60x40x10): 30 steps crossing a day boundary."""
     cfg = configs.box()
