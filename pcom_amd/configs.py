@@ -14,9 +14,10 @@ analytically:
 * ``global`` - quasi-global 1 degree 360x180x30, zonally periodic, seeded
                Gaussian-smoothed bathymetry scaled to 0-5500 m, land where
                shallower than 50 m and poleward of 70 degrees (barotropic
-               CFL of dtbt = 60 s), tanh latitudinal T profile plus
-               N(0, 0.05 K) noise, smooth S in 34-36 psu, analytic annual
-               wind and restoring SST.
+               CFL of dtbt = 60 s), 30 levels stretched from 20 m,
+               tanh latitudinal T profile plus N(0, 0.05 K) noise, smooth
+               S in 34-36 psu, analytic annual wind (0.05 N m-2, tapered
+               on the equator) and restoring SST.
 
 Arrays are Fortran-ordered (i fastest) float64, without halo, as
 pcom_inputs in include/pcom_gpu.h expects them.
@@ -126,7 +127,7 @@ def _smooth(field2d, sigma):
     return gaussian_filter(field2d, sigma=sigma, mode=("wrap", "reflect"))
 
 
-def stretched_levels(nk, depth, top=10.0, power=1.5):
+def stretched_levels(nk, depth, top=20.0, power=1.5):
     """Layer thicknesses growing from `top` m, summing to `depth` m."""
     s = (np.arange(nk, dtype=np.float64) / max(nk - 1, 1)) ** power
     c = (depth - top * nk) / s.sum()
@@ -166,7 +167,11 @@ def global_1deg(glo_ni=360, glo_nj=180, nk=30, depth=5500.0, polar_cut=70.0,
     months = np.arange(12, dtype=np.float64)
     season = np.cos(2.0 * np.pi * months / 12.0)
     hemi = np.sign(lat)
-    tx = -0.1 * np.cos(3.0 * np.pi * lat / 140.0)
+    # trades and westerlies of 0.05 N m-2, tapered to zero on the equator:
+    # int_ts steps T/S forward with centred advection (mod_int.f90:484,
+    # mod_op.f90:280-325), stable only while ah >= u^2 dtbc / 2, so the
+    # configuration keeps currents well below 1 m/s
+    tx = -0.05 * np.cos(3.0 * np.pi * lat / 140.0) * (1.0 - np.exp(-(lat / 8.0) ** 2))
     taux = tx[None, :, None] * (1.0 + 0.2 * hemi[None, :, None] * season[None, None, :])
     taux = np.broadcast_to(taux, (glo_ni, glo_nj, 12)).copy()
     bct = tsurf[None, :, None] - 2.0 * hemi[None, :, None] * season[None, None, :]

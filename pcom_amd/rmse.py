@@ -1,0 +1,96 @@
+"""Error of the mixed-precision form against the FP64 form.
+
+Runs both forms of the time stepper side by side from the same initial
+state and reports, at chosen simulated times, the area-weighted RMSE of
+every prognostic field over sea points (weights rh*dx1*dx2 of the T grid,
+mask from lev; the U-grid fields use the U mask).  The FP64 form is
+bit-identical to the FP64 oracle (tests/test_gpu_parity.py), so it stands in
+for the oracle over runs the CPU cannot finish.
+
+    python -m pcom_amd.rmse --days 365 --every 30
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import abi, configs
+from .lib import Model
+
+FIELDS_T = ["tc", "sc"]
+FIELDS_U = ["uc", "vc"]
+FIELDS_2D = ["chc", "ssh", "buc", "bvc"]
+
+
+def weights(m, cfg):
+    area = m.metric(1, 0) * m.metric(1, 2) * m.metric(1, 3)
+    lev_t = np.zeros((m.ni, m.nj), dtype=np.int32)
+    lev_t[1:-1, 1:-1] = cfg.itn
+    inner = np.zeros((m.ni, m.nj), dtype=bool)
+    inner[1:-1, 1:-1] = True
+    prh_u = m.get("prh_u")
+    wet_u = (prh_u != np.float64(np.float32(0.1))) & inner
+    return area, lev_t, inner, wet_u
+
+
+def rmse_report(ma, mb, cfg):
+    area, lev_t, inner, wet_u = weights(ma, cfg)
+    out = {}
+    sea1 = (lev_t >= 1) & inner
+    for name in FIELDS_T:
+        a, b = ma.get(name), mb.get(name)
+        w = area[..., None] * ((lev_t[..., None] >= np.arange(1, ma.nk + 1)) & inner[..., None])
+        out[name] = float(np.sqrt(np.sum(w * (a - b) ** 2) / np.sum(w)))
+        if name == "tc":
+            d = (a[:, :, 0] - b[:, :, 0])[sea1]
+            out["sst"] = float(np.sqrt(np.sum(area[sea1] * d * d) / np.sum(area[sea1])))
+    for name in FIELDS_U:
+        a, b = ma.get(name), mb.get(name)
+        w = area[..., None] * wet_u[..., None]
+        out[name] = float(np.sqrt(np.sum(w * (a - b) ** 2) / np.sum(w)))
+    for name in FIELDS_2D:
+        a, b = ma.get(name), mb.get(name)
+        msk = sea1 if name in ("chc", "ssh") else wet_u
+        d = (a - b)[msk]
+        out[name] = float(np.sqrt(np.sum(area[msk] * d * d) / np.sum(area[msk])))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--config", default="global", choices=list(configs.CONFIGS))
+    ap.add_argument("--days", type=int, default=365)
+    ap.add_argument("--every", type=int, default=30, help="report every N days")
+    ap.add_argument("--out", default=None, help="write the JSON report here")
+    args = ap.parse_args(argv)
+    cfg = configs.CONFIGS[args.config]()
+    ref = Model(cfg, precision=abi.PCOM_FP64)
+    mix = Model(cfg, precision=abi.PCOM_MIXED)
+    steps_per_day = 86400 // cfg.dtbc
+    report = {"config": cfg.name, "dims": [cfg.glo_ni, cfg.glo_nj, cfg.nk], "records": []}
+    t0 = time.time()
+    day = 0
+    while day < args.days:
+        n = min(args.every, args.days - day)
+        ref.step(n * steps_per_day)
+        mix.step(n * steps_per_day)
+        day += n
+        rec = {"day": day, "finite": bool(ref.check_finite() and mix.check_finite())}
+        rec.update(rmse_report(mix, ref, cfg))
+        report["records"].append(rec)
+        print(json.dumps(rec), flush=True)
+    report["wall_s"] = time.time() - t0
+    last = report["records"][-1]
+    report["gate"] = {"sst_K": 1e-3, "ssh_m": 1e-3,
+                      "pass": bool(last["finite"] and last["sst"] <= 1e-3 and last["ssh"] <= 1e-3)}
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(report, f, indent=1)
+    print(json.dumps(report["gate"]))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -151,3 +151,16 @@ def test_box_many_steps_bitwise(gpu_lib):
     bad = _compare(m, o, STEP_FIELDS)
     assert not bad, "\n".join(bad)
     assert m.check_finite()
+
+
+def test_global_1deg_bitwise_across_a_day(gpu_lib):
+    """The full 1-degree workload (360x180x30): 25 steps, crossing the day
+    boundary (daily forcing and den_prho), bit-identical to the oracle."""
+    cfg = configs.global_1deg()
+    m = Model(cfg)
+    o = Oracle(cfg)
+    m.step(25)
+    o.step(25)
+    bad = _compare(m, o, ["tc", "sc", "uc", "vc", "chc", "mch", "buc", "bvc", "am", "ssh", "acch", "act",
+                          "acu", "acw", "acr", "wm", "fx", "auc", "bphi_xn", "bgra_yw"])
+    assert not bad, "\n".join(bad)
