@@ -25,36 +25,19 @@ namespace pcom {
 
 namespace {
 
-constexpr int kTropBlock = 1024;
+constexpr int kTropBlock = 1024;  // one region point per thread: RW * RHt <= 1024
 
-// Tile coordinates: x is an interior column counted from 0 (Fortran
-// i = 2 + (x mod glo_ni)), j is the Fortran row.  Periodicity is exact in
-// these coordinates, so canon() is the identity.  Everything a substep reads
-// lives in LDS: the horizontal metrics as 1-D vectors (rh and dx%x(2) depend
-// on the row only, dx%x(1) on the column only, because the start-up chain
-// builds them from the 1-D lon/lat inputs, main.f90:347-497; the host
-// checks this), the 2-D statics and the prognostic fields on the region.
-struct TileGrid {
-  using real = double;
-  int ni, njm, bx0, by0, RW, RHt;
-  const int* colmap;   // [RW] Fortran column - 1 of each region column
-  const double* rhj;   // [5][RHt]
-  const double* dx2j;  // [5][RHt]
-  const double* dx1i;  // [5][RW]
-  __device__ __forceinline__ int canon(int x) const { return x; }
-  __device__ __forceinline__ int cx(int x) const {
-    int b = x - bx0;
-    return b < 0 ? 0 : (b >= RW ? RW - 1 : b);
-  }
-  __device__ __forceinline__ int cy(int j) const {
-    int b = j - by0;
-    return b < 0 ? 0 : (b >= RHt ? RHt - 1 : b);
-  }
-  __device__ __forceinline__ int lid(int x, int j) const { return cx(x) + RW * cy(j); }
-  __device__ __forceinline__ double RH(int g, int x, int j) const { return rhj[g * RHt + cy(j)]; }
-  __device__ __forceinline__ double DX1(int g, int x, int j) const { return dx1i[g * RW + cx(x)]; }
-  __device__ __forceinline__ double DX2(int g, int x, int j) const { return dx2j[g * RHt + cy(j)]; }
-};
+// a / b for b with rb = RN(1/b): q = RN(a*rb) is within one ulp of a/b, and
+// one FMA residual step rounds it correctly (Markstein, IBM J. Res. Dev.
+// 34(1), 1990), so the result is the IEEE quotient for every finite,
+// non-subnormal quotient.  All divisors of a substep are static, so their
+// reciprocals are computed once per launch and each division costs a
+// multiply and two FMAs instead of the iterative IEEE sequence.
+__device__ __forceinline__ double qdiv(double a, double b, double rb) {
+  const double q = a * rb;
+  const double r = __builtin_fma(-q, b, a);
+  return __builtin_fma(r, rb, q);
+}
 
 struct TropTileArgs {
   int ni, nj, G;
@@ -74,16 +57,21 @@ struct TropTileArgs {
 constexpr int kFlagStride = 32;                 // 256 B between flags
 constexpr unsigned long long kSpinLimit = 1ull << 22;  // ~seconds; a normal wait is microseconds
 
-// LDS layout of one tile: region arrays of the prognostic fields and of
-// every intermediate a substep reuses, owned-point accumulators, 1-D metrics
-struct TileLds {
-  int NR, NO, RW, RHt;
-  static constexpr int kArr = 15;  // CHP PR Ua Va Ub Vb SC S1 S2 WAu WBu WAv WBv GX GY
-  size_t bytes() const {
-    size_t d = (size_t)kArr * NR + 4 * (size_t)NO + 11 * (size_t)RHt + 5 * (size_t)RW;
-    return d * 8 + (2 * (size_t)NR + RW) * 4;
-  }
+// LDS of one tile: 12 region arrays, a table of per-row metrics (rh,
+// dx%x(2) and cor depend on the row only) and one of per-column metrics
+// (dx%x(1) depends on the column only): the start-up chain builds them
+// from the 1-D lon/lat inputs (main.f90:347-497) and the host checks it.
+constexpr int kRegionArrays = 12;  // CHP PR PU PV S1 S2 WAu WBu WAv WBv GX GY
+enum RowTab {
+  R_RH3, R_DX21, R_DX21I, R_ARH1, R_ARH1I, R_RH4, R_RH4I, R_DX22, R_DX22I, R_RH2, R_ARH2, R_ARH2I,
+  R_DX24, R_DX24I, R_DX23, R_DX23I, R_DEN, R_DENI, R_COR, R_DXX, R_DXXI, R_N
 };
+enum ColTab { C_DX11, C_DX11I, C_DX14, C_DX14I, C_DX12, C_DX12I, C_DX13, C_DX13I, C_N };
+
+size_t trop_lds_bytes(int TW, int TH, int S) {
+  const size_t RW = TW + 5 * S, RHt = TH + 5 * S;
+  return 8 * (kRegionArrays * RW * RHt + R_N * RHt + C_N * RW);
+}
 
 // Publish this tile's round and wait until every tile whose owned points
 // lie in this tile's region has published it.  Waits are bounded: past
@@ -119,11 +107,21 @@ __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int 
   __syncthreads();
 }
 
+// Each workgroup owns a TW x TH tile and holds the tile plus its halo (the
+// "region", RW x RHt points, one per thread) in LDS.  Tile coordinates: x
+// counts interior columns from 0 (Fortran i = 2 + (x mod G)), y is the
+// Fortran row; periodicity is exact in x.  Each thread keeps the static
+// fields of its own point in registers for the whole launch.  Every
+// statement below evaluates the reference's expression for its point with
+// the reference's operation order (ter/p1/p2/div/gra inlined), so the
+// result is independent of the tiling; points near the region edge read
+// clamped neighbours and are garbage, but no owned point depends on them
+// within S substeps.
 __global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
   extern __shared__ double lds[];
   __shared__ int abort_flag;
   if (threadIdx.x == 0) abort_flag = 0;
-  const int G = a.G, nj = a.nj, ni = a.ni;
+  const int G = a.G, nj = a.nj, ni = a.ni, njm = a.nj - 1;
   const int TW = a.TW, TH = a.TH;
   const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
   const int x0 = tx * TW;
@@ -137,272 +135,284 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
   const unsigned long long base = a.epoch << 16;
   const int RW = TW + HL + HR, RHt = TH + HL + HR;
   const int bx0 = x0 - HL, by0 = y0 - HL;
-  const int NR = RW * RHt, NO = TW * TH;
+  const int NR = RW * RHt;
   double* CHP = lds;
   double* PR = CHP + NR;  // predictor ch
-  double* Ua = PR + NR;
-  double* Va = Ua + NR;
-  double* Ub = Va + NR;
-  double* Vb = Ub + NR;
-  double* SC = Vb + NR;   // sqrt(ter(ch)) on U points
-  double* S1 = SC + NR;   // prh*sch*ub fluxes
+  double* PU = PR + NR;   // barotropic velocity
+  double* PV = PU + NR;
+  double* S1 = PV + NR;   // prh*sch*ub fluxes
   double* S2 = S1 + NR;
   double* WAu = S2 + NR;  // Laplacian face fluxes of u and v
   double* WBu = WAu + NR;
   double* WAv = WBu + NR;
   double* WBv = WAv + NR;
-  double* GXa = WBv + NR; // gradients of the predictor ch
-  double* GYa = GXa + NR;
-  double* AM = GYa + NR;  // owned: mch, mch2, upb1, upb2
-  double* AM2 = AM + NO;
-  double* AU = AM2 + NO;
-  double* AV = AU + NO;
-  double* rhj = AV + NO;         // [5][RHt]
-  double* dx2j = rhj + 5 * RHt;  // [5][RHt]
-  double* corj = dx2j + 5 * RHt; // [RHt]
-  double* dx1i = corj + RHt;     // [5][RW]
-  int* levt = reinterpret_cast<int*>(dx1i + 5 * RW);
-  int* levu = levt + NR;
-  int* colmap = levu + NR;
-
-  TileGrid g;
-  g.ni = ni; g.njm = nj - 1; g.bx0 = bx0; g.by0 = by0; g.RW = RW; g.RHt = RHt;
-  g.colmap = colmap; g.rhj = rhj; g.dx2j = dx2j; g.dx1i = dx1i;
+  double* GX = WBv + NR;  // gradients of the predictor ch
+  double* GY = GX + NR;
+  double* rowt = GY + NR;         // [R_N][RHt]
+  double* colt = rowt + R_N * RHt; // [C_N][RW]
   const double bt = a.bt;
   const int nstep = a.nstep;
 
-  // ---- stage the 1-D metrics, masks and column map
+  // ---- per-row and per-column metric tables with the reciprocals of
+  // every divisor (rows outside 1..nj repeat the nearest row; only
+  // region-edge garbage reads them)
+  for (int b = threadIdx.x; b < RHt; b += blockDim.x) {
+    int y = by0 + b;
+    y = y < 1 ? 1 : (y > nj ? nj : y);
+    const long p = 1 + (long)ni * (y - 1);  // column 2
+    double* r = rowt + b;
+    const double cor = a.cor[p];
+    const double xx = 0.5 * cor * bt;
+    r[R_RH3 * RHt] = a.rh[3][p];
+    r[R_DX21 * RHt] = a.dx2[1][p];
+    r[R_ARH1 * RHt] = ARAD * a.rh[1][p];
+    r[R_RH4 * RHt] = a.rh[4][p];
+    r[R_DX22 * RHt] = a.dx2[2][p];
+    r[R_RH2 * RHt] = a.rh[2][p];
+    r[R_ARH2 * RHt] = ARAD * a.rh[2][p];
+    r[R_DX24 * RHt] = a.dx2[4][p];
+    r[R_DX23 * RHt] = a.dx2[3][p];
+    r[R_DEN * RHt] = ARAD * ARAD * a.rh[3][p];
+    r[R_COR * RHt] = cor;
+    r[R_DXX * RHt] = 1 + xx * xx;
+    const int div[] = {R_DX21, R_ARH1, R_RH4, R_DX22, R_ARH2, R_DX24, R_DX23, R_DEN, R_DXX};
+    for (int d : div) r[(d + 1) * RHt] = 1.0 / r[d * RHt];
+  }
   for (int b = threadIdx.x; b < RW; b += blockDim.x) {
     int w = (bx0 + b) % G;
     if (w < 0) w += G;
-    colmap[b] = w + 1;
+    const long p = (long)(w + 1) + (long)ni;  // row 2
+    double* c = colt + b;
+    c[C_DX11 * RW] = a.dx1[1][p];
+    c[C_DX14 * RW] = a.dx1[4][p];
+    c[C_DX12 * RW] = a.dx1[2][p];
+    c[C_DX13 * RW] = a.dx1[3][p];
+    const int div[] = {C_DX11, C_DX14, C_DX12, C_DX13};
+    for (int d : div) c[(d + 1) * RW] = 1.0 / c[d * RW];
   }
-  __syncthreads();
-  for (int b = threadIdx.x; b < RW; b += blockDim.x)
-    for (int s = 1; s <= 4; ++s) dx1i[s * RW + b] = a.dx1[s][colmap[b] + (long)ni * 1];  // row 2
-  for (int b = threadIdx.x; b < RHt; b += blockDim.x) {
-    int y = by0 + b;
-    if (y < 1 || y > nj) continue;
-    long p = 1 + (long)ni * (y - 1);  // column 2
-    for (int s = 1; s <= 4; ++s) {
-      rhj[s * RHt + b] = a.rh[s][p];
-      dx2j[s * RHt + b] = a.dx2[s][p];
+
+  // ---- this thread's point, its neighbours and its static fields
+  const int q = threadIdx.x;
+  const bool inreg = q < NR;
+  const int bx = inreg ? q % RW : 0, by = inreg ? q / RW : 0;
+  const int x = bx0 + bx, y = by0 + by;
+  const bool live = inreg && y >= 1 && y <= nj;
+  const int bxe = bx + 1 < RW ? bx + 1 : RW - 1, bxw = bx > 0 ? bx - 1 : 0;
+  const int byn = by + 1 < RHt ? by + 1 : RHt - 1, bys = by > 0 ? by - 1 : 0;
+  const int qe = bxe + RW * by, qw = bxw + RW * by, qn = bx + RW * byn, qs = bx + RW * bys;
+  const int qne = bxe + RW * byn, qsw = bxw + RW * bys;
+  int w = x % G;
+  if (w < 0) w += G;
+  const int we = w + 1 < G ? w + 1 : 0;
+  const int yc = y < 1 ? 1 : (y > nj ? nj : y);
+  const long p = (long)(w + 1) + (long)ni * (yc - 1);
+  const long pe = (long)(we + 1) + (long)ni * (yc - 1);
+  const long pn = y < nj ? p + ni : p;
+  const bool owned = live && bx >= HL && bx < HL + TW && by >= HL && by < HL + TH && x < G;
+  int levt = 0, levu = 0;
+  double prh_u = 0, prh_t = 1, rprh_t = 1, chc0 = 0, am1 = 0, z1 = 0, z2 = 0, ghx = 0, ghy = 0, ut = 0, vt = 0;
+  double bph0 = 0, bph1 = 0, bph2 = 0, bph3 = 0, bgr0 = 0, bgr1 = 0, bgr2 = 0, bgr3 = 0;
+  double acm = 0, acm2 = 0, acu = 0, acv = 0;
+  if (live) {
+    levt = a.lev_t[p];
+    levu = a.lev_u[p];
+    prh_u = a.prh_u[p];
+    prh_t = a.prh_t[p];
+    rprh_t = 1.0 / prh_t;
+    chc0 = a.chc[p];
+    am1 = a.am1[p];
+    z1 = a.vz1[p];
+    z2 = a.vz2[p];
+    ghx = a.ghx[p];
+    ghy = a.ghy[p];
+    ut = a.but[p];
+    vt = a.bvt[p];
+    bph0 = a.bphi[0][p];
+    bph2 = a.bphi[2][p];
+    bgr0 = a.bgra[0][p];
+    bgr2 = a.bgra[2][p];
+    bph1 = a.bphi[1][pn];
+    bgr1 = a.bgra[1][pn];
+    bph3 = a.bphi[3][pe];
+    bgr3 = a.bgra[3][pe];
+    CHP[q] = a.chp[p];
+    PU[q] = a.bup[p];
+    PV[q] = a.bvp[p];
+    if (owned) {
+      acm = chc0;
+      acm2 = chc0;
+      acu = a.buc[p];
+      acv = a.bvc[p];
     }
-    corj[b] = a.cor[p];
   }
-  auto gidx = [&](int q) -> long {
-    int bx = q % RW, by = q / RW;
-    return colmap[bx] + (long)ni * (by0 + by - 1);
-  };
-  for (int q = threadIdx.x; q < NR; q += blockDim.x) {
-    int y = by0 + q / RW;
-    bool in = y >= 1 && y <= nj;
-    levt[q] = in ? a.lev_t[gidx(q)] : 0;
-    levu[q] = in ? a.lev_u[gidx(q)] : 0;
-  }
-
-  auto load = [&](const double* c, const double* u, const double* v) {
-    for (int q = threadIdx.x; q < NR; q += blockDim.x) {
-      int y = by0 + q / RW;
-      if (y < 1 || y > nj) continue;
-      long p = gidx(q);
-      CHP[q] = c[p];
-      Ua[q] = u[p];
-      Va[q] = v[p];
+  const double* rw = rowt + by;
+  const double* rws = rowt + bys;
+  const double* cl = colt + bx;
+#define ROW(n) rw[(n) * RHt]
+#define COL(n) cl[(n) * RW]
+  // divg<3,3,1>(F1, F2) (div_r2d, mod_op.f90:813-848): p1 and p2 with the
+  // pre-interpolations ter_ns(g3->g2) and ter_ew(g3->g4), / (a*rh_g1)
+  auto div331 = [&]() -> double {
+    const bool r2 = y >= 2 && y <= nj;
+    const double t1 = r2 ? 0.5 * (S1[qs] + S1[q]) : 0.0;
+    const double t0 = r2 ? 0.5 * (S1[qsw] + S1[qw]) : 0.0;
+    const double a1 = qdiv(t1 - t0, COL(C_DX11), COL(C_DX11I));
+    double d = 0.0;
+    if (r2) {
+      const double rh3 = ROW(R_RH3), rh3s = rws[R_RH3 * RHt];
+      const double u1 = 0.5 * (S2[qw] * rh3 + S2[q] * rh3);
+      const double u0 = 0.5 * (S2[qsw] * rh3s + S2[qs] * rh3s);
+      d = u1 - u0;
     }
+    const double b1 = qdiv(d, ROW(R_DX21), ROW(R_DX21I));
+    return qdiv(a1 + b1, ROW(R_ARH1), ROW(R_ARH1I));
   };
-  auto owned = [&](int o, int& x, int& y) -> bool {
-    x = x0 + o % TW;
-    y = y0 + o / TW;
-    return x < G && y <= nj;
+  // sqrt(ter(v, g1 -> g3)) on sea U points, 1 on land
+  const bool rin = y >= 1 && y <= njm;
+  auto sch_of = [&](const double* v) -> double {
+    if (levu <= 0) return 1.0;
+    const double t = rin ? (v[q] + v[qn] + v[qe] + v[qne]) * 0.25 : 0.0;
+    return sqrt(t);
   };
-
-  load(a.chp, a.bup, a.bvp);
-  for (int o = threadIdx.x; o < NO; o += blockDim.x) {
-    int x, y;
-    if (!owned(o, x, y)) continue;
-    long p = (long)(1 + x) + (long)ni * (y - 1);
-    AU[o] = a.buc[p];
-    AV[o] = a.bvc[p];
-    AM[o] = a.chc[p];
-    AM2[o] = a.chc[p];
-  }
+  const double su = levu >= 1 ? 1.0 : 0.0;
+  const double mt = levt >= 1 ? 1.0 : 0.0;
   __syncthreads();
 
-  double* PU = Ua;
-  double* PV = Va;
-  double* QU = Ub;
-  double* QV = Vb;
   int round = 0;
-  auto L = [&](const double* arr) { return [=, &g](int m, int jj) -> double { return arr[g.lid(m, jj)]; }; };
-  auto CH = L(CHP);
-  auto PRED = L(PR);
-  auto F1 = L(S1);
-  auto F2 = L(S2);
-  // every stage visits the region points on rows 1..nj
-#define FOR_REGION(...)                                        \
-  for (int q = threadIdx.x; q < NR; q += blockDim.x) {        \
-    const int x = bx0 + q % RW, y = by0 + q / RW;             \
-    if (y < 1 || y > nj) continue;                            \
-    (void)x;                                                  \
-    __VA_ARGS__                                               \
-  }                                                           \
-  __syncthreads();
-
   for (int nt = 1; nt <= nstep; ++nt) {
-    auto FU = L(PU);
-    auto FV = L(PV);
     // A) predictor (mod_int.f90:184-193): sch = sqrt(ter(chp)) (1 on land),
     // fluxes prh*sch*ub_p, chc = chp - gamma_b*dtbt*div/prh on sea
-    FOR_REGION({
-      const double sc = levu[q] > 0 ? sqrt(ter_di<1>(g, CH, x, y, 0.0)) : 1.0;
-      const double pu = a.prh_u[gidx(q)];
-      S1[q] = pu * sc * PU[q];
-      S2[q] = pu * sc * PV[q];
-    })
-    FOR_REGION({
-      double wk = divg<3, 3, 1>(g, F1, F2, x, y);
-      wk = CHP[q] - wk * bt * GAMMA_B / a.prh_t[gidx(q)];
+    if (live) {
+      const double sc = sch_of(CHP);
+      S1[q] = prh_u * sc * PU[q];
+      S2[q] = prh_u * sc * PV[q];
+    }
+    __syncthreads();
+    if (live) {
+      double wk = div331();
+      wk = CHP[q] - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
       // land keeps its ch: the entry value in the first substep, afterwards
       // the corrector's chp - 0 (mod_int.f90:193, 267)
-      PR[q] = levt[q] > 0 ? wk : (nt == 1 ? a.chc[gidx(q)] : CHP[q]);
-    })
+      PR[q] = levt > 0 ? wk : (nt == 1 ? chc0 : CHP[q]);
+    }
+    __syncthreads();
     // B) barotropic momentum (mod_int.f90:195-261).  Face fluxes of op_lap
     // (mod_op.f90:913-920) and gradients of the predictor ch per point.
-    FOR_REGION({
-      SC[q] = levu[q] > 0 ? sqrt(ter_di<1>(g, PRED, x, y, 0.0)) : 1.0;
-      WAu[q] = p1<3, 4>(g, FU, x, y) / g.RH(4, x, y);
-      WBu[q] = p2<3, 2>(g, FU, x, y) * g.RH(2, x, y);
-      WAv[q] = p1<3, 4>(g, FV, x, y) / g.RH(4, x, y);
-      WBv[q] = p2<3, 2>(g, FV, x, y) * g.RH(2, x, y);
-      GXa[q] = grax<1, 2>(g, PRED, x, y);
-      GYa[q] = gray<1, 4>(g, PRED, x, y);
-    })
-    {
-      auto WA1 = L(WAu);
-      auto WB1 = L(WBu);
-      auto WA2 = L(WAv);
-      auto WB2 = L(WBv);
-      FOR_REGION({
-        const long p = gidx(q);
-        const double su = levu[q] >= 1 ? 1.0 : 0.0;
-        const double den = ARAD * ARAD * g.RH(3, x, y);
-        double l1 = (p1<4, 3>(g, WA1, x, y) + p2<2, 3>(g, WB1, x, y)) / den;
-        double l2 = (p1<4, 3>(g, WA2, x, y) + p2<2, 3>(g, WB2, x, y)) / den;
-        double am1 = a.am1[p];
-        double vs1 = su * am1 * l1;
-        double vs2 = su * am1 * l2;
-        const double z1 = a.vz1[p], z2 = a.vz2[p];
-        double gi1 = 0.0, ci1 = 0.0;  // row nj of grachint/chgraint%x(1) is never written
-        if (y <= g.njm) {
-          const int qn = g.lid(x, y + 1);
-          const long pn = gidx(qn);
-          gi1 = 0.5 * (GXa[q] * a.bphi[0][p] + GXa[qn] * a.bphi[1][pn]);
-          ci1 = 0.5 * (ter_ew<1>(g, PRED, x, y) * a.bgra[0][p] + ter_ew<1>(g, PRED, x, y + 1) * a.bgra[1][pn]);
-        }
-        const int qe = g.lid(x + 1, y);
-        const long pe = gidx(qe);
-        double gi2 = 0.5 * (GYa[q] * a.bphi[2][p] + GYa[qe] * a.bphi[3][pe]);
-        double ci2 = 0.5 * (ter_ns<1>(g, PRED, x, y, 0.0) * a.bgra[2][p] +
-                            ter_ns<1>(g, PRED, x + 1, y, 0.0) * a.bgra[3][pe]);
-        const double sch = SC[q];
-        const double cor = corj[g.cy(y)];
-        double pg1 = (gi1 + ci1 + a.ghx[p]) * su * sch;
-        double pg2 = (gi2 + ci2 + a.ghy[p]) * su * sch;
-        const double up = PU[q], vp = PV[q];
-        const double ut = a.but[p], vt = a.bvt[p];
-        double pu1 = su * (vs1 - z1 + ut - pg1 + 0.5 * cor * vp);
-        double pu2 = su * (vs2 - z2 + vt - pg2 - 0.5 * cor * up);
-        double w1 = pu1 * bt + up;
-        double w2 = pu2 * bt + vp;
-        double xx = 0.5 * cor * bt;
-        double uc = (w1 + 0.5 * cor * bt * w2) / (1 + xx * xx);
-        double vc = (w2 - 0.5 * cor * bt * w1) / (1 + xx * xx);
-        uc = 0.5 * (uc + up);
-        vc = 0.5 * (vc + vp);
-        pu1 = su * (vs1 - z1 - pg1 + ut + cor * vc);
-        pu2 = su * (vs2 - z2 - pg2 + vt - cor * uc);
-        const double un = pu1 * bt + up, vn = pu2 * bt + vp;
-        QU[q] = un;
-        QV[q] = vn;
-        // corrector fluxes prh*sch*ub (mod_int.f90:264-265)
-        const double pr = a.prh_u[p];
-        S1[q] = pr * sch * un;
-        S2[q] = pr * sch * vn;
-      })
+    double sch = 1.0;
+    if (live) {
+      sch = sch_of(PR);
+      const bool r2 = y >= 2 && y <= nj;
+      const double pu = PU[q], pv = PV[q];
+      WAu[q] = qdiv(qdiv(pu - PU[qw], COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I));
+      WAv[q] = qdiv(qdiv(pv - PV[qw], COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I));
+      WBu[q] = qdiv(r2 ? pu - PU[qs] : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2);
+      WBv[q] = qdiv(r2 ? pv - PV[qs] : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2);
+      GX[q] = qdiv(qdiv(PR[qe] - PR[q], COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I));
+      GY[q] = qdiv(qdiv(rin ? PR[qn] - PR[q] : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD);
     }
+    __syncthreads();
+    double un = 0, vn = 0;
+    if (live) {
+      const double l1 = qdiv(qdiv(WAu[qe] - WAu[q], COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv(rin ? WBu[qn] - WBu[q] : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+                             ROW(R_DEN), ROW(R_DENI));
+      const double l2 = qdiv(qdiv(WAv[qe] - WAv[q], COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv(rin ? WBv[qn] - WBv[q] : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+                             ROW(R_DEN), ROW(R_DENI));
+      const double vs1 = su * am1 * l1;
+      const double vs2 = su * am1 * l2;
+      double gi1 = 0.0, ci1 = 0.0;  // row nj of grachint/chgraint%x(1) is never written
+      if (y <= njm) {
+        gi1 = 0.5 * (GX[q] * bph0 + GX[qn] * bph1);
+        ci1 = 0.5 * (0.5 * (PR[q] + PR[qe]) * bgr0 + 0.5 * (PR[qn] + PR[qne]) * bgr1);
+      }
+      const double gi2 = 0.5 * (GY[q] * bph2 + GY[qe] * bph3);
+      const double ci2 = 0.5 * ((rin ? 0.5 * (PR[q] + PR[qn]) : 0.0) * bgr2 +
+                                (rin ? 0.5 * (PR[qe] + PR[qne]) : 0.0) * bgr3);
+      const double cor = ROW(R_COR);
+      const double pg1 = (gi1 + ci1 + ghx) * su * sch;
+      const double pg2 = (gi2 + ci2 + ghy) * su * sch;
+      const double up = PU[q], vp = PV[q];
+      double pu1 = su * (vs1 - z1 + ut - pg1 + 0.5 * cor * vp);
+      double pu2 = su * (vs2 - z2 + vt - pg2 - 0.5 * cor * up);
+      const double w1 = pu1 * bt + up;
+      const double w2 = pu2 * bt + vp;
+      double uc = qdiv(w1 + 0.5 * cor * bt * w2, ROW(R_DXX), ROW(R_DXXI));
+      double vc = qdiv(w2 - 0.5 * cor * bt * w1, ROW(R_DXX), ROW(R_DXXI));
+      uc = 0.5 * (uc + up);
+      vc = 0.5 * (vc + vp);
+      pu1 = su * (vs1 - z1 - pg1 + ut + cor * vc);
+      pu2 = su * (vs2 - z2 - pg2 + vt - cor * uc);
+      un = pu1 * bt + up;
+      vn = pu2 * bt + vp;
+      // the new velocity is the next substep's ub_p; no thread reads PU
+      // or PV at another point in this stage
+      PU[q] = un;
+      PV[q] = vn;
+      // corrector fluxes prh*sch*ub (mod_int.f90:264-265)
+      S1[q] = prh_u * sch * un;
+      S2[q] = prh_u * sch * vn;
+    }
+    __syncthreads();
     // C) corrector ch and the time-filter accumulators (mod_int.f90:263-273)
-    FOR_REGION({
-      double wk = divg<3, 3, 1>(g, F1, F2, x, y);
-      double mt = levt[q] >= 1 ? 1.0 : 0.0;
-      CHP[q] = CHP[q] - mt * wk * bt / a.prh_t[gidx(q)];
-    })
-    for (int o = threadIdx.x; o < NO; o += blockDim.x) {
-      int x, y;
-      if (!owned(o, x, y)) continue;
-      int q = g.lid(x, y);
-      double c = CHP[q];
-      if (nt <= nstep / 2) AM[o] = AM[o] + c;
-      AU[o] = AU[o] + QU[q];
-      AV[o] = AV[o] + QV[q];
-      AM2[o] = AM2[o] + c;
+    double cn = 0;
+    if (live) {
+      const double wk = div331();
+      cn = CHP[q] - qdiv(mt * wk * bt, prh_t, rprh_t);
+      CHP[q] = cn;
+      if (owned) {
+        if (nt <= nstep / 2) acm = acm + cn;
+        acu = acu + un;
+        acv = acv + vn;
+        acm2 = acm2 + cn;
+      }
     }
-    // the new velocity is the next substep's ub_p
-    double* t;
-    t = PU; PU = QU; QU = t;
-    t = PV; PV = QV; QV = t;
     if (nt < nstep && nt % a.S == 0) {
-      double* xc = a.xc[round & 1];
-      double* xu = a.xu[round & 1];
-      double* xv = a.xv[round & 1];
-      for (int o = threadIdx.x; o < NO; o += blockDim.x) {
-        int x, y;
-        if (!owned(o, x, y)) continue;
-        int q = g.lid(x, y);
-        long p = (long)(1 + x) + (long)ni * (y - 1);
-        xc[p] = CHP[q];
-        xu[p] = PU[q];
-        xv[p] = PV[q];
+      const int rb = round & 1;
+      if (owned) {
+        a.xc[rb][p] = cn;
+        a.xu[rb][p] = un;
+        a.xv[rb][p] = vn;
       }
       tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
       if (abort_flag) return;
-      // reload the whole region (own points included) into the A buffers
-      PU = Ua; PV = Va; QU = Ub; QV = Vb;
-      load(xc, xu, xv);
+      // reload the whole region (own point included)
+      if (live) {
+        CHP[q] = a.xc[rb][p];
+        PU[q] = a.xu[rb][p];
+        PV[q] = a.xv[rb][p];
+      }
       ++round;
     }
     __syncthreads();
   }
-#undef FOR_REGION
+#undef ROW
+#undef COL
 
   // every tile of the region has read its entry fields before they are
   // overwritten below
   tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
-  if (abort_flag) return;
+  if (abort_flag || !owned) return;
   // end of the cycle (mod_int.f90:276-287), owned points and their halo
   // copies (column 1 mirrors column ni-1, column ni mirrors column 2)
-  for (int o = threadIdx.x; o < NO; o += blockDim.x) {
-    int x, y;
-    if (!owned(o, x, y)) continue;
-    double u = AU[o] / (double)(nstep + 1);
-    double v = AV[o] / (double)(nstep + 1);
-    double m1 = AM[o] / (double)(nstep / 2 + 1);
-    double m2 = AM2[o] / (double)(nstep + 1);
-    int i = 2 + x;
-    int cols[2] = {i, i == G + 1 ? 1 : (i == 2 ? G + 2 : 0)};
-    for (int cidx = 0; cidx < 2; ++cidx) {
-      if (cols[cidx] == 0) continue;
-      long p = (long)(cols[cidx] - 1) + (long)ni * (y - 1);
-      a.buc[p] = u;
-      a.bvc[p] = v;
-      a.bup[p] = u;
-      a.bvp[p] = v;
-      a.mch[p] = m1;
-      a.mch2[p] = m2;
-      a.chc[p] = m2;
-      a.chp[p] = m2;
-      a.acch[p] = a.acch[p] + m2;
-    }
+  const double u = acu / (double)(nstep + 1);
+  const double v = acv / (double)(nstep + 1);
+  const double m1 = acm / (double)(nstep / 2 + 1);
+  const double m2 = acm2 / (double)(nstep + 1);
+  const int i = 2 + x;
+  const int cols[2] = {i, i == G + 1 ? 1 : (i == 2 ? G + 2 : 0)};
+  for (int cidx = 0; cidx < 2; ++cidx) {
+    if (cols[cidx] == 0) continue;
+    const long o = (long)(cols[cidx] - 1) + (long)ni * (y - 1);
+    a.buc[o] = u;
+    a.bvc[o] = v;
+    a.bup[o] = u;
+    a.bvp[o] = v;
+    a.mch[o] = m1;
+    a.mch2[o] = m2;
+    a.chc[o] = m2;
+    a.chp[o] = m2;
+    a.acch[o] = a.acch[o] + m2;
   }
 }
 
@@ -423,14 +433,6 @@ __global__ void k_vz(Grid<double> g, const R* __restrict__ am, const double* __r
   am1[p] = a1;
 }
 
-size_t trop_lds_bytes(int TW, int TH, int S) {
-  TileLds l;
-  l.RW = TW + 5 * S;
-  l.RHt = TH + 5 * S;
-  l.NR = l.RW * l.RHt;
-  l.NO = TW * TH;
-  return l.bytes();
-}
 
 }  // namespace
 
@@ -471,27 +473,37 @@ std::string trop_configure(Ctx& c) {
     if (c.trop_th <= 0) c.trop_th = 12;
     if (c.trop_s <= 0) c.trop_s = 2;
   } else {
-    // default: 2 substeps per exchange and the tile with the fewest region
-    // points (owned + halo) whose tiles are all co-resident
+    // default: the tiling of least modelled cost among those whose tiles
+    // are all co-resident.  A substep costs about one unit per wavefront of
+    // region points, an exchange about kExchangeWaves units.
+    constexpr long kExchangeWaves = 14;
+    const long nstep = (long)c.dtbc / c.dtbt * 2;
     long best = -1;
-    for (int tw = 4; tw <= 64; tw += 2)
-      for (int th = 4; th <= 32; th += 2) {
-        size_t lds = trop_lds_bytes(tw, th, 2);
-        long resident;
-        if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
-        long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
-        if (tiles > resident) continue;
-        long region = (long)(tw + 10) * (th + 10);
-        if (best < 0 || region < best) {
-          best = region;
-          c.trop_tw = tw;
-          c.trop_th = th;
+    for (int s = 1; s <= 4; ++s)
+      for (int tw = 4; tw <= 64; ++tw)
+        for (int th = 4; th <= 64; ++th) {
+          const long nr = (long)(tw + 5 * s) * (th + 5 * s);
+          if (nr > kTropBlock) continue;
+          const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
+          size_t lds = trop_lds_bytes(tw, th, s);
+          long resident;
+          if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
+          if (tiles > resident) continue;
+          const long cost = nstep * ((nr + 63) / 64) + (nstep / s) * kExchangeWaves;
+          if (best < 0 || cost < best) {
+            best = cost;
+            c.trop_tw = tw;
+            c.trop_th = th;
+            c.trop_s = s;
+          }
         }
-      }
     if (best < 0) return "no barotropic tiling fits this device";
-    c.trop_s = 2;
   }
   if (c.trop_s > 16) return "at most 16 barotropic substeps per exchange";
+  if ((long)(c.trop_tw + 5 * c.trop_s) * (c.trop_th + 5 * c.trop_s) > kTropBlock)
+    return "barotropic tile " + std::to_string(c.trop_tw) + "x" + std::to_string(c.trop_th) + "x" +
+           std::to_string(c.trop_s) + ": the tile plus its halo of 5 points per substep must hold at most " +
+           std::to_string(kTropBlock) + " points";
   size_t lds = trop_lds_bytes(c.trop_tw, c.trop_th, c.trop_s);
   long resident;
   if (!(err = trop_resident(c, lds, &resident)).empty()) return err;

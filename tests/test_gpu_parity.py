@@ -127,7 +127,7 @@ def test_step_bitwise(gpu_lib, cfgname, nsteps):
         assert getattr(cm, f) == getattr(co, f), f
 
 
-@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (8, 8, 4), (24, 12, 2), (7, 9, 5), (8, 6, 5)])
+@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (8, 8, 4), (24, 12, 2), (7, 7, 5), (8, 6, 5), (20, 14, 3)])
 def test_subcycle_tiling_is_exact(gpu_lib, tile):
     """The LDS-tiled barotropic subcycle gives the reference's bits for any
     tile shape and any number of substeps between exchanges."""
