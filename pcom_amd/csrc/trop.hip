@@ -74,17 +74,19 @@ size_t trop_lds_bytes(int TW, int TH, int S) {
 }
 
 // Publish this tile's round and wait until every tile whose owned points
-// lie in this tile's region has published it.  Waits are bounded: past
+// lie in this tile's region has published it.  The exchange values are
+// written and read with agent-scope atomic accesses, the flag is stored
+// with release and each poller ends with an agent-scope acquire fence, so
+// only the flags pay for cache maintenance (full __threadfence on both
+// sides costs several times more per exchange).  Waits are bounded: past
 // kSpinLimit polls the tile records an error and stops waiting, so the
 // launch always completes (the host then reports the failure).
 __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int nx_hi, int ny_lo, int ny_hi,
                           unsigned long long value, int* abort_flag) {
-  __syncthreads();  // this tile's exchange stores are complete
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(&a.flags[(size_t)(tx + a.tiles_x * ty) * kFlagStride], value, __ATOMIC_RELAXED,
+  __syncthreads();  // every exchange store of this tile precedes the release below
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&a.flags[(size_t)(tx + a.tiles_x * ty) * kFlagStride], value, __ATOMIC_RELEASE,
                        __HIP_MEMORY_SCOPE_AGENT);
-  }
   const int nxs = nx_hi - nx_lo + 1;
   const int nys = ny_hi - ny_lo + 1;
   for (int n = threadIdx.x; n < nxs * nys; n += blockDim.x) {
@@ -102,9 +104,16 @@ __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int 
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
+}
+
+__device__ __forceinline__ void xstore(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double xload(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Each workgroup owns a TW x TH tile and holds the tile plus its halo (the
@@ -370,17 +379,17 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
     if (nt < nstep && nt % a.S == 0) {
       const int rb = round & 1;
       if (owned) {
-        a.xc[rb][p] = cn;
-        a.xu[rb][p] = un;
-        a.xv[rb][p] = vn;
+        xstore(&a.xc[rb][p], cn);
+        xstore(&a.xu[rb][p], un);
+        xstore(&a.xv[rb][p], vn);
       }
       tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
       if (abort_flag) return;
       // reload the whole region (own point included)
       if (live) {
-        CHP[q] = a.xc[rb][p];
-        PU[q] = a.xu[rb][p];
-        PV[q] = a.xv[rb][p];
+        CHP[q] = xload(&a.xc[rb][p]);
+        PU[q] = xload(&a.xu[rb][p]);
+        PV[q] = xload(&a.xv[rb][p]);
       }
       ++round;
     }
