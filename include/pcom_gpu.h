@@ -81,8 +81,9 @@ typedef struct pcom_opts {
   int precision;   /* PCOM_FP64 or PCOM_MIXED                              */
   int restart_run; /* namelist restart_run: 1 skips the AB3 bootstrap      */
   int device;      /* HIP device ordinal                                   */
-  int trop_tile_i, trop_tile_j; /* barotropic subcycle tile (0 = 16, 16)   */
-  int trop_substeps;            /* substeps between tile exchanges (0 = 2) */
+  int trop_tile_i, trop_tile_j; /* barotropic subcycle tile (0, 0 and 0
+                                   substeps = the library's choice)        */
+  int trop_substeps;            /* substeps between tile exchanges         */
 } pcom_opts;
 
 typedef struct pcom_ctx pcom_ctx;
@@ -144,6 +145,37 @@ int pcom_get_field(pcom_ctx *ctx, int id, double *host, size_t n);
 int pcom_set_field(pcom_ctx *ctx, int id, const double *host, size_t n);
 
 int pcom_get_counters(pcom_ctx *ctx, pcom_counters *c);
+
+/* Restart clock (main.f90:128-141): after loading a restart file, set the
+ * index of the last completed step (tctr%ist - 1), the current date
+ * (tctr%ct = cdate) and the previous date (tctr%pt = pdate), each date as
+ * {year, month, day, hour, minute, second}.  The next pcom_step is step
+ * index+1: it refreshes the daily forcing only on a day change, as the
+ * reference does for i /= 1, and skips the AB3 bootstrap when the context
+ * was created with restart_run = 1.                                     */
+int pcom_set_clock(pcom_ctx *ctx, long long step, const int cdate[6], const int pdate[6]);
+
+/* missing_float of mod_param.f90:62 (a float32 constant) */
+#define PCOM_MISSING_FLOAT (-9.99999e30f)
+
+/* Time means of one output interval, as main.f90:726-760 (output) with
+ * mod_mympi.f90:178-343 computes them: each accumulator divided by its
+ * record count; acr masked by the T mask; acu, acv and acw by the U mask;
+ * acch and SSH by the surface T mask (land = PCOM_MISSING_FLOAT); SSH
+ * minus its global mean weighted by rh*msk.  Arrays are the global
+ * interior (glo_ni, glo_nj[, nk]), FP64, column-major, as mod_io writes
+ * them; a NULL pointer skips that copy.                                  */
+typedef struct pcom_means {
+  double *pt, *sa, *rho;   /* eqts%act, acs, acr / eqts%n        (3-D) */
+  double *u, *v, *w;       /* equv%acu, acv / equv%n, eqw%acw / eqw%n  */
+  double *ch;              /* eqch%acch / eqch%n                  (2-D) */
+  double *ssh;             /* acssh / acssh%n, minus the mean     (2-D) */
+} pcom_means;
+
+/* Compute the means, copy them out, then reset every accumulator and its
+ * count to zero (the reference resets after each output).  Fails with
+ * PCOM_ERR_STATE when a count is zero (no step since the last output).  */
+int pcom_output_means(pcom_ctx *ctx, pcom_means *out);
 
 /* Horizontal metric of stagger g (1..4) from the start-up chain
  * (main.f90:408-497): which = 0 rh = cos(lat), 1 tn = tan(lat),

@@ -40,6 +40,9 @@ def lib():
                   "orc_advance_time"):
             getattr(L, n).argtypes = [vp]
         L.orc_set_step_index.argtypes = [vp, C.c_longlong]
+        L.orc_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
+        L.orc_output_means.restype = C.c_int
         L.orc_op_ter.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp]
         L.orc_op_px1.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_op_px2.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
@@ -143,6 +146,22 @@ class Oracle:
 
     def set_step_index(self, i):
         lib().orc_set_step_index(self.h, i)
+
+    def set_clock(self, step, cdate, pdate):
+        c = (C.c_int * 6)(*cdate)
+        p = (C.c_int * 6)(*pdate)
+        lib().orc_set_clock(self.h, C.c_longlong(step), c, p)
+
+    def output_means(self, names=abi.MEAN_NAMES):
+        g = (self.ni - 2, self.nj - 2)
+        out, m = {}, abi.PcomMeans()
+        for n in names:
+            shp = g if n in ("ch", "ssh") else g + (self.nk,)
+            out[n] = np.empty(shp, order="F")
+            setattr(m, n, _dp(out[n]))
+        if lib().orc_output_means(self.h, C.byref(m)) != 0:
+            raise ValueError("orc_output_means: a record count is zero")
+        return out
 
     def metric(self, g, which):
         out = np.empty((self.ni, self.nj), order="F")

@@ -1099,6 +1099,106 @@ void orc_advance_time(orc *o) {
 
 void orc_set_step_index(orc *o, long long i) { o->step_i = i; }
 
+/* main.f90:128-141: a restart run continues from tctr%ist = step + 1 with
+ * tctr%ct and tctr%pt from the restart file. */
+void orc_set_clock(orc *o, long long step, const int c[6], const int p[6]) {
+  o->nsteps = step;
+  o->step_i = step;
+  otime *t[2] = {&o->ct, &o->pt};
+  const int *d[2] = {c, p};
+  for (int w = 0; w < 2; ++w) {
+    t[w]->y = d[w][0]; t[w]->m = d[w][1]; t[w]->d = d[w][2];
+    t[w]->h = d[w][3]; t[w]->mi = d[w][4]; t[w]->s = d[w][5];
+    t[w]->mm = days_month(d[w][0], d[w][1]);
+  }
+}
+
+/* ---- output(): main.f90:726-760 with mod_mympi.f90:178-343 ----------- */
+/* interior of a halo'd array into a (glo_ni, glo_nj, nd3) array: the
+ * merge_out_* gather of one rank, var(2:ni-1, 2:nj-1, :) */
+static void to_glo(const orc *o, const double *a, int nd3, double *out) {
+  size_t q = 0;
+  for (int k = 1; k <= nd3; ++k)
+    for (int j = 2; j <= NJ - 1; ++j)
+      for (int i = 2; i <= NI - 1; ++i) out[q++] = a[I3(i, j, k)];
+}
+
+int orc_output_means(orc *o, pcom_means *m) {
+  const double missing = (double)PCOM_MISSING_FLOAT;
+  const size_t n3 = o->n3, n2 = o->n2;
+  if (((m->pt || m->sa || m->rho) && o->n_ts == 0) || ((m->u || m->v) && o->n_uv == 0) ||
+      (m->w && o->n_w == 0) || (m->ch && o->n_ch == 0) || (m->ssh && o->n_ssh == 0))
+    return -1;
+  /* merge_out_eqts (178-199) */
+  if (o->n_ts) {
+    for (size_t p = 0; p < n3; ++p) {
+      o->act[p] = o->act[p] / o->n_ts;
+      o->acs[p] = o->acs[p] / o->n_ts;
+      o->acr[p] = o->acr[p] / o->n_ts;
+      if (o->msk_t[p] == 0) o->acr[p] = missing;
+    }
+    if (m->pt) to_glo(o, o->act, NK, m->pt);
+    if (m->sa) to_glo(o, o->acs, NK, m->sa);
+    if (m->rho) to_glo(o, o->acr, NK, m->rho);
+  }
+  dfill(o->act, n3, 0.0); dfill(o->acs, n3, 0.0); dfill(o->acr, n3, 0.0);
+  o->n_ts = 0;
+  /* merge_out_equvw (201-226): all three masked by the U mask */
+  for (size_t p = 0; p < n3; ++p) {
+    if (o->n_uv) {
+      o->acu[p] = o->acu[p] / o->n_uv;
+      o->acv[p] = o->acv[p] / o->n_uv;
+    }
+    if (o->n_w) o->acw[p] = o->acw[p] / o->n_w;
+    if (o->msk_u[p] == 0) {
+      o->acu[p] = missing;
+      o->acv[p] = missing;
+      o->acw[p] = missing;
+    }
+  }
+  if (m->u) to_glo(o, o->acu, NK, m->u);
+  if (m->v) to_glo(o, o->acv, NK, m->v);
+  if (m->w) to_glo(o, o->acw, NK, m->w);
+  dfill(o->acu, n3, 0.0); dfill(o->acv, n3, 0.0); dfill(o->acw, n3, 0.0);
+  o->n_uv = 0;
+  o->n_w = 0;
+  /* eqch%acch (main.f90:743-749) */
+  if (o->n_ch) {
+    for (size_t p = 0; p < n2; ++p) {
+      o->acch[p] = o->acch[p] / o->n_ch;
+      if (o->msk_t[p] == 0) o->acch[p] = missing;
+    }
+    if (m->ch) to_glo(o, o->acch, 1, m->ch);
+  }
+  dfill(o->acch, n2, 0.0);
+  o->n_ch = 0;
+  /* merge_out_accu_gr2d_weight + merge_out_r2d_weight (251-306): acssh/n,
+   * masked, minus sum(var*wg)/sum(wg) with wg = rh(g1)*msk(:,:,1) */
+  if (o->n_ssh) {
+    double *wg = dalloc(n2);
+    for (size_t p = 0; p < n2; ++p) {
+      o->acssh[p] = o->acssh[p] / o->n_ssh;
+      if (o->msk_t[p] == 0) o->acssh[p] = missing;
+      wg[p] = o->rh[1][p] * (double)o->msk_t[p];
+    }
+    const size_t g2 = (size_t)o->glo_ni * o->glo_nj;
+    double *gv = dalloc(g2), *gw = dalloc(g2);
+    to_glo(o, o->acssh, 1, gv);
+    to_glo(o, wg, 1, gw);
+    double s1 = 0.0, s2 = 0.0;
+    for (size_t q = 0; q < g2; ++q) s1 += gv[q] * gw[q];
+    for (size_t q = 0; q < g2; ++q) s2 += gw[q];
+    const double mean = s1 / s2;
+    for (size_t q = 0; q < g2; ++q)
+      if (gv[q] != missing) gv[q] = gv[q] - mean;
+    if (m->ssh) memcpy(m->ssh, gv, g2 * sizeof(double));
+    free(wg); free(gv); free(gw);
+  }
+  dfill(o->acssh, n2, 0.0);
+  o->n_ssh = 0;
+  return 0;
+}
+
 /* main.f90:158-227 */
 void orc_step(orc *o, int n) {
   for (int s = 0; s < n; ++s) {

@@ -47,6 +47,10 @@ void orc_int_bnd(orc *o);
 void orc_advance_time(orc *o);
 /* Set tctr%i (the step index the AB3 bootstrap reads, mod_int.f90:122). */
 void orc_set_step_index(orc *o, long long i);
+/* Restart clock, as pcom_set_clock (main.f90:128-141). */
+void orc_set_clock(orc *o, long long step, const int cdate[6], const int pdate[6]);
+/* output() time means, as pcom_output_means; -1 when a count is zero. */
+int orc_output_means(orc *o, pcom_means *m);
 
 /* Single operators on the context's grid (halo'd arrays, nd3 levels). */
 void orc_op_ter(orc *o, int ga, int gb, int nd3, double dft,

@@ -48,6 +48,15 @@ class PcomCounters(C.Structure):
     ]
 
 
+PCOM_MISSING_FLOAT = float(__import__("numpy").float32(-9.99999e30))
+
+MEAN_NAMES = ["pt", "sa", "rho", "u", "v", "w", "ch", "ssh"]
+
+
+class PcomMeans(C.Structure):
+    _fields_ = [(n, _DP) for n in MEAN_NAMES]
+
+
 # Field ids, in header order.
 FIELD_NAMES = [
     "tc", "sc", "tp", "sp",

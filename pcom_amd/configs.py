@@ -80,6 +80,22 @@ class Config:
         self._keep = keep
         return s
 
+    def write_raw(self, path):
+        """Write the inputs as the raw stream the Fortran driver reads
+        (pcom_amd/fortran/pcom_main.f90): int32 glo_ni, glo_nj, nk,
+        has_vmix; float64 lon, lat, z; int32 itn; float64 tmn, smn, pt, sa,
+        [vmix,] taux, tauy, bct, bcs, pa, fw; Fortran order, no halo."""
+        with open(path, "wb") as f:
+            hv = 0 if self.vmix is None else 1
+            f.write(np.array([self.glo_ni, self.glo_nj, self.nk, hv], dtype="<i4").tobytes())
+            for a in (self.lon, self.lat, self.z):
+                f.write(np.asarray(a, dtype="<f8").tobytes())
+            f.write(np.asfortranarray(self.itn, dtype="<i4").tobytes(order="F"))
+            arrs = [self.tmn, self.smn, self.pt, self.sa] + ([self.vmix] if hv else [])
+            arrs += [self.taux, self.tauy, self.bct, self.bcs, self.pa, self.fw]
+            for a in arrs:
+                f.write(np.asfortranarray(a, dtype="<f8").tobytes(order="F"))
+
     def c_opts(self, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0)):
         o = abi.PcomOpts()
         o.trop_tile_i, o.trop_tile_j, o.trop_substeps = trop_tile
@@ -205,3 +221,12 @@ def small(glo_ni=8, glo_nj=6, nk=4, seed=7):
 
 
 CONFIGS = {"box": box, "global": global_1deg, "small": small}
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 3 or sys.argv[1] not in CONFIGS:
+        print(f"usage: python -m pcom_amd.configs {{{'|'.join(CONFIGS)}}} OUT.bin  "
+              "(raw inputs for the Fortran driver)", file=sys.stderr)
+        sys.exit(2)
+    CONFIGS[sys.argv[1]]().write_raw(sys.argv[2])

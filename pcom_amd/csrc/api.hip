@@ -473,6 +473,48 @@ int pcom_get_counters(pcom_ctx* ctx, pcom_counters* k) {
   return PCOM_OK;
 }
 
+int pcom_set_clock(pcom_ctx* ctx, long long step, const int cdate[6], const int pdate[6]) {
+  if (!ctx || !cdate || !pdate || step < 0) return fail(PCOM_ERR_ARG, "pcom_set_clock: bad argument");
+  Ctx* c = C(ctx);
+  Calendar t[2];
+  const int* d[2] = {cdate, pdate};
+  for (int w = 0; w < 2; ++w) {
+    const int* x = d[w];
+    if (x[0] < 1 || x[1] < 1 || x[1] > 12 || x[2] < 1 || x[2] > days_in_month(x[0], x[1]) || x[3] < 0 ||
+        x[3] > 23 || x[4] < 0 || x[4] > 59 || x[5] < 0 || x[5] > 59)
+      return fail(PCOM_ERR_ARG, "pcom_set_clock: invalid date");
+    t[w].y = x[0]; t[w].m = x[1]; t[w].d = x[2]; t[w].h = x[3]; t[w].mi = x[4]; t[w].s = x[5];
+    t[w].mm = days_in_month(x[0], x[1]);
+  }
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  HIPCHK(hipStreamSynchronize(c->stream), "sync");
+  c->step_i = step;
+  c->ct = t[0];
+  c->pt = t[1];
+  return PCOM_OK;
+}
+
+int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
+  if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_output_means: NULL argument");
+  Ctx* c = C(ctx);
+  if ((out->pt || out->sa || out->rho) && c->n_ts == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no T/S records");
+  if ((out->u || out->v) && c->n_uv == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no u/v records");
+  if (out->w && c->n_w == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no w records");
+  if (out->ch && c->n_ch == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no ch records");
+  if (out->ssh && c->n_ssh == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no ssh records");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  if (int rc = check_async_error(*c)) return rc;
+  if (!c->stage) {
+    void* p = nullptr;
+    size_t n = (size_t)(c->ni - 2) * (c->nj - 2) * c->nk + 2;
+    if (int rc = dev_alloc(*c, &p, n, sizeof(double))) return rc;
+    c->stage = static_cast<double*>(p);
+  }
+  std::string e = launch_output_means(*c, out, c->stage);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_output_means: " + e);
+  return PCOM_OK;
+}
+
 int pcom_get_metric(const pcom_ctx* ctx, int g, int which, double* out, size_t n) {
   if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_get_metric: NULL argument");
   const Ctx* c = C(ctx);

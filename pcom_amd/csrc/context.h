@@ -68,6 +68,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double t_ms[3] = {0, 0, 0};
   int t_steps = 0;
+  double* stage = nullptr;  // output staging, allocated on first use
   size_t rsize() const { return precision == PCOM_FP64 ? sizeof(double) : sizeof(float); }
 };
 
@@ -75,6 +76,9 @@ struct Ctx {
 std::string launch_step(Ctx& c);                 // one baroclinic step
 std::string launch_global_sums(Ctx& c, double out[3]);
 std::string launch_check_finite(Ctx& c, int* bad);
+// time means into `stage` (>= glo_ni*glo_nj*nk + 2 doubles), copied to the
+// host pointers of o, then accumulators and counts reset
+std::string launch_output_means(Ctx& c, pcom_means* o, double* stage);
 std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
 std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
 // single-operator entry points on device buffers (FP64)

@@ -810,6 +810,62 @@ __global__ void k_cell_area(Grid<double> g, double* w) {
   w[p] = g.rh[1][p] * g.dx1[1][p] * g.dx2[1][p] * ARAD * ARAD;
 }
 
+// ---------------------------------------------------------------------------
+// time-mean output (main.f90:726-760, mod_mympi.f90:178-343): interior
+// points of acc / n into a (glo_ni, glo_nj, nk) array, land masked
+template <typename R>
+__global__ void k_mean3(Grid<R> g, const R* __restrict__ acc, int n, int mask, double* out) {
+  const long gni = g.ni - 2, gnj = g.nj - 2;
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= gni * gnj * g.nk) return;
+  const int i = (int)(p % gni) + 2;
+  const long r = p / gni;
+  const int j = (int)(r % gnj) + 2, k = (int)(r / gnj) + 1;
+  double v = (double)acc[g.ix(i, j, k)] / (double)n;
+  if ((mask == 1 && g.mskt(i, j, k) == 0) || (mask == 2 && g.msku(i, j, k) == 0)) v = (double)PCOM_MISSING_FLOAT;
+  out[p] = v;
+}
+
+__global__ void k_mean2(Grid<double> g, const double* __restrict__ acc, int n, double* out) {
+  const long gni = g.ni - 2, gnj = g.nj - 2;
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= gni * gnj) return;
+  const int i = (int)(p % gni) + 2, j = (int)(p / gni) + 2;
+  double v = acc[g.ix(i, j)] / (double)n;
+  if (g.lev_t[g.ix(i, j)] < 1) v = (double)PCOM_MISSING_FLOAT;
+  out[p] = v;
+}
+
+// sum(glo_var*glo_wg) and sum(glo_wg), wg = rh_g1*msk, each accumulated in
+// Fortran's column-major order by one lane so the FP64 result is the
+// reference's; the two sums run on two lanes side by side
+__global__ void k_ssh_mean(Grid<double> g, const double* __restrict__ var, double* sums) {
+  const int t = threadIdx.x;
+  if (t > 1) return;
+  const long gni = g.ni - 2, gnj = g.nj - 2;
+  double s = 0.0;
+  for (long jj = 0; jj < gnj; ++jj)
+    for (long ii = 0; ii < gni; ++ii) {
+      const long q = g.ix((int)ii + 2, (int)jj + 2);
+      const double w = g.rh[1][q] * (double)(g.lev_t[q] >= 1 ? 1 : 0);
+      s += t == 0 ? var[ii + gni * jj] * w : w;
+    }
+  sums[t] = s;
+}
+
+__global__ void k_ssh_anom(long n, const double* __restrict__ sums, double* var) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double mean = sums[0] / sums[1];
+  if (var[p] != (double)PCOM_MISSING_FLOAT) var[p] = var[p] - mean;
+}
+
+template <typename R>
+__global__ void k_fill(R* a, long n) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) a[p] = (R)0;
+}
+
 template <typename R>
 inline R* P(void* v) {
   return static_cast<R*>(v);
@@ -1050,6 +1106,56 @@ std::string launch_check_finite(Ctx& c, int* bad) {
   e = hipMemcpyAsync(bad, c.f2.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream);
   if (e != hipSuccess) return hip_err(e, "copy flag");
   return hip_err(hipStreamSynchronize(c.stream), "sync flag");
+}
+
+template <typename R>
+std::string output_means_t(Ctx& c, pcom_means* o, double* stage) {
+  const Grid<R>& g = grid_of<R>(c);
+  Fields3& f = c.f3;
+  const long gni = c.ni - 2, gnj = c.nj - 2;
+  const size_t m3 = (size_t)(gni * gnj * c.nk), m2 = (size_t)(gni * gnj);
+  auto out3 = [&](void* acc, int n, int mask, double* host) -> std::string {
+    if (!host) return "";
+    LAUNCH(k_mean3<R>, m3, g, P<R>(acc), n, mask, stage);
+    hipError_t e = hipMemcpyAsync(host, stage, m3 * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+    if (e != hipSuccess) return hip_err(e, "copy means");
+    return hip_err(hipStreamSynchronize(c.stream), "sync means");
+  };
+  std::string err;
+  if (!(err = out3(f.act, c.n_ts, 0, o->pt)).empty()) return err;
+  if (!(err = out3(f.acs, c.n_ts, 0, o->sa)).empty()) return err;
+  if (!(err = out3(f.acr, c.n_ts, 1, o->rho)).empty()) return err;
+  if (!(err = out3(f.acu, c.n_uv, 2, o->u)).empty()) return err;
+  if (!(err = out3(f.acv, c.n_uv, 2, o->v)).empty()) return err;
+  if (!(err = out3(f.acw, c.n_w, 2, o->w)).empty()) return err;
+  auto out2 = [&](const double* acc, int n, bool anomaly, double* host) -> std::string {
+    if (!host) return "";
+    LAUNCH(k_mean2, m2, c.gd, acc, n, stage);
+    if (anomaly) {
+      double* sums = stage + m2;
+      hipLaunchKernelGGL(k_ssh_mean, dim3(1), dim3(64), 0, c.stream, c.gd, stage, sums);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return hip_err(e, "k_ssh_mean");
+      LAUNCH(k_ssh_anom, m2, (long)m2, sums, stage);
+    }
+    hipError_t e = hipMemcpyAsync(host, stage, m2 * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+    if (e != hipSuccess) return hip_err(e, "copy means");
+    return hip_err(hipStreamSynchronize(c.stream), "sync means");
+  };
+  if (!(err = out2(c.f2.acch, c.n_ch, false, o->ch)).empty()) return err;
+  if (!(err = out2(c.f2.acssh, c.n_ssh, true, o->ssh)).empty()) return err;
+  // reset accumulators and counts
+  void* acc3[] = {f.act, f.acs, f.acr, f.acu, f.acv, f.acw};
+  for (void* a : acc3) LAUNCH(k_fill<R>, c.n3, P<R>(a), (long)c.n3);
+  LAUNCH(k_fill<double>, c.n2, c.f2.acch, (long)c.n2);
+  LAUNCH(k_fill<double>, c.n2, c.f2.acssh, (long)c.n2);
+  c.n_ts = c.n_uv = c.n_w = c.n_ch = c.n_ssh = 0;
+  return hip_err(hipStreamSynchronize(c.stream), "sync reset");
+}
+
+std::string launch_output_means(Ctx& c, pcom_means* o, double* stage) {
+  if (c.precision == PCOM_FP64) return output_means_t<double>(c, o, stage);
+  return output_means_t<float>(c, o, stage);
 }
 
 #define TER_CASE(A, B)                                                              \

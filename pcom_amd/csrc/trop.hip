@@ -1,6 +1,6 @@
 // trop.hip - int_trop (mod_int.f90:153-296): the barotropic predictor-
-// corrector subcycle of 2*dtbc/dtbt substeps, in FP64, as one cooperative
-// launch of LDS-resident tiles with temporal blocking.
+// corrector subcycle of 2*dtbc/dtbt substeps, in FP64, as one launch of
+// co-resident LDS tiles with temporal blocking.
 //
 // Each workgroup owns a TW x TH tile of the (i, j) index space and keeps
 // the tile plus a halo of 2S (west/south) and 3S (east/north) points of
@@ -9,17 +9,14 @@
 // (predictor div -> sch -> viscosity / pressure gradient -> corrector div),
 // so S substeps run on LDS alone, recomputing the halo redundantly; then
 // tiles exchange their owned points through double-buffered global arrays
-// and meet at one grid barrier.  The arithmetic per point is the same
-// sequence of IEEE operations as the reference's array statements, so the
-// result does not depend on the tiling.
-#include <hip/hip_cooperative_groups.h>
+// and wait for their neighbour tiles' flags.  The arithmetic per point is
+// the same sequence of IEEE operations as the reference's array
+// statements, so the result does not depend on the tiling.
 #include <hip/hip_runtime.h>
 
 #include <string>
 
 #include "context.h"
-
-namespace cg = cooperative_groups;
 
 namespace pcom {
 
@@ -583,10 +580,15 @@ std::string launch_trop_t(Ctx& c) {
   a.error = c.trop_error;
   a.epoch = ++c.trop_epoch;
   a.bt = (double)c.dtbt;
-  void* args[] = {&a};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), args,
-                                            (unsigned)c.trop_lds, c.stream);
-  if (e != hipSuccess) return std::string("k_trop_tiles (cooperative launch): ") + hipGetErrorString(e);
+  // An ordinary launch: trop_configure keeps the grid within the device's
+  // co-resident capacity (one workgroup per CU by its LDS), so every tile
+  // a wait depends on gets a CU once other work on the device drains, and
+  // every wait is bounded.  A cooperative launch would add nothing but its
+  // per-process global-wave-sync resource, which fails when another
+  // process on the device holds it.
+  hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
   return "";
 }
 

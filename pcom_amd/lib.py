@@ -22,6 +22,7 @@ EXPORTS = [
     "pcom_set_field", "pcom_get_counters", "pcom_global_sums", "pcom_check_finite", "pcom_sync",
     "pcom_phase_times", "pcom_destroy", "pcom_last_error", "pcom_op_ter", "pcom_op_px1",
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
+    "pcom_set_clock", "pcom_output_means",
 ]
 
 
@@ -61,6 +62,8 @@ def lib():
         L.pcom_op_lap.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.pcom_den_rho.argtypes = [vp, sz, dp, dp, dp, dp]
         L.pcom_get_metric.argtypes = [vp, C.c_int, C.c_int, dp, sz]
+        L.pcom_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.pcom_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
         for name in EXPORTS:
             if name not in ("pcom_field_size", "pcom_last_error"):
                 getattr(L, name).restype = C.c_int
@@ -136,6 +139,25 @@ class Model:
         c = abi.PcomCounters()
         _check(lib().pcom_get_counters(self.h, C.byref(c)))
         return c
+
+    def set_clock(self, step, cdate, pdate):
+        """Restart clock (main.f90:128-141): last completed step index and
+        the current / previous dates as (y, m, d, h, mi, s)."""
+        c = (C.c_int * 6)(*cdate)
+        p = (C.c_int * 6)(*pdate)
+        _check(lib().pcom_set_clock(self.h, int(step), c, p))
+
+    def output_means(self, names=abi.MEAN_NAMES):
+        """Time means of the output interval (main.f90:726-760); resets the
+        accumulators.  Returns {name: array (glo_ni, glo_nj[, nk])}."""
+        g = (self.ni - 2, self.nj - 2)
+        out, m = {}, abi.PcomMeans()
+        for n in names:
+            shp = g if n in ("ch", "ssh") else g + (self.nk,)
+            out[n] = np.empty(shp, order="F")
+            setattr(m, n, _dp(out[n]))
+        _check(lib().pcom_output_means(self.h, C.byref(m)))
+        return out
 
     def global_sums(self):
         out = np.zeros(3)

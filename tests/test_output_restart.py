@@ -1,0 +1,117 @@
+"""Time-mean output (main.f90:726-760, mod_mympi.f90:178-343) and restart
+(mod_io.f90:59-124, main.f90:128-141) of the oracle, pinned by their
+invariants; the GPU library against the oracle, bit for bit."""
+import numpy as np
+import pytest
+
+from pcom_amd import abi, configs, restart
+
+from oracle.oracle import Oracle
+
+MISSING = abi.PCOM_MISSING_FLOAT
+
+
+def test_means_masks_anomaly_and_reset():
+    cfg = configs.box()
+    o = Oracle(cfg)
+    o.step(5)
+    acc = {n: o.get(n) for n in ("act", "acs", "acr", "acu", "acch", "acssh")}
+    m = o.output_means()
+    lev = np.zeros((62, 42), np.int32)
+    lev[1:-1, 1:-1] = cfg.itn
+    sea = lev[1:-1, 1:-1] >= 1
+    # pt/sa: plain accumulator / count, unmasked
+    np.testing.assert_array_equal(m["pt"], acc["act"][1:-1, 1:-1, :] / 5)
+    np.testing.assert_array_equal(m["sa"], acc["acs"][1:-1, 1:-1, :] / 5)
+    # rho masked by the T mask; ch and ssh by the surface T mask
+    assert np.all(m["rho"][~sea, 0] == MISSING) and np.all(m["rho"][sea, 0] != MISSING)
+    assert np.all(m["ch"][~sea] == MISSING)
+    np.testing.assert_array_equal(m["ch"][sea], (acc["acch"][1:-1, 1:-1] / 5)[sea])
+    # ssh: the rh-weighted mean over sea is removed
+    rh = o.metric(1, 0)[1:-1, 1:-1]
+    ssh = m["ssh"][sea]
+    w = rh[sea]
+    assert abs(np.sum(ssh * w) / np.sum(w)) < 1e-15 * max(1.0, np.abs(acc["acssh"]).max())
+    raw = (acc["acssh"][1:-1, 1:-1] / 5)[sea]
+    np.testing.assert_allclose(ssh - raw, (ssh - raw)[0], rtol=0, atol=1e-15)
+    # u/v/w masked by the U mask
+    lev_u = o.igrid(3)[1:-1, 1:-1]
+    assert np.all(m["u"][lev_u < 1, 0] == MISSING) and np.all(m["w"][lev_u < 1, 0] == MISSING)
+    # everything reset
+    c = o.counters()
+    assert (c.n_ts, c.n_uv, c.n_w, c.n_ch, c.n_ssh) == (0, 0, 0, 0, 0)
+    assert not np.any(o.get("act")) and not np.any(o.get("acssh"))
+    with pytest.raises(ValueError):
+        o.output_means()
+
+
+def _interior(a):
+    return a[1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("cfgname,n1,n2", [("small", 3, 2), ("box", 20, 10)])
+def test_oracle_restart_continues_exactly(tmp_path, cfgname, n1, n2):
+    """Writing the 29 restart fields after n1 steps and continuing in a new
+    restart-run context for n2 steps gives the uninterrupted run's state
+    (the reference's tools/debug_restart.sh check, on interiors)."""
+    cfg = configs.CONFIGS[cfgname]()
+    a = Oracle(cfg)
+    a.step(n1)
+    c = a.counters()
+    cd = (c.year, c.month, c.day, c.hour, c.minute, c.second)
+    pd = (c.year, c.month, c.day, c.hour - 1, c.minute, c.second) if c.hour > 0 else None
+    if pd is None:
+        pytest.skip("restart point on a day boundary")
+    path = str(tmp_path / "rst.bin")
+    restart.write(path, a, cd, pd, c.step)
+    head, fields = restart.read(path)
+    assert head["step"] == n1 and head["cdate"] == cd
+    for n in restart.RESTART_FIELDS:
+        np.testing.assert_array_equal(fields[n], _interior(a.get(n)))
+    b = Oracle(cfg, restart_run=1)
+    restart.load(path, b)
+    a.step(n2)
+    b.step(n2)
+    for n in ("tc", "sc", "uc", "vc", "chc", "buc", "bvc", "am", "auc", "mch", "ssh"):
+        np.testing.assert_array_equal(_interior(b.get(n)), _interior(a.get(n)), err_msg=n)
+
+
+@pytest.mark.gpu
+def test_gpu_means_bitwise(gpu_lib):
+    from pcom_amd.lib import Model
+    for cfgname, n in (("small", 3), ("box", 26)):
+        cfg = configs.CONFIGS[cfgname]()
+        m, o = Model(cfg), Oracle(cfg)
+        m.step(n)
+        o.step(n)
+        gm, om = m.output_means(), o.output_means()
+        for k in abi.MEAN_NAMES:
+            np.testing.assert_array_equal(gm[k], om[k], err_msg=f"{cfgname} {k}")
+        m.step(2)
+        o.step(2)
+        gm, om = m.output_means(["pt", "ssh"]), o.output_means(["pt", "ssh"])
+        for k in gm:
+            np.testing.assert_array_equal(gm[k], om[k], err_msg=f"{cfgname} second interval {k}")
+
+
+@pytest.mark.gpu
+def test_gpu_restart_bitwise(gpu_lib, tmp_path):
+    """GPU restart run == GPU uninterrupted run == oracle restart run."""
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    a = Model(cfg)
+    a.step(20)
+    c = a.counters()
+    cd = (c.year, c.month, c.day, c.hour, c.minute, c.second)
+    pd = (c.year, c.month, c.day, c.hour - 1, c.minute, c.second)
+    path = str(tmp_path / "rst.bin")
+    restart.write(path, a, cd, pd, c.step)
+    b = Model(cfg, restart_run=1)
+    restart.load(path, b)
+    o = Oracle(cfg, restart_run=1)
+    restart.load(path, o)
+    for x in (a, b, o):
+        x.step(10)
+    for n in ("tc", "sc", "uc", "vc", "chc", "buc", "bvc", "am", "auc", "mch", "ssh"):
+        np.testing.assert_array_equal(_interior(b.get(n)), _interior(a.get(n)), err_msg=f"restart vs run {n}")
+        np.testing.assert_array_equal(_interior(b.get(n)), _interior(o.get(n)), err_msg=f"gpu vs oracle {n}")
