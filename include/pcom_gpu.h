@@ -196,7 +196,9 @@ int pcom_sync(pcom_ctx *ctx);
 
 /* Average device duration (ms) of each phase over the steps since the last
  * reset, measured with HIP events on the context's stream; phases are
- * 0 = whole step, 1 = 3-D baroclinic kernels, 2 = barotropic subcycle.  */
+ * 0 = whole step, 1 = everything but the subcycle kernel (3-D baroclinic
+ * kernels and the 2-D set-up of the subcycle), 2 = the barotropic
+ * subcycle kernel alone (k_trop_tiles, one launch per step).            */
 int pcom_phase_times(pcom_ctx *ctx, int enable, double out_ms[3], int *nsteps);
 
 int pcom_destroy(pcom_ctx *ctx);

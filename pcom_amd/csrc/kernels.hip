@@ -954,8 +954,7 @@ std::string step_t(Ctx& c) {
   }
   LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
   LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
-  if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
-  // int_trop
+  // int_trop (launch_trop records ev[1] right before the subcycle kernel)
   {
     std::string te = launch_trop(c);
     if (!te.empty()) return te;

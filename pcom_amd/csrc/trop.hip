@@ -586,6 +586,7 @@ std::string launch_trop_t(Ctx& c) {
   // every wait is bounded.  A cooperative launch would add nothing but its
   // per-process global-wave-sync resource, which fails when another
   // process on the device holds it.
+  if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
   hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
