@@ -6,8 +6,13 @@ Prints ONE JSON line.  `value` is SYPD of the reported form over exactly
 gradient integrals, advection/friction, the 120-substep barotropic cycle,
 baroclinic momentum, SSH, tracers, convection), state resident in HBM.
 Alongside: the other precision form, a per-phase breakdown from HIP events
-on the library's stream, the HBM roofline of the step, and the CPU FP64
-oracle timed on a bounded sample on the host.
+on the library's stream, the roofline of the dominant kernel (the
+barotropic subcycle, k_trop_tiles: FP64 flops per launch over its
+event-timed duration), the HBM roofline of the whole step, and the CPU
+FP64 oracle timed on a bounded sample on the host.
+
+    python bench.py [--steps K] [--warmup W] [--form fp64|mixed]
+    python bench.py --calibrate-copy    # a plain 1 GiB copy, for PMC scaling
 """
 import argparse
 import json
@@ -20,7 +25,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SYPD at 1° 360x180x30 on 1 GPU, mixed+surrogate vs plain FP64, at stated RMSE"
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
 YEAR_S = 31536000.0
+# FP64 operations (+ - * / sqrt) of one barotropic substep at one point, as
+# the reference's int_trop statements count them (mod_int.f90:184-273):
+# predictor sch and fluxes 9, predictor div and ch 22, viscosity fluxes,
+# sch and gradients 23, momentum with pressure gradient, Coriolis and
+# corrector fluxes 104, corrector div, ch and accumulators 26.
+TROP_FLOPS_PER_POINT = 184
+# HBM bytes per k_trop_tiles launch at 1 degree from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a plain 1 GiB copy
+# (profiles/k_trop_tiles_global1deg_pmc.md); None until measured.
+TROP_TRAFFIC_BYTES = None
 
 
 def sypd(ms_per_step, dtbc):
@@ -48,14 +64,29 @@ def run_form(cfg, precision, steps, warmup):
     m.sync()
     wall = time.perf_counter() - t0
     # per-phase device time from HIP events on the library's stream
+    # (phase 2 brackets the subcycle kernel alone)
     m.phase_times(True)
-    m.step(min(steps, 20))
+    m.step(min(steps, 48))
     ph, n = m.phase_times(False)
     ok = m.check_finite()
     sums = m.global_sums()
     m.close()
-    return {"ms_per_step": 1000.0 * wall / steps, "phase_ms": {"step": ph[0], "3d": ph[1], "subcycle": ph[2]},
+    return {"ms_per_step": 1000.0 * wall / steps,
+            "phase_ms": {"step": ph[0], "other_kernels": ph[1], "subcycle_kernel": ph[2]},
             "phase_steps": n, "finite": bool(ok), "heat_salt_mass": [float(x) for x in sums]}
+
+
+def calibrate_copy():
+    """A plain device copy of 1 GiB (reads 1 GiB, writes 1 GiB), run a few
+    times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": a.numel() * 8, "calibrate_copy_bytes_written": b.numel() * 8}))
 
 
 def cpu_baseline(cfg, nsteps):
@@ -76,11 +107,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64",
-                    help="form reported as `value`; the other is measured alongside")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32"], default="fp64",
+                    help="form reported as `value`; the others are measured alongside")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--config", default="global", choices=["global", "box"])
+    ap.add_argument("--calibrate-copy", action="store_true", help=calibrate_copy.__doc__.split(":")[0])
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (one MI355X holds the whole 1-degree domain); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -90,13 +125,19 @@ def main():
     from pcom_amd import abi, configs
     cfg = configs.CONFIGS[args.config]()
     forms = {}
-    for name, prec in (("fp64", abi.PCOM_FP64), ("mixed", abi.PCOM_MIXED)):
+    for name, prec in abi.FORMS.items():
         forms[name] = run_form(cfg, prec, args.steps, args.warmup)
         forms[name]["sypd"] = round(sypd(forms[name]["ms_per_step"], cfg.dtbc), 2)
     rep = forms[args.form]
-    s3 = 8 if args.form == "fp64" else 4
+    s3 = 8 if args.form == "fp64" else 4  # the 31 compulsory 3-D transfers; mixed keeps T,S,u,v in FP64
+    if args.form != "fp64":
+        s3 = (8 * 8 + 23 * 4) / 31
     b = step_bytes(cfg, s3)
-    achieved = b / (rep["ms_per_step"] / 1000.0) / 1e9
+    step_gbs = b / (rep["ms_per_step"] / 1000.0) / 1e9
+    nstep = 2 * cfg.dtbc // cfg.dtbt
+    flops = TROP_FLOPS_PER_POINT * cfg.glo_ni * cfg.glo_nj * nstep
+    trop_ms = rep["phase_ms"]["subcycle_kernel"]
+    trop_tflops = flops / (trop_ms / 1000.0) / 1e12
     out = {
         "metric": METRIC,
         "value": rep["sypd"],
@@ -108,16 +149,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32+f64",
+        "dtype": "f64" if args.form == "fp64" else "f32 stencils + f64 state",
         "data": "synthetic: seeded analytic quasi-global configuration (no input files)",
         "config": {"workload": f"{cfg.name} {cfg.glo_ni}x{cfg.glo_nj}x{cfg.nk}, dtbt={cfg.dtbt} s, "
-                               f"dtbc={cfg.dtbc} s, 120 barotropic substeps/step",
+                               f"dtbc={cfg.dtbc} s, {nstep} barotropic substeps/step",
                    "form": args.form, "parallelism": "single GPU, whole domain"},
         "forms": forms,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-                     "bytes_per_step": b,
-                     "definition": "algorithmic bytes per baroclinic step (N3*31*s + N2*50*8) / wall time per step"},
+        "roofline": {"bound": "mfma", "kernel": "k_trop_tiles (barotropic subcycle, FP64)",
+                     "achieved": round(trop_tflops, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5), "traffic": TROP_TRAFFIC_BYTES,
+                     "flops_per_launch": flops, "launch_ms": round(trop_ms, 4),
+                     "definition": f"{TROP_FLOPS_PER_POINT} FP64 flops per point per substep x "
+                                   f"{cfg.glo_ni}x{cfg.glo_nj} points x {nstep} substeps / event-timed "
+                                   "kernel duration; peak = MI355X FP64 (vector and matrix) 78.6 TFLOP/s"},
+        "step_roofline": {"bound": "hbm", "achieved": round(step_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / PEAK_HBM_GBS, 5), "bytes_per_step": b,
+                          "definition": "algorithmic bytes per baroclinic step (N3*31*s + N2*50*8) / wall time"},
     }
     if rank == 0 and args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)

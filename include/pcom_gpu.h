@@ -50,11 +50,17 @@ enum {
   PCOM_ERR_NOGPU = -5     /* no HIP device visible                        */
 };
 
-/* Arithmetic of the 3-D baroclinic fields. */
+/* Arithmetic of the 3-D baroclinic fields.  In every form the prognostic
+ * state (T, S, u, v) is stored and stepped in FP64, and the barotropic
+ * subcycle, its time filter and the global sums are FP64. */
 enum {
-  PCOM_FP64 = 0,   /* everything FP64, bit-identical to the FP64 oracle   */
-  PCOM_MIXED = 1   /* 3-D stencils and EOS in FP32; barotropic subcycle,
-                      time filter and global sums in FP64                  */
+  PCOM_FP64 = 0,        /* everything FP64, bit-identical to the oracle    */
+  PCOM_MIXED = 1,       /* 3-D tendency stencils (advection, friction,
+                           diffusion, upwelling, Smagorinsky) in FP32; the
+                           equation of state and the pressure integral and
+                           gradient in FP64                                */
+  PCOM_MIXED_EOS32 = 2  /* as PCOM_MIXED, with the equation of state and
+                           the pressure integral in FP32 as well           */
 };
 
 /* Raw inputs, as read from initial.nc, forcing.nc and vmix.nc
@@ -140,7 +146,7 @@ int pcom_get_dims(const pcom_ctx *ctx, int *ni, int *nj, int *nk);
 size_t pcom_field_size(const pcom_ctx *ctx, int id);
 
 /* Copy a field to / from host memory (FP64, halo'd, column-major).  In
- * PCOM_MIXED, 3-D fields are converted from / to their FP32 device copy. */
+ * the mixed forms, fields stored in FP32 on the device are converted.   */
 int pcom_get_field(pcom_ctx *ctx, int id, double *host, size_t n);
 int pcom_set_field(pcom_ctx *ctx, int id, const double *host, size_t n);
 

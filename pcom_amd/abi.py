@@ -14,6 +14,8 @@ PCOM_ERR_NOGPU = -5
 
 PCOM_FP64 = 0
 PCOM_MIXED = 1
+PCOM_MIXED_EOS32 = 2
+FORMS = {"fp64": PCOM_FP64, "mixed": PCOM_MIXED, "mixed_eos32": PCOM_MIXED_EOS32}
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)

@@ -142,7 +142,7 @@ int build_grid(Ctx& c, Grid<R>& g, bool as_float, const double* prh_t_d, const d
 struct FieldRef {
   void* ptr = nullptr;
   size_t n = 0;
-  bool is3 = false;  // stored in R (float in the mixed form)
+  bool is_float = false;  // stored in FP32 on the device
   const std::vector<double>* host = nullptr;  // host-only static fields
 };
 
@@ -150,20 +150,31 @@ FieldRef field_ref(Ctx& c, int id) {
   FieldRef r;
   Fields3& f = c.f3;
   Fields2& h = c.f2;
+  // 3-D fields: the FP64 state, E (EOS and pressure) or R (the rest)
+  const bool rf = c.precision != PCOM_FP64, ef = c.precision == PCOM_MIXED_EOS32;
+  auto state = [&](void* p, size_t n) {
+    r.ptr = p;
+    r.n = n;
+  };
+  auto eos = [&](void* p, size_t n) {
+    r.ptr = p;
+    r.n = n;
+    r.is_float = ef;
+  };
   auto three = [&](void* p, size_t n) {
     r.ptr = p;
     r.n = n;
-    r.is3 = true;
+    r.is_float = rf;
   };
   auto two = [&](double* p) {
     r.ptr = p;
     r.n = c.n2;
   };
   switch (id) {
-    case PCOM_F_TC: case PCOM_F_TP: three(f.T[c.cur], c.n3); break;
-    case PCOM_F_SC: case PCOM_F_SP: three(f.S[c.cur], c.n3); break;
-    case PCOM_F_UC: case PCOM_F_UP: three(f.U, c.n3); break;
-    case PCOM_F_VC: case PCOM_F_VP: three(f.V, c.n3); break;
+    case PCOM_F_TC: case PCOM_F_TP: state(f.T[c.cur], c.n3); break;
+    case PCOM_F_SC: case PCOM_F_SP: state(f.S[c.cur], c.n3); break;
+    case PCOM_F_UC: case PCOM_F_UP: state(f.U, c.n3); break;
+    case PCOM_F_VC: case PCOM_F_VP: state(f.V, c.n3); break;
     case PCOM_F_AUC: three(f.AU[c.ab], c.n3); break;
     case PCOM_F_AVC: three(f.AV[c.ab], c.n3); break;
     case PCOM_F_AUP: three(f.AU[(c.ab + 2) % 3], c.n3); break;
@@ -173,11 +184,11 @@ FieldRef field_ref(Ctx& c, int id) {
     case PCOM_F_FX: three(f.fx, c.n3); break;
     case PCOM_F_FY: three(f.fy, c.n3); break;
     case PCOM_F_AM: three(f.am, c.n3); break;
-    case PCOM_F_ALPHA: three(f.alpha, c.n3); break;
-    case PCOM_F_ADP: three(f.adp, c.n3); break;
+    case PCOM_F_ALPHA: eos(f.alpha, c.n3); break;
+    case PCOM_F_ADP: eos(f.adp, c.n3); break;
     case PCOM_F_WM: three(f.wm, c.n3p); break;
-    case PCOM_F_PRHO_T: three(f.prho1, c.n3); break;
-    case PCOM_F_PRHO_S: three(f.prho2, c.n3); break;
+    case PCOM_F_PRHO_T: eos(f.prho1, c.n3); break;
+    case PCOM_F_PRHO_S: eos(f.prho2, c.n3); break;
     case PCOM_F_ACT: three(f.act, c.n3); break;
     case PCOM_F_ACS: three(f.acs, c.n3); break;
     case PCOM_F_ACR: three(f.acr, c.n3); break;
@@ -236,8 +247,8 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if (opts->trop_tile_i < 0 || opts->trop_tile_j < 0 || opts->trop_substeps < 0 || opts->trop_tile_i > 256 ||
       opts->trop_tile_j > 256 || opts->trop_substeps > 16)
     return fail(PCOM_ERR_ARG, "barotropic tiling out of range (tile <= 256, substeps per exchange <= 16)");
-  if (opts->precision != PCOM_FP64 && opts->precision != PCOM_MIXED)
-    return fail(PCOM_ERR_ARG, "precision must be PCOM_FP64 or PCOM_MIXED");
+  if (opts->precision != PCOM_FP64 && opts->precision != PCOM_MIXED && opts->precision != PCOM_MIXED_EOS32)
+    return fail(PCOM_ERR_ARG, "precision must be PCOM_FP64, PCOM_MIXED or PCOM_MIXED_EOS32");
   if (opts->month < 1 || opts->month > 12 || opts->day < 1 || opts->day > 31 || opts->year < 1)
     return fail(PCOM_ERR_ARG, "bad start date");
   int ndev = 0;
@@ -267,7 +278,8 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(PCOM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
 
-  const bool fl = c->precision == PCOM_MIXED;
+  const bool fl = c->precision != PCOM_FP64;             // R = float
+  const bool el = c->precision == PCOM_MIXED_EOS32;      // E = float
   const size_t n2 = c->n2, n3 = c->n3, n3p = c->n3p;
   int rc;
   // 2-D static (always FP64) and integer grids
@@ -288,14 +300,14 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = build_grid<double>(*c, c->gd, false, prh_t, prh_u, phih_t, lev_t, lev_u))) return rc;
   if (fl && (rc = build_grid<float>(*c, c->gf, true, prh_t, prh_u, phih_t, lev_t, lev_u))) return rc;
 
-  // 3-D state in R
+  // 3-D prognostic state: FP64 in every form
   Fields3& f = c->f3;
   for (int q = 0; q < 2; ++q) {
-    if ((rc = make(*c, &f.T[q], n3, fl, hs.tc.data()))) return rc;
-    if ((rc = make(*c, &f.S[q], n3, fl, hs.sc.data()))) return rc;
+    if ((rc = make(*c, &f.T[q], n3, false, hs.tc.data()))) return rc;
+    if ((rc = make(*c, &f.S[q], n3, false, hs.sc.data()))) return rc;
   }
-  if ((rc = make(*c, &f.U, n3, fl, nullptr))) return rc;
-  if ((rc = make(*c, &f.V, n3, fl, nullptr))) return rc;
+  if ((rc = make(*c, &f.U, n3, false, nullptr))) return rc;
+  if ((rc = make(*c, &f.V, n3, false, nullptr))) return rc;
   for (int q = 0; q < 3; ++q) {
     if ((rc = make(*c, &f.AU[q], n3, fl, nullptr))) return rc;
     if ((rc = make(*c, &f.AV[q], n3, fl, nullptr))) return rc;
@@ -303,11 +315,11 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = make(*c, &f.fx, n3, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.fy, n3, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.am, n3, fl, nullptr, AM_C))) return rc;
-  if ((rc = make(*c, &f.alpha, n3, fl, nullptr, (double)(1.0f / 1000.0f)))) return rc;
-  if ((rc = make(*c, &f.adp, n3, fl, nullptr))) return rc;
+  if ((rc = make(*c, &f.alpha, n3, el, nullptr, (double)(1.0f / 1000.0f)))) return rc;
+  if ((rc = make(*c, &f.adp, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.wm, n3p, fl, nullptr))) return rc;
-  if ((rc = make(*c, &f.prho1, n3, fl, nullptr))) return rc;
-  if ((rc = make(*c, &f.prho2, n3, fl, nullptr))) return rc;
+  if ((rc = make(*c, &f.prho1, n3, el, nullptr))) return rc;
+  if ((rc = make(*c, &f.prho2, n3, el, nullptr))) return rc;
   void** acc[] = {&f.act, &f.acs, &f.acr, &f.acu, &f.acv, &f.acw};
   for (void** a : acc)
     if ((rc = make(*c, a, n3, fl, nullptr))) return rc;
@@ -449,7 +461,7 @@ int pcom_get_field(pcom_ctx* ctx, int id, double* host, size_t n) {
     return PCOM_OK;
   }
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
-  return download(*c, host, r.ptr, n, r.is3 && c->precision == PCOM_MIXED);
+  return download(*c, host, r.ptr, n, r.is_float);
 }
 
 int pcom_set_field(pcom_ctx* ctx, int id, const double* host, size_t n) {
@@ -460,7 +472,7 @@ int pcom_set_field(pcom_ctx* ctx, int id, const double* host, size_t n) {
   if (n != r.n) return fail(PCOM_ERR_ARG, "pcom_set_field: size " + std::to_string(n) + " != " + std::to_string(r.n));
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   HIPCHK(hipStreamSynchronize(c->stream), "sync before upload");
-  return upload(*c, r.ptr, host, n, r.is3 && c->precision == PCOM_MIXED);
+  return upload(*c, r.ptr, host, n, r.is_float);
 }
 
 int pcom_get_counters(pcom_ctx* ctx, pcom_counters* k) {

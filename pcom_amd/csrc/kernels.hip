@@ -3,8 +3,12 @@
 // Each kernel evaluates one or more reference routines per grid point with
 // the pointwise operators of pcom_dev.h, so intermediates the reference
 // materialises as whole arrays (and halo-swaps) stay in registers / cache.
-// 3-D kernels are templated on R (double; float in the mixed form); the
-// barotropic subcycle is FP64 always.
+// Precision forms: the prognostic state T, S, u, v is FP64 in every form;
+// 3-D tendency stencils compute in R and store their fields in R; the
+// equation of state and the pressure integral (alpha, adp, prho, the
+// face integrals of int_pgra, the pressure gradient of int_clin, int_ssh)
+// compute in E.  FP64: R = E = double.  PCOM_MIXED: R = float, E = double.
+// PCOM_MIXED_EOS32: R = E = float.  The barotropic subcycle is FP64 always.
 //
 // Thread mapping: point kernels use one thread per (i, j, k) with i fastest
 // (coalesced along the Fortran-contiguous dimension); column kernels use
@@ -52,7 +56,7 @@ struct Six {
 };
 
 __global__ void k_bnd(long n2, Six s, int m1, int m2, double f) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n2) return;
 #pragma unroll
   for (int v = 0; v < 6; ++v) {
@@ -64,7 +68,7 @@ __global__ void k_bnd(long n2, Six s, int m1, int m2, double f) {
 // atmospheric pressure gradient (mod_int.f90:100-103): gra_r2d_b(pa, g1)
 // then ter to g3.  bnd%pa changes only with int_bnd, so this runs daily.
 __global__ void k_pa(Grid<double> g, const double* __restrict__ pa, double* pax, double* pay) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   auto PA = [&](int m, int jj) -> double { return pa[g.ix(m, jj)]; };
@@ -85,7 +89,7 @@ __global__ void k_pa(Grid<double> g, const double* __restrict__ pa, double* pax,
 //                     schm = sqrt(ter(mch, dft=1)), 1 on land (mod_int.f90:318-320).
 __global__ void k_sch(Grid<double> g, const double* __restrict__ chc, const double* __restrict__ chp,
                       const double* __restrict__ mch, double* schr, double* sqp, double* schs, double* schm) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   auto CHC = [&](int m, int jj) -> double { return chc[g.ix(m, jj)]; };
@@ -103,14 +107,14 @@ __global__ void k_sch(Grid<double> g, const double* __restrict__ chc, const doub
 
 // den_prho (mod_den.f90:120-153)
 template <typename R>
-__global__ void k_prho(Grid<R> g, const R* __restrict__ tc, const R* __restrict__ sc, const double* __restrict__ chc,
-                       R* prho1, R* prho2) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_prho(Grid<R> g, const double* __restrict__ tc, const double* __restrict__ sc,
+                       const double* __restrict__ chc, R* prho1, R* prho2) {
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   if (g.mskt(i, j, k) > 0) {
-    R t = tc[p], s = sc[p];
+    R t = (R)tc[p], s = (R)sc[p];
     R pr = (R)chc[g.ix(i, j)] * g.pr2[k - 1];
     R ra = den_rho<R>(t + (R)DELTAT, s, pr);
     R rb = den_rho<R>(t - (R)DELTAT, s, pr);
@@ -124,13 +128,13 @@ __global__ void k_prho(Grid<R> g, const R* __restrict__ tc, const R* __restrict_
 // den_alpha (mod_den.f90:33-55): alpha = 1/rho where mask == 1; land is
 // left as it is (1/1000.0 from mod_arrays.f90:298).
 template <typename R>
-__global__ void k_alpha(Grid<R> g, const R* __restrict__ t, const R* __restrict__ s, const double* __restrict__ ch,
-                        R* alpha) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_alpha(Grid<R> g, const double* __restrict__ t, const double* __restrict__ s,
+                        const double* __restrict__ ch, R* alpha) {
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
-  if (g.mskt(i, j, k) == 1) alpha[p] = (R)1.0 / den_rho<R>(t[p], s[p], (R)ch[g.ix(i, j)] * g.pr2[k - 1]);
+  if (g.mskt(i, j, k) == 1) alpha[p] = (R)1.0 / den_rho<R>((R)t[p], (R)s[p], (R)ch[g.ix(i, j)] * g.pr2[k - 1]);
 }
 
 // int_tobot (mod_op.f90:927-944): adp(k) = t(k)/2 + sum(t(k+1:nk)),
@@ -140,7 +144,7 @@ template <typename R>
 __global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
   __shared__ R t[kNKMAX][kCW];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)blockIdx.x * kCW + tx;
+  const long p = (long)xcd_block() * kCW + tx;
   if (p < g.n2) {
     const int lev = g.lev_t[p];
     for (int k = 1 + kl; k <= nk; k += kKL)
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restric
 template <typename R>
 __global__ void k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict__ alpha, double* b0, double* b1,
                        double* b2, double* b3, double* c0, double* c1, double* c2, double* c3) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   int ic = g.canon(i);
@@ -208,23 +212,24 @@ __global__ void k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict
 
 // upwelling (mod_int.f90:626-647): wm from the continuity equation;
 // with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).
-template <typename R>
-__global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc,
-                                                   const double* __restrict__ sqp, R* wm, R* acw,
-                                                   const R* __restrict__ alpha, const double* __restrict__ mch2) {
+template <typename R, typename E>
+__global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const double* __restrict__ uc,
+                                                   const double* __restrict__ vc, const double* __restrict__ sqp, R* wm,
+                                                   R* acw, const E* __restrict__ alpha,
+                                                   const double* __restrict__ mch2) {
   __shared__ R w3[kNKMAX][kCW];
   __shared__ R wv[kNKMAX][kCW];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)blockIdx.x * kCW + tx;
+  const long p = (long)xcd_block() * kCW + tx;
   const bool ok = p < g.n2;
   const int i = ok ? (int)(p % g.ni) + 1 : 1, j = ok ? (int)(p / g.ni) + 1 : 1;
   auto SQ = [&](int m, int jj) -> R { return (R)sqp[g.ix(m, jj)]; };
   if (ok) {
     for (int k = 1 + kl; k <= nk; k += kKL) {
-      const R* u = uc + g.n2 * (k - 1);
-      const R* v = vc + g.n2 * (k - 1);
-      auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * u[g.ix(m, jj)]; };
-      auto VB = [&](int m, int jj) -> R { return SQ(m, jj) * v[g.ix(m, jj)]; };
+      const double* u = uc + g.n2 * (k - 1);
+      const double* v = vc + g.n2 * (k - 1);
+      auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * (R)u[g.ix(m, jj)]; };
+      auto VB = [&](int m, int jj) -> R { return SQ(m, jj) * (R)v[g.ix(m, jj)]; };
       w3[k - 1][tx] = divg<3, 3, 1>(g, VA, VB, i, j);
     }
   }
@@ -248,7 +253,7 @@ __global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const R* __restric
     long q = p + g.n2 * (k - 1);
     R w = wv[k - 1][tx];
     if (k >= 2) wm[q] = w;
-    if (acw) acw[q] = acw[q] + w * alpha[q] / ((R)GRAV * (R)mch2[p]);
+    if (acw) acw[q] = acw[q] + w * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
   }
 }
 
@@ -260,19 +265,20 @@ __device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* schr, int
 
 // op_adv (mod_op.f90:224-278) for uc and vc at one point.
 template <typename R>
-__global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ wm,
-                      const double* __restrict__ chc, const double* __restrict__ schr, R* auc, R* avc) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_adv(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
+                      const R* __restrict__ wm, const double* __restrict__ chc, const double* __restrict__ schr,
+                      R* auc, R* avc) {
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   const long ko = g.n2 * (k - 1);
-  const R* u = uc + ko;
-  const R* v = vc + ko;
+  const double* u = uc + ko;
+  const double* v = vc + ko;
   auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
-  auto X = [&](int m, int jj) -> R { return u[g.ix(m, jj)] / SCH(m, jj); };
+  auto X = [&](int m, int jj) -> R { return (R)u[g.ix(m, jj)] / SCH(m, jj); };
   auto Y = [&](int m, int jj) -> R {
-    R y = v[g.ix(m, jj)] / SCH(m, jj);
+    R y = (R)v[g.ix(m, jj)] / SCH(m, jj);
     return y * g.RH(3, m, jj);
   };
   auto UU = [&](int m, int jj) -> R { return ter_ew<3>(g, X, m, jj); };          // g3 -> g4
@@ -293,21 +299,21 @@ __global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
   R out[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const R* var = c == 0 ? uc : vc;
-    const R* vk = var + ko;
-    auto VK = [&](int m, int jj) -> R { return vk[g.ix(m, jj)]; };
+    const double* var = c == 0 ? uc : vc;
+    const double* vk = var + ko;
+    auto VK = [&](int m, int jj) -> R { return (R)vk[g.ix(m, jj)]; };
     auto M1 = [&](int m, int jj) -> R { return ter_ew<3>(g, VK, m, jj) * UU(m, jj); };
     auto M2 = [&](int m, int jj) -> R { return ter_ns<3>(g, VK, m, jj, (R)0) * VV(m, jj); };
     R ans = (p1<4, 3>(g, M1, i, j) + p2<2, 3>(g, M2, i, j)) / ((R)ARAD * rh3);
     // vertical flux through the U-cell faces, zero unless both layers wet
     auto WB = [&](int kk) -> R {
       if (kk < 2 || kk > g.nk) return (R)0;
-      R a = (var[g.ix(i, j, kk - 1)] + var[g.ix(i, j, kk)]) * (R)0.5;
+      R a = ((R)var[g.ix(i, j, kk - 1)] + (R)var[g.ix(i, j, kk)]) * (R)0.5;
       a = a * (kk == k ? w_k : w_k1);
       return a * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk);
     };
     ans = ans + (WB(k) - WB(k + 1)) / dpr;
-    ans = ans - (R)0.5 * vk[g.ix(i, j)] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
+    ans = ans - (R)0.5 * (R)vk[g.ix(i, j)] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
     out[c] = ans * mu;
   }
   auc[p] = out[0];
@@ -318,7 +324,7 @@ __global__ void k_adv(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
 // metric terms, vertical viscosity with wind stress and bottom drag.
 // v / vcc are callables (unweighted velocity and the "vc" argument).
 template <typename R, typename FV, typename FC>
-__device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v, const R* vp, FC vcc, R tau,
+__device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v, const double* vp, FC vcc, R tau,
                                        R am, R sch, int i, int j, int k) {
   auto GX = [&](int m, int jj) -> R { return grax<3, 4>(g, v, m, jj); };
   auto GY = [&](int m, int jj) -> R { return gray<3, 2>(g, v, m, jj); };
@@ -328,7 +334,7 @@ __device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v
   R wka = divg<4, 2, 3>(g, VA, VB, i, j);
   long q = g.ix(i, j);
   R ans = am / sch * wka;
-  ans = ans + am * g.cv1[q] * vp[g.ix(i, j, k)];
+  ans = ans + am * g.cv1[q] * (R)vp[g.ix(i, j, k)];
   int ic = g.canon(i);
   R dx = (R)ARAD * g.RH(4, ic, j) * (g.DX1(4, ic, j) + g.DX1(4, ic + 1, j));
   R pxc = (vcc(ic + 1, j, k) - vcc(ic - 1, j, k)) / dx;
@@ -342,8 +348,8 @@ __device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v
       return vv * (R)GRAV * (R)CDBOT * mag / sch;
     }
     if (kk > g.nk) return (R)0;
-    R pz = (vp[g.ix(i, j, kk - 1)] - vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk) /
-           g.dz1[kk - 1];
+    R pz = ((R)vp[g.ix(i, j, kk - 1)] - (R)vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) *
+           (R)g.msku(i, j, kk) / g.dz1[kk - 1];
     return (R)GRAV * (R)RHO0 * g.km[g.ix(i, j, kk)] * pz / (sch * sch);
   };
   R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
@@ -353,17 +359,18 @@ __device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v
 // op_fri (mod_op.f90:169-184): fx from (u, uc, -v, taux), fy from
 // (v, vc, u, tauy) with u = uc/sch, v = vc/sch.
 template <typename R>
-__global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc, const R* __restrict__ am,
+__global__ void k_fri(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
+                      const R* __restrict__ am,
                       const double* __restrict__ chc, const double* __restrict__ schr,
                       const double* __restrict__ taux, const double* __restrict__ tauy, R* fx, R* fy) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
   // 3-D callables (m, jj, kk)
-  auto U3 = [&](int m, int jj, int kk) -> R { return uc[g.ix(m, jj, kk)] / SCH(m, jj); };
-  auto V3 = [&](int m, int jj, int kk) -> R { return vc[g.ix(m, jj, kk)] / SCH(m, jj); };
+  auto U3 = [&](int m, int jj, int kk) -> R { return (R)uc[g.ix(m, jj, kk)] / SCH(m, jj); };
+  auto V3 = [&](int m, int jj, int kk) -> R { return (R)vc[g.ix(m, jj, kk)] / SCH(m, jj); };
   auto MV3 = [&](int m, int jj, int kk) -> R { return -V3(m, jj, kk); };
   const long q = g.ix(i, j);
   const R sch = SCH(i, j);
@@ -374,16 +381,16 @@ __global__ void k_fri(Grid<R> g, const R* __restrict__ uc, const R* __restrict__
     const Grid<R>& g;
     int k;
     bool isu;
-    const R* uc;
-    const R* vc;
+    const double* uc;
+    const double* vc;
     const double* schr;
     __device__ R operator()(int m, int jj) const {
       R s = sch_ready<R>(g, schr, m, jj);
-      return (isu ? uc : vc)[g.ix(m, jj, k)] / s;
+      return (R)(isu ? uc : vc)[g.ix(m, jj, k)] / s;
     }
     __device__ R operator()(int m, int jj, int kk) const {
       R s = sch_ready<R>(g, schr, m, jj);
-      return (isu ? uc : vc)[g.ix(m, jj, kk)] / s;
+      return (R)(isu ? uc : vc)[g.ix(m, jj, kk)] / s;
     }
   };
   View vu{g, k, true, uc, vc, schr};
@@ -399,7 +406,7 @@ template <typename R>
 __global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__ avc, const R* __restrict__ fx,
                      const R* __restrict__ fy, const double* __restrict__ schr, const double* __restrict__ pax,
                      const double* __restrict__ pay, double* ut, double* vt) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   const R sch = sch_ready<R>(g, schr, i, j);
@@ -419,88 +426,92 @@ __global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__
 
 // ---------------------------------------------------------------------------
 // int_clin (mod_int.f90:314-368): baroclinic momentum at one point,
-// updated in place (uc == up; every read of uc/vc here is pointwise).
-template <typename R>
+// updated in place (uc == up; every read of uc/vc here is pointwise).  The
+// pressure gradient is evaluated in E on the E grid; the AB3 combination,
+// friction, Coriolis and the update of the FP64 state in double.
+template <typename E>
 struct ClinArgs {
-  Grid<R> g;
-  R *uc, *vc;
-  const R *auc, *aup, *aupp, *avc, *avp, *avpp, *fx, *fy, *adp, *alpha;
-  const double *mch, *schm, *pax, *pay;
+  Grid<E> g;
+  double *uc, *vc;
+  const void *auc, *aup, *aupp, *avc, *avp, *avpp, *fx, *fy;  // in R
+  const E *adp, *alpha;
+  const double *mch, *schm, *pax, *pay, *cor;
   double bc;
 };
 
-template <typename R>
-__global__ void k_clin(ClinArgs<R> a) {
-  const Grid<R>& g = a.g;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <typename R, typename E>
+__global__ void k_clin(ClinArgs<E> a) {
+  const Grid<E>& g = a.g;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
-  const R c_tc = (R)(23.0f / 12.0f), c_tp = (R)(-(16.0f / 12.0f)), c_tpp = (R)(5.0f / 12.0f);
+  const double c_tc = (double)(23.0f / 12.0f), c_tp = (double)(-(16.0f / 12.0f)), c_tpp = (double)(5.0f / 12.0f);
   const long q = g.ix(i, j);
   const long ko = g.n2 * (k - 1);
-  auto MCH = [&](int m, int jj) -> R { return (R)a.mch[g.ix(m, jj)]; };
-  const R sch = (R)a.schm[q];
-  const R* ad = a.adp + ko;
-  const R* al = a.alpha + ko;
-  const R pr = g.pr2[k - 1];
-  auto WK = [&](int m, int jj) -> R {
+  auto MCH = [&](int m, int jj) -> E { return (E)a.mch[g.ix(m, jj)]; };
+  const double sch = a.schm[q];
+  const E* ad = a.adp + ko;
+  const E* al = a.alpha + ko;
+  const E pr = g.pr2[k - 1];
+  auto WK = [&](int m, int jj) -> E {
     long r = g.ix(m, jj);
-    return (R)g.phih_t[r] + (R)a.mch[r] * ad[r];
+    return (E)g.phih_t[r] + (E)a.mch[r] * ad[r];
   };
-  auto AL = [&](int m, int jj) -> R { return al[g.ix(m, jj)]; };
-  auto PG1 = [&](int m, int jj) -> R {
+  auto AL = [&](int m, int jj) -> E { return al[g.ix(m, jj)]; };
+  auto PG1 = [&](int m, int jj) -> E {
     return grax<1, 2>(g, WK, m, jj) + ter_ew<1>(g, AL, m, jj) * pr * grax<1, 2>(g, MCH, m, jj);
   };
-  auto PG2 = [&](int m, int jj) -> R {
-    return gray<1, 4>(g, WK, m, jj) + ter_ns<1>(g, AL, m, jj, (R)0) * pr * gray<1, 4>(g, MCH, m, jj);
+  auto PG2 = [&](int m, int jj) -> E {
+    return gray<1, 4>(g, WK, m, jj) + ter_ns<1>(g, AL, m, jj, (E)0) * pr * gray<1, 4>(g, MCH, m, jj);
   };
-  R wk = ter_ns<2>(g, PG1, i, j, (R)0);
-  R pu1 = -((wk + (R)A0 * (R)a.pax[q]) * sch);
-  wk = ter_ew<4>(g, PG2, i, j);
-  R pu2 = -((wk + (R)A0 * (R)a.pay[q]) * sch);
-  pu1 = pu1 - (c_tc * a.auc[p] + c_tp * a.aup[p] + c_tpp * a.aupp[p]);
-  pu2 = pu2 - (c_tc * a.avc[p] + c_tp * a.avp[p] + c_tpp * a.avpp[p]);
-  pu1 = pu1 + a.fx[p];
-  pu2 = pu2 + a.fy[p];
-  const R cor = g.cor[q];
-  const R u0 = a.uc[p], v0 = a.vc[p];
+  const R* auc = static_cast<const R*>(a.auc);
+  const R* aup = static_cast<const R*>(a.aup);
+  const R* aupp = static_cast<const R*>(a.aupp);
+  const R* avc = static_cast<const R*>(a.avc);
+  const R* avp = static_cast<const R*>(a.avp);
+  const R* avpp = static_cast<const R*>(a.avpp);
+  double wk = (double)ter_ns<2>(g, PG1, i, j, (E)0);
+  double pu1 = -((wk + A0 * a.pax[q]) * sch);
+  wk = (double)ter_ew<4>(g, PG2, i, j);
+  double pu2 = -((wk + A0 * a.pay[q]) * sch);
+  pu1 = pu1 - (c_tc * (double)auc[p] + c_tp * (double)aup[p] + c_tpp * (double)aupp[p]);
+  pu2 = pu2 - (c_tc * (double)avc[p] + c_tp * (double)avp[p] + c_tpp * (double)avpp[p]);
+  pu1 = pu1 + (double)static_cast<const R*>(a.fx)[p];
+  pu2 = pu2 + (double)static_cast<const R*>(a.fy)[p];
+  const double cor = a.cor[q];
+  const double u0 = a.uc[p], v0 = a.vc[p];
   pu1 = pu1 + cor * v0;
   pu2 = pu2 - cor * u0;
-  const R x1 = pu1, x2 = pu2;
-  const R c = (R)0.5 * cor * (R)a.bc;
+  const double x1 = pu1, x2 = pu2;
+  const double c = 0.5 * cor * a.bc;
   pu1 = x1 + c * x2;
   pu2 = x2 - c * x1;
-  const R d = (R)1 + c * c;
-  const R mu = (R)g.msku(i, j, k);
+  const double d = 1.0 + c * c;
+  const double mu = (double)g.msku(i, j, k);
   pu1 = pu1 * mu / d;
   pu2 = pu2 * mu / d;
-  a.uc[p] = u0 + pu1 * (R)a.bc;
-  a.vc[p] = v0 + pu2 * (R)a.bc;
+  a.uc[p] = u0 + pu1 * a.bc;
+  a.vc[p] = v0 + pu2 * a.bc;
 }
 
-// barotropic / baroclinic mode coupling (mod_int.f90:372-380)
-template <typename R>
-__global__ void k_clin_cpl(Grid<R> g, R* uc, R* vc, const double* __restrict__ buc, const double* __restrict__ bvc) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// barotropic / baroclinic mode coupling (mod_int.f90:372-380), on the
+// FP64 state
+__global__ void k_clin_cpl(Grid<double> g, double* uc, double* vc, const double* __restrict__ buc,
+                           const double* __restrict__ bvc) {
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  R s1 = 0, s2 = 0;
+  double s1 = 0, s2 = 0;
   for (int k = 1; k <= g.nk; ++k) {
     long q = p + g.n2 * (k - 1);
-    R mu = (R)g.msku(i, j, k);
+    double mu = (double)g.msku(i, j, k);
     s1 += uc[q] * g.dpr2[k - 1] * mu;
     s2 += vc[q] * g.dpr2[k - 1] * mu;
   }
-  R m1, m2;
-  if constexpr (sizeof(R) == sizeof(double)) {
-    m1 = s1 / g.prh_u_r[p];
-    m2 = s2 / g.prh_u_r[p];
-  } else {
-    m1 = (R)((double)s1 / g.prh_u[p]);
-    m2 = (R)((double)s2 / g.prh_u[p]);
-  }
-  const R bu = (R)buc[p], bv = (R)bvc[p];
+  const double m1 = s1 / g.prh_u_r[p];
+  const double m2 = s2 / g.prh_u_r[p];
+  const double bu = buc[p], bv = bvc[p];
   for (int k = 1; k <= g.nk; ++k) {
     if (g.msku(i, j, k) > 0) {
       long q = p + g.n2 * (k - 1);
@@ -512,17 +523,17 @@ __global__ void k_clin_cpl(Grid<R> g, R* uc, R* vc, const double* __restrict__ b
 
 // unweighted velocity, its accumulator and smag (mod_int.f90:384-440)
 template <typename R>
-__global__ void k_smag(Grid<R> g, const R* __restrict__ uc, const R* __restrict__ vc,
+__global__ void k_smag(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
                        const double* __restrict__ mch2, const double* __restrict__ schs, R* acu, R* acv, R* am,
                        int dtbc) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   const long ko = g.n2 * (k - 1);
   auto S2 = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
-  auto U1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
-  auto U2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
+  auto U1 = [&](int m, int jj) -> R { return (R)uc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
+  auto U2 = [&](int m, int jj) -> R { return (R)vc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
   acu[p] = acu[p] + U1(i, j);
   acv[p] = acv[p] + U2(i, j);
   if (j >= 2 && j <= g.nj - 1) {
@@ -544,12 +555,13 @@ __global__ void k_smag(Grid<R> g, const R* __restrict__ uc, const R* __restrict_
 
 // int_ssh (mod_int.f90:602-624): alpha from (tp, sp, mch), column SSH
 template <typename R>
-__global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const R* __restrict__ tp, const R* __restrict__ sp,
+__global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const double* __restrict__ tp,
+                                                   const double* __restrict__ sp,
                                                    const double* __restrict__ mch, R* alpha, double* ssh,
                                                    double* acssh) {
   __shared__ R term[kNKMAX][kCW];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)blockIdx.x * kCW + tx;
+  const long p = (long)xcd_block() * kCW + tx;
   if (p < g.n2) {
     const R m = (R)mch[p];
     const int lev = g.lev_t[p];
@@ -557,7 +569,7 @@ __global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const R* __restric
       long q = p + g.n2 * (k - 1);
       R al = alpha[q];
       if (lev >= k) {
-        al = (R)1.0 / den_rho<R>(tp[q], sp[q], m * g.pr2[k - 1]);
+        al = (R)1.0 / den_rho<R>((R)tp[q], (R)sp[q], m * g.pr2[k - 1]);
         alpha[q] = al;
       }
       term[k - 1][tx] = al * g.dpr2[k - 1] * m * (R)(lev >= k ? 1 : 0);
@@ -584,19 +596,21 @@ __global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const R* __restric
 template <typename R>
 struct TsArgs {
   Grid<R> g;
-  const R *t0, *s0, *wm, *uc, *vc;
+  const double *t0, *s0, *uc, *vc;
+  const R* wm;
   const double *schs, *mch, *bndt, *bnds;
-  R *t1, *s1;
+  double *t1, *s1;
   double bc;
 };
 
 template <typename R, typename FV>
-__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* varp, const R* wm, const double* schs,
-                                          const R* uc, const R* vc, int i, int j, int k) {
+__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const double* varp, const R* wm,
+                                          const double* schs, const double* uc, const double* vc, int i, int j,
+                                          int k) {
   const long ko = g.n2 * (k - 1);
   auto SCH = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
-  auto UM1 = [&](int m, int jj) -> R { return uc[ko + g.ix(m, jj)] * SCH(m, jj); };
-  auto UM2 = [&](int m, int jj) -> R { return vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
+  auto UM1 = [&](int m, int jj) -> R { return (R)uc[ko + g.ix(m, jj)] * SCH(m, jj); };
+  auto UM2 = [&](int m, int jj) -> R { return (R)vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
   auto X = [&](int m, int jj) -> R { return ter_ns<3>(g, UM1, m, jj, (R)0) * p1<1, 2>(g, var, m, jj); };
   auto Y = [&](int m, int jj) -> R { return ter_ew<3>(g, UM2, m, jj) * p2<1, 4>(g, var, m, jj); };
   const R rh1 = g.RH(1, i, j);
@@ -604,7 +618,7 @@ __device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* var
   ans = ans + ter_ns<4>(g, Y, i, j, (R)0) / ((R)ARAD * rh1);
   auto DX3 = [&](int kk) -> R {
     if (kk < 2 || kk > g.nk) return (R)0;
-    return varp[g.ix(i, j, kk - 1)] - varp[g.ix(i, j, kk)];
+    return (R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)];
   };
   R wka = (wm[g.ix(i, j, k)] * DX3(k) + wm[g.ix(i, j, k + 1)] * DX3(k + 1)) * (R)0.5;
   wka = wka / g.dpr2[k - 1];
@@ -612,8 +626,8 @@ __device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const R* var
 }
 
 template <typename R, typename FV>
-__device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const R* varp, const double* ch, int i, int j,
-                                       int k) {
+__device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const double* varp, const double* ch, int i,
+                                       int j, int k) {
   auto MX = [&](int m, int jj) -> R {
     int mc = g.canon(m);
     R d = (var(mc + 1, jj) - var(mc, jj)) * (R)g.mskt(mc + 1, jj, k) * (R)g.mskt(mc, jj, k);
@@ -633,8 +647,8 @@ __device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const R* varp, 
   auto WKP = [&](int kk) -> R {
     R v = (R)0;
     if (kk >= 2 && kk <= g.nk)
-      v = (varp[g.ix(i, j, kk - 1)] - varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) * (R)g.mskt(i, j, kk) /
-          g.dz1[kk - 1];
+      v = ((R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) *
+          (R)g.mskt(i, j, kk) / g.dz1[kk - 1];
     return (R)GRAV * (R)RHO0 * g.kh[g.ix(i, j, kk)] * v;
   };
   return ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
@@ -643,14 +657,14 @@ __device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const R* varp, 
 template <typename R>
 __global__ void k_ts(TsArgs<R> a) {
   const Grid<R>& g = a.g;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
   int i, j, k;
   ijk(g, p, i, j, k);
   const long ko = g.n2 * (k - 1);
   const long q = g.ix(i, j);
-  auto TV = [&](int m, int jj) -> R { return a.t0[ko + g.ix(m, jj)]; };
-  auto SV = [&](int m, int jj) -> R { return a.s0[ko + g.ix(m, jj)]; };
+  auto TV = [&](int m, int jj) -> R { return (R)a.t0[ko + g.ix(m, jj)]; };
+  auto SV = [&](int m, int jj) -> R { return (R)a.s0[ko + g.ix(m, jj)]; };
   R pt1 = -adv_ts_point<R>(g, TV, a.t0, a.wm, a.schs, a.uc, a.vc, i, j, k);
   R pt2 = -adv_ts_point<R>(g, SV, a.s0, a.wm, a.schs, a.uc, a.vc, i, j, k);
   pt1 = pt1 + dif_point<R>(g, TV, a.t0, a.mch, i, j, k);
@@ -658,12 +672,13 @@ __global__ void k_ts(TsArgs<R> a) {
   const R mk = (R)g.mskt(i, j, k);
   const R m = (R)a.mch[q];
   if (k == 1) {
-    pt1 = pt1 + (R)GAMMA_T * ((R)a.bndt[q] - a.t0[p]) * mk / g.dpr2[0];
-    pt2 = pt2 + (R)GAMMA_S * ((R)a.bnds[q] - a.s0[p]) * mk * m;
+    pt1 = pt1 + (R)GAMMA_T * ((R)a.bndt[q] - (R)a.t0[p]) * mk / g.dpr2[0];
+    pt2 = pt2 + (R)GAMMA_S * ((R)a.bnds[q] - (R)a.s0[p]) * mk * m;
   }
-  R tn = a.t0[p] + pt1 * (R)a.bc * mk / m;
-  R sn = a.s0[p] + pt2 * (R)a.bc * mk / m;
-  if (g.mskt(i, j, k) > 0 && tn < (R)TICE) tn = (R)TICE;
+  // the forward step of the FP64 state (mod_int.f90:484-487)
+  double tn = a.t0[p] + (double)pt1 * a.bc * (double)mk / (double)m;
+  double sn = a.s0[p] + (double)pt2 * a.bc * (double)mk / (double)m;
+  if (g.mskt(i, j, k) > 0 && tn < TICE) tn = TICE;
   a.t1[p] = tn;
   a.s1[p] = sn;
 }
@@ -672,24 +687,29 @@ __global__ void k_ts(TsArgs<R> a) {
 // rhoa(k) = prho(k).(t(k), s(k)) and rhob(k) = prho(k+1).(t(k), s(k)) are
 // kept equal to those expressions of the current column by mix(), and
 // every entry the search reads is one mix() has kept current, so they are
-// evaluated on demand instead of stored.
-template <typename R>
+// evaluated on demand instead of stored.  The column lives in LDS (stride
+// `st` between levels) while the data-dependent search runs.
+template <typename E>
 struct Column {
-  R* t;
-  R* s;
-  const R* p1;
-  const R* p2;
-  const R* dpr;
-  long st;
-  __device__ R rhoa(int k) const { return p1[st * (k - 1)] * t[st * (k - 1)] + p2[st * (k - 1)] * s[st * (k - 1)]; }
-  __device__ R rhob(int k) const { return p1[st * k] * t[st * (k - 1)] + p2[st * k] * s[st * (k - 1)]; }
+  double* t;
+  double* s;
+  const E* p1;
+  const E* p2;
+  const double* dpr;
+  int st;
+  __device__ double rhoa(int k) const {
+    return (double)p1[st * (k - 1)] * t[st * (k - 1)] + (double)p2[st * (k - 1)] * s[st * (k - 1)];
+  }
+  __device__ double rhob(int k) const {
+    return (double)p1[st * k] * t[st * (k - 1)] + (double)p2[st * k] * s[st * (k - 1)];
+  }
   __device__ void mix(int na, int nb) {
-    R sumt = 0, sumd = 0, sums = 0;
+    double sumt = 0, sumd = 0, sums = 0;
     for (int k = na; k <= nb; ++k) sumt += t[st * (k - 1)] * dpr[k - 1];
     for (int k = na; k <= nb; ++k) sumd += dpr[k - 1];
-    R tv = sumt / sumd;
+    double tv = sumt / sumd;
     for (int k = na; k <= nb; ++k) sums += s[st * (k - 1)] * dpr[k - 1];
-    R sv = sums / sumd;
+    double sv = sums / sumd;
     for (int k = na; k <= nb; ++k) {
       t[st * (k - 1)] = tv;
       s[st * (k - 1)] = sv;
@@ -697,16 +717,37 @@ struct Column {
   }
 };
 
-template <typename R>
-__global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<R> g, R* tc, R* sc, const R* __restrict__ prho1,
-                                                    const R* __restrict__ prho2, const double* __restrict__ chc,
+// kCW columns per workgroup; the kKL k-lanes stage t, s (FP64 state) and
+// prho (E) of each column into LDS (dynamic: 2*nk*kCW doubles + 2*nk*kCW
+// E), lane 0 runs the convective adjustment, then all lanes write back and
+// accumulate act, acs and acr (mod_int.f90:493-505) in R.
+template <typename R, typename E>
+__global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<E> g, const double* __restrict__ dpr, double* tc,
+                                                    double* sc, const E* __restrict__ prho1,
+                                                    const E* __restrict__ prho2, const double* __restrict__ chc,
                                                     R* act, R* acs, R* acr) {
-  const int tx = threadIdx.x, kl = threadIdx.y;
-  const long p = (long)blockIdx.x * kCW + tx;
+  extern __shared__ double conv_lds[];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  double* lt = conv_lds;
+  double* ls = lt + nk * kCW;
+  E* l1 = reinterpret_cast<E*>(ls + nk * kCW);
+  E* l2 = l1 + nk * kCW;
+  const long p = (long)xcd_block() * kCW + tx;
   const bool ok = p < g.n2;
   const int nlev = ok ? g.lev_t[p] : 0;
+  if (ok) {
+    for (int k = 1 + kl; k <= nk; k += kKL) {
+      const long q = p + g.n2 * (k - 1);
+      const int o = (k - 1) * kCW + tx;
+      lt[o] = tc[q];
+      ls[o] = sc[q];
+      l1[o] = prho1[q];
+      l2[o] = prho2[q];
+    }
+  }
+  __syncthreads();
   if (ok && kl == 0 && nlev > 1) {
-    Column<R> c{tc + p, sc + p, prho1 + p, prho2 + p, g.dpr2, g.n2};
+    Column<E> c{lt + tx, ls + tx, l1 + tx, l2 + tx, dpr, kCW};
     int kcnt = 1;
     while (kcnt < nlev) {
       if (c.rhob(kcnt) > c.rhoa(kcnt + 1)) {
@@ -735,15 +776,18 @@ __global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<R> g, R* tc, R* sc, con
   }
   __syncthreads();  // the column's new T/S are visible to the other k-lanes
   if (!ok) return;
-  const R ch = (R)chc[p];
-  for (int k = 1 + kl; k <= g.nk; k += kKL) {
-    long q = p + g.n2 * (k - 1);
-    R t = tc[q], s = sc[q];
-    act[q] = act[q] + t;
-    acs[q] = acs[q] + s;
+  const E ch = (E)chc[p];
+  for (int k = 1 + kl; k <= nk; k += kKL) {
+    const long q = p + g.n2 * (k - 1);
+    const int o = (k - 1) * kCW + tx;
+    const double t = lt[o], s = ls[o];
+    tc[q] = t;
+    sc[q] = s;
+    act[q] = act[q] + (R)t;
+    acs[q] = acs[q] + (R)s;
     if (nlev >= k) {
-      R wk = (R)1.0 / den_rho<R>(t, s, ch * g.pr2[k - 1]);
-      acr[q] = acr[q] + (R)1.0 / wk;
+      E wk = (E)1.0 / den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]);
+      acr[q] = acr[q] + (R)((E)1.0 / wk);
     }
   }
 }
@@ -755,7 +799,7 @@ template <typename R>
 __global__ void k_sums_partial(Grid<R> g, const R* __restrict__ t, const R* __restrict__ s,
                                const double* __restrict__ chc, const double* __restrict__ w2, double* part) {
   __shared__ double sh[3][kBlock];
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   double a = 0, b = 0, c = 0;
   if (p < g.n2) {
     int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
@@ -795,17 +839,17 @@ __global__ void k_sums_final(const double* part, int nb, double* out) {
 
 template <typename R>
 __global__ void k_finite(const R* a, long n, int* flag) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p < n && !isfinite((double)a[p])) atomicOr(flag, 1);
 }
 
 __global__ void k_finite_d(const double* a, long n, int* flag) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p < n && !isfinite(a[p])) atomicOr(flag, 1);
 }
 
 __global__ void k_cell_area(Grid<double> g, double* w) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   w[p] = g.rh[1][p] * g.dx1[1][p] * g.dx2[1][p] * ARAD * ARAD;
 }
@@ -816,7 +860,7 @@ __global__ void k_cell_area(Grid<double> g, double* w) {
 template <typename R>
 __global__ void k_mean3(Grid<R> g, const R* __restrict__ acc, int n, int mask, double* out) {
   const long gni = g.ni - 2, gnj = g.nj - 2;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= gni * gnj * g.nk) return;
   const int i = (int)(p % gni) + 2;
   const long r = p / gni;
@@ -828,7 +872,7 @@ __global__ void k_mean3(Grid<R> g, const R* __restrict__ acc, int n, int mask, d
 
 __global__ void k_mean2(Grid<double> g, const double* __restrict__ acc, int n, double* out) {
   const long gni = g.ni - 2, gnj = g.nj - 2;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= gni * gnj) return;
   const int i = (int)(p % gni) + 2, j = (int)(p / gni) + 2;
   double v = acc[g.ix(i, j)] / (double)n;
@@ -854,7 +898,7 @@ __global__ void k_ssh_mean(Grid<double> g, const double* __restrict__ var, doubl
 }
 
 __global__ void k_ssh_anom(long n, const double* __restrict__ sums, double* var) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double mean = sums[0] / sums[1];
   if (var[p] != (double)PCOM_MISSING_FLOAT) var[p] = var[p] - mean;
@@ -862,7 +906,7 @@ __global__ void k_ssh_anom(long n, const double* __restrict__ sums, double* var)
 
 template <typename R>
 __global__ void k_fill(R* a, long n) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p < n) a[p] = (R)0;
 }
 
@@ -896,16 +940,17 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
-template <typename R>
+template <typename R, typename E>
 std::string step_t(Ctx& c) {
   const Grid<R>& g = grid_of<R>(c);
+  const Grid<E>& ge = grid_of<E>(c);
   Fields3& f = c.f3;
   Fields2& h = c.f2;
   const size_t n2 = c.n2, n3 = c.n3;
-  R* T = P<R>(f.T[c.cur]);
-  R* S = P<R>(f.S[c.cur]);
-  R* U = P<R>(f.U);
-  R* V = P<R>(f.V);
+  double* T = P<double>(f.T[c.cur]);
+  double* S = P<double>(f.S[c.cur]);
+  double* U = P<double>(f.U);
+  double* V = P<double>(f.V);
   hipError_t e;
 
   c.step_i += 1;
@@ -922,16 +967,16 @@ std::string step_t(Ctx& c) {
     }
     LAUNCH(k_bnd, n2, (long)n2, s6, m1, m2, fw);
     LAUNCH(k_pa, n2, c.gd, h.bnd[4], h.pax, h.pay);
-    LAUNCH(k_prho<R>, n3, g, T, S, h.chc, P<R>(f.prho1), P<R>(f.prho2));
+    LAUNCH(k_prho<E>, n3, ge, T, S, h.chc, P<E>(f.prho1), P<E>(f.prho2));
   }
   // den_alpha, int_pgra
-  LAUNCH(k_alpha<R>, n3, g, T, S, h.chc, P<R>(f.alpha));
-  LAUNCH_COL(k_adp<R>, g, P<R>(f.alpha), P<R>(f.adp));
-  LAUNCH(k_bphi<R>, n2, g, P<R>(f.adp), P<R>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
+  LAUNCH(k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
+  LAUNCH_COL(k_adp<E>, ge, P<E>(f.alpha), P<E>(f.adp));
+  LAUNCH(k_bphi<E>, n2, ge, P<E>(f.adp), P<E>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
          h.bgra[1], h.bgra[2], h.bgra[3]);
   // int_readyc
   LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
-  LAUNCH_COL(k_upw<R>, g, U, V, h.sqp, P<R>(f.wm), (R*)nullptr, (const R*)nullptr, (const double*)nullptr);
+  LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), (R*)nullptr, (const E*)nullptr, (const double*)nullptr);
   c.ab = (c.ab + 1) % 3;
   R* au = P<R>(f.AU[c.ab]);
   R* av = P<R>(f.AV[c.ab]);
@@ -964,32 +1009,45 @@ std::string step_t(Ctx& c) {
   LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, (double*)nullptr, h.sqp, h.schs, h.schm);
   // int_clin
   {
-    ClinArgs<R> ca;
-    ca.g = g; ca.uc = U; ca.vc = V;
+    ClinArgs<E> ca;
+    ca.g = ge; ca.uc = U; ca.vc = V;
     ca.auc = au; ca.aup = aup; ca.aupp = aupp; ca.avc = av; ca.avp = avp; ca.avpp = avpp;
-    ca.fx = P<R>(f.fx); ca.fy = P<R>(f.fy); ca.adp = P<R>(f.adp); ca.alpha = P<R>(f.alpha);
-    ca.mch = h.mch; ca.schm = h.schm; ca.pax = h.pax; ca.pay = h.pay; ca.bc = (double)c.dtbc;
-    LAUNCH(k_clin<R>, n3, ca);
+    ca.fx = f.fx; ca.fy = f.fy; ca.adp = P<E>(f.adp); ca.alpha = P<E>(f.alpha);
+    ca.mch = h.mch; ca.schm = h.schm; ca.pax = h.pax; ca.pay = h.pay; ca.cor = c.gd.cor;
+    ca.bc = (double)c.dtbc;
+    LAUNCH((k_clin<R, E>), n3, ca);
   }
-  LAUNCH(k_clin_cpl<R>, n2, g, U, V, h.buc, h.bvc);
+  LAUNCH(k_clin_cpl, n2, c.gd, U, V, h.buc, h.bvc);
   LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, h.schs, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
   c.n_uv += 1;
   // int_ssh
-  LAUNCH_COL(k_ssh<R>, g, T, S, h.mch, P<R>(f.alpha), h.ssh, h.acssh);
+  LAUNCH_COL(k_ssh<E>, ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
   c.n_ssh += 1;
   // int_ts
-  LAUNCH_COL(k_upw<R>, g, U, V, h.sqp, P<R>(f.wm), P<R>(f.acw), (const R*)P<R>(f.alpha), (const double*)h.mch2);
+  LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), P<R>(f.acw), (const E*)P<E>(f.alpha),
+             (const double*)h.mch2);
   c.n_w += 1;
   {
     TsArgs<R> a;
     a.g = g; a.t0 = T; a.s0 = S; a.wm = P<R>(f.wm); a.uc = U; a.vc = V;
     a.schs = h.schs; a.mch = h.mch; a.bndt = h.bnd[2]; a.bnds = h.bnd[3];
-    a.t1 = P<R>(f.T[1 - c.cur]); a.s1 = P<R>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
+    a.t1 = P<double>(f.T[1 - c.cur]); a.s1 = P<double>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
     LAUNCH(k_ts<R>, n3, a);
   }
   c.cur = 1 - c.cur;
-  LAUNCH_COL(k_conv<R>, g, P<R>(f.T[c.cur]), P<R>(f.S[c.cur]), P<R>(f.prho1), P<R>(f.prho2), h.chc, P<R>(f.act),
-         P<R>(f.acs), P<R>(f.acr));
+  {
+    const size_t lds = 2 * (size_t)c.nk * kCW * (sizeof(double) + sizeof(E));
+    if (lds > 65536) {
+      hipError_t ea =
+          hipFuncSetAttribute((const void*)k_conv<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (ea != hipSuccess) return hip_err(ea, "k_conv LDS size");
+    }
+    hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), lds, c.stream, ge,
+                       c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
+                       P<R>(f.act), P<R>(f.acs), P<R>(f.acr));
+    hipError_t e2 = hipGetLastError();
+    if (e2 != hipSuccess) return hip_err(e2, "k_conv");
+  }
   c.n_ts += 1;
   if (c.timing) {
     (void)hipEventRecord(c.ev[3], c.stream);
@@ -1014,7 +1072,7 @@ std::string step_t(Ctx& c) {
 // ---- single operators (FP64) for parity tests ------------------------------
 template <int GA, int GB>
 __global__ void k_op_ter(Grid<double> g, int nd3, double dft, const double* in, double* out) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * nd3) return;
   int i = (int)(p % g.ni) + 1;
   long r = p / g.ni;
@@ -1030,7 +1088,7 @@ __global__ void k_op_ter(Grid<double> g, int nd3, double dft, const double* in, 
 
 template <int DIR, int GA, int GB>
 __global__ void k_op_px(Grid<double> g, int nd3, const double* in, double* out) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * nd3) return;
   int i = (int)(p % g.ni) + 1;
   long r = p / g.ni;
@@ -1042,7 +1100,7 @@ __global__ void k_op_px(Grid<double> g, int nd3, const double* in, double* out) 
 
 template <int GA, int GB, int GC>
 __global__ void k_op_div(Grid<double> g, int nd3, const double* va, const double* vb, double* out) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * nd3) return;
   int i = (int)(p % g.ni) + 1;
   long r = p / g.ni;
@@ -1055,7 +1113,7 @@ __global__ void k_op_div(Grid<double> g, int nd3, const double* va, const double
 }
 
 __global__ void k_op_lap33(Grid<double> g, const double* in, double* out) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   auto F = [&](int m, int jj) -> double { return in[g.ix(m, jj)]; };
@@ -1063,15 +1121,16 @@ __global__ void k_op_lap33(Grid<double> g, const double* in, double* out) {
 }
 
 __global__ void k_den_rho(long n, const double* t, const double* s, const double* pr, double* rho) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p < n) rho[p] = den_rho<double>(t[p], s[p], pr[p]);
 }
 
 }  // namespace
 
 std::string launch_step(Ctx& c) {
-  if (c.precision == PCOM_FP64) return step_t<double>(c);
-  return step_t<float>(c);
+  if (c.precision == PCOM_FP64) return step_t<double, double>(c);
+  if (c.precision == PCOM_MIXED) return step_t<float, double>(c);
+  return step_t<float, float>(c);
 }
 
 std::string launch_global_sums(Ctx& c, double out[3]) {
@@ -1081,10 +1140,7 @@ std::string launch_global_sums(Ctx& c, double out[3]) {
   double* part = c.f2.red + n2;         // 3 * nb
   double* res = part + 3 * (size_t)nb;  // 3
   LAUNCH(k_cell_area, n2, c.gd, area);
-  if (c.precision == PCOM_FP64)
-    LAUNCH(k_sums_partial<double>, n2, c.gd, P<double>(c.f3.T[c.cur]), P<double>(c.f3.S[c.cur]), c.f2.chc, area, part);
-  else
-    LAUNCH(k_sums_partial<float>, n2, c.gf, P<float>(c.f3.T[c.cur]), P<float>(c.f3.S[c.cur]), c.f2.chc, area, part);
+  LAUNCH(k_sums_partial<double>, n2, c.gd, P<double>(c.f3.T[c.cur]), P<double>(c.f3.S[c.cur]), c.f2.chc, area, part);
   hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, c.stream, part, (int)nb, res);
   hipError_t e = hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream);
   if (e != hipSuccess) return hip_err(e, "copy sums");
@@ -1096,10 +1152,7 @@ std::string launch_check_finite(Ctx& c, int* bad) {
   hipError_t e = hipMemsetAsync(c.f2.flag, 0, sizeof(int), c.stream);
   if (e != hipSuccess) return hip_err(e, "memset flag");
   void* f3s[] = {c.f3.T[c.cur], c.f3.S[c.cur], c.f3.U, c.f3.V};
-  for (void* a : f3s) {
-    if (c.precision == PCOM_FP64) LAUNCH(k_finite<double>, c.n3, P<double>(a), (long)c.n3, c.f2.flag);
-    else LAUNCH(k_finite<float>, c.n3, P<float>(a), (long)c.n3, c.f2.flag);
-  }
+  for (void* a : f3s) LAUNCH(k_finite<double>, c.n3, P<double>(a), (long)c.n3, c.f2.flag);
   double* f2s[] = {c.f2.chc, c.f2.buc, c.f2.bvc};
   for (double* a : f2s) LAUNCH(k_finite_d, c.n2, a, (long)c.n2, c.f2.flag);
   e = hipMemcpyAsync(bad, c.f2.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream);

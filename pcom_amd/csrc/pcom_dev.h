@@ -40,6 +40,20 @@ constexpr double AM_C = 1.0e3;
 constexpr double SMAG_C = (double)0.56f;
 constexpr double AH_C = 1.0e3;
 
+// ---- XCD-aware workgroup order.  MI355X deals the workgroups of a launch
+// round-robin to its 8 XCDs (workgroup b runs on XCD b % 8), each with its
+// own L2.  Stencil kernels read the rows next to their own, so consecutive
+// workgroup ids are remapped to consecutive tiles of ONE XCD: XCD x takes
+// the contiguous range [start(x), start(x+1)) of logical tiles.  The map is
+// a bijection of [0, n) for any n.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, n = gridDim.x;
+  const unsigned q = n / kXcds, rem = n % kXcds;
+  const unsigned x = b % kXcds, r = b / kXcds;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
 // ---- stagger tables (mod_arrays.f90:148-166, link_grid 524-552):
 // offsets g%i, g%j and the ew / ns / di neighbours of g1..g4.
 __host__ __device__ constexpr int goi(int g) { return (g == 2 || g == 3) ? 1 : 0; }

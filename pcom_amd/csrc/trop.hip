@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
 template <typename R>
 __global__ void k_vz(Grid<double> g, const R* __restrict__ am, const double* __restrict__ bup,
                      const double* __restrict__ bvp, double* vz1, double* vz2, double* am1) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
   int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
   const double su = g.lev_u[p] >= 1 ? 1.0 : 0.0;

@@ -13,7 +13,7 @@ module pcom_gpu
 
   integer(c_int), parameter :: PCOM_OK = 0, PCOM_ERR_ARG = -1, PCOM_ERR_HIP = -2, &
                                PCOM_ERR_NAN = -3, PCOM_ERR_STATE = -4, PCOM_ERR_NOGPU = -5
-  integer(c_int), parameter :: PCOM_FP64 = 0, PCOM_MIXED = 1
+  integer(c_int), parameter :: PCOM_FP64 = 0, PCOM_MIXED = 1, PCOM_MIXED_EOS32 = 2
 
   ! field ids, in the order of the enum in include/pcom_gpu.h
   enum, bind(c)

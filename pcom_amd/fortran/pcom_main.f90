@@ -28,7 +28,7 @@ program pcom_main
   character(len=256) :: fname_ini = 'input/inputs.bin', fname_rst = '', out_dir = 'output/'
   character(len=16) :: out_per = 'day'
   integer :: restart_per = 1, restart_run = 0
-  character(len=8) :: precision = 'fp64'
+  character(len=12) :: precision = 'fp64'
   integer :: device = 0, tile_i = 0, tile_j = 0, tile_s = 0
   namelist /time_ctrl/ bdate, edate, dtbt, dtbc
   namelist /io_ctrl/ fname_ini, fname_rst, out_dir, out_per, restart_per, restart_run
@@ -104,6 +104,7 @@ program pcom_main
   opts%year = bd(1); opts%month = bd(2); opts%day = bd(3); opts%hour = bd(4)
   opts%precision = PCOM_FP64
   if (trim(precision) == 'mixed') opts%precision = PCOM_MIXED
+  if (trim(precision) == 'mixed_eos32') opts%precision = PCOM_MIXED_EOS32
   opts%restart_run = restart_run
   opts%device = device
   opts%trop_tile_i = tile_i; opts%trop_tile_j = tile_j; opts%trop_substeps = tile_s
