@@ -232,6 +232,66 @@ int pcom_op_lap(pcom_ctx *ctx, int ga, int gb, const double *in, double *out);
 int pcom_den_rho(pcom_ctx *ctx, size_t n, const double *t, const double *s,
                  const double *p, double *rho);
 
+/* ---- Learned barotropic surrogate.  New: the reference has no such part.
+ * A BF16 convolutional network (six 3x3 layers, 64 hidden channels, FP32
+ * accumulation on the matrix cores) predicts what int_trop returns
+ * (mod_int.f90:153-296): the step-mean free surface mch2 and mch and the
+ * step-mean barotropic velocity, from chc, buc, bvc, but, bvt, the face
+ * integrals bphi and bgraphi of int_pgra, sea masks and Coriolis.  It is
+ * trained on the device from snapshots of the plain subcycle.  When in
+ * use, every check_every-th step still runs the plain subcycle, which is
+ * the fallback and measures the network's error there; an error above tol
+ * turns the surrogate off.                                               */
+typedef struct pcom_nn_stats {
+  long long recorded;        /* snapshots held in the store                */
+  long long surrogate_steps; /* steps whose subcycle the network replaced  */
+  long long plain_steps;     /* steps that ran the plain subcycle          */
+  long long checks;          /* plain steps that also judged a prediction  */
+  long long fallbacks;       /* checks that turned the surrogate off       */
+  int in_use;
+  double last_check_rmse[4]; /* sea RMSE of mch2, mch (dimensionless ch),  */
+  double max_check_rmse[4];  /* ubt, vbt (m/s) at the last / worst check   */
+  double last_persist_rmse[4]; /* same for persistence (no change over the
+                                  step) at the last check: the method
+                                  without the network                      */
+} pcom_nn_stats;
+
+/* Attach a network with seeded random weights and a device store of up to
+ * `capacity` snapshots (0: no store).  Needs glo_ni*glo_nj % 16 == 0.   */
+int pcom_nn_create(pcom_ctx *ctx, int capacity, unsigned long long seed);
+/* Record (inputs, outputs) of every plain subcycle from now on (1) or not. */
+int pcom_nn_record(pcom_ctx *ctx, int on);
+/* Adam training on the recorded snapshots: `iters` minibatches of `batch`
+ * snapshots drawn with `seed`, learning rate lr decayed to 0 by a cosine
+ * over the call; loss[iters] (mean squared error in normalised units) may
+ * be NULL.                                                               */
+int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long long seed, double *loss);
+/* Use the network in pcom_step with a plain, judging subcycle every
+ * check_every steps (0 turns the surrogate off); tol bounds the sea RMSE of
+ * each checked output (0 = never fall back).                             */
+int pcom_nn_use(pcom_ctx *ctx, int check_every, double tol);
+/* Root-mean-square error over sea points of the network on recorded
+ * snapshots [first, first+count) (in recording order), in physical units:
+ * rmse[0..3] for the mch2, mch, ubt, vbt increments and rmse[4..7] for
+ * persistence (no change over the step).  Held-out snapshots, recorded
+ * after training, measure what the network gains.                       */
+int pcom_nn_eval(pcom_ctx *ctx, int first, int count, double rmse[8]);
+int pcom_nn_get_stats(pcom_ctx *ctx, pcom_nn_stats *s);
+/* Parameters: per layer the FP32 master weights [cout][3][3][cin] (channels
+ * padded: input 32 (16 used), hidden 64, output 32 (4 used)) and biases
+ * [cout], then 31 normalisation values (means and sds of the 13 recorded
+ * inputs, output scales 4, Coriolis scale).  Setting them marks the
+ * network as trained.                                                    */
+size_t pcom_nn_param_count(const pcom_ctx *ctx);
+int pcom_nn_get_params(pcom_ctx *ctx, float *host, size_t n);
+int pcom_nn_set_params(pcom_ctx *ctx, const float *host, size_t n);
+/* One forward and backward pass on normalised host tensors, for gradient
+ * checks: x [batch][glo_nj][glo_ni][16], target [batch][glo_nj][glo_ni][4];
+ * out gets the output (same shape as target), grads the loss gradient in
+ * parameter order (zero for the normalisation values).                  */
+int pcom_nn_forward_backward(pcom_ctx *ctx, int batch, const float *x, const float *target, float *out,
+                             float *grads, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,15 @@ class PcomMeans(C.Structure):
     _fields_ = [(n, _DP) for n in MEAN_NAMES]
 
 
+class PcomNnStats(C.Structure):
+    _fields_ = [
+        ("recorded", C.c_longlong), ("surrogate_steps", C.c_longlong), ("plain_steps", C.c_longlong),
+        ("checks", C.c_longlong), ("fallbacks", C.c_longlong), ("in_use", C.c_int),
+        ("last_check_rmse", C.c_double * 4), ("max_check_rmse", C.c_double * 4),
+        ("last_persist_rmse", C.c_double * 4),
+    ]
+
+
 # Field ids, in header order.
 FIELD_NAMES = [
     "tc", "sc", "tp", "sp",

@@ -41,7 +41,10 @@ int dev_alloc(Ctx& c, void** out, size_t count, size_t elem) {
   return PCOM_OK;
 }
 
+// host -> device on the null stream: the context's stream is non-blocking,
+// so its queued work is drained first
 int upload(Ctx& c, void* dst, const double* src, size_t n, bool as_float) {
+  if (c.stream) HIPCHK(hipStreamSynchronize(c.stream), "sync before upload");
   if (!as_float) {
     HIPCHK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice), "upload");
   } else {
@@ -410,6 +413,7 @@ int pcom_destroy(pcom_ctx* ctx) {
   Ctx* c = C(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  nn_destroy(*c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -524,6 +528,101 @@ int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
   }
   std::string e = launch_output_means(*c, out, c->stage);
   if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_output_means: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_create(pcom_ctx* ctx, int capacity, unsigned long long seed) {
+  if (!ctx || capacity < 0) return fail(PCOM_ERR_ARG, "pcom_nn_create: bad argument");
+  Ctx* c = C(ctx);
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_create(*c, capacity, seed);
+  if (!e.empty()) {
+    nn_destroy(*c);
+    return fail(e.find("hip") != std::string::npos ? PCOM_ERR_HIP : PCOM_ERR_ARG, "pcom_nn_create: " + e);
+  }
+  return PCOM_OK;
+}
+
+int pcom_nn_record(pcom_ctx* ctx, int on) {
+  if (!ctx) return fail(PCOM_ERR_ARG, "pcom_nn_record: NULL context");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_record: no surrogate attached");
+  nn_set_record(*c, on != 0);
+  return PCOM_OK;
+}
+
+int pcom_nn_train(pcom_ctx* ctx, int iters, int batch, double lr, unsigned long long seed, double* loss) {
+  if (!ctx || iters < 0 || batch < 1 || !(lr > 0)) return fail(PCOM_ERR_ARG, "pcom_nn_train: bad argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_train: no surrogate attached");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_train(*c, iters, batch, lr, seed, loss);
+  if (!e.empty()) return fail(PCOM_ERR_STATE, "pcom_nn_train: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_use(pcom_ctx* ctx, int check_every, double tol) {
+  if (!ctx || check_every < 0 || tol < 0) return fail(PCOM_ERR_ARG, "pcom_nn_use: bad argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_use: no surrogate attached");
+  std::string e = nn_set_use(*c, check_every, tol);
+  if (!e.empty()) return fail(PCOM_ERR_STATE, "pcom_nn_use: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_eval(pcom_ctx* ctx, int first, int count, double rmse[8]) {
+  if (!ctx || !rmse) return fail(PCOM_ERR_ARG, "pcom_nn_eval: NULL argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_eval: no surrogate attached");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_eval(*c, first, count, rmse);
+  if (!e.empty()) return fail(PCOM_ERR_STATE, "pcom_nn_eval: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_get_stats(pcom_ctx* ctx, pcom_nn_stats* s) {
+  if (!ctx || !s) return fail(PCOM_ERR_ARG, "pcom_nn_get_stats: NULL argument");
+  nn_stats(*C(ctx), s);
+  return PCOM_OK;
+}
+
+size_t pcom_nn_param_count(const pcom_ctx* ctx) {
+  if (!ctx) return 0;
+  const Ctx* c = C(ctx);
+  return c->nn ? nn_param_count(*c) : 0;
+}
+
+int pcom_nn_get_params(pcom_ctx* ctx, float* host, size_t n) {
+  if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_nn_get_params: NULL argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_get_params: no surrogate attached");
+  if (n != pcom_nn_param_count(ctx)) return fail(PCOM_ERR_ARG, "pcom_nn_get_params: size mismatch");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_params(*c, host, false);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_nn_get_params: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_set_params(pcom_ctx* ctx, const float* host, size_t n) {
+  if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_nn_set_params: NULL argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_set_params: no surrogate attached");
+  if (n != pcom_nn_param_count(ctx)) return fail(PCOM_ERR_ARG, "pcom_nn_set_params: size mismatch");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_params(*c, const_cast<float*>(host), true);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_nn_set_params: " + e);
+  return PCOM_OK;
+}
+
+int pcom_nn_forward_backward(pcom_ctx* ctx, int batch, const float* x, const float* target, float* out, float* grads,
+                             double* loss) {
+  if (!ctx || batch < 1 || !x || !target || !out || !grads || !loss)
+    return fail(PCOM_ERR_ARG, "pcom_nn_forward_backward: bad argument");
+  Ctx* c = C(ctx);
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_forward_backward: no surrogate attached");
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  std::string e = nn_forward_backward(*c, batch, x, target, out, grads, loss);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_nn_forward_backward: " + e);
   return PCOM_OK;
 }
 

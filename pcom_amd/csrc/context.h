@@ -11,6 +11,8 @@
 
 namespace pcom {
 
+struct Nn;  // surrogate.hip: the learned barotropic surrogate
+
 // 3-D fields are stored in R (double, or float in the mixed form); 2-D
 // fields (free surface, barotropic mode, forcing) are always double.
 struct Fields3 {
@@ -69,6 +71,7 @@ struct Ctx {
   double t_ms[3] = {0, 0, 0};
   int t_steps = 0;
   double* stage = nullptr;  // output staging, allocated on first use
+  Nn* nn = nullptr;         // learned subcycle surrogate, if attached
   size_t rsize() const { return precision == PCOM_FP64 ? sizeof(double) : sizeof(float); }
 };
 
@@ -81,6 +84,19 @@ std::string launch_check_finite(Ctx& c, int* bad);
 std::string launch_output_means(Ctx& c, pcom_means* o, double* stage);
 std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
 std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
+// surrogate.hip
+std::string nn_create(Ctx& c, int capacity, unsigned long long seed);
+void nn_destroy(Ctx& c);
+void nn_set_record(Ctx& c, bool on);
+std::string nn_set_use(Ctx& c, int every, double tol);
+std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&));
+std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long seed, double* loss);
+size_t nn_param_count(const Ctx& c);
+std::string nn_eval(Ctx& c, int first, int count, double* res);
+std::string nn_params(Ctx& c, float* host, bool set);
+std::string nn_forward_backward(Ctx& c, int batch, const float* x, const float* target, float* out, float* grads,
+                                double* loss);
+void nn_stats(const Ctx& c, pcom_nn_stats* s);
 // single-operator entry points on device buffers (FP64)
 std::string launch_op_ter(Ctx& c, int ga, int gb, int nd3, double dft, const double* in, double* out);
 std::string launch_op_px(Ctx& c, int dir, int ga, int gb, int nd3, const double* in, double* out);

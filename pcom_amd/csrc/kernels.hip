@@ -1001,7 +1001,7 @@ std::string step_t(Ctx& c) {
   LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
   // int_trop (launch_trop records ev[1] right before the subcycle kernel)
   {
-    std::string te = launch_trop(c);
+    std::string te = c.nn ? nn_trop_step(c, launch_trop) : launch_trop(c);
     if (!te.empty()) return te;
     c.n_ch += 1;
   }

@@ -1,0 +1,1185 @@
+// surrogate.hip - learned surrogate of the barotropic subcycle (int_trop,
+// mod_int.f90:153-296): a 3x3 convolutional network in BF16 with FP32
+// accumulation on the gfx950 matrix cores, trained on the device from
+// snapshots the plain subcycle produces, and used in pcom_step to skip most
+// subcycles.  The plain subcycle stays the fallback and the judge: every
+// check_every-th step runs it and measures the network's error there.
+//
+// What the network maps (one baroclinic step, interior glo_nj x glo_ni
+// points, zonally periodic, zero padding at the walls):
+//   inputs (16 channels): chc-1, buc, bvc, but, bvt (the state and the
+//           column-integrated baroclinic forcing int_readyc hands to
+//           int_trop), T-point and U-point sea masks, Coriolis parameter,
+//           and the face integrals bphi and bgraphi of int_pgra (xn, xs,
+//           ye, yw each), through which the density field enters the
+//           barotropic pressure gradient (mod_int.f90:64-88, 224-248)
+//   outputs (4 channels): mch2-chc, mch-chc, ubt-buc, vbt-bvc (the
+//           step-mean free surface of the full and half cycle and the
+//           step-mean barotropic velocity, as increments)
+// each channel normalised by its mean and standard deviation over the
+// recorded snapshots.  Applying a prediction writes what int_trop's end
+// writes (mod_int.f90:276-287) with the area-weighted mean of the free
+// surface increment removed over sea, so the prediction conserves mass.
+//
+// Tensors are NHWC with channels padded to multiples of 32 (the K width of
+// v_mfma_f32_16x16x32_bf16).  Lane layout of that instruction (checked on
+// the device): A row = lane % 16, k = 8 (lane / 16) + e; B column =
+// lane % 16, same k; D row = 4 (lane / 16) + i, column = lane % 16.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "context.h"
+
+namespace pcom {
+
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16_t;
+
+constexpr int kInC = 32;    // padded input channels (16 used)
+constexpr int kHid = 64;    // hidden channels
+constexpr int kOutC = 32;   // padded output channels (4 used)
+// input channels: 0-4 and 8-15 recorded per snapshot (kDyn of them), 5-7
+// static (sea masks, Coriolis)
+constexpr int kInUsed = 16, kOutUsed = 4, kDyn = 13;
+__host__ __device__ constexpr int dyn_channel(int d) { return d < 5 ? d : d + 3; }
+constexpr int kLayers = 6;  // 32->64, 4 x 64->64, 64->32 (last linear)
+
+__host__ __device__ inline bf16_t f2bf(float f) {
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h) {
+  unsigned u = (unsigned)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+struct Geo {
+  int n, H, W;
+  __host__ __device__ long npix() const { return (long)n * H * W; }
+};
+
+__device__ __forceinline__ bf8 load8(const bf16_t* p) {
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf8, v);
+}
+__device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uint4(0, 0, 0, 0)); }
+
+// ---- forward 3x3 convolution: out[p][co] = b[co] + sum W[co][dy][dx][ci]
+// in[p + (dy-1, dx-1)][ci], periodic in x, zero outside 0 <= y < H.  One
+// wave computes 16 pixels x COUT channels; 4 waves per workgroup.
+template <int CIN, int COUT, bool RELU, bool OUTF32>
+__global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
+                                                  const float* __restrict__ bias, void* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long np = g.npix();
+  const long p0 = ((long)xcd_block() * 4 + wave) * 16;
+  if (p0 >= np) return;
+  const int kq = lane >> 4, r = lane & 15;
+  const long pa = p0 + r;
+  const int x = (int)(pa % g.W);
+  const long t = pa / g.W;
+  const int y = (int)(t % g.H), n = (int)(t / g.H);
+  f4 acc[COUT / 16];
+#pragma unroll
+  for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
+#pragma unroll
+  for (int dy = 0; dy < 3; ++dy) {
+    const int yy = y + dy - 1;
+    const bool ok = yy >= 0 && yy < g.H;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+      int xx = x + dx - 1;
+      xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+      const bf16_t* src = in + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * CIN + kq * 8;
+#pragma unroll
+      for (int c = 0; c < CIN / 32; ++c) {
+        const bf8 a = ok ? load8(src + c * 32) : zero8();
+#pragma unroll
+        for (int nb = 0; nb < COUT / 16; ++nb) {
+          const bf8 b = load8(w + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * CIN + c * 32 + kq * 8);
+          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < COUT / 16; ++nb) {
+    const int co = nb * 16 + r;
+    const float bb = bias[co];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long pd = p0 + kq * 4 + i;
+      float v = acc[nb][i] + bb;
+      if (RELU) v = v > 0.f ? v : 0.f;
+      if (OUTF32) static_cast<float*>(out)[pd * COUT + co] = v;
+      else static_cast<bf16_t*>(out)[pd * COUT + co] = f2bf(v);
+    }
+  }
+}
+
+// ---- backward data: din[p][ci] = sum W[co][dy][dx][ci] dpre[p - (dy-1, dx-1)][co]
+// with wt = W transposed to [ci][dy][dx][co].
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __restrict__ dpre,
+                                                       const bf16_t* __restrict__ wt, bf16_t* din) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long np = g.npix();
+  const long p0 = ((long)xcd_block() * 4 + wave) * 16;
+  if (p0 >= np) return;
+  const int kq = lane >> 4, r = lane & 15;
+  const long pa = p0 + r;
+  const int x = (int)(pa % g.W);
+  const long t = pa / g.W;
+  const int y = (int)(t % g.H), n = (int)(t / g.H);
+  f4 acc[CIN / 16];
+#pragma unroll
+  for (int nb = 0; nb < CIN / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
+#pragma unroll
+  for (int dy = 0; dy < 3; ++dy) {
+    const int yy = y - dy + 1;
+    const bool ok = yy >= 0 && yy < g.H;
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+      int xx = x - dx + 1;
+      xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+      const bf16_t* src = dpre + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * COUT + kq * 8;
+#pragma unroll
+      for (int c = 0; c < COUT / 32; ++c) {
+        const bf8 a = ok ? load8(src + c * 32) : zero8();
+#pragma unroll
+        for (int nb = 0; nb < CIN / 16; ++nb) {
+          const bf8 b = load8(wt + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * COUT + c * 32 + kq * 8);
+          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < CIN / 16; ++nb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) din[(p0 + kq * 4 + i) * CIN + nb * 16 + r] = f2bf(acc[nb][i]);
+}
+
+// ---- backward weights, one (dy, dx) per blockIdx.y and one pixel chunk per
+// blockIdx.x: part[chunk][dydx][co][ci] = sum_p dpre[p][co] in[p+(dy-1,dx-1)][ci].
+// GEMM with M = co, N = ci, K = pixels; 32-pixel slices of both operands
+// are staged in LDS and read back transposed.  Wave w owns co rows
+// [16 w', 16 w' + 16) for w' = w, w + 4, ... < COUT/16.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_bwd_w(Geo g, const bf16_t* __restrict__ in,
+                                                    const bf16_t* __restrict__ dpre, float* part, long chunk) {
+  __shared__ bf16_t sD[32][COUT + 2];
+  __shared__ bf16_t sI[32][CIN + 2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kq = lane >> 4, r = lane & 15;
+  const int dydx = blockIdx.y, dy = dydx / 3, dx = dydx % 3;
+  const long np = g.npix();
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = pbeg + chunk < np ? pbeg + chunk : np;
+  constexpr int MB = COUT / 16, NB = CIN / 16;
+  constexpr int MW = (MB + 3) / 4;  // co blocks per wave
+  f4 acc[MW][NB];
+#pragma unroll
+  for (int a = 0; a < MW; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f4{0, 0, 0, 0};
+  for (long ps = pbeg; ps < pend; ps += 32) {
+    // stage 32 pixels of dpre (COUT) and of the shifted input (CIN)
+    for (int e = threadIdx.x; e < 32 * COUT; e += 256) {
+      const int pp = e / COUT, c = e % COUT;
+      const long p = ps + pp;
+      sD[pp][c] = p < pend ? dpre[p * COUT + c] : (bf16_t)0;
+    }
+    for (int e = threadIdx.x; e < 32 * CIN; e += 256) {
+      const int pp = e / CIN, c = e % CIN;
+      const long p = ps + pp;
+      bf16_t v = 0;
+      if (p < pend) {
+        const int x = (int)(p % g.W);
+        const long t = p / g.W;
+        const int y = (int)(t % g.H), n = (int)(t / g.H);
+        const int yy = y + dy - 1;
+        int xx = x + dx - 1;
+        xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+        if (yy >= 0 && yy < g.H) v = in[(((long)n * g.H + yy) * g.W + xx) * CIN + c];
+      }
+      sI[pp][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < MW; ++a) {
+      const int mb = wave + 4 * a;
+      if (mb < MB) {
+        bf16_t av[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) av[e] = sD[kq * 8 + e][mb * 16 + r];
+        bf8 A;
+        __builtin_memcpy(&A, av, 16);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          bf16_t bv[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) bv[e] = sI[kq * 8 + e][b * 16 + r];
+          bf8 B;
+          __builtin_memcpy(&B, bv, 16);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[a][b], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* dst = part + ((long)blockIdx.x * 9 + dydx) * COUT * CIN;
+#pragma unroll
+  for (int a = 0; a < MW; ++a) {
+    const int mb = wave + 4 * a;
+    if (mb < MB)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[(mb * 16 + kq * 4 + i) * CIN + b * 16 + r] = acc[a][b][i];
+  }
+}
+
+// sum the chunk partials in chunk order into gw[co][dy][dx][ci]; bias
+// gradient partials likewise into gb[co]
+__global__ void k_reduce_w(const float* part, int nchunk, int cout, int cin, float* gw) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = 9L * cout * cin;
+  if (e >= m) return;
+  const int ci = (int)(e % cin);
+  const long t = e / cin;
+  const int co = (int)(t % cout), dydx = (int)(t / cout);
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[((long)c * 9 + dydx) * cout * cin + (long)co * cin + ci];
+  gw[((long)co * 9 + dydx) * cin + ci] = s;
+}
+
+__global__ void k_bias_part(Geo g, const bf16_t* __restrict__ dpre, int cout, long chunk, float* part) {
+  const int co = threadIdx.x;
+  if (co >= cout) return;
+  const long np = g.npix();
+  const long pbeg = (long)blockIdx.x * chunk;
+  const long pend = pbeg + chunk < np ? pbeg + chunk : np;
+  float s = 0.f;
+  for (long p = pbeg; p < pend; ++p) s += bf2f(dpre[p * cout + co]);
+  part[(long)blockIdx.x * cout + co] = s;
+}
+
+__global__ void k_bias_reduce(const float* part, int nchunk, int cout, float* gb) {
+  const int co = threadIdx.x;
+  if (co >= cout) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[(long)c * cout + co];
+  gb[co] = s;
+}
+
+// dpre = dpost where post > 0 (ReLU)
+__global__ void k_relu_bwd(long n, const bf16_t* __restrict__ post, const bf16_t* __restrict__ dpost, bf16_t* dpre) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  dpre[e] = bf2f(post[e]) > 0.f ? dpost[e] : (bf16_t)0;
+}
+
+// masked mean-squared error over the 4 used outputs; channel c is masked by
+// input channel 5 (T-point sea) for c < 2 and 6 (U-point sea) otherwise.
+// dpre = d(loss)/d(out) in BF16; per-block partial sums of loss and count.
+__global__ void k_loss(long np, const float* __restrict__ out, const float* __restrict__ tgt,
+                       const bf16_t* __restrict__ x, float inv_count, bf16_t* dpre, double* part) {
+  __shared__ double sh[256];
+  const long p = (long)xcd_block() * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (p < np) {
+    const float mt = bf2f(x[p * kInC + 5]), mu = bf2f(x[p * kInC + 6]);
+    for (int c = 0; c < kOutC; ++c) {
+      float d = 0.f;
+      if (c < kOutUsed) {
+        const float m = c < 2 ? mt : mu;
+        d = m * (out[p * kOutC + c] - tgt[p * kOutUsed + c]);
+        l += (double)d * d;
+      }
+      dpre[p * kOutC + c] = f2bf(2.f * d * inv_count);
+    }
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_parts(const double* part, int n, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *out = s;
+  }
+}
+
+// Adam on the FP32 master weights, then the BF16 copies: w[co][dy][dx][ci]
+// and its transpose wt[ci][dy][dx][co] for the backward-data pass
+__global__ void k_adam(long n, float* w32, float* m, float* v, const float* g, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  const float mm = b1 * m[e] + (1.f - b1) * gg;
+  const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  w32[e] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+}
+
+__global__ void k_w_bf16(int cout, int cin, const float* w32, bf16_t* w, bf16_t* wt) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 9L * cout * cin) return;
+  const int ci = (int)(e % cin);
+  const long t = e / cin;
+  const int dydx = (int)(t % 9), co = (int)(t / 9);
+  const bf16_t h = f2bf(w32[e]);
+  w[e] = h;
+  wt[((long)ci * 9 + dydx) * cout + co] = h;
+}
+
+// ---- data movement between the model state and the network tensors.
+// Static per-pixel channels: sea masks and the normalised Coriolis
+// parameter; pixel q = y * W + x maps to Fortran (i, j) = (x + 2, y + 2).
+struct NnNorm {
+  float in_mean[kDyn], in_std[kDyn], out_std[kOutUsed];
+  float cor_scale;
+};
+constexpr int kNormValues = 2 * kDyn + kOutUsed + 1;
+
+__device__ __forceinline__ long nn_ix(int ni, int x, int y) { return (long)(x + 1) + (long)ni * (y + 1); }
+
+// the kDyn recorded inputs of the live state: chc-1, buc, bvc, but, bvt,
+// bphi xn xs ye yw, bgraphi xn xs ye yw
+struct LiveIn {
+  const double* f[kDyn];
+};
+__device__ __forceinline__ float live_value(const LiveIn& in, int d, long k) {
+  return d == 0 ? (float)(in.f[0][k] - 1.0) : (float)in.f[d][k];
+}
+
+// record the inputs of one plain subcycle (raw values)
+__global__ void k_rec_in(int ni, int H, int W, LiveIn in, float* store) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+#pragma unroll
+  for (int d = 0; d < kDyn; ++d) store[d * hw + q] = live_value(in, d, k);
+}
+
+// and its outputs, as increments over the recorded inputs
+__global__ void k_rec_out(int ni, int H, int W, const double* mch2, const double* mch, const double* buc,
+                          const double* bvc, const float* in, float* tgt) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+  const double c0 = (double)in[q] + 1.0;
+  tgt[q] = (float)(mch2[k] - c0);
+  tgt[hw + q] = (float)(mch[k] - c0);
+  tgt[2 * hw + q] = (float)(buc[k] - (double)in[hw + q]);
+  tgt[3 * hw + q] = (float)(bvc[k] - (double)in[2 * hw + q]);
+}
+
+// channel moments of the store (one block per channel, FP64, fixed order)
+__global__ void k_moments(const float* store, long per_snap, long hw, int nsnap, int chan, double* out) {
+  __shared__ double s1[256], s2[256];
+  const int c = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int sn = 0; sn < nsnap; ++sn)
+    for (long q = threadIdx.x; q < hw; q += blockDim.x) {
+      const double v = store[(long)sn * per_snap + (long)c * hw + q];
+      a += v;
+      b += v * v;
+    }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * c] = s1[0];
+    out[2 * c + 1] = s2[0];
+  }
+  (void)chan;
+}
+
+// pack a minibatch of recorded snapshots into the network input (BF16 NHWC,
+// 32 channels) and targets (FP32, 4 channels, normalised)
+__global__ void k_pack(int H, int W, const float* sin, const float* stg, const int* idx, int batch, NnNorm nm,
+                       const float* stat, bf16_t* x, float* t) {
+  const long hw = (long)H * W;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= hw * batch) return;
+  const int b = (int)(e / hw);
+  const long q = e % hw;
+  const long s = idx[b];
+  const float* in = sin + s * kDyn * hw;
+  const float* tg = stg + s * kOutUsed * hw;
+  bf16_t* xo = x + e * kInC;
+#pragma unroll
+  for (int d = 0; d < kDyn; ++d) xo[dyn_channel(d)] = f2bf((in[d * hw + q] - nm.in_mean[d]) / nm.in_std[d]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xo[5 + c] = f2bf(stat[c * hw + q]);
+  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
+#pragma unroll
+  for (int c = 0; c < kOutUsed; ++c) t[e * kOutUsed + c] = tg[c * hw + q] / nm.out_std[c];
+}
+
+// the live state into the network input (one image)
+__global__ void k_pack_live(int ni, int H, int W, LiveIn in, NnNorm nm, const float* stat, bf16_t* x) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+  bf16_t* xo = x + q * kInC;
+#pragma unroll
+  for (int d = 0; d < kDyn; ++d) xo[dyn_channel(d)] = f2bf((live_value(in, d, k) - nm.in_mean[d]) / nm.in_std[d]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xo[5 + c] = f2bf(stat[c * hw + q]);
+  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
+}
+
+// prediction -> increments (FP64) with per-block sums over sea of
+// area*d(mch2), area*d(mch) and area (rh of the T points)
+__global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const float* stat, const double* area,
+                       double* inc, double* part) {
+  __shared__ double s[3][256];
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (q < hw) {
+    const float mt = stat[q], mu = stat[hw + q];
+#pragma unroll
+    for (int c = 0; c < kOutUsed; ++c)
+      inc[c * hw + q] = (double)(out[q * kOutC + c] * nm.out_std[c]) * (double)(c < 2 ? mt : mu);
+    const double ar = area[nn_ix(ni, (int)(q % W), (int)(q / W))] * (double)mt;
+    v[0] = ar * inc[q];
+    v[1] = ar * inc[hw + q];
+    v[2] = ar;
+  }
+  for (int c = 0; c < 3; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) part[3 * blockIdx.x + c] = s[c][0];
+}
+
+// sea means of the two free-surface increments, then removed from them
+__global__ void k_pred_mean(const double* part, int nb, double* mean) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double a = 0.0, b = 0.0, w = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      a += part[3 * i];
+      b += part[3 * i + 1];
+      w += part[3 * i + 2];
+    }
+    mean[0] = w > 0.0 ? a / w : 0.0;
+    mean[1] = w > 0.0 ? b / w : 0.0;
+  }
+}
+
+__global__ void k_pred_fix(long hw, const double* mean, const float* stat, double* inc) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= hw) return;
+  const double sea = (double)stat[q];
+  inc[q] = inc[q] - mean[0] * sea;
+  inc[hw + q] = inc[hw + q] - mean[1] * sea;
+}
+
+// write the end-of-cycle fields (mod_int.f90:276-287) from the increments,
+// interior points and their periodic halo copies
+__global__ void k_apply(int ni, int H, int W, const double* inc, double* chc, double* chp, double* mch,
+                        double* mch2, double* acch, double* buc, double* bvc, double* bup, double* bvp) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const int x = (int)(q % W), y = (int)(q / W);
+  const long k = nn_ix(ni, x, y);
+  const double c0 = chc[k];
+  const double m2 = c0 + inc[q];
+  const double m1 = c0 + inc[hw + q];
+  const double u = buc[k] + inc[2 * hw + q];
+  const double v = bvc[k] + inc[3 * hw + q];
+  const int i = x + 2;
+  const int cols[2] = {i, i == W + 1 ? 1 : (i == 2 ? W + 2 : 0)};
+  for (int cidx = 0; cidx < 2; ++cidx) {
+    if (cols[cidx] == 0) continue;
+    const long o = (long)(cols[cidx] - 1) + (long)ni * (y + 1);
+    buc[o] = u;
+    bvc[o] = v;
+    bup[o] = u;
+    bvp[o] = v;
+    mch[o] = m1;
+    mch2[o] = m2;
+    chc[o] = m2;
+    chp[o] = m2;
+    acch[o] = acch[o] + m2;
+  }
+}
+
+// root-mean-square error over sea of the prediction against the plain
+// subcycle's result (err[c]), and of persistence, i.e. no change over the
+// step (err[4 + c]); one block per channel, fixed order
+__global__ void k_check(int ni, int H, int W, const double* inc, const double* c0, const double* mch2,
+                        const double* mch, const double* buc, const double* bvc, const float* stat, double* err) {
+  __shared__ double s[256], s0[256], cnt[256];
+  const int c = blockIdx.x;
+  const long hw = (long)H * W;
+  double a = 0.0, a0 = 0.0, n = 0.0;
+  for (long q = threadIdx.x; q < hw; q += blockDim.x) {
+    const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+    const double m = (double)stat[(c < 2 ? 0 : hw) + q];
+    const double* truth = c == 0 ? mch2 : c == 1 ? mch : c == 2 ? buc : bvc;
+    const double base = c0[c * hw + q];
+    const double d = m * (base + inc[c * hw + q] - truth[k]);
+    const double d0 = m * (base - truth[k]);
+    a += d * d;
+    a0 += d0 * d0;
+    n += m;
+  }
+  s[threadIdx.x] = a;
+  s0[threadIdx.x] = a0;
+  cnt[threadIdx.x] = n;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      s[threadIdx.x] += s[threadIdx.x + st];
+      s0[threadIdx.x] += s0[threadIdx.x + st];
+      cnt[threadIdx.x] += cnt[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    err[c] = cnt[0] > 0 ? sqrt(s[0] / cnt[0]) : 0.0;
+    err[4 + c] = cnt[0] > 0 ? sqrt(s0[0] / cnt[0]) : 0.0;
+  }
+}
+
+// squared prediction error and squared target (persistence error) per used
+// output channel, in physical units, summed over sea points of a batch
+__global__ void k_eval(long np, long hw, const float* __restrict__ out, const float* __restrict__ tgt, NnNorm nm,
+                       const float* __restrict__ stat, double* part) {
+  __shared__ double s[9][256];
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < np) {
+    const long q = p % hw;
+    const float mt = stat[q], mu = stat[hw + q];
+    for (int c = 0; c < kOutUsed; ++c) {
+      const double m = c < 2 ? mt : mu;
+      const double d = m * ((double)out[p * kOutC + c] - (double)tgt[p * kOutUsed + c]) * nm.out_std[c];
+      const double t = m * (double)tgt[p * kOutUsed + c] * nm.out_std[c];
+      v[c] = d * d;
+      v[4 + c] = t * t;
+    }
+    v[8] = mt;
+  }
+  for (int c = 0; c < 9; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 9; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 9; ++c) part[9 * blockIdx.x + c] = s[c][0];
+}
+
+__global__ void k_save_base(int ni, int H, int W, const double* chc, const double* buc, const double* bvc,
+                            double* base) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+  base[q] = chc[k];
+  base[hw + q] = chc[k];
+  base[2 * hw + q] = buc[k];
+  base[3 * hw + q] = bvc[k];
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+#define NN_LAUNCH(kern, grid, block, ...)                                        \
+  do {                                                                           \
+    hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__);                   \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) return std::string(#kern ": ") + hipGetErrorString(e_); \
+  } while (0)
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct NnLayer {
+  int cin = 0, cout = 0;
+  bool relu = true;
+  float *w32 = nullptr, *b = nullptr;     // master weights [co][9][ci], bias [co]
+  float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;  // Adam moments
+  float *gw = nullptr, *gb = nullptr;     // gradients
+  bf16_t *w = nullptr, *wt = nullptr;     // BF16 weights and transpose
+  long nw() const { return 9L * cin * cout; }
+};
+
+struct Nn {
+  int H = 0, W = 0, ni = 0;
+  int cap = 0, nrec = 0, cursor = 0;
+  bool recording = false;
+  int use_every = 0;  // surrogate in use: plain subcycle every use_every-th step
+  double tol = 0.0;
+  long long step_count = 0, surrogate_steps = 0, plain_steps = 0, checks = 0, fallbacks = 0;
+  double last_err[4] = {0, 0, 0, 0}, max_err[4] = {0, 0, 0, 0}, last_persist[4] = {0, 0, 0, 0};
+  bool normalised = false;
+  int adam_t = 0;              // Adam steps taken
+  double* misc = nullptr;      // [16]: sea means, check errors
+  NnNorm norm{};
+  NnLayer L[kLayers];
+  float* store_in = nullptr;   // [cap][5][HW]
+  float* store_tg = nullptr;   // [cap][4][HW]
+  float* stat = nullptr;       // [3][HW] sea T, sea U, Coriolis / scale
+  int batch_max = 0;
+  bf16_t* act[kLayers + 1] = {};  // layer inputs / outputs (BF16), act[0] = x
+  float* out = nullptr;        // last layer output (FP32) [batch][HW][32]
+  float* tgt = nullptr;        // [batch][HW][4]
+  bf16_t *g0 = nullptr, *g1 = nullptr;  // gradient ping-pong (BF16, 64 channels)
+  float* wpart = nullptr;      // weight-gradient partials
+  long wchunk = 0;
+  int wnchunk = 0;
+  double* dpart = nullptr;     // loss partials, moments, prediction sums
+  double* inc = nullptr;       // [4][HW] predicted increments
+  double* base = nullptr;      // [4][HW] state before a checked subcycle
+  int* idx = nullptr;          // minibatch indices
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+std::string nn_alloc(Nn& n, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess) return std::string("hipMalloc (surrogate): ") + hipGetErrorString(e);
+  n.allocs.push_back(*p);
+  return "";
+}
+
+template <typename T>
+std::string nn_make(Nn& n, T** p, size_t count) {
+  void* v = nullptr;
+  std::string e = nn_alloc(n, &v, count * sizeof(T));
+  if (!e.empty()) return e;
+  hipError_t h = hipMemset(v, 0, count * sizeof(T));
+  if (h != hipSuccess) return std::string("hipMemset (surrogate): ") + hipGetErrorString(h);
+  *p = static_cast<T*>(v);
+  return "";
+}
+
+std::string nn_batch_buffers(Nn& n, int batch) {
+  if (batch <= n.batch_max) return "";
+  const long np = (long)batch * n.H * n.W;
+  std::string e;
+  for (int l = 0; l <= kLayers; ++l) {
+    const int c = l == 0 ? kInC : (l == kLayers ? kOutC : kHid);
+    if (!(e = nn_make(n, &n.act[l], (size_t)np * c)).empty()) return e;
+  }
+  if (!(e = nn_make(n, &n.out, (size_t)np * kOutC)).empty()) return e;
+  if (!(e = nn_make(n, &n.tgt, (size_t)np * kOutUsed)).empty()) return e;
+  if (!(e = nn_make(n, &n.g0, (size_t)np * kHid)).empty()) return e;
+  if (!(e = nn_make(n, &n.g1, (size_t)np * kHid)).empty()) return e;
+  n.wchunk = 2048;
+  n.wnchunk = (int)((np + n.wchunk - 1) / n.wchunk);
+  if (!(e = nn_make(n, &n.wpart, (size_t)n.wnchunk * 9 * kHid * kHid)).empty()) return e;
+  // block partials: up to 9 per block of 256 pixels (k_eval), with room
+  if (!(e = nn_make(n, &n.dpart, (size_t)(9 * nblk(np) + 64))).empty()) return e;
+  if (!(e = nn_make(n, &n.idx, (size_t)batch)).empty()) return e;
+  n.batch_max = batch;
+  return "";
+}
+
+// forward pass over `batch` images already packed into act[0]
+std::string nn_forward(Nn& n, hipStream_t st, int batch) {
+  Geo g{batch, n.H, n.W};
+  const unsigned blocks = (unsigned)((g.npix() / 16 + 3) / 4);
+  NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
+            (void*)n.act[1]);
+  for (int l = 1; l < kLayers - 1; ++l)
+    NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
+              (void*)n.act[l + 1]);
+  NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
+            n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
+  return "";
+}
+
+// backward pass from dpre of the last layer (in act[kLayers], BF16)
+std::string nn_backward(Nn& n, hipStream_t st, int batch) {
+  Geo g{batch, n.H, n.W};
+  const long np = g.npix();
+  const unsigned blocks = (unsigned)((np / 16 + 3) / 4);
+  const int nch = (int)((np + n.wchunk - 1) / n.wchunk);
+  bf16_t* dpre = n.act[kLayers];  // d loss / d (last pre-activation), 32 channels
+  for (int l = kLayers - 1; l >= 0; --l) {
+    NnLayer& L = n.L[l];
+    // weight and bias gradients
+    if (L.cin == kInC && L.cout == kHid)
+      NN_LAUNCH((k_conv_bwd_w<kInC, kHid>), dim3(nch, 9), dim3(256), g, n.act[l], dpre, n.wpart, n.wchunk);
+    else if (L.cin == kHid && L.cout == kHid)
+      NN_LAUNCH((k_conv_bwd_w<kHid, kHid>), dim3(nch, 9), dim3(256), g, n.act[l], dpre, n.wpart, n.wchunk);
+    else
+      NN_LAUNCH((k_conv_bwd_w<kHid, kOutC>), dim3(nch, 9), dim3(256), g, n.act[l], dpre, n.wpart, n.wchunk);
+    NN_LAUNCH(k_reduce_w, dim3(nblk(L.nw())), dim3(256), n.wpart, nch, L.cout, L.cin, L.gw);
+    NN_LAUNCH(k_bias_part, dim3(nch), dim3(64), g, dpre, L.cout, n.wchunk, (float*)n.wpart);
+    NN_LAUNCH(k_bias_reduce, dim3(1), dim3(64), (const float*)n.wpart, nch, L.cout, L.gb);
+    if (l == 0) break;
+    // gradient of this layer's input = previous layer's post-activation
+    bf16_t* dpost = (dpre == n.g0) ? n.g1 : n.g0;
+    if (L.cout == kOutC)
+      NN_LAUNCH((k_conv_bwd_data<kHid, kOutC>), dim3(blocks), dim3(256), g, dpre, L.wt, dpost);
+    else
+      NN_LAUNCH((k_conv_bwd_data<kHid, kHid>), dim3(blocks), dim3(256), g, dpre, L.wt, dpost);
+    // through the ReLU of layer l-1 (its output is act[l])
+    NN_LAUNCH(k_relu_bwd, dim3(nblk(np * kHid)), dim3(256), np * kHid, n.act[l], dpost, dpost);
+    dpre = dpost;
+  }
+  return "";
+}
+
+std::string nn_loss(Nn& n, hipStream_t st, int batch, double* loss_dev) {
+  const long np = (long)batch * n.H * n.W;
+  // masked count: sum of masks over used channels (computed from the input)
+  const float inv = 1.0f / (float)(np * kOutUsed);
+  NN_LAUNCH(k_loss, dim3(nblk(np)), dim3(256), np, n.out, n.tgt, n.act[0], inv, n.act[kLayers], n.dpart);
+  NN_LAUNCH(k_sum_parts, dim3(1), dim3(64), n.dpart, (int)nblk(np), loss_dev);
+  return "";
+}
+
+std::string nn_adam(Nn& n, hipStream_t st, float lr, int t) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float bc1 = 1.f - std::pow(b1, (float)t), bc2 = 1.f - std::pow(b2, (float)t);
+  for (int l = 0; l < kLayers; ++l) {
+    NnLayer& L = n.L[l];
+    NN_LAUNCH(k_adam, dim3(nblk(L.nw())), dim3(256), L.nw(), L.w32, L.mw, L.vw, L.gw, lr, b1, b2, eps, bc1, bc2);
+    NN_LAUNCH(k_adam, dim3(1), dim3(256), (long)L.cout, L.b, L.mb, L.vb, L.gb, lr, b1, b2, eps, bc1, bc2);
+    NN_LAUNCH(k_w_bf16, dim3(nblk(L.nw())), dim3(256), L.cout, L.cin, L.w32, L.w, L.wt);
+  }
+  return "";
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// entry points used by api.hip
+
+std::string nn_create(Ctx& c, int capacity, unsigned long long seed) {
+  if (c.nn) return "a surrogate is already attached to this context";
+  const int H = c.nj - 2, W = c.ni - 2;
+  if (((long)H * W) % 16 != 0) return "the surrogate needs glo_ni*glo_nj to be a multiple of 16";
+  Nn* n = new Nn();
+  c.nn = n;
+  n->H = H;
+  n->W = W;
+  n->ni = c.ni;
+  n->cap = capacity;
+  const long hw = (long)H * W;
+  std::string e;
+  const int dims[kLayers][2] = {{kInC, kHid}, {kHid, kHid}, {kHid, kHid}, {kHid, kHid}, {kHid, kHid}, {kHid, kOutC}};
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l < kLayers; ++l) {
+    NnLayer& L = n->L[l];
+    L.cin = dims[l][0];
+    L.cout = dims[l][1];
+    L.relu = l < kLayers - 1;
+    const long nw = L.nw();
+    float** f32[] = {&L.w32, &L.mw, &L.vw, &L.gw};
+    for (float** p : f32)
+      if (!(e = nn_make(*n, p, (size_t)nw)).empty()) return e;
+    float** fb[] = {&L.b, &L.mb, &L.vb, &L.gb};
+    for (float** p : fb)
+      if (!(e = nn_make(*n, p, (size_t)L.cout)).empty()) return e;
+    if (!(e = nn_make(*n, &L.w, (size_t)nw)).empty()) return e;
+    if (!(e = nn_make(*n, &L.wt, (size_t)nw)).empty()) return e;
+    // He initialisation over the used input channels; padded channels zero
+    const int cin_used = l == 0 ? kInUsed : L.cin;
+    const int cout_used = l == kLayers - 1 ? kOutUsed : L.cout;
+    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * cin_used)) * (l == kLayers - 1 ? 0.1f : 1.f));
+    std::vector<float> hw32((size_t)nw, 0.f);
+    for (int co = 0; co < cout_used; ++co)
+      for (int k = 0; k < 9; ++k)
+        for (int ci = 0; ci < cin_used; ++ci) hw32[((size_t)co * 9 + k) * L.cin + ci] = nd(rng);
+    if (hipMemcpy(L.w32, hw32.data(), nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return "upload surrogate weights";
+  }
+  // static channels: sea masks and Coriolis of the interior
+  std::vector<float> stat((size_t)3 * hw);
+  double cmax = 0.0;
+  for (long q = 0; q < hw; ++q) cmax = std::max(cmax, std::fabs(c.hs.cor[(q % W) + 1 + (size_t)c.ni * (q / W + 1)]));
+  n->norm.cor_scale = (float)(cmax > 0 ? cmax : 1.0);
+  for (long q = 0; q < hw; ++q) {
+    const size_t k = (size_t)(q % W) + 1 + (size_t)c.ni * (q / W + 1);
+    stat[q] = c.hs.lev_t[k] > 0 ? 1.f : 0.f;
+    stat[hw + q] = c.hs.lev_u[k] > 0 ? 1.f : 0.f;
+    stat[2 * hw + q] = (float)(c.hs.cor[k] / n->norm.cor_scale);
+  }
+  if (!(e = nn_make(*n, &n->stat, (size_t)3 * hw)).empty()) return e;
+  if (hipMemcpy(n->stat, stat.data(), stat.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return "upload surrogate statics";
+  if (capacity > 0) {
+    if (!(e = nn_make(*n, &n->store_in, (size_t)capacity * kDyn * hw)).empty()) return e;
+    if (!(e = nn_make(*n, &n->store_tg, (size_t)capacity * kOutUsed * hw)).empty()) return e;
+  }
+  if (!(e = nn_make(*n, &n->inc, (size_t)4 * hw)).empty()) return e;
+  if (!(e = nn_make(*n, &n->misc, (size_t)24)).empty()) return e;
+  if (!(e = nn_make(*n, &n->base, (size_t)4 * hw)).empty()) return e;
+  if (!(e = nn_batch_buffers(*n, 1)).empty()) return e;
+  hipStream_t st = c.stream;
+  for (int l = 0; l < kLayers; ++l)
+    NN_LAUNCH(k_w_bf16, dim3(nblk(n->L[l].nw())), dim3(256), n->L[l].cout, n->L[l].cin, n->L[l].w32, n->L[l].w,
+              n->L[l].wt);
+  if (hipStreamSynchronize(st) != hipSuccess) return "surrogate set-up sync";
+  return "";
+}
+
+void nn_destroy(Ctx& c) {
+  if (!c.nn) return;
+  for (void* p : c.nn->allocs) (void)hipFree(p);
+  delete c.nn;
+  c.nn = nullptr;
+}
+
+void nn_set_record(Ctx& c, bool on) {
+  if (c.nn) c.nn->recording = on && c.nn->cap > 0;
+}
+
+std::string nn_set_use(Ctx& c, int every, double tol) {
+  if (!c.nn) return "no surrogate attached";
+  if (every > 0 && !c.nn->normalised) return "the surrogate is untrained (no normalisation statistics)";
+  c.nn->use_every = every;
+  c.nn->tol = tol;
+  return "";
+}
+
+namespace {
+LiveIn live_inputs(const Ctx& c) {
+  const Fields2& h = c.f2;
+  LiveIn in;
+  const double* f[kDyn] = {h.chc, h.buc, h.bvc, h.but, h.bvt, h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3],
+                           h.bgra[0], h.bgra[1], h.bgra[2], h.bgra[3]};
+  for (int d = 0; d < kDyn; ++d) in.f[d] = f[d];
+  return in;
+}
+
+// the network's increments for the live state into n->inc (mass-corrected)
+std::string nn_predict(Ctx& c) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  Fields2& h = c.f2;
+  const long hw = (long)n->H * n->W;
+  NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c), n->norm, n->stat,
+            n->act[0]);
+  std::string e = nn_forward(*n, st, 1);
+  if (!e.empty()) return e;
+  NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->stat, c.gd.rh[1], n->inc,
+            n->dpart);
+  NN_LAUNCH(k_pred_mean, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc);
+  NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
+  return "";
+}
+}  // namespace
+
+// the barotropic part of one step: plain subcycle, or the network's
+// prediction; records snapshots of plain subcycles when recording.
+std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  Fields2& h = c.f2;
+  const long hw = (long)n->H * n->W;
+  n->step_count += 1;
+  const bool use = n->use_every > 0;
+  const bool check = use && (n->step_count % n->use_every == 0);
+  if (c.timing) (void)hipEventRecord(c.ev[1], st);
+  if (use && !check) {
+    std::string e = nn_predict(c);
+    if (!e.empty()) return e;
+    NN_LAUNCH(k_apply, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->inc, h.chc, h.chp, h.mch, h.mch2, h.acch,
+              h.buc, h.bvc, h.bup, h.bvp);
+    n->surrogate_steps += 1;
+    return "";
+  }
+  const int slot = n->cursor;
+  if (n->recording)
+    NN_LAUNCH(k_rec_in, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c),
+              n->store_in + (long)slot * kDyn * hw);
+  if (check) {
+    // the prediction the network would have made, judged by the plain cycle
+    NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
+    std::string e = nn_predict(c);
+    if (!e.empty()) return e;
+  }
+  std::string e = plain(c);
+  if (!e.empty()) return e;
+  n->plain_steps += 1;
+  if (n->recording) {
+    NN_LAUNCH(k_rec_out, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.mch2, h.mch, h.buc, h.bvc,
+              n->store_in + (long)slot * kDyn * hw, n->store_tg + (long)slot * kOutUsed * hw);
+    n->cursor = (n->cursor + 1) % n->cap;
+    n->nrec = std::min(n->nrec + 1, n->cap);
+  }
+  if (check) {
+    double* err = n->misc + 12;  // 8 values, read back below
+    NN_LAUNCH(k_check, dim3(4), dim3(256), n->ni, n->H, n->W, n->inc, n->base, h.mch2, h.mch, h.buc, h.bvc, n->stat,
+              err);
+    double he[8];
+    if (hipMemcpyAsync(he, err, sizeof he, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return "surrogate check read-back";
+    n->checks += 1;
+    bool bad = false;
+    for (int q = 0; q < 4; ++q) {
+      n->last_persist[q] = he[4 + q];
+      n->last_err[q] = he[q];
+      n->max_err[q] = std::max(n->max_err[q], he[q]);
+      if (!(he[q] <= n->tol)) bad = true;  // NaN counts as bad
+    }
+    if (bad && n->tol > 0.0) {
+      n->fallbacks += 1;
+      n->use_every = 0;  // the plain subcycle takes over
+    }
+  }
+  return "";
+}
+
+// normalisation statistics from the recorded snapshots
+std::string nn_normalise(Ctx& c) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  if (n->nrec < 1) return "no recorded snapshots";
+  const long hw = (long)n->H * n->W;
+  double mom[2 * (kDyn + kOutUsed)];
+  NN_LAUNCH(k_moments, dim3(kDyn), dim3(256), n->store_in, (long)kDyn * hw, hw, n->nrec, kDyn, n->dpart);
+  NN_LAUNCH(k_moments, dim3(kOutUsed), dim3(256), n->store_tg, (long)kOutUsed * hw, hw, n->nrec, kOutUsed,
+            n->dpart + 2 * kDyn);
+  // the null-stream copy does not wait for the context's (non-blocking) stream
+  if (hipStreamSynchronize(st) != hipSuccess ||
+      hipMemcpy(mom, n->dpart, sizeof mom, hipMemcpyDeviceToHost) != hipSuccess)
+    return "moments read-back";
+  const double cnt = (double)n->nrec * hw;
+  for (int q = 0; q < kDyn + kOutUsed; ++q) {
+    const double mean = mom[2 * q] / cnt;
+    double var = mom[2 * q + 1] / cnt - mean * mean;
+    double sd = var > 0 ? std::sqrt(var) : 0.0;
+    if (!(sd > 1e-12)) sd = 1.0;
+    if (q < kDyn) {
+      n->norm.in_mean[q] = (float)mean;
+      n->norm.in_std[q] = (float)sd;
+    } else {
+      n->norm.out_std[q - kDyn] = (float)std::sqrt(mom[2 * q + 1] / cnt > 0 ? mom[2 * q + 1] / cnt : 1.0);
+    }
+  }
+  n->normalised = true;
+  return "";
+}
+
+std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long seed, double* loss) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  if (n->nrec < 1) return "no recorded snapshots to train on";
+  std::string e;
+  if (!n->normalised && !(e = nn_normalise(c)).empty()) return e;
+  if (!(e = nn_batch_buffers(*n, batch)).empty()) return e;
+  const long hw = (long)n->H * n->W;
+  std::mt19937_64 rng(seed);
+  std::uniform_int_distribution<int> pick(0, n->nrec - 1);
+  std::vector<int> idx(batch);
+  double* ldev = n->misc + 8;
+  for (int it = 0; it < iters; ++it) {
+    for (int b = 0; b < batch; ++b) idx[b] = pick(rng);
+    // synchronous: the host vector is refilled next iteration
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(n->idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return "minibatch upload";
+    NN_LAUNCH(k_pack, dim3(nblk(hw * batch)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->idx, batch, n->norm,
+              n->stat, n->act[0], n->tgt);
+    if (!(e = nn_forward(*n, st, batch)).empty()) return e;
+    if (!(e = nn_loss(*n, st, batch, ldev)).empty()) return e;
+    if (!(e = nn_backward(*n, st, batch)).empty()) return e;
+    // cosine decay of the learning rate over this call's iterations
+    const double lr_it = lr * 0.5 * (1.0 + std::cos(3.14159265358979 * it / (double)iters));
+    if (!(e = nn_adam(*n, st, (float)lr_it, ++n->adam_t)).empty()) return e;
+    if (loss) {
+      double l;
+      if (hipMemcpyAsync(&l, ldev, sizeof l, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return "loss read-back";
+      loss[it] = l / (double)((long)batch * hw * kOutUsed);
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return "training sync";
+  return "";
+}
+
+// root-mean-square error over sea of the network and of persistence on
+// recorded snapshots [first, first+count) of the ring, in physical units:
+// out[0..3] network (mch2, mch, ubt, vbt increments), out[4..7] persistence
+std::string nn_eval(Ctx& c, int first, int count, double* res) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  if (!n->normalised) return "the surrogate is untrained";
+  if (first < 0 || count < 1 || first + count > n->nrec) return "snapshot range outside the recorded ones";
+  std::string e;
+  const int batch = 4;
+  if (!(e = nn_batch_buffers(*n, batch)).empty()) return e;
+  const long hw = (long)n->H * n->W;
+  std::vector<double> acc(9, 0.0), part;
+  for (int s0 = first; s0 < first + count; s0 += batch) {
+    const int b = std::min(batch, first + count - s0);
+    std::vector<int> idx(b);
+    for (int q = 0; q < b; ++q) idx[q] = s0 + q;
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(n->idx, idx.data(), b * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return "index upload";
+    NN_LAUNCH(k_pack, dim3(nblk(hw * b)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->idx, b, n->norm,
+              n->stat, n->act[0], n->tgt);
+    if (!(e = nn_forward(*n, st, b)).empty()) return e;
+    const long np = hw * b;
+    const int nb = (int)nblk(np);
+    NN_LAUNCH(k_eval, dim3(nb), dim3(256), np, hw, n->out, n->tgt, n->norm, n->stat, n->dpart);
+    part.resize((size_t)9 * nb);
+    if (hipMemcpyAsync(part.data(), n->dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return "evaluation read-back";
+    for (int i = 0; i < nb; ++i)
+      for (int q = 0; q < 9; ++q) acc[q] += part[(size_t)9 * i + q];
+  }
+  // U-point counts are not kept separately: both use the T-point sea count
+  for (int q = 0; q < 8; ++q) res[q] = acc[8] > 0 ? std::sqrt(acc[q] / acc[8]) : 0.0;
+  return "";
+}
+
+size_t nn_param_count(const Ctx& c) {
+  if (!c.nn) return 0;
+  size_t s = 0;
+  for (int l = 0; l < kLayers; ++l) s += (size_t)c.nn->L[l].nw() + c.nn->L[l].cout;
+  return s + kNormValues;
+}
+
+// parameters as [layer][w (co, 9, ci) then b (co)], FP32 master values
+std::string nn_params(Ctx& c, float* host, bool set) {
+  Nn* n = c.nn;
+  if (hipStreamSynchronize(c.stream) != hipSuccess) return "sync";
+  size_t off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    NnLayer& L = n->L[l];
+    hipError_t e1, e2;
+    if (set) {
+      e1 = hipMemcpy(L.w32, host + off, L.nw() * sizeof(float), hipMemcpyHostToDevice);
+      e2 = hipMemcpy(L.b, host + off + L.nw(), L.cout * sizeof(float), hipMemcpyHostToDevice);
+    } else {
+      e1 = hipMemcpy(host + off, L.w32, L.nw() * sizeof(float), hipMemcpyDeviceToHost);
+      e2 = hipMemcpy(host + off + L.nw(), L.b, L.cout * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    if (e1 != hipSuccess || e2 != hipSuccess) return "parameter copy";
+    off += (size_t)L.nw() + L.cout;
+  }
+  float* nv[] = {n->norm.in_mean, n->norm.in_std, n->norm.out_std};
+  const int nc[] = {kDyn, kDyn, kOutUsed};
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < nc[a]; ++q, ++off) {
+      if (set) nv[a][q] = host[off];
+      else host[off] = nv[a][q];
+    }
+  if (set) n->norm.cor_scale = host[off];
+  else host[off] = n->norm.cor_scale;
+  if (set) {
+    n->normalised = true;
+    hipStream_t st = c.stream;
+    for (int l = 0; l < kLayers; ++l)
+      NN_LAUNCH(k_w_bf16, dim3(nblk(n->L[l].nw())), dim3(256), n->L[l].cout, n->L[l].cin, n->L[l].w32, n->L[l].w,
+                n->L[l].wt);
+    if (hipStreamSynchronize(st) != hipSuccess) return "parameter sync";
+  }
+  return "";
+}
+
+// one forward + backward pass on caller-provided normalised tensors (host
+// FP32): x [batch][H][W][8], target [batch][H][W][4]; returns the network
+// output [batch][H][W][4], the loss and the gradients in parameter order
+std::string nn_forward_backward(Ctx& c, int batch, const float* x, const float* target, float* out, float* grads,
+                                double* loss) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  std::string e;
+  if (!(e = nn_batch_buffers(*n, batch)).empty()) return e;
+  const long np = (long)batch * n->H * n->W;
+  if (hipStreamSynchronize(st) != hipSuccess) return "sync";
+  std::vector<bf16_t> hx((size_t)np * kInC, 0);
+  for (long p = 0; p < np; ++p)
+    for (int ch = 0; ch < kInUsed; ++ch) hx[(size_t)p * kInC + ch] = f2bf(x[p * kInUsed + ch]);
+  if (hipMemcpy(n->act[0], hx.data(), hx.size() * sizeof(bf16_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(n->tgt, target, (size_t)np * kOutUsed * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return "input upload";
+  if (!(e = nn_forward(*n, st, batch)).empty()) return e;
+  double* ldev = n->misc + 8;
+  if (!(e = nn_loss(*n, st, batch, ldev)).empty()) return e;
+  if (!(e = nn_backward(*n, st, batch)).empty()) return e;
+  if (hipStreamSynchronize(st) != hipSuccess) return "sync";
+  std::vector<float> ho((size_t)np * kOutC);
+  if (hipMemcpy(ho.data(), n->out, ho.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return "output";
+  for (long p = 0; p < np; ++p)
+    for (int ch = 0; ch < kOutUsed; ++ch) out[p * kOutUsed + ch] = ho[(size_t)p * kOutC + ch];
+  double l;
+  if (hipMemcpy(&l, ldev, sizeof l, hipMemcpyDeviceToHost) != hipSuccess) return "loss";
+  *loss = l / (double)(np * kOutUsed);
+  size_t off = 0;
+  for (int ll = 0; ll < kLayers; ++ll) {
+    NnLayer& L = n->L[ll];
+    if (hipMemcpy(grads + off, L.gw, L.nw() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(grads + off + L.nw(), L.gb, L.cout * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return "gradient read-back";
+    off += (size_t)L.nw() + L.cout;
+  }
+  for (int q = 0; q < kNormValues; ++q) grads[off + q] = 0.f;
+  return "";
+}
+
+void nn_stats(const Ctx& c, pcom_nn_stats* s) {
+  std::memset(s, 0, sizeof *s);
+  if (!c.nn) return;
+  const Nn* n = c.nn;
+  s->recorded = n->nrec;
+  s->surrogate_steps = n->surrogate_steps;
+  s->plain_steps = n->plain_steps;
+  s->checks = n->checks;
+  s->fallbacks = n->fallbacks;
+  s->in_use = n->use_every > 0 ? 1 : 0;
+  for (int q = 0; q < 4; ++q) {
+    s->last_check_rmse[q] = n->last_err[q];
+    s->last_persist_rmse[q] = n->last_persist[q];
+    s->max_check_rmse[q] = n->max_err[q];
+  }
+}
+
+}  // namespace pcom

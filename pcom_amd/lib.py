@@ -23,6 +23,8 @@ EXPORTS = [
     "pcom_phase_times", "pcom_destroy", "pcom_last_error", "pcom_op_ter", "pcom_op_px1",
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
     "pcom_set_clock", "pcom_output_means",
+    "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
+    "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
 ]
 
 
@@ -64,8 +66,20 @@ def lib():
         L.pcom_get_metric.argtypes = [vp, C.c_int, C.c_int, dp, sz]
         L.pcom_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pcom_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
+        fp = C.POINTER(C.c_float)
+        L.pcom_nn_create.argtypes = [vp, C.c_int, C.c_ulonglong]
+        L.pcom_nn_record.argtypes = [vp, C.c_int]
+        L.pcom_nn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, dp]
+        L.pcom_nn_use.argtypes = [vp, C.c_int, C.c_double]
+        L.pcom_nn_eval.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.pcom_nn_get_stats.argtypes = [vp, C.POINTER(abi.PcomNnStats)]
+        L.pcom_nn_param_count.argtypes = [vp]
+        L.pcom_nn_param_count.restype = sz
+        L.pcom_nn_get_params.argtypes = [vp, fp, sz]
+        L.pcom_nn_set_params.argtypes = [vp, fp, sz]
+        L.pcom_nn_forward_backward.argtypes = [vp, C.c_int, fp, fp, fp, fp, dp]
         for name in EXPORTS:
-            if name not in ("pcom_field_size", "pcom_last_error"):
+            if name not in ("pcom_field_size", "pcom_last_error", "pcom_nn_param_count"):
                 getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -158,6 +172,55 @@ class Model:
             setattr(m, n, _dp(out[n]))
         _check(lib().pcom_output_means(self.h, C.byref(m)))
         return out
+
+    # ---- learned barotropic surrogate
+    def nn_create(self, capacity=0, seed=1):
+        _check(lib().pcom_nn_create(self.h, int(capacity), int(seed)))
+
+    def nn_record(self, on=True):
+        _check(lib().pcom_nn_record(self.h, int(bool(on))))
+
+    def nn_train(self, iters, batch=4, lr=1e-3, seed=1):
+        loss = np.zeros(max(int(iters), 1))
+        _check(lib().pcom_nn_train(self.h, int(iters), int(batch), float(lr), int(seed), _dp(loss)))
+        return loss[:iters]
+
+    def nn_use(self, check_every, tol=0.0):
+        _check(lib().pcom_nn_use(self.h, int(check_every), float(tol)))
+
+    def nn_eval(self, first, count):
+        """(network RMSE[4], persistence RMSE[4]) on recorded snapshots."""
+        r = np.zeros(8)
+        _check(lib().pcom_nn_eval(self.h, int(first), int(count), _dp(r)))
+        return r[:4], r[4:]
+
+    def nn_stats(self):
+        st = abi.PcomNnStats()
+        _check(lib().pcom_nn_get_stats(self.h, C.byref(st)))
+        return st
+
+    def nn_params(self, values=None):
+        n = lib().pcom_nn_param_count(self.h)
+        fp = C.POINTER(C.c_float)
+        if values is None:
+            out = np.empty(n, np.float32)
+            _check(lib().pcom_nn_get_params(self.h, out.ctypes.data_as(fp), n))
+            return out
+        v = np.ascontiguousarray(values, np.float32)
+        if v.size != n:
+            raise ValueError(f"{v.size} parameters given, the network has {n}")
+        _check(lib().pcom_nn_set_params(self.h, v.ctypes.data_as(fp), n))
+
+    def nn_forward_backward(self, x, target):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        out = np.empty(t.shape, np.float32)
+        grads = np.empty(lib().pcom_nn_param_count(self.h), np.float32)
+        loss = C.c_double()
+        _check(lib().pcom_nn_forward_backward(self.h, x.shape[0], x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                              out.ctypes.data_as(fp), grads.ctypes.data_as(fp), C.byref(loss)))
+        return loss.value, out, grads
 
     def global_sums(self):
         out = np.zeros(3)

@@ -1,0 +1,144 @@
+"""Learned barotropic surrogate: the CPU network restatement pinned by a
+brute-force convolution, BF16 rounding cases and finite differences; the
+HIP forward/backward (BF16 MFMA) against it; training and use in the time
+stepper, judged by the plain subcycle."""
+import numpy as np
+import pytest
+
+from oracle import surrogate_ref as ref
+from pcom_amd import configs
+
+
+def _random_params(rng, scale=1.0):
+    parts = []
+    for ci, co in ref.DIMS:
+        used_ci = ref.IN_USED if ci == ref.IN_C and len(parts) == 0 else ci
+        w = np.zeros((co, 3, 3, ci))
+        w[:, :, :, :used_ci] = rng.standard_normal((co, 3, 3, used_ci)) * scale * np.sqrt(2.0 / (9 * used_ci))
+        parts += [w.ravel(), 0.1 * rng.standard_normal(co)]
+    parts.append(np.ones(ref.N_NORM))
+    return np.concatenate(parts)
+
+
+def test_conv_matches_brute_force():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((2, 3, 5, 4))
+    w = rng.standard_normal((3, 3, 3, 4))
+    b = rng.standard_normal(3)
+    out = ref.conv(x, w, b)
+    bf = np.zeros_like(out)
+    for n in range(2):
+        for y in range(3):
+            for xx in range(5):
+                for co in range(3):
+                    s = b[co]
+                    for dy in range(3):
+                        for dx in range(3):
+                            yy = y + dy - 1
+                            if 0 <= yy < 3:
+                                s += np.dot(w[co, dy, dx], x[n, yy, (xx + dx - 1) % 5])
+                    bf[n, y, xx, co] = s
+    np.testing.assert_allclose(out, bf, rtol=1e-12, atol=1e-12)
+
+
+def test_bf16_rounding_cases():
+    v = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, -2.5, 3.0e-3, 65504.0], np.float32)
+    got = ref.bf16(v)
+    np.testing.assert_array_equal(got, np.array([1.0, 1.0, 1 + 2 ** -6, -2.5, 0.0030059814453125, 65536.0],
+                                                np.float32))
+
+
+def test_gradients_match_finite_differences():
+    """The float64 backward pass against central differences of the loss."""
+    rng = np.random.default_rng(3)
+    p = _random_params(rng)
+    x = rng.standard_normal((1, 3, 4, ref.IN_USED))
+    x[..., 5] = (rng.uniform(size=(1, 3, 4)) > 0.3)
+    x[..., 6] = (rng.uniform(size=(1, 3, 4)) > 0.3)
+    t = rng.standard_normal((1, 3, 4, ref.OUT_USED))
+    loss, _, g = ref.loss_and_grads(p, x, t, rnd=False)
+    off = 0
+    checked = 0
+    for ci, co in ref.DIMS:
+        nw = 9 * ci * co
+        cand = [off + int(k) for k in rng.choice(nw, 40, replace=False)] + [off + nw + int(rng.integers(co))]
+        for k in cand:
+            if p[k] == 0 and g[k] == 0:
+                continue
+            h = 1e-6 * max(1.0, abs(p[k]))
+            pp, pm = p.copy(), p.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (ref.loss_and_grads(pp, x, t, rnd=False)[0] - ref.loss_and_grads(pm, x, t, rnd=False)[0]) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-6 * max(abs(fd), 1e-3 * loss), (k, fd, g[k])
+            checked += 1
+        off += nw + co
+    assert checked > 100
+
+
+def _inputs(cfg, rng, batch):
+    m_h, m_w = cfg.glo_nj, cfg.glo_ni
+    x = rng.standard_normal((batch, m_h, m_w, ref.IN_USED)).astype(np.float32)
+    sea_t = (cfg.itn.T > 0).astype(np.float32)
+    x[..., 5] = sea_t
+    x[..., 6] = sea_t * np.roll(sea_t, -1, axis=1)
+    x[..., 7] = np.linspace(0.3, 1.0, m_h)[None, :, None]
+    t = rng.standard_normal((batch, m_h, m_w, ref.OUT_USED)).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.gpu
+def test_gpu_forward_backward_matches_oracle(gpu_lib):
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    m = Model(cfg)
+    m.nn_create(capacity=0, seed=5)
+    rng = np.random.default_rng(11)
+    p = _random_params(rng).astype(np.float32)
+    m.nn_params(p)
+    x, t = _inputs(cfg, rng, 2)
+    loss, out, g = m.nn_forward_backward(x, t)
+    rl, rout, rg = ref.loss_and_grads(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rnd=True)
+    # BF16 activations rounded from FP32 (device) or float64 (here) sums
+    # differ by one BF16 ulp now and then; tolerances are set by that
+    assert abs(loss - rl) <= 2e-3 * rl
+    assert np.sqrt(np.mean((out - rout) ** 2)) <= 1e-2 * np.sqrt(np.mean(rout ** 2))
+    off = 0
+    for li, (ci, co) in enumerate(ref.DIMS):
+        n = 9 * ci * co + co
+        gg, rr = g[off:off + n], rg[off:off + n]
+        assert np.linalg.norm(gg - rr) <= 3e-2 * np.linalg.norm(rr), f"layer {li}"
+        off += n
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
+    """Snapshots from plain box steps, a short training run, then steps
+    with the network judged by the plain subcycle every 4th step."""
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    m = Model(cfg)
+    m.nn_create(capacity=96, seed=2)
+    m.nn_record(True)
+    m.step(96)
+    m.nn_record(False)
+    assert m.nn_stats().recorded == 96
+    loss = m.nn_train(3000, batch=4, lr=2e-3, seed=3)
+    assert np.all(np.isfinite(loss))
+    assert loss[-100:].mean() < 0.2 * loss[:10].mean()
+    plain = Model(cfg)
+    plain.step(96)
+    m.nn_use(4)
+    m.step(16)
+    plain.step(16)
+    st = m.nn_stats()
+    assert (st.surrogate_steps, st.checks) == (12, 4)
+    assert m.check_finite()
+    err = np.array(st.last_check_rmse[:])
+    per = np.array(st.last_persist_rmse[:])
+    assert np.all(np.isfinite(err))
+    # the network's free surface and transport beat "no change over the step"
+    assert err[0] < per[0] and err[2] < per[2] and err[3] < per[3], (err, per)
+    # mass: the surrogate removes its mean free-surface increment
+    s_nn, s_pl = m.global_sums()[2], plain.global_sums()[2]
+    assert abs(s_nn - s_pl) <= 1e-9 * abs(s_pl)
