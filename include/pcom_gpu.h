@@ -267,8 +267,10 @@ int pcom_nn_record(pcom_ctx *ctx, int on);
  * be NULL.                                                               */
 int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long long seed, double *loss);
 /* Use the network in pcom_step with a plain, judging subcycle every
- * check_every steps (0 turns the surrogate off); tol bounds the sea RMSE of
- * each checked output (0 = never fall back).                             */
+ * check_every steps (0 turns the surrogate off).  At each check, tol > 0
+ * bounds the sea RMSE of every output, tol < 0 bounds it by -tol times the
+ * RMSE of persistence (no change over the step), tol = 0 never falls back;
+ * a failed check turns the surrogate off.                                */
 int pcom_nn_use(pcom_ctx *ctx, int check_every, double tol);
 /* Root-mean-square error over sea points of the network on recorded
  * snapshots [first, first+count) (in recording order), in physical units:

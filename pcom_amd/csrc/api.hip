@@ -562,7 +562,7 @@ int pcom_nn_train(pcom_ctx* ctx, int iters, int batch, double lr, unsigned long 
 }
 
 int pcom_nn_use(pcom_ctx* ctx, int check_every, double tol) {
-  if (!ctx || check_every < 0 || tol < 0) return fail(PCOM_ERR_ARG, "pcom_nn_use: bad argument");
+  if (!ctx || check_every < 0 || !(tol == tol)) return fail(PCOM_ERR_ARG, "pcom_nn_use: bad argument");
   Ctx* c = C(ctx);
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_use: no surrogate attached");
   std::string e = nn_set_use(*c, check_every, tol);

@@ -961,9 +961,12 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
       n->last_persist[q] = he[4 + q];
       n->last_err[q] = he[q];
       n->max_err[q] = std::max(n->max_err[q], he[q]);
-      if (!(he[q] <= n->tol)) bad = true;  // NaN counts as bad
+      // tol > 0 bounds the error, tol < 0 bounds it relative to persistence;
+      // NaN counts as a failed check
+      const double bound = n->tol > 0.0 ? n->tol : -n->tol * he[4 + q];
+      if (!(he[q] <= bound)) bad = true;
     }
-    if (bad && n->tol > 0.0) {
+    if (bad && n->tol != 0.0) {
       n->fallbacks += 1;
       n->use_every = 0;  // the plain subcycle takes over
     }

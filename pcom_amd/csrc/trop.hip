@@ -14,6 +14,8 @@
 // statements, so the result does not depend on the tiling.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "context.h"
@@ -528,6 +530,16 @@ std::string trop_configure(Ctx& c) {
   return "";
 }
 
+// process-wide order of subcycle launches per device (never destroyed: the
+// events outlive every context)
+struct TropChain {
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int last = 0;
+  bool used = false;
+};
+std::mutex g_trop_mutex;
+std::map<int, TropChain> g_trop_chain;
+
 template <typename R>
 std::string launch_trop_t(Ctx& c) {
   Fields2& h = c.f2;
@@ -583,13 +595,31 @@ std::string launch_trop_t(Ctx& c) {
   // An ordinary launch: trop_configure keeps the grid within the device's
   // co-resident capacity (one workgroup per CU by its LDS), so every tile
   // a wait depends on gets a CU once other work on the device drains, and
-  // every wait is bounded.  A cooperative launch would add nothing but its
-  // per-process global-wave-sync resource, which fails when another
-  // process on the device holds it.
+  // every wait is bounded.  Two such launches running at once could each
+  // hold CUs the other's tiles need, so launches of all contexts of this
+  // process on a device are chained: each waits for the previous one.  (A
+  // cooperative launch would add nothing but its per-process global-wave-
+  // sync resource, which fails when another process on the device holds it.)
+  std::lock_guard<std::mutex> lock(g_trop_mutex);
+  TropChain& ch = g_trop_chain[c.device];
+  if (!ch.ev[0]) {
+    for (hipEvent_t& ev : ch.ev) {
+      hipError_t ee = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (ee != hipSuccess) return std::string("hipEventCreate: ") + hipGetErrorString(ee);
+    }
+  }
+  if (ch.used) {
+    hipError_t ew = hipStreamWaitEvent(c.stream, ch.ev[ch.last], 0);
+    if (ew != hipSuccess) return std::string("hipStreamWaitEvent: ") + hipGetErrorString(ew);
+  }
   if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
   hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
+  ch.last = 1 - ch.last;
+  e = hipEventRecord(ch.ev[ch.last], c.stream);
+  if (e != hipSuccess) return std::string("hipEventRecord: ") + hipGetErrorString(e);
+  ch.used = true;
   return "";
 }
 

@@ -7,7 +7,12 @@ mask from lev; the U-grid fields use the U mask).  The FP64 form is
 bit-identical to the FP64 oracle (tests/test_gpu_parity.py), so it stands in
 for the oracle over runs the CPU cannot finish.
 
-    python -m pcom_amd.rmse --days 365 --every 30 [--forms mixed,mixed_eos32]
+    python -m pcom_amd.rmse --days 365 --every 30 [--forms mixed,mixed+surrogate]
+
+A form "<form>+surrogate" records the first --train-days of plain
+subcycles, trains the learned barotropic surrogate on them (pcom_amd.
+surrogate) and then runs with it, the plain subcycle judging every
+--check-every-th step.
 """
 import argparse
 import json
@@ -63,26 +68,58 @@ def main(argv=None):
     ap.add_argument("--config", default="global", choices=list(configs.CONFIGS))
     ap.add_argument("--days", type=int, default=365)
     ap.add_argument("--every", type=int, default=30, help="report every N days")
-    ap.add_argument("--forms", default="mixed", help="comma-separated mixed forms to compare with FP64")
+    ap.add_argument("--forms", default="mixed", help="comma-separated forms to compare with FP64")
+    ap.add_argument("--train-days", type=int, default=20, help="surrogate forms: days recorded for training")
+    ap.add_argument("--iters", type=int, default=9000, help="surrogate forms: training iterations")
+    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--tol", type=float, default=0.0,
+                    help="surrogate fallback bound (see pcom_nn_use; <0: relative to persistence)")
     ap.add_argument("--out", default=None, help="write the JSON report here")
     args = ap.parse_args(argv)
     cfg = configs.CONFIGS[args.config]()
     forms = args.forms.split(",")
     ref = Model(cfg, precision=abi.PCOM_FP64)
-    mixed = {f: Model(cfg, precision=abi.FORMS[f]) for f in forms}
+    models = {}
     steps_per_day = 86400 // cfg.dtbc
-    report = {"config": cfg.name, "dims": [cfg.glo_ni, cfg.glo_nj, cfg.nk], "records": {f: [] for f in forms}}
+    for f in forms:
+        base = f.split("+")[0]
+        models[f] = Model(cfg, precision=abi.FORMS[base])
+        if f.endswith("+surrogate"):
+            models[f].nn_create(capacity=args.train_days * steps_per_day, seed=1)
+            models[f].nn_record(True)
+    report = {"config": cfg.name, "dims": [cfg.glo_ni, cfg.glo_nj, cfg.nk], "records": {f: [] for f in forms},
+              "surrogate": {}}
     t0 = time.time()
     day = 0
     while day < args.days:
         n = min(args.every, args.days - day)
+        if day < args.train_days:
+            n = min(n, args.train_days - day)
         ref.step(n * steps_per_day)
-        for f, m in mixed.items():
+        for m in models.values():
             m.step(n * steps_per_day)
         day += n
-        for f, m in mixed.items():
+        if day == args.train_days:
+            for f, m in models.items():
+                if f.endswith("+surrogate"):
+                    m.nn_record(False)
+                    for chunk in range(0, args.iters, 1000):
+                        it = min(1000, args.iters - chunk)
+                        lr = 1e-3 if chunk < args.iters // 3 else 5e-4
+                        loss = m.nn_train(it, batch=8, lr=lr, seed=2 + chunk)
+                        print(json.dumps({"form": f, "training_iters": chunk + it, "loss": float(loss[-50:].mean())}),
+                              flush=True)
+                    m.nn_use(args.check_every, args.tol)
+        for f, m in models.items():
             rec = {"form": f, "day": day, "finite": bool(ref.check_finite() and m.check_finite())}
             rec.update(rmse_report(m, ref, cfg))
+            if f.endswith("+surrogate"):
+                st = m.nn_stats()
+                rec["surrogate_steps"] = st.surrogate_steps
+                rec["in_use"] = st.in_use
+                rec["fallbacks"] = st.fallbacks
+                rec["check_rmse"] = list(st.last_check_rmse)
+                rec["persist_rmse"] = list(st.last_persist_rmse)
             report["records"][f].append(rec)
             print(json.dumps(rec), flush=True)
     report["wall_s"] = time.time() - t0

@@ -1,0 +1,112 @@
+"""The learned barotropic surrogate workflow on one configuration.
+
+1. spin the model up for `spin_days` with the plain subcycle;
+2. record `snapshots` consecutive steps (inputs and outputs of the plain
+   subcycle) into the device store;
+3. train the BF16 network on them (pcom_nn_train);
+4. record `holdout` further steps and measure, on those, the network's
+   RMSE against the plain subcycle beside the RMSE of persistence (the
+   step without the network: no change of the barotropic mode);
+5. time plain steps and surrogate steps (plain judging subcycle every
+   `check_every`-th step).
+
+    python -m pcom_amd.surrogate --config global --spin-days 10 \\
+        --snapshots 480 --iters 9000 --out report.json
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import abi, configs
+from .lib import Model
+
+OUTPUTS = ["mch2", "mch", "ubt", "vbt"]
+
+
+def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=48, iters=9000, batch=8,
+            lr=1e-3, seed=1, log=print):
+    """Spin up, record, train and evaluate; returns (model, report)."""
+    spd = 86400 // cfg.dtbc
+    m = Model(cfg, precision=precision)
+    m.step(spin_days * spd)
+    m.nn_create(capacity=snapshots + holdout, seed=seed)
+    m.nn_record(True)
+    m.step(snapshots)
+    m.nn_record(False)
+    t0 = time.time()
+    parts = []
+    for chunk in range(0, iters, 1000):
+        it = min(1000, iters - chunk)
+        parts.append(m.nn_train(it, batch=batch, lr=lr if chunk < iters // 3 else lr / 2, seed=seed + 1 + chunk))
+        log(f"  {chunk + it} iterations, loss {parts[-1][-50:].mean():.3e}")
+    loss = np.concatenate(parts)
+    train_s = time.time() - t0
+    log(f"trained {iters} iterations in {train_s:.1f} s, loss {loss[:20].mean():.3e} -> {loss[-100:].mean():.3e}")
+    m.nn_record(True)
+    m.step(holdout)
+    m.nn_record(False)
+    net, per = m.nn_eval(snapshots, holdout)
+    report = {
+        "config": cfg.name, "spin_days": spin_days, "snapshots": snapshots, "holdout": holdout,
+        "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
+        "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-100:].mean()),
+        "holdout_rmse_network": dict(zip(OUTPUTS, map(float, net))),
+        "holdout_rmse_persistence": dict(zip(OUTPUTS, map(float, per))),
+        "holdout_gain": dict(zip(OUTPUTS, map(float, per / np.maximum(net, 1e-300)))),
+    }
+    log("held-out gain over persistence: " + ", ".join(f"{k} {v:.3f}" for k, v in report["holdout_gain"].items()))
+    return m, report
+
+
+def time_steps(m, nsteps, check_every):
+    """ms per step of plain steps, then of steps with the surrogate."""
+    m.nn_use(0)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(nsteps)
+    m.sync()
+    plain = 1000.0 * (time.perf_counter() - t0) / nsteps
+    m.nn_use(check_every)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(nsteps)
+    m.sync()
+    sur = 1000.0 * (time.perf_counter() - t0) / nsteps
+    return plain, sur
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--config", default="global", choices=list(configs.CONFIGS))
+    ap.add_argument("--form", default="mixed", choices=list(abi.FORMS))
+    ap.add_argument("--spin-days", type=int, default=10)
+    ap.add_argument("--snapshots", type=int, default=480)
+    ap.add_argument("--holdout", type=int, default=48)
+    ap.add_argument("--iters", type=int, default=9000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--time-steps", type=int, default=240)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    cfg = configs.CONFIGS[args.config]()
+    m, rep = prepare(cfg, abi.FORMS[args.form], args.spin_days, args.snapshots, args.holdout, args.iters,
+                     args.batch, args.lr)
+    plain, sur = time_steps(m, args.time_steps, args.check_every)
+    st = m.nn_stats()
+    rep.update({"form": args.form, "check_every": args.check_every, "ms_per_step_plain": plain,
+                "ms_per_step_surrogate": sur, "speedup": plain / sur,
+                "last_check_rmse": list(st.last_check_rmse), "last_persist_rmse": list(st.last_persist_rmse),
+                "finite": bool(m.check_finite())})
+    print(json.dumps(rep), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(rep, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

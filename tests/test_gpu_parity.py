@@ -164,3 +164,19 @@ def test_global_1deg_bitwise_across_a_day(gpu_lib):
     bad = _compare(m, o, ["tc", "sc", "uc", "vc", "chc", "mch", "buc", "bvc", "am", "ssh", "acch", "act",
                           "acu", "acw", "acr", "wm", "fx", "auc", "bphi_xn", "bgra_yw"])
     assert not bad, "\n".join(bad)
+
+
+def test_two_contexts_share_the_device(gpu_lib):
+    """Two 1-degree contexts stepping on their own streams at once (each
+    subcycle launch needs most of the device's CUs; launches are chained
+    across contexts) give the same bits as one context alone."""
+    cfg = configs.global_1deg()
+    a, b = Model(cfg), Model(cfg, precision=abi.PCOM_MIXED)
+    for _ in range(12):
+        a.step(1)
+        b.step(1)
+    solo = Model(cfg)
+    solo.step(12)
+    bad = _compare(a, solo, ["tc", "uc", "chc", "buc", "mch"])
+    assert not bad, "\n".join(bad)
+    assert b.check_finite()
