@@ -33,10 +33,10 @@ YEAR_S = 31536000.0
 # sch and gradients 23, momentum with pressure gradient, Coriolis and
 # corrector fluxes 104, corrector div, ch and accumulators 26.
 TROP_FLOPS_PER_POINT = 184
-# HBM bytes per k_trop_tiles launch at 1 degree from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a plain 1 GiB copy
-# (profiles/k_trop_tiles_global1deg_pmc.md); None until measured.
-TROP_TRAFFIC_BYTES = None
+# HBM bytes per k_trop_tiles launch at 1 degree: rocprofv3 --pmc FETCH_SIZE
+# (x 2.000) + WRITE_SIZE (x 1.000), the factors being what the counters read
+# for a plain 1 GiB copy (profiles/k_trop_tiles_global1deg_pmc.md)
+TROP_TRAFFIC_BYTES = 518_920_000
 
 
 def sypd(ms_per_step, dtbc):
@@ -52,12 +52,22 @@ def step_bytes(cfg, s3):
     return n3 * 31 * s3 + n2 * 50 * 8
 
 
-def run_form(cfg, precision, steps, warmup):
-    import ctypes as C
-
+def run_form(cfg, precision, steps, warmup, surrogate=False, check_every=10):
     from pcom_amd.lib import Model
     m = Model(cfg, precision=precision)
     m.step(warmup)
+    sur = None
+    if surrogate:
+        # a network trained briefly on 48 recorded subcycles: throughput does
+        # not depend on the weights (accuracy: python -m pcom_amd.surrogate)
+        m.nn_create(capacity=48, seed=1)
+        m.nn_record(True)
+        m.step(48)
+        m.nn_record(False)
+        t0 = time.perf_counter()
+        m.nn_train(200, batch=8, lr=1e-3, seed=2)
+        sur = {"train_s_200_iters": round(time.perf_counter() - t0, 2), "check_every": check_every}
+        m.nn_use(check_every)
     m.sync()
     t0 = time.perf_counter()
     m.step(steps)
@@ -70,10 +80,15 @@ def run_form(cfg, precision, steps, warmup):
     ph, n = m.phase_times(False)
     ok = m.check_finite()
     sums = m.global_sums()
+    out = {"ms_per_step": 1000.0 * wall / steps,
+           "phase_ms": {"step": ph[0], "other_kernels": ph[1], "subcycle_kernel": ph[2]},
+           "phase_steps": n, "finite": bool(ok), "heat_salt_mass": [float(x) for x in sums]}
+    if sur is not None:
+        st = m.nn_stats()
+        sur.update({"surrogate_steps": st.surrogate_steps, "plain_steps": st.plain_steps})
+        out["surrogate"] = sur
     m.close()
-    return {"ms_per_step": 1000.0 * wall / steps,
-            "phase_ms": {"step": ph[0], "other_kernels": ph[1], "subcycle_kernel": ph[2]},
-            "phase_steps": n, "finite": bool(ok), "heat_salt_mass": [float(x) for x in sums]}
+    return out
 
 
 def calibrate_copy():
@@ -107,10 +122,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32"], default="fp64",
+    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32", "mixed+surrogate"], default="fp64",
                     help="form reported as `value`; the others are measured alongside")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--config", default="global", choices=["global", "box"])
+    ap.add_argument("--forms", default="fp64,mixed,mixed_eos32,mixed+surrogate",
+                    help="forms to measure (comma-separated; must include --form)")
     ap.add_argument("--calibrate-copy", action="store_true", help=calibrate_copy.__doc__.split(":")[0])
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -124,14 +141,17 @@ def main():
 
     from pcom_amd import abi, configs
     cfg = configs.CONFIGS[args.config]()
+    wanted = args.forms.split(",")
+    if args.form not in wanted:
+        wanted.insert(0, args.form)
     forms = {}
-    for name, prec in abi.FORMS.items():
-        forms[name] = run_form(cfg, prec, args.steps, args.warmup)
+    for name in wanted:
+        base, sur = name.split("+")[0], name.endswith("+surrogate")
+        forms[name] = run_form(cfg, abi.FORMS[base], args.steps, args.warmup, surrogate=sur)
         forms[name]["sypd"] = round(sypd(forms[name]["ms_per_step"], cfg.dtbc), 2)
     rep = forms[args.form]
-    s3 = 8 if args.form == "fp64" else 4  # the 31 compulsory 3-D transfers; mixed keeps T,S,u,v in FP64
-    if args.form != "fp64":
-        s3 = (8 * 8 + 23 * 4) / 31
+    # the 31 compulsory 3-D transfers; the mixed forms keep T, S, u, v in FP64
+    s3 = 8 if args.form == "fp64" else (8 * 8 + 23 * 4) / 31
     b = step_bytes(cfg, s3)
     step_gbs = b / (rep["ms_per_step"] / 1000.0) / 1e9
     nstep = 2 * cfg.dtbc // cfg.dtbt
@@ -149,15 +169,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32 stencils + f64 state",
+        "dtype": "f64" if args.form == "fp64" else "f32 stencils + f64 state"
+                 + (" + bf16 surrogate" if "surrogate" in args.form else ""),
         "data": "synthetic: seeded analytic quasi-global configuration (no input files)",
         "config": {"workload": f"{cfg.name} {cfg.glo_ni}x{cfg.glo_nj}x{cfg.nk}, dtbt={cfg.dtbt} s, "
                                f"dtbc={cfg.dtbc} s, {nstep} barotropic substeps/step",
                    "form": args.form, "parallelism": "single GPU, whole domain"},
         "forms": forms,
+        "accuracy": "1-year SST/SSH RMSE of each form against FP64: profiles/rmse_1yr_global1deg.md",
         "roofline": {"bound": "mfma", "kernel": "k_trop_tiles (barotropic subcycle, FP64)",
                      "achieved": round(trop_tflops, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5), "traffic": TROP_TRAFFIC_BYTES,
+                     "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5), "traffic": TROP_TRAFFIC_BYTES if cfg.name == "global" else None,
                      "flops_per_launch": flops, "launch_ms": round(trop_ms, 4),
                      "definition": f"{TROP_FLOPS_PER_POINT} FP64 flops per point per substep x "
                                    f"{cfg.glo_ni}x{cfg.glo_nj} points x {nstep} substeps / event-timed "

@@ -12,7 +12,10 @@ for the oracle over runs the CPU cannot finish.
 A form "<form>+surrogate" records the first --train-days of plain
 subcycles, trains the learned barotropic surrogate on them (pcom_amd.
 surrogate) and then runs with it, the plain subcycle judging every
---check-every-th step.
+--check-every-th step.  The form "fp64+f32init" is the FP64 form started
+from the initial T and S rounded to FP32: its distance from FP64 is the
+model's own sensitivity to perturbations at FP32 rounding level, the floor
+any non-bit-identical form meets.
 """
 import argparse
 import json
@@ -84,6 +87,9 @@ def main(argv=None):
     for f in forms:
         base = f.split("+")[0]
         models[f] = Model(cfg, precision=abi.FORMS[base])
+        if f.endswith("+f32init"):
+            for name in ("tc", "sc"):
+                models[f].set(name, models[f].get(name).astype(np.float32).astype(np.float64))
         if f.endswith("+surrogate"):
             models[f].nn_create(capacity=args.train_days * steps_per_day, seed=1)
             models[f].nn_record(True)
