@@ -232,6 +232,29 @@ int pcom_op_lap(pcom_ctx *ctx, int ga, int gb, const double *in, double *out);
 int pcom_den_rho(pcom_ctx *ctx, size_t n, const double *t, const double *s,
                  const double *p, double *rho);
 
+/* ---- Ensembles and the subcycle autotuner.  New: the reference runs one
+ * ocean per (MPI) job.
+ *
+ * pcom_ensemble_step advances every member by nsteps steps with the same
+ * results as pcom_step on each.  Each member's 3-D kernels run on the
+ * member's own stream; the members' barotropic subcycles run in packed
+ * launches, as many members per launch as the device holds co-resident
+ * (first fit by tile count), so small members fill the device together.
+ * Members may differ in grid size and form and must outlive the ensemble;
+ * a member with a surrogate in use runs its own barotropic part.        */
+typedef struct pcom_ensemble pcom_ensemble;
+int pcom_ensemble_create(pcom_ctx *const *members, int count, pcom_ensemble **out);
+int pcom_ensemble_step(pcom_ensemble *ens, int nsteps);
+/* Packing of the last pcom_ensemble_step: number of launches and the
+ * launch of each member (-1: the member runs alone).                    */
+int pcom_ensemble_packs(pcom_ensemble *ens, int *npacks, int *pack_of_member);
+int pcom_ensemble_destroy(pcom_ensemble *ens);
+/* Autotuner: time the subcycle on this context's state under the `trials`
+ * tilings (tile i, tile j, substeps per exchange) a cost model ranks best,
+ * restoring the state after each, and keep the fastest.  Returns the chosen
+ * tiling and its time per launch.  The state is unchanged.               */
+int pcom_tune_subcycle(pcom_ctx *ctx, int trials, int tile[3], double *ms);
+
 /* ---- Learned barotropic surrogate.  New: the reference has no such part.
  * A BF16 convolutional network (six 3x3 layers, 64 hidden channels, FP32
  * accumulation on the matrix cores) predicts what int_trop returns

@@ -358,11 +358,17 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   c->trop_tw = opts->trop_tile_i;
   c->trop_th = opts->trop_tile_j;
   c->trop_s = opts->trop_substeps;
+  {
+    // exchange flags for as many tiles as the device can hold co-resident
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+    c->trop_flag_cap = (long)prop.multiProcessorCount * 8;
+  }
   std::string te = trop_configure(*c);
   if (!te.empty()) return fail(PCOM_ERR_HIP, te);
   {
     void* p;
-    size_t nflag = (size_t)c->trop_blocks * 32 + 32;
+    size_t nflag = (size_t)c->trop_flag_cap * 32 + 32;
     if ((rc = dev_alloc(*c, &p, nflag, sizeof(unsigned long long)))) return rc;
     HIPCHK(hipMemset(p, 0, nflag * sizeof(unsigned long long)), "zero tile flags");
     c->trop_flags = (unsigned long long*)p;
@@ -528,6 +534,50 @@ int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
   }
   std::string e = launch_output_means(*c, out, c->stage);
   if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_output_means: " + e);
+  return PCOM_OK;
+}
+
+int pcom_ensemble_create(pcom_ctx* const* members, int count, pcom_ensemble** out) {
+  if (!members || count < 1 || !out) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: bad argument");
+  std::vector<Ctx*> m;
+  for (int q = 0; q < count; ++q) {
+    if (!members[q]) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: NULL member");
+    for (Ctx* x : m)
+      if (x == C(members[q])) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: a member is listed twice");
+    m.push_back(C(members[q]));
+  }
+  Ensemble* en = nullptr;
+  std::string e = ensemble_create(m, &en);
+  if (!e.empty()) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: " + e);
+  *out = reinterpret_cast<pcom_ensemble*>(en);
+  return PCOM_OK;
+}
+
+int pcom_ensemble_step(pcom_ensemble* ens, int nsteps) {
+  if (!ens || nsteps < 0) return fail(PCOM_ERR_ARG, "pcom_ensemble_step: bad argument");
+  std::string e = ensemble_step(*reinterpret_cast<Ensemble*>(ens), nsteps);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_ensemble_step: " + e);
+  return PCOM_OK;
+}
+
+int pcom_ensemble_packs(pcom_ensemble* ens, int* npacks, int* pack_of_member) {
+  if (!ens || !npacks || !pack_of_member) return fail(PCOM_ERR_ARG, "pcom_ensemble_packs: NULL argument");
+  ensemble_packs(*reinterpret_cast<Ensemble*>(ens), npacks, pack_of_member);
+  return PCOM_OK;
+}
+
+int pcom_ensemble_destroy(pcom_ensemble* ens) {
+  ensemble_destroy(reinterpret_cast<Ensemble*>(ens));
+  return PCOM_OK;
+}
+
+int pcom_tune_subcycle(pcom_ctx* ctx, int trials, int tile[3], double* ms) {
+  if (!ctx || trials < 1 || !tile || !ms) return fail(PCOM_ERR_ARG, "pcom_tune_subcycle: bad argument");
+  Ctx* c = C(ctx);
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  if (int rc = check_async_error(*c)) return rc;
+  std::string e = tune_subcycle(*c, trials, tile, ms);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_tune_subcycle: " + e);
   return PCOM_OK;
 }
 

@@ -11,7 +11,8 @@
 
 namespace pcom {
 
-struct Nn;  // surrogate.hip: the learned barotropic surrogate
+struct Nn;        // surrogate.hip: the learned barotropic surrogate
+struct Ensemble;  // trop.hip: contexts stepped together
 
 // 3-D fields are stored in R (double, or float in the mixed form); 2-D
 // fields (free surface, barotropic mode, forcing) are always double.
@@ -63,6 +64,7 @@ struct Ctx {
   int trop_tiles_x = 0, trop_blocks = 0;
   size_t trop_lds = 0;
   unsigned long long* trop_flags = nullptr;  // per-tile exchange flags (zeroed at create)
+  long trop_flag_cap = 0;                    // tiles the flag array holds
   int* trop_error = nullptr;                 // set by a timed-out neighbour wait
   unsigned long long trop_epoch = 0;
   // phase timing (HIP events on `stream`)
@@ -77,6 +79,8 @@ struct Ctx {
 
 // kernels.hip
 std::string launch_step(Ctx& c);                 // one baroclinic step
+std::string launch_step_pre(Ctx& c);             // up to int_trop
+std::string launch_step_post(Ctx& c);            // after int_trop
 std::string launch_global_sums(Ctx& c, double out[3]);
 std::string launch_check_finite(Ctx& c, int* bad);
 // time means into `stage` (>= glo_ni*glo_nj*nk + 2 doubles), copied to the
@@ -84,6 +88,12 @@ std::string launch_check_finite(Ctx& c, int* bad);
 std::string launch_output_means(Ctx& c, pcom_means* o, double* stage);
 std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
 std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
+std::string launch_vz(Ctx& c);                   // trop.hip: the subcycle's set-up kernel
+std::string ensemble_create(const std::vector<Ctx*>& members, Ensemble** out);
+std::string ensemble_step(Ensemble& en, int nsteps);
+void ensemble_packs(const Ensemble& en, int* npacks, int* pack_of);
+void ensemble_destroy(Ensemble* en);
+std::string tune_subcycle(Ctx& c, int trials, int out[3], double* best_ms);
 // surrogate.hip
 std::string nn_create(Ctx& c, int capacity, unsigned long long seed);
 void nn_destroy(Ctx& c);

@@ -940,8 +940,10 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
+// main.f90:164-211: daily forcing, den_alpha, int_pgra, int_readyc (the
+// step up to the barotropic subcycle)
 template <typename R, typename E>
-std::string step_t(Ctx& c) {
+std::string step_pre_t(Ctx& c) {
   const Grid<R>& g = grid_of<R>(c);
   const Grid<E>& ge = grid_of<E>(c);
   Fields3& f = c.f3;
@@ -999,12 +1001,28 @@ std::string step_t(Ctx& c) {
   }
   LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
   LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
-  // int_trop (launch_trop records ev[1] right before the subcycle kernel)
-  {
-    std::string te = c.nn ? nn_trop_step(c, launch_trop) : launch_trop(c);
-    if (!te.empty()) return te;
-    c.n_ch += 1;
-  }
+  return "";
+}
+
+// main.f90:214-226 after int_trop: int_clin, int_ssh, int_ts, calendar
+template <typename R, typename E>
+std::string step_post_t(Ctx& c) {
+  const Grid<R>& g = grid_of<R>(c);
+  const Grid<E>& ge = grid_of<E>(c);
+  Fields3& f = c.f3;
+  Fields2& h = c.f2;
+  const size_t n2 = c.n2, n3 = c.n3;
+  double* T = P<double>(f.T[c.cur]);
+  double* S = P<double>(f.S[c.cur]);
+  double* U = P<double>(f.U);
+  double* V = P<double>(f.V);
+  R* au = P<R>(f.AU[c.ab]);
+  R* av = P<R>(f.AV[c.ab]);
+  R* aup = P<R>(f.AU[(c.ab + 2) % 3]);
+  R* avp = P<R>(f.AV[(c.ab + 2) % 3]);
+  R* aupp = P<R>(f.AU[(c.ab + 1) % 3]);
+  R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
+  (void)n3;
   if (c.timing) (void)hipEventRecord(c.ev[2], c.stream);
   LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, (double*)nullptr, h.sqp, h.schs, h.schm);
   // int_clin
@@ -1127,10 +1145,25 @@ __global__ void k_den_rho(long n, const double* t, const double* s, const double
 
 }  // namespace
 
+std::string launch_step_pre(Ctx& c) {
+  if (c.precision == PCOM_FP64) return step_pre_t<double, double>(c);
+  if (c.precision == PCOM_MIXED) return step_pre_t<float, double>(c);
+  return step_pre_t<float, float>(c);
+}
+
+std::string launch_step_post(Ctx& c) {
+  if (c.precision == PCOM_FP64) return step_post_t<double, double>(c);
+  if (c.precision == PCOM_MIXED) return step_post_t<float, double>(c);
+  return step_post_t<float, float>(c);
+}
+
 std::string launch_step(Ctx& c) {
-  if (c.precision == PCOM_FP64) return step_t<double, double>(c);
-  if (c.precision == PCOM_MIXED) return step_t<float, double>(c);
-  return step_t<float, float>(c);
+  std::string e = launch_step_pre(c);
+  if (!e.empty()) return e;
+  e = c.nn ? nn_trop_step(c, launch_trop) : launch_trop(c);
+  if (!e.empty()) return e;
+  c.n_ch += 1;
+  return launch_step_post(c);
 }
 
 std::string launch_global_sums(Ctx& c, double out[3]) {

@@ -895,7 +895,6 @@ LiveIn live_inputs(const Ctx& c) {
 std::string nn_predict(Ctx& c) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
-  Fields2& h = c.f2;
   const long hw = (long)n->H * n->W;
   NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c), n->norm, n->stat,
             n->act[0]);

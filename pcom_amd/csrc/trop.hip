@@ -14,9 +14,11 @@
 // statements, so the result does not depend on the tiling.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "context.h"
 
@@ -25,6 +27,10 @@ namespace pcom {
 namespace {
 
 constexpr int kTropBlock = 1024;  // one region point per thread: RW * RHt <= 1024
+
+inline std::string hip_error_string(hipError_t e, const char* what) {
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
 
 // a / b for b with rb = RN(1/b): q = RN(a*rb) is within one ulp of a/b, and
 // one FMA residual step rounds it correctly (Markstein, IBM J. Res. Dev.
@@ -125,13 +131,13 @@ __device__ __forceinline__ double xload(const double* p) {
 // result is independent of the tiling; points near the region edge read
 // clamped neighbours and are garbage, but no owned point depends on them
 // within S substeps.
-__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
+__device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile) {
   extern __shared__ double lds[];
   __shared__ int abort_flag;
   if (threadIdx.x == 0) abort_flag = 0;
   const int G = a.G, nj = a.nj, ni = a.ni, njm = a.nj - 1;
   const int TW = a.TW, TH = a.TH;
-  const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
+  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   const int x0 = tx * TW;
   const int y0 = 1 + ty * TH;
   const int HL = 2 * a.S, HR = 3 * a.S;
@@ -424,6 +430,28 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body(a, blockIdx.x); }
+
+// The subcycles of several contexts (ensemble members) in one launch:
+// workgroups [start[m], start[m+1]) are member m's tiles, each member with
+// its own fields, flags and epoch.  Tiles wait only for tiles of their own
+// member, so members of any grid sizes share the launch.
+constexpr int kMaxPack = 16;
+struct MultiLaunch {
+  int n;
+  int start[kMaxPack + 1];
+  unsigned long long epoch[kMaxPack];
+};
+
+__global__ void __launch_bounds__(kTropBlock) k_trop_multi(const TropTileArgs* __restrict__ args, MultiLaunch ml) {
+  const int b = blockIdx.x;
+  int m = 0;
+  while (m + 1 < ml.n && b >= ml.start[m + 1]) ++m;
+  TropTileArgs a = args[m];
+  a.epoch = ml.epoch[m];
+  trop_tiles_body(a, b - ml.start[m]);
+}
+
 // vs at the first substep (mod_int.f90:198-207): seau*am(:,:,1)*lap(ub),
 // and am(:,:,1) in FP64 for the substeps.
 template <typename R>
@@ -521,6 +549,9 @@ std::string trop_configure(Ctx& c) {
   long tiles_x = (G + c.trop_tw - 1) / c.trop_tw;
   long tiles_y = (c.nj + c.trop_th - 1) / c.trop_th;
   long need = tiles_x * tiles_y;
+  if (c.trop_flag_cap > 0 && need > c.trop_flag_cap)
+    return "barotropic tiling needs " + std::to_string(need) + " tiles, more than the " +
+           std::to_string(c.trop_flag_cap) + " the context holds flags for";
   if (need > resident)
     return "barotropic tiling needs " + std::to_string(need) + " co-resident workgroups, the device holds " +
            std::to_string(resident) + "; choose larger tiles";
@@ -541,15 +572,24 @@ std::mutex g_trop_mutex;
 std::map<int, TropChain> g_trop_chain;
 
 template <typename R>
-std::string launch_trop_t(Ctx& c) {
+std::string launch_vz_t(Ctx& c) {
   Fields2& h = c.f2;
+  hipLaunchKernelGGL(k_vz<R>, dim3((unsigned)((c.n2 + 255) / 256)), dim3(256), 0, c.stream, c.gd,
+                     static_cast<const R*>(c.f3.am), h.bup, h.bvp, h.vz1, h.vz2, h.am1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("k_vz: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string launch_vz(Ctx& c) {
+  if (c.precision == PCOM_FP64) return launch_vz_t<double>(c);
+  return launch_vz_t<float>(c);
+}
+
+// the subcycle's arguments for the context's current tiling (epoch 0)
+TropTileArgs trop_args(const Ctx& c) {
+  const Fields2& h = c.f2;
   const Grid<double>& gd = c.gd;
-  {
-    hipLaunchKernelGGL(k_vz<R>, dim3((unsigned)((c.n2 + 255) / 256)), dim3(256), 0, c.stream, gd,
-                       static_cast<const R*>(c.f3.am), h.bup, h.bvp, h.vz1, h.vz2, h.am1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return std::string("k_vz: ") + hipGetErrorString(e);
-  }
   TropTileArgs a;
   a.ni = c.ni;
   a.nj = c.nj;
@@ -590,18 +630,23 @@ std::string launch_trop_t(Ctx& c) {
   a.tiles_y = c.trop_blocks / c.trop_tiles_x;
   a.flags = c.trop_flags;
   a.error = c.trop_error;
-  a.epoch = ++c.trop_epoch;
+  a.epoch = 0;
   a.bt = (double)c.dtbt;
-  // An ordinary launch: trop_configure keeps the grid within the device's
-  // co-resident capacity (one workgroup per CU by its LDS), so every tile
-  // a wait depends on gets a CU once other work on the device drains, and
-  // every wait is bounded.  Two such launches running at once could each
-  // hold CUs the other's tiles need, so launches of all contexts of this
-  // process on a device are chained: each waits for the previous one.  (A
-  // cooperative launch would add nothing but its per-process global-wave-
-  // sync resource, which fails when another process on the device holds it.)
+  return a;
+}
+
+// An ordinary launch: trop_configure keeps the grid within the device's
+// co-resident capacity (one workgroup per CU by its LDS), so every tile a
+// wait depends on gets a CU once other work on the device drains, and every
+// wait is bounded.  Two such launches running at once could each hold CUs
+// the other's tiles need, so launches of all contexts of this process on a
+// device are chained: each waits for the previous one.  (A cooperative
+// launch would add nothing but its per-process global-wave-sync resource,
+// which fails when another process on the device holds it.)
+template <typename F>
+std::string chained_launch(int device, hipStream_t st, F launch) {
   std::lock_guard<std::mutex> lock(g_trop_mutex);
-  TropChain& ch = g_trop_chain[c.device];
+  TropChain& ch = g_trop_chain[device];
   if (!ch.ev[0]) {
     for (hipEvent_t& ev : ch.ev) {
       hipError_t ee = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
@@ -609,23 +654,330 @@ std::string launch_trop_t(Ctx& c) {
     }
   }
   if (ch.used) {
-    hipError_t ew = hipStreamWaitEvent(c.stream, ch.ev[ch.last], 0);
+    hipError_t ew = hipStreamWaitEvent(st, ch.ev[ch.last], 0);
     if (ew != hipSuccess) return std::string("hipStreamWaitEvent: ") + hipGetErrorString(ew);
   }
-  if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
-  hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
+  std::string err = launch();
+  if (!err.empty()) return err;
   ch.last = 1 - ch.last;
-  e = hipEventRecord(ch.ev[ch.last], c.stream);
+  hipError_t e = hipEventRecord(ch.ev[ch.last], st);
   if (e != hipSuccess) return std::string("hipEventRecord: ") + hipGetErrorString(e);
   ch.used = true;
   return "";
 }
 
+std::string launch_trop_kernel(Ctx& c) {
+  TropTileArgs a = trop_args(c);
+  a.epoch = ++c.trop_epoch;
+  return chained_launch(c.device, c.stream, [&]() -> std::string {
+    if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
+    hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
+    return "";
+  });
+}
+
 std::string launch_trop(Ctx& c) {
-  if (c.precision == PCOM_FP64) return launch_trop_t<double>(c);
-  return launch_trop_t<float>(c);
+  std::string e = launch_vz(c);
+  if (!e.empty()) return e;
+  return launch_trop_kernel(c);
+}
+
+}  // namespace pcom
+
+// ---------------------------------------------------------------------------
+// Ensembles: several contexts stepped together.  Each member's 3-D kernels
+// run on the member's own stream; the members' subcycles run in packed
+// k_trop_multi launches on the ensemble's stream, first-fit by tile count
+// against the device's co-resident capacity, so small members fill the
+// device together instead of running one after another.
+
+namespace pcom {
+
+struct Ensemble {
+  std::vector<Ctx*> m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::vector<int>> packs;  // member indices
+  std::vector<size_t> pack_lds;
+  std::vector<int> pack_of;             // member -> pack (-1: runs alone)
+  TropTileArgs* dargs = nullptr;        // device, in pack order
+  std::vector<int> pack_first;          // first dargs index of each pack
+  std::vector<hipEvent_t> ready, done;
+};
+
+namespace {
+
+std::string multi_capacity(int device, size_t lds, long* cap) {
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+  e = hipFuncSetAttribute((const void*)k_trop_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
+  int per_cu = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop_multi, kTropBlock, lds);
+  if (e != hipSuccess) return std::string("occupancy of k_trop_multi: ") + hipGetErrorString(e);
+  *cap = (long)per_cu * prop.multiProcessorCount;
+  return "";
+}
+
+// packs and device arguments for the members' current tilings
+std::string ensemble_plan(Ensemble& en) {
+  const int n = (int)en.m.size();
+  std::vector<int> order;
+  for (int q = 0; q < n; ++q)
+    if (!en.m[q]->nn) order.push_back(q);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return en.m[a]->trop_blocks > en.m[b]->trop_blocks; });
+  en.packs.clear();
+  en.pack_lds.clear();
+  en.pack_of.assign(n, -1);
+  std::vector<long> used;
+  for (int q : order) {
+    const Ctx& c = *en.m[q];
+    bool placed = false;
+    for (size_t p = 0; p < en.packs.size() && !placed; ++p) {
+      if (en.packs[p].size() >= (size_t)kMaxPack) continue;
+      const size_t lds = std::max(en.pack_lds[p], c.trop_lds);
+      long cap;
+      std::string e = multi_capacity(en.device, lds, &cap);
+      if (!e.empty()) return e;
+      if (used[p] + c.trop_blocks <= cap) {
+        en.packs[p].push_back(q);
+        en.pack_lds[p] = lds;
+        used[p] += c.trop_blocks;
+        en.pack_of[q] = (int)p;
+        placed = true;
+      }
+    }
+    if (!placed) {
+      long cap;
+      std::string e = multi_capacity(en.device, c.trop_lds, &cap);
+      if (!e.empty()) return e;
+      if (c.trop_blocks > cap) return "an ensemble member's subcycle tiling does not fit the device";
+      en.packs.push_back({q});
+      en.pack_lds.push_back(c.trop_lds);
+      used.push_back(c.trop_blocks);
+      en.pack_of[q] = (int)en.packs.size() - 1;
+    }
+  }
+  // device arguments in pack order
+  std::vector<TropTileArgs> host;
+  en.pack_first.clear();
+  for (auto& pk : en.packs) {
+    en.pack_first.push_back((int)host.size());
+    for (int q : pk) host.push_back(trop_args(*en.m[q]));
+  }
+  if (en.dargs) (void)hipFree(en.dargs);
+  en.dargs = nullptr;
+  if (!host.empty()) {
+    hipError_t e = hipMalloc(&en.dargs, host.size() * sizeof(TropTileArgs));
+    if (e != hipSuccess) return std::string("hipMalloc: ") + hipGetErrorString(e);
+    e = hipMemcpy(en.dargs, host.data(), host.size() * sizeof(TropTileArgs), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return std::string("upload ensemble arguments: ") + hipGetErrorString(e);
+  }
+  for (hipEvent_t& ev : en.done) (void)hipEventDestroy(ev);
+  en.done.assign(en.packs.size(), nullptr);
+  for (hipEvent_t& ev : en.done) {
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return std::string("hipEventCreate: ") + hipGetErrorString(e);
+  }
+  return "";
+}
+
+}  // namespace
+
+std::string ensemble_create(const std::vector<Ctx*>& members, Ensemble** out) {
+  *out = nullptr;
+  if (members.empty()) return "an ensemble needs at least one member";
+  Ensemble* en = new Ensemble();
+  en->m = members;
+  en->device = members[0]->device;
+  for (Ctx* c : members)
+    if (c->device != en->device) {
+      delete en;
+      return "all ensemble members must be on one device";
+    }
+  hipError_t e = hipSetDevice(en->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&en->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete en;
+    return std::string("hipStreamCreate: ") + hipGetErrorString(e);
+  }
+  en->ready.assign(members.size(), nullptr);
+  for (hipEvent_t& ev : en->ready) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  *out = en;
+  return "";
+}
+
+void ensemble_destroy(Ensemble* en) {
+  if (!en) return;
+  (void)hipSetDevice(en->device);
+  if (en->stream) (void)hipStreamSynchronize(en->stream);
+  for (hipEvent_t ev : en->ready) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : en->done) (void)hipEventDestroy(ev);
+  if (en->dargs) (void)hipFree(en->dargs);
+  if (en->stream) (void)hipStreamDestroy(en->stream);
+  delete en;
+}
+
+void ensemble_packs(const Ensemble& en, int* npacks, int* pack_of) {
+  *npacks = (int)en.packs.size();
+  for (size_t q = 0; q < en.m.size(); ++q) pack_of[q] = en.pack_of.empty() ? -1 : en.pack_of[q];
+}
+
+std::string ensemble_step(Ensemble& en, int nsteps) {
+  hipError_t e = hipSetDevice(en.device);
+  if (e != hipSuccess) return std::string("hipSetDevice: ") + hipGetErrorString(e);
+  // tilings may have changed since the last call (autotuner)
+  for (Ctx* c : en.m) (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(en.stream);
+  std::string err = ensemble_plan(en);
+  if (!err.empty()) return err;
+  for (int s = 0; s < nsteps; ++s) {
+    // members: the step up to the subcycle
+    for (size_t q = 0; q < en.m.size(); ++q) {
+      Ctx& c = *en.m[q];
+      if (!(err = launch_step_pre(c)).empty()) return err;
+      if (c.nn) {  // a member with a surrogate runs its own barotropic part
+        if (!(err = nn_trop_step(c, launch_trop)).empty()) return err;
+        continue;
+      }
+      if (!(err = launch_vz(c)).empty()) return err;
+      if ((e = hipEventRecord(en.ready[q], c.stream)) != hipSuccess) return hip_error_string(e, "hipEventRecord");
+    }
+    // packed subcycles
+    for (size_t p = 0; p < en.packs.size(); ++p) {
+      MultiLaunch ml{};
+      ml.n = (int)en.packs[p].size();
+      int blocks = 0;
+      for (int k = 0; k < ml.n; ++k) {
+        Ctx& c = *en.m[en.packs[p][k]];
+        if ((e = hipStreamWaitEvent(en.stream, en.ready[en.packs[p][k]], 0)) != hipSuccess)
+          return hip_error_string(e, "hipStreamWaitEvent");
+        ml.start[k] = blocks;
+        ml.epoch[k] = ++c.trop_epoch;
+        blocks += c.trop_blocks;
+      }
+      ml.start[ml.n] = blocks;
+      const TropTileArgs* args = en.dargs + en.pack_first[p];
+      const size_t lds = en.pack_lds[p];
+      err = chained_launch(en.device, en.stream, [&]() -> std::string {
+        hipError_t ea = hipFuncSetAttribute((const void*)k_trop_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)lds);
+        if (ea != hipSuccess) return hip_error_string(ea, "hipFuncSetAttribute");
+        hipLaunchKernelGGL(k_trop_multi, dim3(blocks), dim3(kTropBlock), lds, en.stream, args, ml);
+        ea = hipGetLastError();
+        if (ea != hipSuccess) return hip_error_string(ea, "k_trop_multi");
+        return "";
+      });
+      if (!err.empty()) return err;
+      if ((e = hipEventRecord(en.done[p], en.stream)) != hipSuccess) return hip_error_string(e, "hipEventRecord");
+    }
+    // members: the rest of the step
+    for (size_t q = 0; q < en.m.size(); ++q) {
+      Ctx& c = *en.m[q];
+      if (!c.nn && (e = hipStreamWaitEvent(c.stream, en.done[en.pack_of[q]], 0)) != hipSuccess)
+        return hip_error_string(e, "hipStreamWaitEvent");
+      c.n_ch += 1;
+      if (!(err = launch_step_post(c)).empty()) return err;
+    }
+  }
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: time the subcycle of this context under the tilings the cost
+// model ranks best, on the context's own state (snapshotted and restored
+// around every trial), and keep the fastest.
+std::string tune_subcycle(Ctx& c, int trials, int out[3], double* best_ms) {
+  const int G = c.ni - 2;
+  const long nstep = (long)c.dtbc / c.dtbt * 2;
+  struct Cand {
+    long cost;
+    int tw, th, s;
+  };
+  std::vector<Cand> cands;
+  std::string err;
+  for (int s = 1; s <= 4; ++s)
+    for (int tw = 4; tw <= 64; ++tw)
+      for (int th = 4; th <= 64; ++th) {
+        const long nr = (long)(tw + 5 * s) * (th + 5 * s);
+        if (nr > kTropBlock) continue;
+        const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
+        if (tiles > c.trop_flag_cap) continue;
+        long resident;
+        if (!(err = trop_resident(c, trop_lds_bytes(tw, th, s), &resident)).empty()) return err;
+        if (tiles > resident) continue;
+        cands.push_back({nstep * ((nr + 63) / 64) + (nstep / s) * 14, tw, th, s});
+      }
+  if (cands.empty()) return "no barotropic tiling fits this device";
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.cost < b.cost; });
+  if ((int)cands.size() > trials) cands.resize(trials);
+  // snapshot of what the subcycle writes
+  Fields2& h = c.f2;
+  double* fields[] = {h.chc, h.chp, h.mch, h.mch2, h.acch, h.buc, h.bvc, h.bup, h.bvp};
+  const int nf = 9;
+  double* snap = nullptr;
+  hipError_t e = hipMalloc(&snap, sizeof(double) * c.n2 * nf);
+  if (e != hipSuccess) return hip_error_string(e, "hipMalloc");
+  auto copy = [&](bool save) -> hipError_t {
+    for (int q = 0; q < nf; ++q) {
+      hipError_t ec = save ? hipMemcpyAsync(snap + q * c.n2, fields[q], sizeof(double) * c.n2,
+                                            hipMemcpyDeviceToDevice, c.stream)
+                           : hipMemcpyAsync(fields[q], snap + q * c.n2, sizeof(double) * c.n2,
+                                            hipMemcpyDeviceToDevice, c.stream);
+      if (ec != hipSuccess) return ec;
+    }
+    return hipSuccess;
+  };
+  hipEvent_t ea, eb;
+  (void)hipEventCreate(&ea);
+  (void)hipEventCreate(&eb);
+  const int keep[3] = {c.trop_tw, c.trop_th, c.trop_s};
+  double best = 1e300;
+  Cand win = cands[0];
+  if ((e = copy(true)) != hipSuccess) err = hip_error_string(e, "snapshot");
+  for (const Cand& cd : cands) {
+    if (!err.empty()) break;
+    c.trop_tw = cd.tw;
+    c.trop_th = cd.th;
+    c.trop_s = cd.s;
+    if (!(err = trop_configure(c)).empty()) break;
+    double t = 1e300;
+    for (int r = 0; r < 3 && err.empty(); ++r) {
+      if ((e = copy(false)) != hipSuccess) { err = hip_error_string(e, "restore"); break; }
+      (void)hipEventRecord(ea, c.stream);
+      if (!(err = launch_trop(c)).empty()) break;
+      (void)hipEventRecord(eb, c.stream);
+      if ((e = hipEventSynchronize(eb)) != hipSuccess) { err = hip_error_string(e, "sync"); break; }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ea, eb);
+      t = std::min(t, (double)ms);
+    }
+    if (err.empty() && t < best) {
+      best = t;
+      win = cd;
+    }
+  }
+  // the chosen tiling (or the previous one after a failure), state restored
+  c.trop_tw = err.empty() ? win.tw : keep[0];
+  c.trop_th = err.empty() ? win.th : keep[1];
+  c.trop_s = err.empty() ? win.s : keep[2];
+  std::string e2 = trop_configure(c);
+  if (copy(false) != hipSuccess && err.empty()) err = "restore after tuning";
+  (void)hipStreamSynchronize(c.stream);
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  (void)hipFree(snap);
+  if (!err.empty()) return err;
+  if (!e2.empty()) return e2;
+  out[0] = c.trop_tw;
+  out[1] = c.trop_th;
+  out[2] = c.trop_s;
+  *best_ms = best;
+  return "";
 }
 
 }  // namespace pcom

@@ -210,6 +210,76 @@ module pcom_gpu
       real(c_double), intent(out) :: rho(*)
     end function pcom_den_rho
 
+    ! ensembles and the subcycle autotuner
+    integer(c_int) function pcom_ensemble_create(members, count, ens) bind(C, name='pcom_ensemble_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: count
+      type(c_ptr), intent(out) :: ens
+    end function pcom_ensemble_create
+
+    integer(c_int) function pcom_ensemble_step(ens, nsteps) bind(C, name='pcom_ensemble_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nsteps
+    end function pcom_ensemble_step
+
+    integer(c_int) function pcom_ensemble_packs(ens, npacks, pack_of_member) bind(C, name='pcom_ensemble_packs')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), intent(out) :: npacks, pack_of_member(*)
+    end function pcom_ensemble_packs
+
+    integer(c_int) function pcom_ensemble_destroy(ens) bind(C, name='pcom_ensemble_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function pcom_ensemble_destroy
+
+    integer(c_int) function pcom_tune_subcycle(ctx, trials, tile, ms) bind(C, name='pcom_tune_subcycle')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: trials
+      integer(c_int), intent(out) :: tile(3)
+      real(c_double), intent(out) :: ms
+    end function pcom_tune_subcycle
+
+    ! learned barotropic surrogate
+    integer(c_int) function pcom_nn_create(ctx, capacity, seed) bind(C, name='pcom_nn_create')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: capacity
+      integer(c_long_long), value :: seed
+    end function pcom_nn_create
+
+    integer(c_int) function pcom_nn_record(ctx, on) bind(C, name='pcom_nn_record')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function pcom_nn_record
+
+    integer(c_int) function pcom_nn_train(ctx, iters, batch, lr, seed, loss) bind(C, name='pcom_nn_train')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iters, batch
+      real(c_double), value :: lr
+      integer(c_long_long), value :: seed
+      real(c_double), intent(out) :: loss(*)
+    end function pcom_nn_train
+
+    integer(c_int) function pcom_nn_use(ctx, check_every, tol) bind(C, name='pcom_nn_use')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: check_every
+      real(c_double), value :: tol
+    end function pcom_nn_use
+
+    integer(c_int) function pcom_nn_eval(ctx, first, count, rmse) bind(C, name='pcom_nn_eval')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: first, count
+      real(c_double), intent(out) :: rmse(8)
+    end function pcom_nn_eval
+
     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s

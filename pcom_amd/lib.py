@@ -25,6 +25,8 @@ EXPORTS = [
     "pcom_set_clock", "pcom_output_means",
     "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
+    "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
+    "pcom_tune_subcycle",
 ]
 
 
@@ -78,6 +80,12 @@ def lib():
         L.pcom_nn_get_params.argtypes = [vp, fp, sz]
         L.pcom_nn_set_params.argtypes = [vp, fp, sz]
         L.pcom_nn_forward_backward.argtypes = [vp, C.c_int, fp, fp, fp, fp, dp]
+        ip = C.POINTER(C.c_int)
+        L.pcom_ensemble_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.pcom_ensemble_step.argtypes = [vp, C.c_int]
+        L.pcom_ensemble_packs.argtypes = [vp, ip, ip]
+        L.pcom_ensemble_destroy.argtypes = [vp]
+        L.pcom_tune_subcycle.argtypes = [vp, C.c_int, ip, dp]
         for name in EXPORTS:
             if name not in ("pcom_field_size", "pcom_last_error", "pcom_nn_param_count"):
                 getattr(L, name).restype = C.c_int
@@ -92,6 +100,34 @@ def _check(rc):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Ensemble:
+    """Several Models stepped together (pcom_ensemble_*)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        _check(lib().pcom_ensemble_create(arr, len(self.members), C.byref(h)))
+        self.h = h
+
+    def step(self, n=1):
+        _check(lib().pcom_ensemble_step(self.h, int(n)))
+
+    def packs(self):
+        npk = C.c_int()
+        of = (C.c_int * len(self.members))()
+        _check(lib().pcom_ensemble_packs(self.h, C.byref(npk), of))
+        return npk.value, list(of)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _LIB is not None:
+            _LIB.pcom_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def den_rho(t, s, p):
@@ -172,6 +208,13 @@ class Model:
             setattr(m, n, _dp(out[n]))
         _check(lib().pcom_output_means(self.h, C.byref(m)))
         return out
+
+    def tune_subcycle(self, trials=8):
+        """Autotune the subcycle tiling; returns ((tile_i, tile_j, substeps), ms)."""
+        t = (C.c_int * 3)()
+        ms = C.c_double()
+        _check(lib().pcom_tune_subcycle(self.h, int(trials), t, C.byref(ms)))
+        return tuple(t), ms.value
 
     # ---- learned barotropic surrogate
     def nn_create(self, capacity=0, seed=1):
