@@ -254,6 +254,10 @@ int pcom_ensemble_destroy(pcom_ensemble *ens);
  * restoring the state after each, and keep the fastest.  Returns the chosen
  * tiling and its time per launch.  The state is unchanged.               */
 int pcom_tune_subcycle(pcom_ctx *ctx, int trials, int tile[3], double *ms);
+/* Set the subcycle tiling {tile i, tile j, substeps per exchange} (as
+ * pcom_opts.trop_tile_* at create; all zero = the cost model's choice) or
+ * read it (set = 0).  The results do not depend on the tiling.           */
+int pcom_subcycle_tiling(pcom_ctx *ctx, int set, int tile[3]);
 
 /* ---- Learned barotropic surrogate.  New: the reference has no such part.
  * A BF16 convolutional network (six 3x3 layers, 64 hidden channels, FP32

@@ -581,6 +581,33 @@ int pcom_tune_subcycle(pcom_ctx* ctx, int trials, int tile[3], double* ms) {
   return PCOM_OK;
 }
 
+int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[3]) {
+  if (!ctx || !tile) return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: NULL argument");
+  Ctx* c = C(ctx);
+  if (set) {
+    if (tile[0] < 0 || tile[1] < 0 || tile[2] < 0 || tile[0] > 256 || tile[1] > 256 || tile[2] > 16)
+      return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: tiling out of range");
+    HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+    HIPCHK(hipStreamSynchronize(c->stream), "sync");
+    const int keep[3] = {c->trop_tw, c->trop_th, c->trop_s};
+    c->trop_tw = tile[0];
+    c->trop_th = tile[1];
+    c->trop_s = tile[2];
+    std::string e = trop_configure(*c);
+    if (!e.empty()) {
+      c->trop_tw = keep[0];
+      c->trop_th = keep[1];
+      c->trop_s = keep[2];
+      (void)trop_configure(*c);
+      return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: " + e);
+    }
+  }
+  tile[0] = c->trop_tw;
+  tile[1] = c->trop_th;
+  tile[2] = c->trop_s;
+  return PCOM_OK;
+}
+
 int pcom_nn_create(pcom_ctx* ctx, int capacity, unsigned long long seed) {
   if (!ctx || capacity < 0) return fail(PCOM_ERR_ARG, "pcom_nn_create: bad argument");
   Ctx* c = C(ctx);

@@ -243,6 +243,13 @@ module pcom_gpu
       real(c_double), intent(out) :: ms
     end function pcom_tune_subcycle
 
+    integer(c_int) function pcom_subcycle_tiling(ctx, set, tile) bind(C, name='pcom_subcycle_tiling')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: set
+      integer(c_int), intent(inout) :: tile(3)
+    end function pcom_subcycle_tiling
+
     ! learned barotropic surrogate
     integer(c_int) function pcom_nn_create(ctx, capacity, seed) bind(C, name='pcom_nn_create')
       import :: c_int, c_ptr, c_long_long

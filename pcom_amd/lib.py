@@ -26,7 +26,7 @@ EXPORTS = [
     "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
-    "pcom_tune_subcycle",
+    "pcom_tune_subcycle", "pcom_subcycle_tiling",
 ]
 
 
@@ -86,6 +86,7 @@ def lib():
         L.pcom_ensemble_packs.argtypes = [vp, ip, ip]
         L.pcom_ensemble_destroy.argtypes = [vp]
         L.pcom_tune_subcycle.argtypes = [vp, C.c_int, ip, dp]
+        L.pcom_subcycle_tiling.argtypes = [vp, C.c_int, ip]
         for name in EXPORTS:
             if name not in ("pcom_field_size", "pcom_last_error", "pcom_nn_param_count"):
                 getattr(L, name).restype = C.c_int
@@ -215,6 +216,12 @@ class Model:
         ms = C.c_double()
         _check(lib().pcom_tune_subcycle(self.h, int(trials), t, C.byref(ms)))
         return tuple(t), ms.value
+
+    def tiling(self, tile=None):
+        """The subcycle tiling (tile_i, tile_j, substeps); sets it if given."""
+        t = (C.c_int * 3)(*(tile or (0, 0, 0)))
+        _check(lib().pcom_subcycle_tiling(self.h, int(tile is not None), t))
+        return tuple(t)
 
     # ---- learned barotropic surrogate
     def nn_create(self, capacity=0, seed=1):

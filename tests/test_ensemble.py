@@ -56,3 +56,25 @@ def test_autotuner_keeps_state_and_bits(gpu_lib):
     b.step(3)
     bad = _same(a, b)
     assert not bad, "; ".join(bad)
+
+
+def test_tiling_set_and_read_back(gpu_lib):
+    """pcom_subcycle_tiling: a tiling set on a live context gives the bits
+    of a context created with it; a tiling that does not fit is refused and
+    the old one kept."""
+    from pcom_amd.lib import Model, PcomError
+    cfg = configs.global_1deg(glo_ni=180, glo_nj=90)
+    a = Model(cfg)
+    b = Model(cfg, trop_tile=(26, 16, 2))
+    default = a.tiling()
+    assert all(t > 0 for t in default)
+    a.step(2)
+    b.step(2)
+    assert a.tiling((26, 16, 2)) == (26, 16, 2)
+    with pytest.raises(PcomError):
+        a.tiling((40, 40, 4))  # tile plus halo above one workgroup's points
+    assert a.tiling() == (26, 16, 2)
+    a.step(3)
+    b.step(3)
+    assert not _same(a, b)
+    assert a.tiling((0, 0, 0)) == default
