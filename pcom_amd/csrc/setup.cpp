@@ -95,6 +95,8 @@ void ter2_ns(const Dims& d, std::vector<double>& ans, const std::vector<double>&
 
 std::string build_setup(const pcom_inputs& in, HostSetup& hs) {
   if (in.glo_ni < 3 || in.glo_nj < 3 || in.nk < 2) return "grid too small (need glo_ni, glo_nj >= 3, nk >= 2)";
+  if ((long long)(in.glo_ni + 2) * (in.glo_nj + 2) >= (1ll << 31))
+    return "grid too large (the halo'd 2-D size must stay below 2^31 points)";
   if (!in.lon || !in.lat || !in.z || !in.itn || !in.tmn || !in.smn || !in.pt || !in.sa || !in.taux ||
       !in.tauy || !in.bct || !in.bcs || !in.pa || !in.fw)
     return "pcom_inputs has a NULL array (only vmix may be NULL)";

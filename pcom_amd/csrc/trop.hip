@@ -62,35 +62,59 @@ struct TropTileArgs {
 constexpr int kFlagStride = 32;                 // 256 B between flags
 constexpr unsigned long long kSpinLimit = 1ull << 22;  // ~seconds; a normal wait is microseconds
 
-// LDS of one tile: 12 region arrays, a table of per-row metrics (rh,
-// dx%x(2) and cor depend on the row only) and one of per-column metrics
-// (dx%x(1) depends on the column only): the start-up chain builds them
-// from the 1-D lon/lat inputs (main.f90:347-497) and the host checks it.
-constexpr int kRegionArrays = 12;  // CHP PR PU PV S1 S2 WAu WBu WAv WBv GX GY
+// LDS of one tile: six region arrays of (double, double) pairs, then a
+// table of per-row metrics (rh, dx%x(2) and cor depend on the row only) and
+// one of per-column metrics (dx%x(1) depends on the column only): the
+// start-up chain builds them from the 1-D lon/lat inputs (main.f90:347-497)
+// and the host checks it.  Every array sits at a fixed offset with a fixed
+// stride, so an access is one address register per neighbour plus an
+// immediate offset, and the two components a stage reads at the same point
+// come in one 16-byte access.
+enum RegionPair {
+  P_CH,  // (corrector ch, predictor ch)
+  P_UV,  // barotropic velocity
+  P_SF,  // prh*sch*ub fluxes
+  P_GR,  // gradients of the predictor ch
+  P_WA,  // Laplacian face fluxes of (u, v), east-west
+  P_WB,  // the same, north-south
+  P_N
+};
 enum RowTab {
   R_RH3, R_DX21, R_DX21I, R_ARH1, R_ARH1I, R_RH4, R_RH4I, R_DX22, R_DX22I, R_RH2, R_ARH2, R_ARH2I,
   R_DX24, R_DX24I, R_DX23, R_DX23I, R_DEN, R_DENI, R_COR, R_DXX, R_DXXI, R_N
 };
 enum ColTab { C_DX11, C_DX11I, C_DX14, C_DX14I, C_DX12, C_DX12I, C_DX13, C_DX13I, C_N };
+// entries per metric table: at least the widest region side (a region of at
+// most kTropBlock points with sides of at least 6 is at most 170 long)
+constexpr int kTabStride = 176;
 
-size_t trop_lds_bytes(int TW, int TH, int S) {
-  const size_t RW = TW + 5 * S, RHt = TH + 5 * S;
-  return 8 * (kRegionArrays * RW * RHt + R_N * RHt + C_N * RW);
-}
+size_t trop_lds_bytes(int, int, int) { return 16 * P_N * kTropBlock + 8 * (R_N + C_N) * kTabStride; }
 
 // Publish this tile's round and wait until every tile whose owned points
-// lie in this tile's region has published it.  The exchange values are
-// written and read with agent-scope atomic accesses, the flag is stored
-// with release and each poller ends with an agent-scope acquire fence, so
-// only the flags pay for cache maintenance (full __threadfence on both
-// sides costs several times more per exchange).  Waits are bounded: past
-// kSpinLimit polls the tile records an error and stops waiting, so the
-// launch always completes (the host then reports the failure).
+// lie in this tile's region has published it.
+//
+// The exchange values and the flags are written and read only with
+// agent-scope atomic accesses.  Those carry the device-scope cache policy
+// (sc1) that keeps each of them coherent across the XCDs' L2 caches on its
+// own, which is what makes relaxed agent-scope atomics coherent at all.
+// Order between locations comes from completion: every wave waits for all
+// its memory accesses (s_waitcnt with every counter at zero) before the
+// workgroup barrier that precedes the flag store, so the tile's exchange
+// stores, and its loads of the entry fields and of the last round, are
+// complete before its flag can be seen; a consumer issues its data loads
+// only after the barrier that follows its flag polls returned.  A release
+// store and an acquire fence would add an L2 writeback and an L2
+// invalidate per exchange, which only non-atomic accesses need.
+//
+// Waits are bounded: past kSpinLimit polls the tile records an error and
+// stops waiting, so the launch always completes (the host then reports the
+// failure).
 __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int nx_hi, int ny_lo, int ny_hi,
                           unsigned long long value, int* abort_flag) {
-  __syncthreads();  // every exchange store of this tile precedes the release below
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
   if (threadIdx.x == 0)
-    __hip_atomic_store(&a.flags[(size_t)(tx + a.tiles_x * ty) * kFlagStride], value, __ATOMIC_RELEASE,
+    __hip_atomic_store(&a.flags[(size_t)(tx + a.tiles_x * ty) * kFlagStride], value, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   const int nxs = nx_hi - nx_lo + 1;
   const int nys = ny_hi - ny_lo + 1;
@@ -109,7 +133,6 @@ __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int 
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
@@ -131,8 +154,9 @@ __device__ __forceinline__ double xload(const double* p) {
 // result is independent of the tiling; points near the region edge read
 // clamped neighbours and are garbage, but no owned point depends on them
 // within S substeps.
-__device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile,
+                                                const unsigned long long epoch) {
+  extern __shared__ double2 lds2[];
   __shared__ int abort_flag;
   if (threadIdx.x == 0) abort_flag = 0;
   const int G = a.G, nj = a.nj, ni = a.ni, njm = a.nj - 1;
@@ -146,24 +170,18 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const int ny_lo = -((HL + TH - 1) / TH), ny_hi = (HR + TH - 1) / TH;
   const int nx_span = nx_hi - nx_lo + 1 > a.tiles_x ? 0 : 1;  // region wraps the whole circle
   const int wx_lo = nx_span ? nx_lo : 0, wx_hi = nx_span ? nx_hi : a.tiles_x - 1;
-  const unsigned long long base = a.epoch << 16;
+  const unsigned long long base = epoch << 16;
   const int RW = TW + HL + HR, RHt = TH + HL + HR;
   const int bx0 = x0 - HL, by0 = y0 - HL;
   const int NR = RW * RHt;
-  double* CHP = lds;
-  double* PR = CHP + NR;  // predictor ch
-  double* PU = PR + NR;   // barotropic velocity
-  double* PV = PU + NR;
-  double* S1 = PV + NR;   // prh*sch*ub fluxes
-  double* S2 = S1 + NR;
-  double* WAu = S2 + NR;  // Laplacian face fluxes of u and v
-  double* WBu = WAu + NR;
-  double* WAv = WBu + NR;
-  double* WBv = WAv + NR;
-  double* GX = WBv + NR;  // gradients of the predictor ch
-  double* GY = GX + NR;
-  double* rowt = GY + NR;         // [R_N][RHt]
-  double* colt = rowt + R_N * RHt; // [C_N][RW]
+  double2* CH = lds2 + P_CH * kTropBlock;
+  double2* UV = lds2 + P_UV * kTropBlock;
+  double2* SF = lds2 + P_SF * kTropBlock;
+  double2* GR = lds2 + P_GR * kTropBlock;
+  double2* WA = lds2 + P_WA * kTropBlock;
+  double2* WB = lds2 + P_WB * kTropBlock;
+  double* rowt = reinterpret_cast<double*>(lds2 + P_N * kTropBlock);  // [R_N][kTabStride]
+  double* colt = rowt + R_N * kTabStride;                             // [C_N][kTabStride]
   const double bt = a.bt;
   const int nstep = a.nstep;
 
@@ -177,32 +195,32 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     double* r = rowt + b;
     const double cor = a.cor[p];
     const double xx = 0.5 * cor * bt;
-    r[R_RH3 * RHt] = a.rh[3][p];
-    r[R_DX21 * RHt] = a.dx2[1][p];
-    r[R_ARH1 * RHt] = ARAD * a.rh[1][p];
-    r[R_RH4 * RHt] = a.rh[4][p];
-    r[R_DX22 * RHt] = a.dx2[2][p];
-    r[R_RH2 * RHt] = a.rh[2][p];
-    r[R_ARH2 * RHt] = ARAD * a.rh[2][p];
-    r[R_DX24 * RHt] = a.dx2[4][p];
-    r[R_DX23 * RHt] = a.dx2[3][p];
-    r[R_DEN * RHt] = ARAD * ARAD * a.rh[3][p];
-    r[R_COR * RHt] = cor;
-    r[R_DXX * RHt] = 1 + xx * xx;
+    r[R_RH3 * kTabStride] = a.rh[3][p];
+    r[R_DX21 * kTabStride] = a.dx2[1][p];
+    r[R_ARH1 * kTabStride] = ARAD * a.rh[1][p];
+    r[R_RH4 * kTabStride] = a.rh[4][p];
+    r[R_DX22 * kTabStride] = a.dx2[2][p];
+    r[R_RH2 * kTabStride] = a.rh[2][p];
+    r[R_ARH2 * kTabStride] = ARAD * a.rh[2][p];
+    r[R_DX24 * kTabStride] = a.dx2[4][p];
+    r[R_DX23 * kTabStride] = a.dx2[3][p];
+    r[R_DEN * kTabStride] = ARAD * ARAD * a.rh[3][p];
+    r[R_COR * kTabStride] = cor;
+    r[R_DXX * kTabStride] = 1 + xx * xx;
     const int div[] = {R_DX21, R_ARH1, R_RH4, R_DX22, R_ARH2, R_DX24, R_DX23, R_DEN, R_DXX};
-    for (int d : div) r[(d + 1) * RHt] = 1.0 / r[d * RHt];
+    for (int d : div) r[(d + 1) * kTabStride] = 1.0 / r[d * kTabStride];
   }
   for (int b = threadIdx.x; b < RW; b += blockDim.x) {
     int w = (bx0 + b) % G;
     if (w < 0) w += G;
     const long p = (long)(w + 1) + (long)ni;  // row 2
     double* c = colt + b;
-    c[C_DX11 * RW] = a.dx1[1][p];
-    c[C_DX14 * RW] = a.dx1[4][p];
-    c[C_DX12 * RW] = a.dx1[2][p];
-    c[C_DX13 * RW] = a.dx1[3][p];
+    c[C_DX11 * kTabStride] = a.dx1[1][p];
+    c[C_DX14 * kTabStride] = a.dx1[4][p];
+    c[C_DX12 * kTabStride] = a.dx1[2][p];
+    c[C_DX13 * kTabStride] = a.dx1[3][p];
     const int div[] = {C_DX11, C_DX14, C_DX12, C_DX13};
-    for (int d : div) c[(d + 1) * RW] = 1.0 / c[d * RW];
+    for (int d : div) c[(d + 1) * kTabStride] = 1.0 / c[d * kTabStride];
   }
 
   // ---- this thread's point, its neighbours and its static fields
@@ -219,9 +237,10 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   if (w < 0) w += G;
   const int we = w + 1 < G ? w + 1 : 0;
   const int yc = y < 1 ? 1 : (y > nj ? nj : y);
-  const long p = (long)(w + 1) + (long)ni * (yc - 1);
-  const long pe = (long)(we + 1) + (long)ni * (yc - 1);
-  const long pn = y < nj ? p + ni : p;
+  // 2-D offsets fit in int (ni * nj < 2^31 is checked at create)
+  const int p = (w + 1) + ni * (yc - 1);
+  const int pe = (we + 1) + ni * (yc - 1);
+  const int pn = y < nj ? p + ni : p;
   const bool owned = live && bx >= HL && bx < HL + TW && by >= HL && by < HL + TH && x < G;
   int levt = 0, levu = 0;
   double prh_u = 0, prh_t = 1, rprh_t = 1, chc0 = 0, am1 = 0, z1 = 0, z2 = 0, ghx = 0, ghy = 0, ut = 0, vt = 0;
@@ -249,9 +268,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     bgr1 = a.bgra[1][pn];
     bph3 = a.bphi[3][pe];
     bgr3 = a.bgra[3][pe];
-    CHP[q] = a.chp[p];
-    PU[q] = a.bup[p];
-    PV[q] = a.bvp[p];
+    CH[q].x = a.chp[p];
+    UV[q] = make_double2(a.bup[p], a.bvp[p]);
     if (owned) {
       acm = chc0;
       acm2 = chc0;
@@ -262,33 +280,37 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const double* rw = rowt + by;
   const double* rws = rowt + bys;
   const double* cl = colt + bx;
-#define ROW(n) rw[(n) * RHt]
-#define COL(n) cl[(n) * RW]
-  // divg<3,3,1>(F1, F2) (div_r2d, mod_op.f90:813-848): p1 and p2 with the
-  // pre-interpolations ter_ns(g3->g2) and ter_ew(g3->g4), / (a*rh_g1)
+#define ROW(n) rw[(n) * kTabStride]
+#define COL(n) cl[(n) * kTabStride]
+  // divg<3,3,1>(F1, F2) (div_r2d, mod_op.f90:813-848) of the fluxes in SF:
+  // p1 and p2 with the pre-interpolations ter_ns(g3->g2) and ter_ew(g3->g4),
+  // / (a*rh_g1)
   auto div331 = [&]() -> double {
     const bool r2 = y >= 2 && y <= nj;
-    const double t1 = r2 ? 0.5 * (S1[qs] + S1[q]) : 0.0;
-    const double t0 = r2 ? 0.5 * (S1[qsw] + S1[qw]) : 0.0;
+    const double2 fs = SF[qs], fq = SF[q], fsw = SF[qsw], fw = SF[qw];
+    const double t1 = r2 ? 0.5 * (fs.x + fq.x) : 0.0;
+    const double t0 = r2 ? 0.5 * (fsw.x + fw.x) : 0.0;
     const double a1 = qdiv(t1 - t0, COL(C_DX11), COL(C_DX11I));
     double d = 0.0;
     if (r2) {
-      const double rh3 = ROW(R_RH3), rh3s = rws[R_RH3 * RHt];
-      const double u1 = 0.5 * (S2[qw] * rh3 + S2[q] * rh3);
-      const double u0 = 0.5 * (S2[qsw] * rh3s + S2[qs] * rh3s);
+      const double rh3 = ROW(R_RH3), rh3s = rws[R_RH3 * kTabStride];
+      const double u1 = 0.5 * (fw.y * rh3 + fq.y * rh3);
+      const double u0 = 0.5 * (fsw.y * rh3s + fs.y * rh3s);
       d = u1 - u0;
     }
     const double b1 = qdiv(d, ROW(R_DX21), ROW(R_DX21I));
     return qdiv(a1 + b1, ROW(R_ARH1), ROW(R_ARH1I));
   };
-  // sqrt(ter(v, g1 -> g3)) on sea U points, 1 on land
+  // sqrt(ter(v, g1 -> g3)) on sea U points, 1 on land, from v at the point
+  // and its north, east and north-east neighbours
   const bool rin = y >= 1 && y <= njm;
-  auto sch_of = [&](const double* v) -> double {
+  auto sch_of = [&](double v, double vn, double ve, double vne) -> double {
     if (levu <= 0) return 1.0;
-    const double t = rin ? (v[q] + v[qn] + v[qe] + v[qne]) * 0.25 : 0.0;
+    const double t = rin ? (v + vn + ve + vne) * 0.25 : 0.0;
     return sqrt(t);
   };
   const double su = levu >= 1 ? 1.0 : 0.0;
+  const double sam1 = su * am1;  // su * am1 * lap, left to right
   const double mt = levt >= 1 ? 1.0 : 0.0;
   __syncthreads();
 
@@ -297,56 +319,62 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     // A) predictor (mod_int.f90:184-193): sch = sqrt(ter(chp)) (1 on land),
     // fluxes prh*sch*ub_p, chc = chp - gamma_b*dtbt*div/prh on sea
     if (live) {
-      const double sc = sch_of(CHP);
-      S1[q] = prh_u * sc * PU[q];
-      S2[q] = prh_u * sc * PV[q];
+      const double sc = sch_of(CH[q].x, CH[qn].x, CH[qe].x, CH[qne].x);
+      const double2 uv = UV[q];
+      SF[q] = make_double2(prh_u * sc * uv.x, prh_u * sc * uv.y);
     }
     __syncthreads();
     if (live) {
       double wk = div331();
-      wk = CHP[q] - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
+      const double chp = CH[q].x;
+      wk = chp - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
       // land keeps its ch: the entry value in the first substep, afterwards
       // the corrector's chp - 0 (mod_int.f90:193, 267)
-      PR[q] = levt > 0 ? wk : (nt == 1 ? chc0 : CHP[q]);
+      CH[q].y = levt > 0 ? wk : (nt == 1 ? a.chc[p] : chp);
     }
     __syncthreads();
     // B) barotropic momentum (mod_int.f90:195-261).  Face fluxes of op_lap
     // (mod_op.f90:913-920) and gradients of the predictor ch per point.
     double sch = 1.0;
+    double2 uv = make_double2(0.0, 0.0);
     if (live) {
-      sch = sch_of(PR);
+      const double pr = CH[q].y, pre = CH[qe].y, prn = CH[qn].y;
+      sch = sch_of(pr, prn, pre, CH[qne].y);
       const bool r2 = y >= 2 && y <= nj;
-      const double pu = PU[q], pv = PV[q];
-      WAu[q] = qdiv(qdiv(pu - PU[qw], COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I));
-      WAv[q] = qdiv(qdiv(pv - PV[qw], COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I));
-      WBu[q] = qdiv(r2 ? pu - PU[qs] : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2);
-      WBv[q] = qdiv(r2 ? pv - PV[qs] : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2);
-      GX[q] = qdiv(qdiv(PR[qe] - PR[q], COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I));
-      GY[q] = qdiv(qdiv(rin ? PR[qn] - PR[q] : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD);
+      uv = UV[q];
+      const double2 uvw = UV[qw], uvs = UV[qs];
+      WA[q] = make_double2(qdiv(qdiv(uv.x - uvw.x, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)),
+                           qdiv(qdiv(uv.y - uvw.y, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)));
+      WB[q] = make_double2(qdiv(r2 ? uv.x - uvs.x : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2),
+                           qdiv(r2 ? uv.y - uvs.y : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2));
+      GR[q] = make_double2(qdiv(qdiv(pre - pr, COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I)),
+                           qdiv(qdiv(rin ? prn - pr : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD));
     }
     __syncthreads();
     double un = 0, vn = 0;
     if (live) {
-      const double l1 = qdiv(qdiv(WAu[qe] - WAu[q], COL(C_DX13), COL(C_DX13I)) +
-                                 qdiv(rin ? WBu[qn] - WBu[q] : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+      const double2 wa = WA[q], wae = WA[qe], wb = WB[q], wbn = WB[qn];
+      const double l1 = qdiv(qdiv(wae.x - wa.x, COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv(rin ? wbn.x - wb.x : 0.0, ROW(R_DX23), ROW(R_DX23I)),
                              ROW(R_DEN), ROW(R_DENI));
-      const double l2 = qdiv(qdiv(WAv[qe] - WAv[q], COL(C_DX13), COL(C_DX13I)) +
-                                 qdiv(rin ? WBv[qn] - WBv[q] : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+      const double l2 = qdiv(qdiv(wae.y - wa.y, COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv(rin ? wbn.y - wb.y : 0.0, ROW(R_DX23), ROW(R_DX23I)),
                              ROW(R_DEN), ROW(R_DENI));
-      const double vs1 = su * am1 * l1;
-      const double vs2 = su * am1 * l2;
+      const double vs1 = sam1 * l1;
+      const double vs2 = sam1 * l2;
+      const double2 g = GR[q], gn = GR[qn], ge = GR[qe];
+      const double pr = CH[q].y, pre = CH[qe].y, prn = CH[qn].y, prne = CH[qne].y;
       double gi1 = 0.0, ci1 = 0.0;  // row nj of grachint/chgraint%x(1) is never written
       if (y <= njm) {
-        gi1 = 0.5 * (GX[q] * bph0 + GX[qn] * bph1);
-        ci1 = 0.5 * (0.5 * (PR[q] + PR[qe]) * bgr0 + 0.5 * (PR[qn] + PR[qne]) * bgr1);
+        gi1 = 0.5 * (g.x * bph0 + gn.x * bph1);
+        ci1 = 0.5 * (0.5 * (pr + pre) * bgr0 + 0.5 * (prn + prne) * bgr1);
       }
-      const double gi2 = 0.5 * (GY[q] * bph2 + GY[qe] * bph3);
-      const double ci2 = 0.5 * ((rin ? 0.5 * (PR[q] + PR[qn]) : 0.0) * bgr2 +
-                                (rin ? 0.5 * (PR[qe] + PR[qne]) : 0.0) * bgr3);
+      const double gi2 = 0.5 * (g.y * bph2 + ge.y * bph3);
+      const double ci2 = 0.5 * ((rin ? 0.5 * (pr + prn) : 0.0) * bgr2 + (rin ? 0.5 * (pre + prne) : 0.0) * bgr3);
       const double cor = ROW(R_COR);
       const double pg1 = (gi1 + ci1 + ghx) * su * sch;
       const double pg2 = (gi2 + ci2 + ghy) * su * sch;
-      const double up = PU[q], vp = PV[q];
+      const double up = uv.x, vp = uv.y;
       double pu1 = su * (vs1 - z1 + ut - pg1 + 0.5 * cor * vp);
       double pu2 = su * (vs2 - z2 + vt - pg2 - 0.5 * cor * up);
       const double w1 = pu1 * bt + up;
@@ -359,21 +387,19 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       pu2 = su * (vs2 - z2 - pg2 + vt - cor * uc);
       un = pu1 * bt + up;
       vn = pu2 * bt + vp;
-      // the new velocity is the next substep's ub_p; no thread reads PU
-      // or PV at another point in this stage
-      PU[q] = un;
-      PV[q] = vn;
+      // the new velocity is the next substep's ub_p; no thread reads UV at
+      // another point in this stage
+      UV[q] = make_double2(un, vn);
       // corrector fluxes prh*sch*ub (mod_int.f90:264-265)
-      S1[q] = prh_u * sch * un;
-      S2[q] = prh_u * sch * vn;
+      SF[q] = make_double2(prh_u * sch * un, prh_u * sch * vn);
     }
     __syncthreads();
     // C) corrector ch and the time-filter accumulators (mod_int.f90:263-273)
     double cn = 0;
     if (live) {
       const double wk = div331();
-      cn = CHP[q] - qdiv(mt * wk * bt, prh_t, rprh_t);
-      CHP[q] = cn;
+      cn = CH[q].x - qdiv(mt * wk * bt, prh_t, rprh_t);
+      CH[q].x = cn;
       if (owned) {
         if (nt <= nstep / 2) acm = acm + cn;
         acu = acu + un;
@@ -382,19 +408,22 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       }
     }
     if (nt < nstep && nt % a.S == 0) {
-      const int rb = round & 1;
+      // uniform selects keep the buffer addresses in scalar registers
+      const bool odd = round & 1;
+      double* const xc = (odd ? a.xc[1] : a.xc[0]) + p;
+      double* const xu = (odd ? a.xu[1] : a.xu[0]) + p;
+      double* const xv = (odd ? a.xv[1] : a.xv[0]) + p;
       if (owned) {
-        xstore(&a.xc[rb][p], cn);
-        xstore(&a.xu[rb][p], un);
-        xstore(&a.xv[rb][p], vn);
+        xstore(xc, cn);
+        xstore(xu, un);
+        xstore(xv, vn);
       }
       tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
-      if (abort_flag) return;
-      // reload the whole region (own point included)
-      if (live) {
-        CHP[q] = xload(&a.xc[rb][p]);
-        PU[q] = xload(&a.xu[rb][p]);
-        PV[q] = xload(&a.xv[rb][p]);
+      if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
+      // the halo from the neighbours (owned points already hold these values)
+      if (live && !owned) {
+        CH[q].x = xload(xc);
+        UV[q] = make_double2(xload(xu), xload(xv));
       }
       ++round;
     }
@@ -406,7 +435,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   // every tile of the region has read its entry fields before they are
   // overwritten below
   tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
-  if (abort_flag || !owned) return;
+  if (__builtin_amdgcn_readfirstlane(abort_flag) || !owned) return;
   // end of the cycle (mod_int.f90:276-287), owned points and their halo
   // copies (column 1 mirrors column ni-1, column ni mirrors column 2)
   const double u = acu / (double)(nstep + 1);
@@ -430,7 +459,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   }
 }
 
-__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body(a, blockIdx.x, a.epoch); }
 
 // The subcycles of several contexts (ensemble members) in one launch:
 // workgroups [start[m], start[m+1]) are member m's tiles, each member with
@@ -447,9 +476,7 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_multi(const TropTileArgs* _
   const int b = blockIdx.x;
   int m = 0;
   while (m + 1 < ml.n && b >= ml.start[m + 1]) ++m;
-  TropTileArgs a = args[m];
-  a.epoch = ml.epoch[m];
-  trop_tiles_body(a, b - ml.start[m]);
+  trop_tiles_body(args[m], b - ml.start[m], ml.epoch[m]);
 }
 
 // vs at the first substep (mod_int.f90:198-207): seau*am(:,:,1)*lap(ub),
@@ -540,6 +567,8 @@ std::string trop_configure(Ctx& c) {
     return "barotropic tile " + std::to_string(c.trop_tw) + "x" + std::to_string(c.trop_th) + "x" +
            std::to_string(c.trop_s) + ": the tile plus its halo of 5 points per substep must hold at most " +
            std::to_string(kTropBlock) + " points";
+  if (c.trop_tw + 5 * c.trop_s > kTabStride || c.trop_th + 5 * c.trop_s > kTabStride)
+    return "barotropic tile sides plus halo must be at most " + std::to_string(kTabStride);
   size_t lds = trop_lds_bytes(c.trop_tw, c.trop_th, c.trop_s);
   long resident;
   if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
