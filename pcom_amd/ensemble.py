@@ -17,7 +17,7 @@ from the same inputs, and the fastest combination is applied to the
 ensemble. Throughput is reported as aggregate simulated years per day over
 all members, beside the same members stepped one after another.
 
-    python -m pcom_amd.ensemble --steps 48 [--n2 24 --n1 8] [--out report.json]
+    python -m pcom_amd.ensemble --days 365 [--n2 24 --n1 8] [--out report.json]
 """
 import argparse
 import itertools
@@ -50,17 +50,39 @@ def member_configs(n2=24, n1=8, seed0=100, sd=0.01):
     return out
 
 
+def member_spread(members, cfgs):
+    """Per grid size, the area-weighted RMS over the sea of the members'
+    SST standard deviation (K): how far the perturbed members have drifted
+    apart."""
+    out = {}
+    for deg in sorted({d for d, _ in cfgs}):
+        sst = np.stack([m.get("tc")[1:-1, 1:-1, 0] for (d, _), m in zip(cfgs, members) if d == deg])
+        cfg = next(c for d, c in cfgs if d == deg)
+        wet = cfg.itn > 0
+        w = np.cos(np.deg2rad(cfg.lat))[None, :] * wet
+        sd = sst.std(axis=0)
+        out[f"{deg}deg"] = float(np.sqrt((w * sd ** 2).sum() / w.sum()))
+    return out
+
+
 def sypd(n_steps, dtbc, wall_s):
     return (n_steps * dtbc / YEAR_S) / (wall_s / 86400.0)
 
 
-def time_ensemble(members, steps, warmup=1):
+def time_ensemble(members, steps, warmup=1, chunk=0, log=None):
     ens = Ensemble(members)
     ens.step(warmup)
     for m in members:
         m.sync()
     t0 = time.perf_counter()
-    ens.step(steps)
+    done = 0
+    while done < steps:
+        n = min(chunk or steps, steps - done)
+        ens.step(n)
+        done += n
+        if log and done < steps:
+            members[0].sync()
+            log(f"  {done} of {steps} ensemble steps, {time.perf_counter() - t0:.1f} s")
     for m in members:
         m.sync()
     wall = time.perf_counter() - t0
@@ -111,14 +133,20 @@ def main(argv=None):
     ap.add_argument("--n2", type=int, default=24, help="members at 2 degrees")
     ap.add_argument("--n1", type=int, default=8, help="members at 1 degree")
     ap.add_argument("--form", default="mixed", choices=list(abi.FORMS))
-    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=48, help="ensemble steps timed")
+    ap.add_argument("--days", type=float, default=0, help="simulated days instead of --steps (365 = one year)")
+    ap.add_argument("--compare-steps", type=int, default=48,
+                    help="steps of the same members stepped one after another, for comparison")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     prec = abi.FORMS[args.form]
     cfgs = member_configs(args.n2, args.n1)
     dtbc = cfgs[0][1].dtbc
-    report = {"members": {"2deg": args.n2, "1deg": args.n1}, "form": args.form, "steps": args.steps}
+    if args.days > 0:
+        args.steps = int(round(args.days * 86400 / dtbc))
+    report = {"members": {"2deg": args.n2, "1deg": args.n1}, "form": args.form, "steps": args.steps,
+              "simulated_days": args.steps * dtbc / 86400}
     choice = None
     if not args.no_tune:
         t0 = time.time()
@@ -129,23 +157,29 @@ def main(argv=None):
     if choice:
         for (deg, _), m in zip(cfgs, members):
             m.tiling(choice[deg])
-    wall, npk = time_ensemble(members, args.steps)
-    report["ensemble"] = {"ms_per_step": 1000.0 * wall / args.steps, "subcycle_launches_per_step": npk,
+    print(f"stepping {len(members)} members for {args.steps} steps", flush=True)
+    wall, npk = time_ensemble(members, args.steps, chunk=500, log=lambda s: print(s, flush=True))
+    sums = [m.global_sums() for m in members]
+    report["ensemble"] = {"ms_per_step": 1000.0 * wall / args.steps, "wall_s": wall,
+                          "subcycle_launches_per_step": npk,
                           "aggregate_sypd": len(members) * sypd(args.steps, dtbc, wall),
-                          "finite": all(m.check_finite() for m in members)}
+                          "finite": all(m.check_finite() for m in members),
+                          "heat_salt_mass_first_member": [float(x) for x in sums[0]],
+                          "sst_spread_k": member_spread(members, cfgs)}
     # the same members one after another (each pcom_step in turn)
+    ns = args.compare_steps
     for m in members:
         m.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(ns):
         for m in members:
             m.step(1)
     for m in members:
         m.sync()
     wall_seq = time.perf_counter() - t0
-    report["one_after_another"] = {"ms_per_step": 1000.0 * wall_seq / args.steps,
-                                   "aggregate_sypd": len(members) * sypd(args.steps, dtbc, wall_seq)}
-    report["speedup"] = wall_seq / wall
+    report["one_after_another"] = {"steps": ns, "ms_per_step": 1000.0 * wall_seq / ns,
+                                   "aggregate_sypd": len(members) * sypd(ns, dtbc, wall_seq)}
+    report["speedup"] = (wall_seq / ns) / (wall / args.steps)
     print(json.dumps(report), flush=True)
     if args.out:
         with open(args.out, "w") as f:
