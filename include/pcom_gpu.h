@@ -90,7 +90,16 @@ typedef struct pcom_opts {
   int trop_tile_i, trop_tile_j; /* barotropic subcycle tile (0, 0 and 0
                                    substeps = the library's choice)        */
   int trop_substeps;            /* substeps between tile exchanges         */
+  int trop_mode;   /* PCOM_TROP_AUTO, _PERSISTENT or _LAUNCHES             */
 } pcom_opts;
+
+/* How the barotropic subcycle runs.  PERSISTENT: one launch per baroclinic
+ * step whose tiles stay resident and exchange halos through device memory
+ * (needs every tile co-resident: grids up to about 1 degree).  LAUNCHES:
+ * one launch per `substeps` substeps, the kernel boundary being the
+ * exchange (any grid size).  AUTO: PERSISTENT when the grid fits, else
+ * LAUNCHES.  The results are the same bits in every mode.               */
+enum { PCOM_TROP_AUTO = 0, PCOM_TROP_PERSISTENT = 1, PCOM_TROP_LAUNCHES = 2 };
 
 typedef struct pcom_ctx pcom_ctx;
 
@@ -254,10 +263,12 @@ int pcom_ensemble_destroy(pcom_ensemble *ens);
  * restoring the state after each, and keep the fastest.  Returns the chosen
  * tiling and its time per launch.  The state is unchanged.               */
 int pcom_tune_subcycle(pcom_ctx *ctx, int trials, int tile[3], double *ms);
-/* Set the subcycle tiling {tile i, tile j, substeps per exchange} (as
- * pcom_opts.trop_tile_* at create; all zero = the cost model's choice) or
- * read it (set = 0).  The results do not depend on the tiling.           */
-int pcom_subcycle_tiling(pcom_ctx *ctx, int set, int tile[3]);
+/* Set the subcycle tiling {tile i, tile j, substeps per exchange, mode}
+ * (as pcom_opts.trop_tile_*, trop_substeps and trop_mode at create; zero
+ * tile sizes and substeps = the cost model's choice for the mode) or read
+ * the tiling in use, its mode resolved (set = 0).  The results do not
+ * depend on the tiling or the mode.                                      */
+int pcom_subcycle_tiling(pcom_ctx *ctx, int set, int tile[4]);
 
 /* ---- Learned barotropic surrogate.  New: the reference has no such part.
  * A BF16 convolutional network (six 3x3 layers, 64 hidden channels, FP32

@@ -15,6 +15,10 @@ PCOM_ERR_NOGPU = -5
 PCOM_FP64 = 0
 PCOM_MIXED = 1
 PCOM_MIXED_EOS32 = 2
+
+PCOM_TROP_AUTO = 0
+PCOM_TROP_PERSISTENT = 1
+PCOM_TROP_LAUNCHES = 2
 FORMS = {"fp64": PCOM_FP64, "mixed": PCOM_MIXED, "mixed_eos32": PCOM_MIXED_EOS32}
 
 _DP = C.POINTER(C.c_double)
@@ -37,6 +41,7 @@ class PcomOpts(C.Structure):
         ("year", C.c_int), ("month", C.c_int), ("day", C.c_int), ("hour", C.c_int),
         ("precision", C.c_int), ("restart_run", C.c_int), ("device", C.c_int),
         ("trop_tile_i", C.c_int), ("trop_tile_j", C.c_int), ("trop_substeps", C.c_int),
+        ("trop_mode", C.c_int),
     ]
 
 

@@ -96,9 +96,11 @@ class Config:
             for a in arrs:
                 f.write(np.asfortranarray(a, dtype="<f8").tobytes(order="F"))
 
-    def c_opts(self, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0)):
+    def c_opts(self, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0),
+               trop_mode=abi.PCOM_TROP_AUTO):
         o = abi.PcomOpts()
         o.trop_tile_i, o.trop_tile_j, o.trop_substeps = trop_tile
+        o.trop_mode = trop_mode
         o.dtbt, o.dtbc = self.dtbt, self.dtbc
         o.year, o.month, o.day, o.hour = self.start
         o.precision = precision

@@ -250,6 +250,8 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if (opts->trop_tile_i < 0 || opts->trop_tile_j < 0 || opts->trop_substeps < 0 || opts->trop_tile_i > 256 ||
       opts->trop_tile_j > 256 || opts->trop_substeps > 16)
     return fail(PCOM_ERR_ARG, "barotropic tiling out of range (tile <= 256, substeps per exchange <= 16)");
+  if (opts->trop_mode < PCOM_TROP_AUTO || opts->trop_mode > PCOM_TROP_LAUNCHES)
+    return fail(PCOM_ERR_ARG, "trop_mode must be PCOM_TROP_AUTO, PCOM_TROP_PERSISTENT or PCOM_TROP_LAUNCHES");
   if (opts->precision != PCOM_FP64 && opts->precision != PCOM_MIXED && opts->precision != PCOM_MIXED_EOS32)
     return fail(PCOM_ERR_ARG, "precision must be PCOM_FP64, PCOM_MIXED or PCOM_MIXED_EOS32");
   if (opts->month < 1 || opts->month > 12 || opts->day < 1 || opts->day > 31 || opts->year < 1)
@@ -335,7 +337,8 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   for (double** a : ones)
     if ((rc = make2(*c, a, n2, nullptr, 1.0))) return rc;
   double** zeros[] = {&h.acch, &h.acssh, &h.ssh, &h.buc, &h.bvc, &h.bup, &h.bvp, &h.but, &h.bvt, &h.pax, &h.pay,
-                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1]};
+                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1],
+                      &h.xacc[0], &h.xacc[1], &h.xacc[2], &h.xacc[3]};
   for (double** a : zeros)
     if ((rc = make2(*c, a, n2, nullptr))) return rc;
   for (int q = 0; q < 4; ++q) {
@@ -358,6 +361,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   c->trop_tw = opts->trop_tile_i;
   c->trop_th = opts->trop_tile_j;
   c->trop_s = opts->trop_substeps;
+  c->trop_mode = opts->trop_mode;
   {
     // exchange flags for as many tiles as the device can hold co-resident
     hipDeviceProp_t prop;
@@ -581,23 +585,26 @@ int pcom_tune_subcycle(pcom_ctx* ctx, int trials, int tile[3], double* ms) {
   return PCOM_OK;
 }
 
-int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[3]) {
+int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[4]) {
   if (!ctx || !tile) return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: NULL argument");
   Ctx* c = C(ctx);
   if (set) {
-    if (tile[0] < 0 || tile[1] < 0 || tile[2] < 0 || tile[0] > 256 || tile[1] > 256 || tile[2] > 16)
+    if (tile[0] < 0 || tile[1] < 0 || tile[2] < 0 || tile[0] > 256 || tile[1] > 256 || tile[2] > 16 ||
+        tile[3] < PCOM_TROP_AUTO || tile[3] > PCOM_TROP_LAUNCHES)
       return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: tiling out of range");
     HIPCHK(hipSetDevice(c->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(c->stream), "sync");
-    const int keep[3] = {c->trop_tw, c->trop_th, c->trop_s};
+    const int keep[4] = {c->trop_tw, c->trop_th, c->trop_s, c->trop_mode};
     c->trop_tw = tile[0];
     c->trop_th = tile[1];
     c->trop_s = tile[2];
+    c->trop_mode = tile[3];
     std::string e = trop_configure(*c);
     if (!e.empty()) {
       c->trop_tw = keep[0];
       c->trop_th = keep[1];
       c->trop_s = keep[2];
+      c->trop_mode = keep[3];
       (void)trop_configure(*c);
       return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: " + e);
     }
@@ -605,6 +612,7 @@ int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[3]) {
   tile[0] = c->trop_tw;
   tile[1] = c->trop_th;
   tile[2] = c->trop_s;
+  tile[3] = c->trop_launches ? PCOM_TROP_LAUNCHES : PCOM_TROP_PERSISTENT;
   return PCOM_OK;
 }
 

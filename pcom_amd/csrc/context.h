@@ -38,6 +38,7 @@ struct Fields2 {
   double *vz1, *vz2, *am1;                      // subcycle statics
   double *schr, *sqp, *schs, *schm;             // sqrt(ter(ch)) factors on g3
   double *xc[2], *xu[2], *xv[2];                // subcycle tile exchange
+  double *xacc[4];  // subcycle accumulators between launches (PCOM_TROP_LAUNCHES)
   double *red;                                  // reduction scratch
   int *flag;
 };
@@ -61,6 +62,8 @@ struct Ctx {
   Fields2 f2{};
   std::vector<void*> allocs;
   int trop_tw = 0, trop_th = 0, trop_s = 0;   // barotropic tile and substeps/exchange
+  int trop_mode = 0;                          // requested PCOM_TROP_* mode
+  bool trop_launches = false;                 // mode in use: one launch per exchange
   int trop_tiles_x = 0, trop_blocks = 0;
   size_t trop_lds = 0;
   unsigned long long* trop_flags = nullptr;  // per-tile exchange flags (zeroed at create)

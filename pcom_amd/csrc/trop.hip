@@ -9,9 +9,14 @@
 // (predictor div -> sch -> viscosity / pressure gradient -> corrector div),
 // so S substeps run on LDS alone, recomputing the halo redundantly; then
 // tiles exchange their owned points through double-buffered global arrays
-// and wait for their neighbour tiles' flags.  The arithmetic per point is
-// the same sequence of IEEE operations as the reference's array
-// statements, so the result does not depend on the tiling.
+// and wait for their neighbour tiles' flags.  That needs every tile
+// co-resident, which holds up to about 1 degree; larger grids run the same
+// kernel in launch mode instead: one launch per S substeps over all tiles,
+// each tile loading its region from the previous launch's buffer and
+// storing its owned points for the next, the kernel boundary being the
+// exchange.  The arithmetic per point is the same sequence of IEEE
+// operations as the reference's array statements, so the result depends on
+// neither the tiling nor the mode.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,10 +57,15 @@ struct TropTileArgs {
   const double *bphi[4], *bgra[4];
   double *chc, *chp, *mch, *mch2, *acch, *buc, *bvc, *bup, *bvp;
   double *xc[2], *xu[2], *xv[2];
+  double* acc[4];             // accumulators between launches (launch mode)
   unsigned long long* flags;  // per tile, kFlagStride apart: epoch * 2^16 + round
   int* error;                 // set when a neighbour wait exceeds its bound
   unsigned long long epoch;
   int nstep, S, TW, TH, tiles_x, tiles_y;
+  // launch mode (k_trop_block): substeps nt0..nt1 of this launch; the state
+  // enters from buffer `src` of xc/xu/xv (nt0 > 1) and leaves into buffer
+  // `dst` (nt1 < nstep)
+  int nt0, nt1, src, dst;
   double bt;
 };
 
@@ -154,6 +164,7 @@ __device__ __forceinline__ double xload(const double* p) {
 // result is independent of the tiling; points near the region edge read
 // clamped neighbours and are garbage, but no owned point depends on them
 // within S substeps.
+template <bool kLaunches>
 __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile,
                                                 const unsigned long long epoch) {
   extern __shared__ double2 lds2[];
@@ -268,13 +279,25 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     bgr1 = a.bgra[1][pn];
     bph3 = a.bphi[3][pe];
     bgr3 = a.bgra[3][pe];
-    CH[q].x = a.chp[p];
-    UV[q] = make_double2(a.bup[p], a.bvp[p]);
-    if (owned) {
-      acm = chc0;
-      acm2 = chc0;
-      acu = a.buc[p];
-      acv = a.bvc[p];
+    if (!kLaunches || a.nt0 == 1) {
+      CH[q].x = a.chp[p];
+      UV[q] = make_double2(a.bup[p], a.bvp[p]);
+      if (owned) {
+        acm = chc0;
+        acm2 = chc0;
+        acu = a.buc[p];
+        acv = a.bvc[p];
+      }
+    } else {  // launch mode: the previous launch's state
+      const bool s1 = a.src != 0;
+      CH[q].x = (s1 ? a.xc[1] : a.xc[0])[p];
+      UV[q] = make_double2((s1 ? a.xu[1] : a.xu[0])[p], (s1 ? a.xv[1] : a.xv[0])[p]);
+      if (owned) {
+        acm = a.acc[0][p];
+        acm2 = a.acc[1][p];
+        acu = a.acc[2][p];
+        acv = a.acc[3][p];
+      }
     }
   }
   const double* rw = rowt + by;
@@ -315,7 +338,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   __syncthreads();
 
   int round = 0;
-  for (int nt = 1; nt <= nstep; ++nt) {
+  const int nt1 = kLaunches ? a.nt1 : nstep;
+  for (int nt = kLaunches ? a.nt0 : 1; nt <= nt1; ++nt) {
     // A) predictor (mod_int.f90:184-193): sch = sqrt(ter(chp)) (1 on land),
     // fluxes prh*sch*ub_p, chc = chp - gamma_b*dtbt*div/prh on sea
     if (live) {
@@ -407,7 +431,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
         acm2 = acm2 + cn;
       }
     }
-    if (nt < nstep && nt % a.S == 0) {
+    if (!kLaunches && nt < nt1 && nt % a.S == 0) {
       // uniform selects keep the buffer addresses in scalar registers
       const bool odd = round & 1;
       double* const xc = (odd ? a.xc[1] : a.xc[0]) + p;
@@ -432,9 +456,25 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
 #undef ROW
 #undef COL
 
+  if (kLaunches && nt1 < nstep) {  // hand the owned points to the next launch
+    if (owned) {
+      const bool d1 = a.dst != 0;
+      const double2 uv = UV[q];
+      (d1 ? a.xc[1] : a.xc[0])[p] = CH[q].x;
+      (d1 ? a.xu[1] : a.xu[0])[p] = uv.x;
+      (d1 ? a.xv[1] : a.xv[0])[p] = uv.y;
+      a.acc[0][p] = acm;
+      a.acc[1][p] = acm2;
+      a.acc[2][p] = acu;
+      a.acc[3][p] = acv;
+    }
+    return;
+  }
   // every tile of the region has read its entry fields before they are
-  // overwritten below
-  tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
+  // overwritten below (in launch mode the entry fields were read by an
+  // earlier launch)
+  if (!kLaunches)
+    tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
   if (__builtin_amdgcn_readfirstlane(abort_flag) || !owned) return;
   // end of the cycle (mod_int.f90:276-287), owned points and their halo
   // copies (column 1 mirrors column ni-1, column ni mirrors column 2)
@@ -459,7 +499,13 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   }
 }
 
-__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body(a, blockIdx.x, a.epoch); }
+__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body<false>(a, blockIdx.x, a.epoch); }
+
+// launch mode: tiles of a launch do not wait for each other, so neighbouring
+// tiles go to one XCD, whose L2 then serves their shared halo
+__global__ void __launch_bounds__(kTropBlock) k_trop_block(TropTileArgs a) {
+  trop_tiles_body<true>(a, (int)xcd_block(), 0);
+}
 
 // The subcycles of several contexts (ensemble members) in one launch:
 // workgroups [start[m], start[m+1]) are member m's tiles, each member with
@@ -476,7 +522,7 @@ __global__ void __launch_bounds__(kTropBlock) k_trop_multi(const TropTileArgs* _
   const int b = blockIdx.x;
   int m = 0;
   while (m + 1 < ml.n && b >= ml.start[m + 1]) ++m;
-  trop_tiles_body(args[m], b - ml.start[m], ml.epoch[m]);
+  trop_tiles_body<false>(args[m], b - ml.start[m], ml.epoch[m]);
 }
 
 // vs at the first substep (mod_int.f90:198-207): seau*am(:,:,1)*lap(ub),
@@ -500,25 +546,77 @@ __global__ void k_vz(Grid<double> g, const R* __restrict__ am, const double* __r
 }  // namespace
 
 namespace {
-// co-resident workgroups of k_trop_tiles with `lds` bytes of LDS each
+// co-resident workgroups of k_trop_tiles (and of k_trop_block, whichever
+// is fewer) with `lds` bytes of LDS each
 std::string trop_resident(const Ctx& c, size_t lds, long* resident) {
   hipDeviceProp_t prop;
   hipError_t e = hipGetDeviceProperties(&prop, c.device);
   if (e != hipSuccess) return std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
   *resident = 0;
   if (lds > (size_t)prop.sharedMemPerBlock) return "";
-  e = hipFuncSetAttribute((const void*)k_trop_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
-  int per_cu = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop_tiles, kTropBlock, lds);
-  if (e != hipSuccess) return std::string("occupancy of k_trop_tiles: ") + hipGetErrorString(e);
+  int per_cu = 1 << 30;
+  for (const void* k : {(const void*)k_trop_tiles, (const void*)k_trop_block}) {
+    e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
+    int n = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kTropBlock, lds);
+    if (e != hipSuccess) return std::string("occupancy of the subcycle kernels: ") + hipGetErrorString(e);
+    per_cu = std::min(per_cu, n);
+  }
   *resident = (long)per_cu * prop.multiProcessorCount;
+  return "";
+}
+}  // namespace
+
+namespace {
+// A candidate tiling and its modelled cost per cycle, in units of one
+// wavefront of region points through one substep.  Persistent mode: every
+// substep costs the region's wavefronts and an exchange about
+// kExchangeWaves units.  Launch mode: each launch runs its tiles in waves
+// of co-resident workgroups, each paying its substeps plus about
+// kLoadSubsteps substeps for loading its region and statics, and a launch
+// adds about kLaunchWaves units.
+struct TropCand {
+  long cost;
+  int tw, th, s;
+};
+constexpr long kExchangeWaves = 14, kLoadSubsteps = 2, kLaunchWaves = 14;
+
+std::string trop_candidates(const Ctx& c, bool launches, std::vector<TropCand>* out) {
+  const int G = c.ni - 2;
+  const long nstep = (long)c.dtbc / c.dtbt * 2;
+  long resident;
+  std::string err = trop_resident(c, trop_lds_bytes(0, 0, 0), &resident);
+  if (!err.empty()) return err;
+  out->clear();
+  if (resident == 0) return "";
+  for (int s = 1; s <= 4; ++s) {
+    if (launches && s >= nstep) continue;  // a cycle takes at least two launches
+    for (int tw = 4; tw <= 64; ++tw)
+      for (int th = 4; th <= 64; ++th) {
+        const long nr = (long)(tw + 5 * s) * (th + 5 * s);
+        if (nr > kTropBlock) continue;
+        const long waves = (nr + 63) / 64;
+        const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
+        long cost;
+        if (!launches) {
+          if (tiles > resident || (c.trop_flag_cap > 0 && tiles > c.trop_flag_cap)) continue;
+          cost = nstep * waves + (nstep / s) * kExchangeWaves;
+        } else {
+          const long nl = (nstep + s - 1) / s, rounds = (tiles + resident - 1) / resident;
+          cost = nl * (rounds * (s + kLoadSubsteps) * waves + kLaunchWaves);
+        }
+        out->push_back({cost, tw, th, s});
+      }
+  }
+  std::stable_sort(out->begin(), out->end(), [](const TropCand& a, const TropCand& b) { return a.cost < b.cost; });
   return "";
 }
 }  // namespace
 
 std::string trop_configure(Ctx& c) {
   const int G = c.ni - 2;
+  const long nstep = (long)c.dtbc / c.dtbt * 2;
   // the LDS tiles keep rh, dx%x(2) and cor per row and dx%x(1) per column
   const HostSetup& hs = c.hs;
   for (int s = 1; s <= 4; ++s)
@@ -531,36 +629,29 @@ std::string trop_configure(Ctx& c) {
           return "internal: horizontal metrics are not separable in (i, j)";
       }
   std::string err;
+  const size_t lds = trop_lds_bytes(0, 0, 0);
+  long resident;
+  if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
+  if (resident == 0) return "the subcycle's " + std::to_string(lds) + " B of LDS do not fit this device";
+  bool launches = c.trop_mode == PCOM_TROP_LAUNCHES;
   if (c.trop_tw > 0 || c.trop_th > 0 || c.trop_s > 0) {
     if (c.trop_tw <= 0) c.trop_tw = 24;
     if (c.trop_th <= 0) c.trop_th = 12;
     if (c.trop_s <= 0) c.trop_s = 2;
   } else {
-    // default: the tiling of least modelled cost among those whose tiles
-    // are all co-resident.  A substep costs about one unit per wavefront of
-    // region points, an exchange about kExchangeWaves units.
-    constexpr long kExchangeWaves = 14;
-    const long nstep = (long)c.dtbc / c.dtbt * 2;
-    long best = -1;
-    for (int s = 1; s <= 4; ++s)
-      for (int tw = 4; tw <= 64; ++tw)
-        for (int th = 4; th <= 64; ++th) {
-          const long nr = (long)(tw + 5 * s) * (th + 5 * s);
-          if (nr > kTropBlock) continue;
-          const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
-          size_t lds = trop_lds_bytes(tw, th, s);
-          long resident;
-          if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
-          if (tiles > resident) continue;
-          const long cost = nstep * ((nr + 63) / 64) + (nstep / s) * kExchangeWaves;
-          if (best < 0 || cost < best) {
-            best = cost;
-            c.trop_tw = tw;
-            c.trop_th = th;
-            c.trop_s = s;
-          }
-        }
-    if (best < 0) return "no barotropic tiling fits this device";
+    // default: the candidate of least modelled cost, persistent if any fits
+    std::vector<TropCand> cands;
+    if (!launches) {
+      if (!(err = trop_candidates(c, false, &cands)).empty()) return err;
+      if (cands.empty() && c.trop_mode == PCOM_TROP_PERSISTENT)
+        return "no persistent barotropic tiling fits this device for this grid";
+      launches = cands.empty();
+    }
+    if (launches && !(err = trop_candidates(c, true, &cands)).empty()) return err;
+    if (cands.empty()) return "no barotropic tiling fits this device";
+    c.trop_tw = cands[0].tw;
+    c.trop_th = cands[0].th;
+    c.trop_s = cands[0].s;
   }
   if (c.trop_s > 16) return "at most 16 barotropic substeps per exchange";
   if ((long)(c.trop_tw + 5 * c.trop_s) * (c.trop_th + 5 * c.trop_s) > kTropBlock)
@@ -569,21 +660,19 @@ std::string trop_configure(Ctx& c) {
            std::to_string(kTropBlock) + " points";
   if (c.trop_tw + 5 * c.trop_s > kTabStride || c.trop_th + 5 * c.trop_s > kTabStride)
     return "barotropic tile sides plus halo must be at most " + std::to_string(kTabStride);
-  size_t lds = trop_lds_bytes(c.trop_tw, c.trop_th, c.trop_s);
-  long resident;
-  if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
-  if (resident == 0)
-    return "barotropic tile " + std::to_string(c.trop_tw) + "x" + std::to_string(c.trop_th) + "x" +
-           std::to_string(c.trop_s) + " needs " + std::to_string(lds) + " B of LDS";
   long tiles_x = (G + c.trop_tw - 1) / c.trop_tw;
   long tiles_y = (c.nj + c.trop_th - 1) / c.trop_th;
   long need = tiles_x * tiles_y;
-  if (c.trop_flag_cap > 0 && need > c.trop_flag_cap)
-    return "barotropic tiling needs " + std::to_string(need) + " tiles, more than the " +
-           std::to_string(c.trop_flag_cap) + " the context holds flags for";
-  if (need > resident)
-    return "barotropic tiling needs " + std::to_string(need) + " co-resident workgroups, the device holds " +
-           std::to_string(resident) + "; choose larger tiles";
+  const bool fits = need <= resident && !(c.trop_flag_cap > 0 && need > c.trop_flag_cap);
+  if (c.trop_mode == PCOM_TROP_AUTO) launches = !fits;
+  if (!launches && !fits)
+    return "barotropic tiling needs " + std::to_string(need) + " co-resident workgroups with exchange flags, " +
+           "the device holds " + std::to_string(std::min(resident, c.trop_flag_cap > 0 ? c.trop_flag_cap : resident)) +
+           "; choose larger tiles or PCOM_TROP_LAUNCHES";
+  if (launches && c.trop_s >= nstep)
+    return "launch mode needs fewer substeps per launch than the " + std::to_string(nstep) + " of a cycle";
+  if (need > (1l << 30)) return "too many barotropic tiles";
+  c.trop_launches = launches;
   c.trop_tiles_x = (int)tiles_x;
   c.trop_blocks = (int)need;
   c.trop_lds = lds;
@@ -651,7 +740,12 @@ TropTileArgs trop_args(const Ctx& c) {
     a.xu[q] = h.xu[q];
     a.xv[q] = h.xv[q];
   }
+  for (int q = 0; q < 4; ++q) a.acc[q] = h.xacc[q];
   a.nstep = c.dtbc / c.dtbt * 2;
+  a.nt0 = 1;
+  a.nt1 = a.nstep;
+  a.src = 0;
+  a.dst = 0;
   a.S = c.trop_s;
   a.TW = c.trop_tw;
   a.TH = c.trop_th;
@@ -697,6 +791,21 @@ std::string chained_launch(int device, hipStream_t st, F launch) {
 
 std::string launch_trop_kernel(Ctx& c) {
   TropTileArgs a = trop_args(c);
+  if (c.trop_launches) {
+    // one launch per S substeps; tiles of a launch never wait for each
+    // other, so the launches need no chaining with other contexts'
+    if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
+    for (int b = 0, nt0 = 1; nt0 <= a.nstep; ++b, nt0 += c.trop_s) {
+      a.nt0 = nt0;
+      a.nt1 = std::min(a.nstep, nt0 + c.trop_s - 1);
+      a.src = (b + 1) & 1;
+      a.dst = b & 1;
+      hipLaunchKernelGGL(k_trop_block, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return std::string("k_trop_block: ") + hipGetErrorString(e);
+    }
+    return "";
+  }
   a.epoch = ++c.trop_epoch;
   return chained_launch(c.device, c.stream, [&]() -> std::string {
     if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
@@ -756,7 +865,7 @@ std::string ensemble_plan(Ensemble& en) {
   const int n = (int)en.m.size();
   std::vector<int> order;
   for (int q = 0; q < n; ++q)
-    if (!en.m[q]->nn) order.push_back(q);
+    if (!en.m[q]->nn && !en.m[q]->trop_launches) order.push_back(q);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return en.m[a]->trop_blocks > en.m[b]->trop_blocks; });
   en.packs.clear();
@@ -873,6 +982,10 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
         if (!(err = nn_trop_step(c, launch_trop)).empty()) return err;
         continue;
       }
+      if (c.trop_launches) {  // so does one whose subcycle runs in launch mode
+        if (!(err = launch_trop(c)).empty()) return err;
+        continue;
+      }
       if (!(err = launch_vz(c)).empty()) return err;
       if ((e = hipEventRecord(en.ready[q], c.stream)) != hipSuccess) return hip_error_string(e, "hipEventRecord");
     }
@@ -907,7 +1020,7 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
     // members: the rest of the step
     for (size_t q = 0; q < en.m.size(); ++q) {
       Ctx& c = *en.m[q];
-      if (!c.nn && (e = hipStreamWaitEvent(c.stream, en.done[en.pack_of[q]], 0)) != hipSuccess)
+      if (en.pack_of[q] >= 0 && (e = hipStreamWaitEvent(c.stream, en.done[en.pack_of[q]], 0)) != hipSuccess)
         return hip_error_string(e, "hipStreamWaitEvent");
       c.n_ch += 1;
       if (!(err = launch_step_post(c)).empty()) return err;
@@ -921,28 +1034,11 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
 // model ranks best, on the context's own state (snapshotted and restored
 // around every trial), and keep the fastest.
 std::string tune_subcycle(Ctx& c, int trials, int out[3], double* best_ms) {
-  const int G = c.ni - 2;
-  const long nstep = (long)c.dtbc / c.dtbt * 2;
-  struct Cand {
-    long cost;
-    int tw, th, s;
-  };
-  std::vector<Cand> cands;
-  std::string err;
-  for (int s = 1; s <= 4; ++s)
-    for (int tw = 4; tw <= 64; ++tw)
-      for (int th = 4; th <= 64; ++th) {
-        const long nr = (long)(tw + 5 * s) * (th + 5 * s);
-        if (nr > kTropBlock) continue;
-        const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
-        if (tiles > c.trop_flag_cap) continue;
-        long resident;
-        if (!(err = trop_resident(c, trop_lds_bytes(tw, th, s), &resident)).empty()) return err;
-        if (tiles > resident) continue;
-        cands.push_back({nstep * ((nr + 63) / 64) + (nstep / s) * 14, tw, th, s});
-      }
+  // candidates of the mode in use, best modelled first
+  std::vector<TropCand> cands;
+  std::string err = trop_candidates(c, c.trop_launches, &cands);
+  if (!err.empty()) return err;
   if (cands.empty()) return "no barotropic tiling fits this device";
-  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.cost < b.cost; });
   if ((int)cands.size() > trials) cands.resize(trials);
   // snapshot of what the subcycle writes
   Fields2& h = c.f2;
@@ -966,9 +1062,9 @@ std::string tune_subcycle(Ctx& c, int trials, int out[3], double* best_ms) {
   (void)hipEventCreate(&eb);
   const int keep[3] = {c.trop_tw, c.trop_th, c.trop_s};
   double best = 1e300;
-  Cand win = cands[0];
+  TropCand win = cands[0];
   if ((e = copy(true)) != hipSuccess) err = hip_error_string(e, "snapshot");
-  for (const Cand& cd : cands) {
+  for (const TropCand& cd : cands) {
     if (!err.empty()) break;
     c.trop_tw = cd.tw;
     c.trop_th = cd.th;

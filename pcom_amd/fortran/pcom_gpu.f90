@@ -14,6 +14,7 @@ module pcom_gpu
   integer(c_int), parameter :: PCOM_OK = 0, PCOM_ERR_ARG = -1, PCOM_ERR_HIP = -2, &
                                PCOM_ERR_NAN = -3, PCOM_ERR_STATE = -4, PCOM_ERR_NOGPU = -5
   integer(c_int), parameter :: PCOM_FP64 = 0, PCOM_MIXED = 1, PCOM_MIXED_EOS32 = 2
+  integer(c_int), parameter :: PCOM_TROP_AUTO = 0, PCOM_TROP_PERSISTENT = 1, PCOM_TROP_LAUNCHES = 2
 
   ! field ids, in the order of the enum in include/pcom_gpu.h
   enum, bind(c)
@@ -45,6 +46,7 @@ module pcom_gpu
     integer(c_int) :: dtbt, dtbc, year, month, day, hour
     integer(c_int) :: precision, restart_run, device
     integer(c_int) :: trop_tile_i, trop_tile_j, trop_substeps
+    integer(c_int) :: trop_mode
   end type pcom_opts
 
   type, bind(C) :: pcom_means
@@ -247,7 +249,7 @@ module pcom_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: set
-      integer(c_int), intent(inout) :: tile(3)
+      integer(c_int), intent(inout) :: tile(4)
     end function pcom_subcycle_tiling
 
     ! learned barotropic surrogate

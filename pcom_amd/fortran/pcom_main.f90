@@ -29,10 +29,10 @@ program pcom_main
   character(len=16) :: out_per = 'day'
   integer :: restart_per = 1, restart_run = 0
   character(len=12) :: precision = 'fp64'
-  integer :: device = 0, tile_i = 0, tile_j = 0, tile_s = 0
+  integer :: device = 0, tile_i = 0, tile_j = 0, tile_s = 0, trop_mode = 0
   namelist /time_ctrl/ bdate, edate, dtbt, dtbc
   namelist /io_ctrl/ fname_ini, fname_rst, out_dir, out_per, restart_per, restart_run
-  namelist /gpu_ctrl/ precision, device, tile_i, tile_j, tile_s
+  namelist /gpu_ctrl/ precision, device, tile_i, tile_j, tile_s, trop_mode
 
   character(len=256) :: nml_file
   integer :: u, ios, has_vmix
@@ -108,6 +108,7 @@ program pcom_main
   opts%restart_run = restart_run
   opts%device = device
   opts%trop_tile_i = tile_i; opts%trop_tile_j = tile_j; opts%trop_substeps = tile_s
+  opts%trop_mode = trop_mode
   call pcom_check(pcom_create(inp, opts, ctx), 'pcom_create')
 
   ! ---- time control (main.f90:123-141)

@@ -144,12 +144,13 @@ def den_rho(t, s, p):
 class Model:
     """One device-resident PCOM time stepper (a pcom_ctx)."""
 
-    def __init__(self, cfg, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0)):
+    def __init__(self, cfg, precision=abi.PCOM_FP64, device=0, restart_run=0, trop_tile=(0, 0, 0),
+                 trop_mode=abi.PCOM_TROP_AUTO):
         self.cfg = cfg
         self.precision = precision
         self._inp = cfg.c_inputs()
         self._opt = cfg.c_opts(precision=precision, device=device, restart_run=restart_run,
-                               trop_tile=trop_tile)
+                               trop_tile=trop_tile, trop_mode=trop_mode)
         h = C.c_void_p()
         _check(lib().pcom_create(C.byref(self._inp), C.byref(self._opt), C.byref(h)))
         self.h = h
@@ -217,11 +218,24 @@ class Model:
         _check(lib().pcom_tune_subcycle(self.h, int(trials), t, C.byref(ms)))
         return tuple(t), ms.value
 
-    def tiling(self, tile=None):
-        """The subcycle tiling (tile_i, tile_j, substeps); sets it if given."""
-        t = (C.c_int * 3)(*(tile or (0, 0, 0)))
-        _check(lib().pcom_subcycle_tiling(self.h, int(tile is not None), t))
-        return tuple(t)
+    def tiling(self, tile=None, mode=None):
+        """The subcycle tiling (tile_i, tile_j, substeps); sets it if given
+        (with `mode`, a PCOM_TROP_* value: default the current mode)."""
+        if tile is None and mode is None:
+            t = (C.c_int * 4)()
+            _check(lib().pcom_subcycle_tiling(self.h, 0, t))
+            return tuple(t)[:3]
+        cur = (C.c_int * 4)()
+        _check(lib().pcom_subcycle_tiling(self.h, 0, cur))
+        t = (C.c_int * 4)(*(tuple(tile or (0, 0, 0)) + (cur[3] if mode is None else mode,)))
+        _check(lib().pcom_subcycle_tiling(self.h, 1, t))
+        return tuple(t)[:3]
+
+    def subcycle_mode(self):
+        """PCOM_TROP_PERSISTENT or PCOM_TROP_LAUNCHES: how the subcycle runs."""
+        t = (C.c_int * 4)()
+        _check(lib().pcom_subcycle_tiling(self.h, 0, t))
+        return t[3]
 
     # ---- learned barotropic surrogate
     def nn_create(self, capacity=0, seed=1):

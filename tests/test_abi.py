@@ -38,7 +38,7 @@ def test_field_ids_match_header():
 def test_struct_layouts():
     # pcom_inputs: 3 ints, then 15 pointers (8-byte aligned)
     assert C.sizeof(abi.PcomInputs) == 16 + 15 * 8
-    assert C.sizeof(abi.PcomOpts) == 12 * 4
+    assert C.sizeof(abi.PcomOpts) == 13 * 4  # 13 ints, trop_mode last
     assert C.sizeof(abi.PcomCounters) == 8 + 11 * 4 + 4
 
 

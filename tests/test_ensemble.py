@@ -29,13 +29,15 @@ def test_ensemble_matches_individual_runs(gpu_lib):
     """Members of three grid sizes and two forms, packed together."""
     from pcom_amd.lib import Ensemble, Model
     cfgs = [_perturbed(configs.box(), 100), _perturbed(configs.box(), 101), configs.small(),
-            _perturbed(configs.global_1deg(glo_ni=180, glo_nj=90), 102)]
-    forms = [abi.PCOM_FP64, abi.PCOM_MIXED, abi.PCOM_FP64, abi.PCOM_FP64]
-    members = [Model(c, precision=f) for c, f in zip(cfgs, forms)]
+            _perturbed(configs.global_1deg(glo_ni=180, glo_nj=90), 102), _perturbed(configs.box(), 103)]
+    forms = [abi.PCOM_FP64, abi.PCOM_MIXED, abi.PCOM_FP64, abi.PCOM_FP64, abi.PCOM_FP64]
+    modes = [abi.PCOM_TROP_AUTO] * 4 + [abi.PCOM_TROP_LAUNCHES]  # the last runs its subcycle alone
+    members = [Model(c, precision=f, trop_mode=md) for c, f, md in zip(cfgs, forms, modes)]
     ens = Ensemble(members)
     ens.step(5)
     npk, of = ens.packs()
-    assert npk < len(members), (npk, of)  # small members share launches
+    assert npk < len(members) - 1, (npk, of)  # small members share launches
+    assert of[-1] == -1
     for c, f, m in zip(cfgs, forms, members):
         solo = Model(c, precision=f)
         solo.step(5)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from pcom_amd import abi, configs
-from pcom_amd.lib import Model, den_rho as gpu_den_rho
+from pcom_amd.lib import Model, PcomError, den_rho as gpu_den_rho
 
 from oracle.oracle import Oracle, den_rho as orc_den_rho
 
@@ -139,6 +139,50 @@ def test_subcycle_tiling_is_exact(gpu_lib, tile):
         o.step(2)
         bad = _compare(m, o, ["chc", "chp", "mch", "mch2", "acch", "buc", "bvc", "bup", "bvp", "tc", "uc"])
         assert not bad, f"{cfgname} tile {tile}:\n" + "\n".join(bad)
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (24, 12, 2), (7, 7, 5)])
+def test_launch_mode_is_exact(gpu_lib, tile):
+    """The subcycle in launch mode (one launch per S substeps, the kernel
+    boundary being the exchange) gives the reference's bits."""
+    for cfgname in ("small", "box"):
+        cfg = configs.CONFIGS[cfgname]()
+        m = Model(cfg, trop_tile=tile, trop_mode=abi.PCOM_TROP_LAUNCHES)
+        assert m.subcycle_mode() == abi.PCOM_TROP_LAUNCHES
+        o = Oracle(cfg)
+        m.step(2)
+        o.step(2)
+        bad = _compare(m, o, ["chc", "chp", "mch", "mch2", "acch", "buc", "bvc", "bup", "bvp", "tc", "uc"])
+        assert not bad, f"{cfgname} tile {tile}:\n" + "\n".join(bad)
+
+
+def test_launch_mode_matches_persistent_at_1deg(gpu_lib):
+    cfg = configs.global_1deg()
+    a = Model(cfg)
+    assert a.subcycle_mode() == abi.PCOM_TROP_PERSISTENT  # the 1-degree grid fits
+    b = Model(cfg, trop_mode=abi.PCOM_TROP_LAUNCHES)
+    a.step(3)
+    b.step(3)
+    for n in STEP_FIELDS:
+        assert np.array_equal(a.get(n), b.get(n)), n
+
+
+def test_large_grid_runs_in_launch_mode(gpu_lib):
+    """A 0.25 degree grid (1440 x 720) has more tiles than the device holds
+    co-resident: the automatic mode is launch mode, the persistent mode is
+    refused, and the bits do not depend on the launch tiling."""
+    cfg = configs.global_1deg(glo_ni=1440, glo_nj=720, nk=5)
+    cfg.dtbt, cfg.dtbc = 15, 900
+    with pytest.raises(PcomError):
+        Model(cfg, trop_mode=abi.PCOM_TROP_PERSISTENT)
+    a = Model(cfg)
+    assert a.subcycle_mode() == abi.PCOM_TROP_LAUNCHES
+    b = Model(cfg, trop_tile=(12, 10, 1))
+    a.step(2)
+    b.step(2)
+    for n in ["chc", "chp", "mch", "mch2", "acch", "buc", "bvc", "tc", "uc"]:
+        assert np.array_equal(a.get(n), b.get(n)), n
+    assert a.check_finite()
 
 
 def test_box_many_steps_bitwise(gpu_lib):
