@@ -57,6 +57,7 @@ SUR_TOL = -1.0  # a check fails when the network's error exceeds persistence's
 
 
 def run_form(cfg, precision, steps, warmup, surrogate=False, check_every=10):
+    from pcom_amd import abi
     from pcom_amd.lib import Model
     m = Model(cfg, precision=precision)
     m.step(warmup)
@@ -89,9 +90,13 @@ def run_form(cfg, precision, steps, warmup, surrogate=False, check_every=10):
     ph, n = m.phase_times(False)
     ok = m.check_finite()
     sums = m.global_sums()
+    mode, tile = m.subcycle_mode(), m.tiling()
+    nstep = 2 * cfg.dtbc // cfg.dtbt
     out = {"ms_per_step": 1000.0 * wall / steps,
            "phase_ms": {"step": ph[0], "other_kernels": ph[1], "subcycle_kernel": ph[2]},
-           "phase_steps": n, "finite": bool(ok), "heat_salt_mass": [float(x) for x in sums]}
+           "phase_steps": n, "finite": bool(ok), "heat_salt_mass": [float(x) for x in sums],
+           "subcycle_mode": mode, "subcycle_tiling": list(tile),
+           "subcycle_launches": 1 if mode == abi.PCOM_TROP_PERSISTENT else -(-nstep // tile[2])}
     if sur is not None:
         st = m.nn_stats()
         sur.update({"surrogate_steps": st.surrogate_steps, "plain_steps": st.plain_steps, "checks": st.checks,
@@ -125,8 +130,8 @@ def cpu_baseline(cfg, nsteps):
     wall = time.perf_counter() - t0
     ms = 1000.0 * wall / nsteps
     return {"value": round(sypd(ms, cfg.dtbc), 4), "unit": "SYPD", "cores": 1, "kind": "port",
-            "sample": f"{nsteps} steady steps of the same 360x180x30 workload on the FP64 C oracle "
-                      f"(1 thread), {ms:.0f} ms/step"}
+            "sample": f"{nsteps} steady steps of the same {cfg.glo_ni}x{cfg.glo_nj}x{cfg.nk} workload on the "
+                      f"FP64 C oracle (1 thread), {ms:.0f} ms/step"}
 
 
 def main():
@@ -137,7 +142,7 @@ def main():
     ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32", "mixed+surrogate"], default="fp64",
                     help="form reported as `value`; the others are measured alongside")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline (0 = skip)")
-    ap.add_argument("--config", default="global", choices=["global", "box"])
+    ap.add_argument("--config", default="global", choices=["global", "box", "eddy"])
     ap.add_argument("--forms", default="fp64,mixed,mixed_eos32,mixed+surrogate",
                     help="forms to measure (comma-separated; must include --form)")
     ap.add_argument("--calibrate-copy", action="store_true", help=calibrate_copy.__doc__.split(":")[0])
@@ -189,7 +194,11 @@ def main():
                    "form": args.form, "parallelism": "single GPU, whole domain"},
         "forms": forms,
         "accuracy": "1-year SST/SSH RMSE of each form against FP64: profiles/rmse_1yr_global1deg.md",
-        "roofline": {"bound": "mfma", "kernel": "k_trop_tiles (barotropic subcycle, FP64)",
+        "roofline": {"bound": "mfma",
+                     "kernel": ("k_trop_tiles (barotropic subcycle, FP64, one persistent launch per step)"
+                                if rep["subcycle_mode"] == abi.PCOM_TROP_PERSISTENT else
+                                f"k_trop_block (barotropic subcycle, FP64, {rep['subcycle_launches']} launches "
+                                "per step, timed together)"),
                      "achieved": round(trop_tflops, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5), "traffic": TROP_TRAFFIC_BYTES if cfg.name == "global" else None,
                      "flops_per_launch": flops, "launch_ms": round(trop_ms, 4),

@@ -222,7 +222,20 @@ def small(glo_ni=8, glo_nj=6, nk=4, seed=7):
     return cfg
 
 
-CONFIGS = {"box": box, "global": global_1deg, "small": small}
+def eddy_025deg():
+    """Eddy-permitting quarter-degree configuration, 1440 x 720 x 50: the
+    quasi-global geometry, stratification and forcing of global_1deg at 0.25
+    degrees (bathymetry smoothed over the same physical distance), with the
+    steps the CFL limits there allow: dtbt = 15 s (gravity waves of about
+    230 m/s over 9.5 km cells at the 70 degree cut) and dtbc = 900 s, 120
+    barotropic substeps per step as at 1 degree."""
+    cfg = global_1deg(glo_ni=1440, glo_nj=720, nk=50)
+    cfg.dtbt, cfg.dtbc = 15, 900
+    cfg.name = "eddy"
+    return cfg
+
+
+CONFIGS = {"box": box, "global": global_1deg, "small": small, "eddy": eddy_025deg}
 
 
 if __name__ == "__main__":
