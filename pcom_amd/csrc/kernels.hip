@@ -217,8 +217,9 @@ __global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const double* __re
                                                    const double* __restrict__ vc, const double* __restrict__ sqp, R* wm,
                                                    R* acw, const E* __restrict__ alpha,
                                                    const double* __restrict__ mch2) {
+  // one LDS column array: the divergence per level, then in place the
+  // upwelling (half the LDS of two arrays, twice the workgroups per CU)
   __shared__ R w3[kNKMAX][kCW];
-  __shared__ R wv[kNKMAX][kCW];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
   const long p = (long)xcd_block() * kCW + tx;
   const bool ok = p < g.n2;
@@ -240,18 +241,20 @@ __global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const double* __re
     for (int k = 1; k <= nk; ++k) s += w3[k - 1][tx] * g.dpr2[k - 1] * (R)(lev >= k ? 1 : 0);
     R wk2 = -(s / g.prh_t_r[p]);
     R w = wm[p];  // wm(1) is never written by upwelling
-    wv[0][tx] = w;
+    R d = w3[0][tx];
+    w3[0][tx] = w;
     for (int k = 2; k <= nk; ++k) {
-      w = w + g.dpr2[k - 2] * (wk2 + w3[k - 2][tx]);
+      w = w + g.dpr2[k - 2] * (wk2 + d);  // d = the divergence of level k-1
       w = w * (R)(lev >= k ? 1 : 0);
-      wv[k - 1][tx] = w;
+      d = w3[k - 1][tx];
+      w3[k - 1][tx] = w;
     }
   }
   __syncthreads();
   if (!ok) return;
   for (int k = 1 + kl; k <= nk; k += kKL) {
     long q = p + g.n2 * (k - 1);
-    R w = wv[k - 1][tx];
+    R w = w3[k - 1][tx];
     if (k >= 2) wm[q] = w;
     if (acw) acw[q] = acw[q] + w * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
   }
@@ -265,7 +268,7 @@ __device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* schr, int
 
 // op_adv (mod_op.f90:224-278) for uc and vc at one point.
 template <typename R>
-__global__ void k_adv(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
+__global__ void __launch_bounds__(kBlock) k_adv(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
                       const R* __restrict__ wm, const double* __restrict__ chc, const double* __restrict__ schr,
                       R* auc, R* avc) {
   long p = (long)xcd_block() * blockDim.x + threadIdx.x;
@@ -359,7 +362,7 @@ __device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v
 // op_fri (mod_op.f90:169-184): fx from (u, uc, -v, taux), fy from
 // (v, vc, u, tauy) with u = uc/sch, v = vc/sch.
 template <typename R>
-__global__ void k_fri(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
+__global__ void __launch_bounds__(kBlock) k_fri(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
                       const R* __restrict__ am,
                       const double* __restrict__ chc, const double* __restrict__ schr,
                       const double* __restrict__ taux, const double* __restrict__ tauy, R* fx, R* fy) {
@@ -655,7 +658,7 @@ __device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const double* v
 }
 
 template <typename R>
-__global__ void k_ts(TsArgs<R> a) {
+__global__ void __launch_bounds__(kBlock) k_ts(TsArgs<R> a) {
   const Grid<R>& g = a.g;
   long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2 * g.nk) return;
