@@ -749,8 +749,16 @@ __global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<E> g, const double* __r
     }
   }
   __syncthreads();
-  if (ok && kl == 0 && nlev > 1) {
-    Column<E> c{lt + tx, ls + tx, l1 + tx, l2 + tx, dpr, kCW};
+  // the searches of the kCW columns are spread over the kKL waves (column
+  // tx + kCW/kKL * kl on lane tx < kCW/kKL of wave kl): a wave then waits for
+  // the longest of kCW/kKL searches instead of kCW
+  constexpr int kPer = kCW / kKL;
+  const int col = tx + kPer * kl;
+  const long pcol = (long)xcd_block() * kCW + col;
+  const int ncol = tx < kPer && pcol < g.n2 ? g.lev_t[pcol] : 0;
+  if (ncol > 1) {
+    Column<E> c{lt + col, ls + col, l1 + col, l2 + col, dpr, kCW};
+    const int nlev = ncol;
     int kcnt = 1;
     while (kcnt < nlev) {
       if (c.rhob(kcnt) > c.rhoa(kcnt + 1)) {
