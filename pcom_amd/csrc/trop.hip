@@ -79,12 +79,13 @@ constexpr unsigned long long kSpinLimit = 1ull << 22;  // ~seconds; a normal wai
 // and the host checks it.  Every array sits at a fixed offset with a fixed
 // stride, so an access is one address register per neighbour plus an
 // immediate offset, and the two components a stage reads at the same point
-// come in one 16-byte access.
+// come in one 16-byte access; components read apart are separate arrays, whose
+// 8-byte lane stride keeps the LDS banks free of conflicts.
 enum RegionPair {
-  P_CH,  // (corrector ch, predictor ch)
+  P_CH,  // corrector ch, then predictor ch: two scalar arrays (read apart)
   P_UV,  // barotropic velocity
   P_SF,  // prh*sch*ub fluxes
-  P_GR,  // gradients of the predictor ch
+  P_GR,  // x, then y gradient of the predictor ch: two scalar arrays
   P_WA,  // Laplacian face fluxes of (u, v), east-west
   P_WB,  // the same, north-south
   P_N
@@ -185,10 +186,12 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const int RW = TW + HL + HR, RHt = TH + HL + HR;
   const int bx0 = x0 - HL, by0 = y0 - HL;
   const int NR = RW * RHt;
-  double2* CH = lds2 + P_CH * kTropBlock;
+  double* CHP = reinterpret_cast<double*>(lds2 + P_CH * kTropBlock);
+  double* PR = CHP + kTropBlock;
   double2* UV = lds2 + P_UV * kTropBlock;
   double2* SF = lds2 + P_SF * kTropBlock;
-  double2* GR = lds2 + P_GR * kTropBlock;
+  double* GX = reinterpret_cast<double*>(lds2 + P_GR * kTropBlock);
+  double* GY = GX + kTropBlock;
   double2* WA = lds2 + P_WA * kTropBlock;
   double2* WB = lds2 + P_WB * kTropBlock;
   double* rowt = reinterpret_cast<double*>(lds2 + P_N * kTropBlock);  // [R_N][kTabStride]
@@ -280,7 +283,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     bph3 = a.bphi[3][pe];
     bgr3 = a.bgra[3][pe];
     if (!kLaunches || a.nt0 == 1) {
-      CH[q].x = a.chp[p];
+      CHP[q] = a.chp[p];
       UV[q] = make_double2(a.bup[p], a.bvp[p]);
       if (owned) {
         acm = chc0;
@@ -290,7 +293,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       }
     } else {  // launch mode: the previous launch's state
       const bool s1 = a.src != 0;
-      CH[q].x = (s1 ? a.xc[1] : a.xc[0])[p];
+      CHP[q] = (s1 ? a.xc[1] : a.xc[0])[p];
       UV[q] = make_double2((s1 ? a.xu[1] : a.xu[0])[p], (s1 ? a.xv[1] : a.xv[0])[p]);
       if (owned) {
         acm = a.acc[0][p];
@@ -343,18 +346,18 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     // A) predictor (mod_int.f90:184-193): sch = sqrt(ter(chp)) (1 on land),
     // fluxes prh*sch*ub_p, chc = chp - gamma_b*dtbt*div/prh on sea
     if (live) {
-      const double sc = sch_of(CH[q].x, CH[qn].x, CH[qe].x, CH[qne].x);
+      const double sc = sch_of(CHP[q], CHP[qn], CHP[qe], CHP[qne]);
       const double2 uv = UV[q];
       SF[q] = make_double2(prh_u * sc * uv.x, prh_u * sc * uv.y);
     }
     __syncthreads();
     if (live) {
       double wk = div331();
-      const double chp = CH[q].x;
+      const double chp = CHP[q];
       wk = chp - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
       // land keeps its ch: the entry value in the first substep, afterwards
       // the corrector's chp - 0 (mod_int.f90:193, 267)
-      CH[q].y = levt > 0 ? wk : (nt == 1 ? a.chc[p] : chp);
+      PR[q] = levt > 0 ? wk : (nt == 1 ? a.chc[p] : chp);
     }
     __syncthreads();
     // B) barotropic momentum (mod_int.f90:195-261).  Face fluxes of op_lap
@@ -362,8 +365,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     double sch = 1.0;
     double2 uv = make_double2(0.0, 0.0);
     if (live) {
-      const double pr = CH[q].y, pre = CH[qe].y, prn = CH[qn].y;
-      sch = sch_of(pr, prn, pre, CH[qne].y);
+      const double pr = PR[q], pre = PR[qe], prn = PR[qn];
+      sch = sch_of(pr, prn, pre, PR[qne]);
       const bool r2 = y >= 2 && y <= nj;
       uv = UV[q];
       const double2 uvw = UV[qw], uvs = UV[qs];
@@ -371,8 +374,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
                            qdiv(qdiv(uv.y - uvw.y, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)));
       WB[q] = make_double2(qdiv(r2 ? uv.x - uvs.x : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2),
                            qdiv(r2 ? uv.y - uvs.y : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2));
-      GR[q] = make_double2(qdiv(qdiv(pre - pr, COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I)),
-                           qdiv(qdiv(rin ? prn - pr : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD));
+      GX[q] = qdiv(qdiv(pre - pr, COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I));
+      GY[q] = qdiv(qdiv(rin ? prn - pr : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD);
     }
     __syncthreads();
     double un = 0, vn = 0;
@@ -386,14 +389,14 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
                              ROW(R_DEN), ROW(R_DENI));
       const double vs1 = sam1 * l1;
       const double vs2 = sam1 * l2;
-      const double2 g = GR[q], gn = GR[qn], ge = GR[qe];
-      const double pr = CH[q].y, pre = CH[qe].y, prn = CH[qn].y, prne = CH[qne].y;
+      const double gx = GX[q], gxn = GX[qn], gy = GY[q], gye = GY[qe];
+      const double pr = PR[q], pre = PR[qe], prn = PR[qn], prne = PR[qne];
       double gi1 = 0.0, ci1 = 0.0;  // row nj of grachint/chgraint%x(1) is never written
       if (y <= njm) {
-        gi1 = 0.5 * (g.x * bph0 + gn.x * bph1);
+        gi1 = 0.5 * (gx * bph0 + gxn * bph1);
         ci1 = 0.5 * (0.5 * (pr + pre) * bgr0 + 0.5 * (prn + prne) * bgr1);
       }
-      const double gi2 = 0.5 * (g.y * bph2 + ge.y * bph3);
+      const double gi2 = 0.5 * (gy * bph2 + gye * bph3);
       const double ci2 = 0.5 * ((rin ? 0.5 * (pr + prn) : 0.0) * bgr2 + (rin ? 0.5 * (pre + prne) : 0.0) * bgr3);
       const double cor = ROW(R_COR);
       const double pg1 = (gi1 + ci1 + ghx) * su * sch;
@@ -422,8 +425,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     double cn = 0;
     if (live) {
       const double wk = div331();
-      cn = CH[q].x - qdiv(mt * wk * bt, prh_t, rprh_t);
-      CH[q].x = cn;
+      cn = CHP[q] - qdiv(mt * wk * bt, prh_t, rprh_t);
+      CHP[q] = cn;
       if (owned) {
         if (nt <= nstep / 2) acm = acm + cn;
         acu = acu + un;
@@ -446,7 +449,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
       // the halo from the neighbours (owned points already hold these values)
       if (live && !owned) {
-        CH[q].x = xload(xc);
+        CHP[q] = xload(xc);
         UV[q] = make_double2(xload(xu), xload(xv));
       }
       ++round;
@@ -460,7 +463,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     if (owned) {
       const bool d1 = a.dst != 0;
       const double2 uv = UV[q];
-      (d1 ? a.xc[1] : a.xc[0])[p] = CH[q].x;
+      (d1 ? a.xc[1] : a.xc[0])[p] = CHP[q];
       (d1 ? a.xu[1] : a.xu[0])[p] = uv.x;
       (d1 ? a.xv[1] : a.xv[0])[p] = uv.y;
       a.acc[0][p] = acm;
