@@ -606,84 +606,129 @@ struct TsArgs {
   double bc;
 };
 
-template <typename R, typename FV>
-__device__ __forceinline__ R adv_ts_point(const Grid<R>& g, FV var, const double* varp, const R* wm,
-                                          const double* schs, const double* uc, const double* vc, int i, int j,
-                                          int k) {
-  const long ko = g.n2 * (k - 1);
-  auto SCH = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
-  auto UM1 = [&](int m, int jj) -> R { return (R)uc[ko + g.ix(m, jj)] * SCH(m, jj); };
-  auto UM2 = [&](int m, int jj) -> R { return (R)vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
-  auto X = [&](int m, int jj) -> R { return ter_ns<3>(g, UM1, m, jj, (R)0) * p1<1, 2>(g, var, m, jj); };
-  auto Y = [&](int m, int jj) -> R { return ter_ew<3>(g, UM2, m, jj) * p2<1, 4>(g, var, m, jj); };
-  const R rh1 = g.RH(1, i, j);
-  R ans = ter_ew<2>(g, X, i, j) / ((R)ARAD * rh1);
-  ans = ans + ter_ns<4>(g, Y, i, j, (R)0) / ((R)ARAD * rh1);
-  auto DX3 = [&](int kk) -> R {
-    if (kk < 2 || kk > g.nk) return (R)0;
-    return (R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)];
-  };
-  R wka = (wm[g.ix(i, j, k)] * DX3(k) + wm[g.ix(i, j, k + 1)] * DX3(k + 1)) * (R)0.5;
-  wka = wka / g.dpr2[k - 1];
-  return ans + wka;
-}
-
-template <typename R, typename FV>
-__device__ __forceinline__ R dif_point(const Grid<R>& g, FV var, const double* varp, const double* ch, int i,
-                                       int j, int k) {
-  auto MX = [&](int m, int jj) -> R {
-    int mc = g.canon(m);
-    R d = (var(mc + 1, jj) - var(mc, jj)) * (R)g.mskt(mc + 1, jj, k) * (R)g.mskt(mc, jj, k);
-    d = d / g.DX1(2, mc, jj);
-    return d / ((R)ARAD * g.RH(2, mc, jj));
-  };
-  auto MY = [&](int m, int jj) -> R {
-    int mc = g.canon(m);
-    R d = (jj > g.njm) ? (R)0 : (var(mc, jj + 1) - var(mc, jj)) * (R)g.mskt(mc, jj + 1, k) * (R)g.mskt(mc, jj, k);
-    d = d / g.DX2(4, mc, jj);
-    return d / (R)ARAD;
-  };
-  auto CH = [&](int m, int jj) -> R { return (R)ch[g.ix(m, jj)]; };
-  auto VA = [&](int m, int jj) -> R { return ter_ew<1>(g, CH, m, jj) * MX(m, jj); };
-  auto VB = [&](int m, int jj) -> R { return ter_ns<1>(g, CH, m, jj, (R)0) * MY(m, jj); };
-  R ans = (R)AH_C * divg<2, 4, 1>(g, VA, VB, i, j);
-  auto WKP = [&](int kk) -> R {
-    R v = (R)0;
-    if (kk >= 2 && kk <= g.nk)
-      v = ((R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) *
-          (R)g.mskt(i, j, kk) / g.dz1[kk - 1];
-    return (R)GRAV * (R)RHO0 * g.kh[g.ix(i, j, kk)] * v;
-  };
-  return ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
-}
+// int_ts's tendency and step (mod_int.f90:463-490), with the face fluxes
+// staged in LDS.  op_adv_ts and op_dif are
+// divergences of fluxes on the faces of a T cell (mod_op.f90:186-222,
+// 280-325): the east-west faces (g2 points) carry X = ter(um)*p1(var) and
+// VA = ter(ch)*masked p1(var)/(a rh), the north-south faces (g4 points) Y
+// and VB*rh.  Computed per cell, every face is evaluated twice (once by
+// each cell beside it); here a workgroup owns a kTsTX x kTsTY tile of
+// canonical columns 2..nim at one level, evaluates each face of the tile
+// once for T and S, and combines them per cell with the same operations
+// in the same order.  The halo columns 1 and ni take their faces from
+// their canonical columns nim and 2 and everything else from themselves,
+// as the per-cell composition does.
+constexpr int kTsTX = 64, kTsTY = 4;
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_ts(TsArgs<R> a) {
+__global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
   const Grid<R>& g = a.g;
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2 * g.nk) return;
-  int i, j, k;
-  ijk(g, p, i, j, k);
+  const int G = g.ni - 2;
+  const int ntx = (G + kTsTX - 1) / kTsTX, nty = (g.nj + kTsTY - 1) / kTsTY;
+  const int b = (int)xcd_block();
+  const int k = b / (ntx * nty) + 1;
+  const int r = b % (ntx * nty);
+  const int c0 = 2 + (r % ntx) * kTsTX, j0 = 1 + (r / ntx) * kTsTY;
+  const int tx = threadIdx.x % kTsTX, ty = threadIdx.x / kTsTX;
+  const int ic = c0 + tx, j = j0 + ty;
   const long ko = g.n2 * (k - 1);
-  const long q = g.ix(i, j);
-  auto TV = [&](int m, int jj) -> R { return (R)a.t0[ko + g.ix(m, jj)]; };
-  auto SV = [&](int m, int jj) -> R { return (R)a.s0[ko + g.ix(m, jj)]; };
-  R pt1 = -adv_ts_point<R>(g, TV, a.t0, a.wm, a.schs, a.uc, a.vc, i, j, k);
-  R pt2 = -adv_ts_point<R>(g, SV, a.s0, a.wm, a.schs, a.uc, a.vc, i, j, k);
-  pt1 = pt1 + dif_point<R>(g, TV, a.t0, a.mch, i, j, k);
-  pt2 = pt2 + dif_point<R>(g, SV, a.s0, a.mch, i, j, k);
-  const R mk = (R)g.mskt(i, j, k);
-  const R m = (R)a.mch[q];
-  if (k == 1) {
-    pt1 = pt1 + (R)GAMMA_T * ((R)a.bndt[q] - (R)a.t0[p]) * mk / g.dpr2[0];
-    pt2 = pt2 + (R)GAMMA_S * ((R)a.bnds[q] - (R)a.s0[p]) * mk * m;
+  // faces [tracer][row][column]: fx/fa east faces, column 0 the tile's
+  // west face; fy/fb north faces, row 0 the tile's south face
+  __shared__ R fx[2][kTsTY][kTsTX + 1], fa[2][kTsTY][kTsTX + 1];
+  __shared__ R fy[2][kTsTY + 1][kTsTX], fb[2][kTsTY + 1][kTsTX];
+
+  auto SCH = [&](int m, int jj) -> R { return (R)a.schs[g.ix(m, jj)]; };
+  auto CH = [&](int m, int jj) -> R { return (R)a.mch[g.ix(m, jj)]; };
+  auto VAR = [&](int v, int m, int jj) -> R { return (R)(v ? a.s0 : a.t0)[ko + g.ix(m, jj)]; };
+  auto UM1 = [&](int m, int jj) -> R { return (R)a.uc[ko + g.ix(m, jj)] * SCH(m, jj); };
+  auto UM2 = [&](int m, int jj) -> R { return (R)a.vc[ko + g.ix(m, jj)] * SCH(m, jj) * g.RH(3, m, jj); };
+  // east-west face at g2 point (m, jj), m canonical: X (op_adv_ts) and
+  // VA (op_dif) of T and S
+  auto face_ew = [&](int m, int jj, int row, int col) {
+    const R um = (jj < 2 || jj > g.nj) ? (R)0 : (R)0.5 * (UM1(m, jj - 1) + UM1(m, jj));  // ter_ns<3>
+    const R chx = (R)0.5 * (CH(m, jj) + CH(m + 1, jj));                                 // ter_ew<1>
+    for (int v = 0; v < 2; ++v) {
+      const R dv = VAR(v, m + 1, jj) - VAR(v, m, jj);
+      fx[v][row][col] = um * (dv / g.DX1(2, m, jj));  // * p1<1,2>
+      R d = dv * (R)g.mskt(m + 1, jj, k) * (R)g.mskt(m, jj, k);
+      d = d / g.DX1(2, m, jj);
+      fa[v][row][col] = chx * (d / ((R)ARAD * g.RH(2, m, jj)));
+    }
+  };
+  // north-south face at g4 point (m, jj): Y and VB * rh_g4
+  auto face_ns = [&](int m, int jj, int row, int col) {
+    const R vm = (R)0.5 * (UM2(m - 1, jj) + UM2(m, jj));                                     // ter_ew<3>
+    const R chy = (jj < 1 || jj > g.njm) ? (R)0 : (R)0.5 * (CH(m, jj) + CH(m, jj + 1));      // ter_ns<1>
+    for (int v = 0; v < 2; ++v) {
+      const R dv = (jj < 1 || jj > g.njm) ? (R)0 : VAR(v, m, jj + 1) - VAR(v, m, jj);
+      fy[v][row][col] = vm * (dv / g.DX2(4, m, jj));  // * p2<1,4>
+      R d = (jj > g.njm) ? (R)0 : (VAR(v, m, jj + 1) - VAR(v, m, jj)) * (R)g.mskt(m, jj + 1, k) * (R)g.mskt(m, jj, k);
+      d = d / g.DX2(4, m, jj);
+      fb[v][row][col] = chy * (d / (R)ARAD) * g.RH(4, m, jj);
+    }
+  };
+  const bool in = ic <= g.nim && j <= g.nj;
+  if (in) {
+    face_ew(ic, j, ty, tx + 1);
+    face_ns(ic, j, ty + 1, tx);
+    if (tx == 0) face_ew(c0 - 1 == 1 ? g.nim : c0 - 1, j, ty, 0);
+    if (ty == 0 && j0 > 1) face_ns(ic, j0 - 1, 0, tx);
   }
-  // the forward step of the FP64 state (mod_int.f90:484-487)
-  double tn = a.t0[p] + (double)pt1 * a.bc * (double)mk / (double)m;
-  double sn = a.s0[p] + (double)pt2 * a.bc * (double)mk / (double)m;
-  if (g.mskt(i, j, k) > 0 && tn < TICE) tn = TICE;
-  a.t1[p] = tn;
-  a.s1[p] = sn;
+  __syncthreads();
+  if (!in) return;
+
+  // one cell (i, j) from the faces of its canonical column ic
+  auto cell = [&](int i) {
+    const long p = ko + g.ix(i, j);
+    const long q = g.ix(i, j);
+    R pt[2];
+    for (int v = 0; v < 2; ++v) {
+      const double* varp = v ? a.s0 : a.t0;
+      // op_adv_ts (mod_op.f90:280-325)
+      const R rh1 = g.RH(1, i, j);
+      R ans = (R)0.5 * (fx[v][ty][tx] + fx[v][ty][tx + 1]) / ((R)ARAD * rh1);                      // ter_ew<2>
+      ans = ans + ((j < 2 || j > g.nj) ? (R)0 : (R)0.5 * (fy[v][ty][tx] + fy[v][ty + 1][tx])) /    // ter_ns<4>
+                      ((R)ARAD * rh1);
+      auto DX3 = [&](int kk) -> R {
+        if (kk < 2 || kk > g.nk) return (R)0;
+        return (R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)];
+      };
+      R wka = (a.wm[g.ix(i, j, k)] * DX3(k) + a.wm[g.ix(i, j, k + 1)] * DX3(k + 1)) * (R)0.5;
+      wka = wka / g.dpr2[k - 1];
+      pt[v] = -(ans + wka);
+    }
+    for (int v = 0; v < 2; ++v) {
+      const double* varp = v ? a.s0 : a.t0;
+      // op_dif (mod_op.f90:186-222): divg<2,4,1> of (VA, VB*rh)
+      const R a1 = (fa[v][ty][tx + 1] - fa[v][ty][tx]) / g.DX1(1, ic, j);
+      const R d = (j < 2 || j > g.nj) ? (R)0 : (fb[v][ty + 1][tx] - fb[v][ty][tx]);
+      const R b1 = d / g.DX2(1, ic, j);
+      R ans = (R)AH_C * ((a1 + b1) / ((R)ARAD * g.RH(1, ic, j)));
+      auto WKP = [&](int kk) -> R {
+        R w = (R)0;
+        if (kk >= 2 && kk <= g.nk)
+          w = ((R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) *
+              (R)g.mskt(i, j, kk) / g.dz1[kk - 1];
+        return (R)GRAV * (R)RHO0 * g.kh[g.ix(i, j, kk)] * w;
+      };
+      pt[v] = pt[v] + (ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1]);
+    }
+    const R mk = (R)g.mskt(i, j, k);
+    const R m = (R)a.mch[q];
+    if (k == 1) {
+      pt[0] = pt[0] + (R)GAMMA_T * ((R)a.bndt[q] - (R)a.t0[p]) * mk / g.dpr2[0];
+      pt[1] = pt[1] + (R)GAMMA_S * ((R)a.bnds[q] - (R)a.s0[p]) * mk * m;
+    }
+    // the forward step of the FP64 state (mod_int.f90:484-487)
+    double tn = a.t0[p] + (double)pt[0] * a.bc * (double)mk / (double)m;
+    double sn = a.s0[p] + (double)pt[1] * a.bc * (double)mk / (double)m;
+    if (g.mskt(i, j, k) > 0 && tn < TICE) tn = TICE;
+    a.t1[p] = tn;
+    a.s1[p] = sn;
+  };
+  cell(ic);
+  if (ic == g.nim) cell(1);
+  if (ic == 2) cell(g.ni);
 }
 
 // convect / adjust / mix (mod_int.f90:509-600) on one column, in place.
@@ -1061,7 +1106,11 @@ std::string step_post_t(Ctx& c) {
     a.g = g; a.t0 = T; a.s0 = S; a.wm = P<R>(f.wm); a.uc = U; a.vc = V;
     a.schs = h.schs; a.mch = h.mch; a.bndt = h.bnd[2]; a.bnds = h.bnd[3];
     a.t1 = P<double>(f.T[1 - c.cur]); a.s1 = P<double>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
-    LAUNCH(k_ts<R>, n3, a);
+    const int G = c.ni - 2;
+    const unsigned nb = (unsigned)(((G + kTsTX - 1) / kTsTX) * ((c.nj + kTsTY - 1) / kTsTY) * c.nk);
+    hipLaunchKernelGGL(k_ts_tiles<R>, dim3(nb), dim3(kTsTX * kTsTY), 0, c.stream, a);
+    hipError_t et = hipGetLastError();
+    if (et != hipSuccess) return hip_err(et, "k_ts_tiles");
   }
   c.cur = 1 - c.cur;
   {
