@@ -323,85 +323,117 @@ __global__ void __launch_bounds__(kBlock) k_adv(Grid<R> g, const double* __restr
   avc[p] = out[1];
 }
 
-// fri_r3d (mod_op.f90:103-167) at one point: horizontal viscosity,
-// metric terms, vertical viscosity with wind stress and bottom drag.
-// v / vcc are callables (unweighted velocity and the "vc" argument).
-template <typename R, typename FV, typename FC>
-__device__ __forceinline__ R fri_point(const Grid<R>& g, const double* chc, FV v, const double* vp, FC vcc, R tau,
-                                       R am, R sch, int i, int j, int k) {
-  auto GX = [&](int m, int jj) -> R { return grax<3, 4>(g, v, m, jj); };
-  auto GY = [&](int m, int jj) -> R { return gray<3, 2>(g, v, m, jj); };
-  auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
-  auto VA = [&](int m, int jj) -> R { return ter_ns<1>(g, CH, m, jj, (R)0) * GX(m, jj); };
-  auto VB = [&](int m, int jj) -> R { return ter_ew<1>(g, CH, m, jj) * GY(m, jj); };
-  R wka = divg<4, 2, 3>(g, VA, VB, i, j);
-  long q = g.ix(i, j);
-  R ans = am / sch * wka;
-  ans = ans + am * g.cv1[q] * (R)vp[g.ix(i, j, k)];
-  int ic = g.canon(i);
-  R dx = (R)ARAD * g.RH(4, ic, j) * (g.DX1(4, ic, j) + g.DX1(4, ic + 1, j));
-  R pxc = (vcc(ic + 1, j, k) - vcc(ic - 1, j, k)) / dx;
-  ans = ans + am * g.cv2[q] * sch * pxc;
-  const int lev = g.lev_u[q];
-  auto WKP = [&](int kk) -> R {
-    if (kk == 1) return (R)GRAV * tau / sch;
-    if (lev > 0 && kk == lev + 1) {
-      R vv = v(i, j, lev), cc = vcc(i, j, lev);
-      R mag = sqrt(vv * vv + cc * cc);
-      return vv * (R)GRAV * (R)CDBOT * mag / sch;
-    }
-    if (kk > g.nk) return (R)0;
-    R pz = ((R)vp[g.ix(i, j, kk - 1)] - (R)vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) *
-           (R)g.msku(i, j, kk) / g.dz1[kk - 1];
-    return (R)GRAV * (R)RHO0 * g.km[g.ix(i, j, kk)] * pz / (sch * sch);
-  };
-  R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
-  return ans * (R)g.msku(i, j, k) + vert;
-}
+// op_fri for both components (mod_op.f90:103-184) with the horizontal
+// viscosity's face fluxes staged in LDS.  fri_r3d's first term is
+// div(ch*gra(v)) on U points: VA = ter(ch)*grax(v) on the east-west faces
+// (g4 points) and VB*rh = ter(ch)*gray(v)*rh on the north-south faces (g2
+// points), each shared by the two U cells beside it.  A workgroup owns a
+// kFriTX x kFriTY tile of canonical columns at one level, evaluates every
+// face once for u and v, then finishes each cell (metric terms, vertical
+// viscosity, wind stress, bottom drag) as fri_point does, with the same
+// operations in the same order.  Halo columns take their faces from their
+// canonical columns and everything else from themselves.
+constexpr int kFriTX = 64, kFriTY = 4;
 
-// op_fri (mod_op.f90:169-184): fx from (u, uc, -v, taux), fy from
-// (v, vc, u, tauy) with u = uc/sch, v = vc/sch.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_fri(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
-                      const R* __restrict__ am,
-                      const double* __restrict__ chc, const double* __restrict__ schr,
-                      const double* __restrict__ taux, const double* __restrict__ tauy, R* fx, R* fy) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2 * g.nk) return;
-  int i, j, k;
-  ijk(g, p, i, j, k);
-  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
-  // 3-D callables (m, jj, kk)
-  auto U3 = [&](int m, int jj, int kk) -> R { return (R)uc[g.ix(m, jj, kk)] / SCH(m, jj); };
-  auto V3 = [&](int m, int jj, int kk) -> R { return (R)vc[g.ix(m, jj, kk)] / SCH(m, jj); };
-  auto MV3 = [&](int m, int jj, int kk) -> R { return -V3(m, jj, kk); };
-  const long q = g.ix(i, j);
-  const R sch = SCH(i, j);
-  const R a = am[p];
-  // the 2-D (m, jj) views at level k that the horizontal operators take,
-  // plus a 3-D view for the bottom drag level
-  struct View {
-    const Grid<R>& g;
-    int k;
-    bool isu;
-    const double* uc;
-    const double* vc;
-    const double* schr;
-    __device__ R operator()(int m, int jj) const {
-      R s = sch_ready<R>(g, schr, m, jj);
-      return (R)(isu ? uc : vc)[g.ix(m, jj, k)] / s;
-    }
-    __device__ R operator()(int m, int jj, int kk) const {
-      R s = sch_ready<R>(g, schr, m, jj);
-      return (R)(isu ? uc : vc)[g.ix(m, jj, kk)] / s;
+__global__ void __launch_bounds__(kFriTX * kFriTY)
+    k_fri_tiles(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc, const R* __restrict__ am,
+                const double* __restrict__ chc, const double* __restrict__ schr, const double* __restrict__ taux,
+                const double* __restrict__ tauy, R* fx, R* fy) {
+  const int G = g.ni - 2;
+  const int ntx = (G + kFriTX - 1) / kFriTX, nty = (g.nj + kFriTY - 1) / kFriTY;
+  const int b = (int)xcd_block();
+  const int k = b / (ntx * nty) + 1;
+  const int r = b % (ntx * nty);
+  const int c0 = 2 + (r % ntx) * kFriTX, j0 = 1 + (r / ntx) * kFriTY;
+  const int tx = threadIdx.x % kFriTX, ty = threadIdx.x / kFriTX;
+  const int ic = c0 + tx, j = j0 + ty;
+  const long ko = g.n2 * (k - 1);
+  // [component][row][face]: fa east-west faces at g4 points c0 + f (face f
+  // is cell f's west face, f + 1 its east face), fb north-south faces at
+  // g2 points of rows j0 + row (row ty is cell ty's south face)
+  __shared__ R fa[2][kFriTY][kFriTX + 1];
+  __shared__ R fb[2][kFriTY + 1][kFriTX];
+  // the horizontal views at level k: v = uc/sch (u) or vc/sch (v)
+  auto VV = [&](int c, int m, int jj) -> R { return (R)(c ? vc : uc)[ko + g.ix(m, jj)] / (R)schr[g.ix(m, jj)]; };
+  auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
+  // VA(m, jj) = ter_ns<1>(ch) * grax<3,4>(v), m canonical
+  auto face_ew = [&](int m, int jj, int row, int f) {
+    const R chy = (jj < 1 || jj > g.njm) ? (R)0 : (R)0.5 * (CH(m, jj) + CH(m, jj + 1));
+    for (int c = 0; c < 2; ++c) {
+      const R gx = (VV(c, m, jj) - VV(c, m - 1, jj)) / g.DX1(4, m, jj) / ((R)ARAD * g.RH(4, m, jj));
+      fa[c][row][f] = chy * gx;
     }
   };
-  View vu{g, k, true, uc, vc, schr};
-  View vv{g, k, false, uc, vc, schr};
-  (void)U3;
-  (void)V3;
-  fx[p] = fri_point<R>(g, chc, vu, uc, MV3, (R)taux[q], a, sch, i, j, k);
-  fy[p] = fri_point<R>(g, chc, vv, vc, vu, (R)tauy[q], a, sch, i, j, k);
+  // VB(m, jj) * rh_g2 = ter_ew<1>(ch) * gray<3,2>(v) * rh_g2
+  auto face_ns = [&](int m, int jj, int row, int f) {
+    const R chx = (R)0.5 * (CH(m, jj) + CH(m + 1, jj));
+    for (int c = 0; c < 2; ++c) {
+      const R d = (jj < 2 || jj > g.nj) ? (R)0 : (VV(c, m, jj) - VV(c, m, jj - 1));
+      const R gy = d / g.DX2(2, m, jj) / (R)ARAD;
+      fb[c][row][f] = chx * gy * g.RH(2, m, jj);
+    }
+  };
+  const bool in = ic <= g.nim && j <= g.nj;
+  if (in) {
+    face_ew(ic, j, ty, tx);
+    face_ns(ic, j, ty, tx);
+    if (tx == kFriTX - 1 || ic == g.nim) face_ew(g.canon(ic + 1), j, ty, tx + 1);
+    if ((ty == kFriTY - 1 || j == g.nj) && j < g.nj) face_ns(ic, j + 1, ty + 1, tx);
+  }
+  __syncthreads();
+  if (!in) return;
+
+  auto cell = [&](int i) {
+    const long p = ko + g.ix(i, j);
+    const long q = g.ix(i, j);
+    const R sch = (R)schr[q];
+    const R a = am[p];
+    const int lev = g.lev_u[q];
+    R out[2];
+    for (int c = 0; c < 2; ++c) {
+      // fx from (u, uc, -v, taux), fy from (v, vc, u, tauy) (mod_op.f90:169-184)
+      const double* vp = c ? vc : uc;
+      const R tau = (R)(c ? tauy : taux)[q];
+      auto v3 = [&](int m, int jj, int kk) -> R {
+        return (R)(c ? vc : uc)[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)];
+      };
+      auto vcc = [&](int m, int jj, int kk) -> R {
+        // -v for fx, u for fy
+        return c ? (R)uc[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)]
+                 : -((R)vc[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)]);
+      };
+      // divg<4,2,3>(VA, VB) (mod_op.f90:813-848)
+      const R a1 = (fa[c][ty][tx + 1] - fa[c][ty][tx]) / g.DX1(3, ic, j);
+      const R d = (j < 1 || j > g.njm) ? (R)0 : (fb[c][ty + 1][tx] - fb[c][ty][tx]);
+      const R b1 = d / g.DX2(3, ic, j);
+      const R wka = (a1 + b1) / ((R)ARAD * g.RH(3, ic, j));
+      R ans = a / sch * wka;
+      ans = ans + a * g.cv1[q] * (R)vp[g.ix(i, j, k)];
+      const R dx = (R)ARAD * g.RH(4, ic, j) * (g.DX1(4, ic, j) + g.DX1(4, ic + 1, j));
+      const R pxc = (vcc(ic + 1, j, k) - vcc(ic - 1, j, k)) / dx;
+      ans = ans + a * g.cv2[q] * sch * pxc;
+      auto WKP = [&](int kk) -> R {
+        if (kk == 1) return (R)GRAV * tau / sch;
+        if (lev > 0 && kk == lev + 1) {
+          R vv = v3(i, j, lev), cc = vcc(i, j, lev);
+          R mag = sqrt(vv * vv + cc * cc);
+          return vv * (R)GRAV * (R)CDBOT * mag / sch;
+        }
+        if (kk > g.nk) return (R)0;
+        R pz = ((R)vp[g.ix(i, j, kk - 1)] - (R)vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) *
+               (R)g.msku(i, j, kk) / g.dz1[kk - 1];
+        return (R)GRAV * (R)RHO0 * g.km[g.ix(i, j, kk)] * pz / (sch * sch);
+      };
+      const R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
+      out[c] = ans * (R)g.msku(i, j, k) + vert;
+    }
+    fx[p] = out[0];
+    fy[p] = out[1];
+  };
+  cell(ic);
+  if (ic == g.nim) cell(1);
+  if (ic == 2) cell(g.ni);
 }
 
 // column integral of the baroclinic tendency (mod_int.f90:136-149)
@@ -1055,7 +1087,14 @@ std::string step_pre_t(Ctx& c) {
       if ((e = hipMemcpyAsync(avpp, avp, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
     }
   }
-  LAUNCH(k_fri<R>, n3, g, U, V, P<R>(f.am), h.chc, h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
+  {
+    const int G = c.ni - 2;
+    const unsigned nb = (unsigned)(((G + kFriTX - 1) / kFriTX) * ((c.nj + kFriTY - 1) / kFriTY) * c.nk);
+    hipLaunchKernelGGL(k_fri_tiles<R>, dim3(nb), dim3(kFriTX * kFriTY), 0, c.stream, g, U, V, P<R>(f.am), h.chc,
+                       h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
+    hipError_t ef = hipGetLastError();
+    if (ef != hipSuccess) return hip_err(ef, "k_fri_tiles");
+  }
   LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
   return "";
 }
