@@ -330,8 +330,8 @@ __global__ void __launch_bounds__(kBlock) k_adv(Grid<R> g, const double* __restr
 // points), each shared by the two U cells beside it.  A workgroup owns a
 // kFriTX x kFriTY tile of canonical columns at one level, evaluates every
 // face once for u and v, then finishes each cell (metric terms, vertical
-// viscosity, wind stress, bottom drag) as fri_point does, with the same
-// operations in the same order.  Halo columns take their faces from their
+// viscosity, wind stress, bottom drag) as fri_r3d orders it, with the
+// same operations in the same order.  Halo columns take their faces from their
 // canonical columns and everything else from themselves.
 constexpr int kFriTX = 64, kFriTY = 4;
 
