@@ -266,61 +266,108 @@ __device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* schr, int
   return (R)schr[g.ix(m, jj)];
 }
 
-// op_adv (mod_op.f90:224-278) for uc and vc at one point.
+// op_adv (mod_op.f90:224-278) for uc and vc, with the face quantities
+// staged in LDS.  On the east-west faces of a U cell (g4 points) live the
+// transport UU = ter(u/sch) and the fluxes M1 = ter(var)*UU, on the
+// north-south faces (g2 points) VV = ter(v/sch*rh) and M2 = ter(var)*VV;
+// each face is shared by the two cells beside it.  A workgroup owns a
+// kAdvTX x kAdvTY tile of canonical columns at one level, evaluates every
+// face once, then finishes each cell (flux divergence, vertical flux,
+// -var*div(v)/2) with the same operations in the same order as the
+// per-cell composition.  Halo columns take their faces from their
+// canonical columns and everything else from themselves.
+constexpr int kAdvTX = 64, kAdvTY = 4;
+
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_adv(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
-                      const R* __restrict__ wm, const double* __restrict__ chc, const double* __restrict__ schr,
-                      R* auc, R* avc) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2 * g.nk) return;
-  int i, j, k;
-  ijk(g, p, i, j, k);
+__global__ void __launch_bounds__(kAdvTX * kAdvTY)
+    k_adv_tiles(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc, const R* __restrict__ wm,
+                const double* __restrict__ chc, const double* __restrict__ schr, R* auc, R* avc) {
+  const int G = g.ni - 2;
+  const int ntx = (G + kAdvTX - 1) / kAdvTX, nty = (g.nj + kAdvTY - 1) / kAdvTY;
+  const int b = (int)xcd_block();
+  const int k = b / (ntx * nty) + 1;
+  const int r = b % (ntx * nty);
+  const int c0 = 2 + (r % ntx) * kAdvTX, j0 = 1 + (r / ntx) * kAdvTY;
+  const int tx = threadIdx.x % kAdvTX, ty = threadIdx.x / kAdvTX;
+  const int ic = c0 + tx, j = j0 + ty;
   const long ko = g.n2 * (k - 1);
   const double* u = uc + ko;
   const double* v = vc + ko;
-  auto SCH = [&](int m, int jj) -> R { return sch_ready<R>(g, schr, m, jj); };
+  // east-west faces [row][f] at g4 points c0 + f (f is cell f's west
+  // face); north-south faces [row][tx] at g2 points of row j0 + row (row
+  // ty is cell ty's south face)
+  __shared__ R fuu[kAdvTY][kAdvTX + 1], fm1[2][kAdvTY][kAdvTX + 1];
+  __shared__ R fvv[kAdvTY + 1][kAdvTX], fm2[2][kAdvTY + 1][kAdvTX];
+  auto SCH = [&](int m, int jj) -> R { return (R)schr[g.ix(m, jj)]; };
   auto X = [&](int m, int jj) -> R { return (R)u[g.ix(m, jj)] / SCH(m, jj); };
   auto Y = [&](int m, int jj) -> R {
     R y = (R)v[g.ix(m, jj)] / SCH(m, jj);
     return y * g.RH(3, m, jj);
   };
-  auto UU = [&](int m, int jj) -> R { return ter_ew<3>(g, X, m, jj); };          // g3 -> g4
-  auto VV = [&](int m, int jj) -> R { return ter_ns<3>(g, Y, m, jj, (R)0); };    // g3 -> g2
-  // w = ter(wm/chc, g1 -> g3) on level kk of nkp
-  auto W = [&](int kk) -> R {
-    const R* w = wm + g.n2 * (kk - 1);
-    auto WW = [&](int m, int jj) -> R { return w[g.ix(m, jj)] / (R)chc[g.ix(m, jj)]; };
-    return ter_di<1>(g, WW, i, j, (R)0);
+  auto VK = [&](int c, int m, int jj) -> R { return (R)(c ? v : u)[g.ix(m, jj)]; };
+  // ter_ew<3> (g3 -> g4) at g4 point (m, jj), m canonical
+  auto face_ew = [&](int m, int jj, int row, int f) {
+    const R uu = (R)0.5 * (X(m - 1, jj) + X(m, jj));
+    fuu[row][f] = uu;
+    for (int c = 0; c < 2; ++c) fm1[c][row][f] = (R)0.5 * (VK(c, m - 1, jj) + VK(c, m, jj)) * uu;
   };
-  const R w_k = W(k), w_k1 = W(k + 1);
-  const R rh3 = g.RH(3, i, j);
-  const R dpr = g.dpr2[k - 1];
-  const R d1 = p1<4, 3>(g, UU, i, j);
-  const R d2 = p2<2, 3>(g, VV, i, j);
-  const R d3 = (w_k - w_k1) / dpr;
-  const R mu = (R)g.msku(i, j, k);
-  R out[2];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const double* var = c == 0 ? uc : vc;
-    const double* vk = var + ko;
-    auto VK = [&](int m, int jj) -> R { return (R)vk[g.ix(m, jj)]; };
-    auto M1 = [&](int m, int jj) -> R { return ter_ew<3>(g, VK, m, jj) * UU(m, jj); };
-    auto M2 = [&](int m, int jj) -> R { return ter_ns<3>(g, VK, m, jj, (R)0) * VV(m, jj); };
-    R ans = (p1<4, 3>(g, M1, i, j) + p2<2, 3>(g, M2, i, j)) / ((R)ARAD * rh3);
-    // vertical flux through the U-cell faces, zero unless both layers wet
-    auto WB = [&](int kk) -> R {
-      if (kk < 2 || kk > g.nk) return (R)0;
-      R a = ((R)var[g.ix(i, j, kk - 1)] + (R)var[g.ix(i, j, kk)]) * (R)0.5;
-      a = a * (kk == k ? w_k : w_k1);
-      return a * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk);
-    };
-    ans = ans + (WB(k) - WB(k + 1)) / dpr;
-    ans = ans - (R)0.5 * (R)vk[g.ix(i, j)] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
-    out[c] = ans * mu;
+  // ter_ns<3> (g3 -> g2, rows 2..nj) at g2 point (m, jj)
+  auto face_ns = [&](int m, int jj, int row, int f) {
+    const bool w = jj >= 2 && jj <= g.nj;
+    const R vv = w ? (R)0.5 * (Y(m, jj - 1) + Y(m, jj)) : (R)0;
+    fvv[row][f] = vv;
+    for (int c = 0; c < 2; ++c) fm2[c][row][f] = (w ? (R)0.5 * (VK(c, m, jj - 1) + VK(c, m, jj)) : (R)0) * vv;
+  };
+  const bool in = ic <= g.nim && j <= g.nj;
+  if (in) {
+    face_ew(ic, j, ty, tx);
+    face_ns(ic, j, ty, tx);
+    if (tx == kAdvTX - 1 || ic == g.nim) face_ew(g.canon(ic + 1), j, ty, tx + 1);
+    if ((ty == kAdvTY - 1 || j == g.nj) && j < g.nj) face_ns(ic, j + 1, ty + 1, tx);
   }
-  auc[p] = out[0];
-  avc[p] = out[1];
+  __syncthreads();
+  if (!in) return;
+
+  auto cell = [&](int i) {
+    const long p = ko + g.ix(i, j);
+    // w = ter(wm/chc, g1 -> g3) on level kk of nkp
+    auto W = [&](int kk) -> R {
+      const R* w = wm + g.n2 * (kk - 1);
+      auto WW = [&](int m, int jj) -> R { return w[g.ix(m, jj)] / (R)chc[g.ix(m, jj)]; };
+      return ter_di<1>(g, WW, i, j, (R)0);
+    };
+    const R w_k = W(k), w_k1 = W(k + 1);
+    const R rh3 = g.RH(3, i, j);
+    const R dpr = g.dpr2[k - 1];
+    const bool rows = j >= 1 && j <= g.njm;  // p2's rows (g2 -> g3)
+    const R d1 = (fuu[ty][tx + 1] - fuu[ty][tx]) / g.DX1(3, ic, j);
+    const R d2 = (rows ? fvv[ty + 1][tx] - fvv[ty][tx] : (R)0) / g.DX2(3, ic, j);
+    const R d3 = (w_k - w_k1) / dpr;
+    const R mu = (R)g.msku(i, j, k);
+    R out[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double* var = c == 0 ? uc : vc;
+      const R p1m = (fm1[c][ty][tx + 1] - fm1[c][ty][tx]) / g.DX1(3, ic, j);
+      const R p2m = (rows ? fm2[c][ty + 1][tx] - fm2[c][ty][tx] : (R)0) / g.DX2(3, ic, j);
+      R ans = (p1m + p2m) / ((R)ARAD * rh3);
+      // vertical flux through the U-cell faces, zero unless both layers wet
+      auto WB = [&](int kk) -> R {
+        if (kk < 2 || kk > g.nk) return (R)0;
+        R a = ((R)var[g.ix(i, j, kk - 1)] + (R)var[g.ix(i, j, kk)]) * (R)0.5;
+        a = a * (kk == k ? w_k : w_k1);
+        return a * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk);
+      };
+      ans = ans + (WB(k) - WB(k + 1)) / dpr;
+      ans = ans - (R)0.5 * (R)var[p] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
+      out[c] = ans * mu;
+    }
+    auc[p] = out[0];
+    avc[p] = out[1];
+  };
+  cell(ic);
+  if (ic == g.nim) cell(1);
+  if (ic == 2) cell(g.ni);
 }
 
 // op_fri for both components (mod_op.f90:103-184) with the horizontal
@@ -1074,7 +1121,14 @@ std::string step_pre_t(Ctx& c) {
   R* avp = P<R>(f.AV[(c.ab + 2) % 3]);
   R* aupp = P<R>(f.AU[(c.ab + 1) % 3]);
   R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
-  LAUNCH(k_adv<R>, n3, g, U, V, P<R>(f.wm), h.chc, h.schr, au, av);
+  {
+    const int G = c.ni - 2;
+    const unsigned nb = (unsigned)(((G + kAdvTX - 1) / kAdvTX) * ((c.nj + kAdvTY - 1) / kAdvTY) * c.nk);
+    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wm), h.chc,
+                       h.schr, au, av);
+    hipError_t ea = hipGetLastError();
+    if (ea != hipSuccess) return hip_err(ea, "k_adv_tiles");
+  }
   if (c.rst != 1) {  // AB3 bootstrap (mod_int.f90:122-132)
     size_t bytes = n3 * sizeof(R);
     if (c.step_i == 1) {
