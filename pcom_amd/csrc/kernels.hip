@@ -844,28 +844,32 @@ struct Column {
   }
 };
 
-// kCW columns per workgroup; the kKL k-lanes stage t, s (FP64 state) and
-// prho (E) of each column into LDS (dynamic: 2*nk*kCW doubles + 2*nk*kCW
+constexpr int kConvCW = 32, kConvKL = 8;
+
+// kConvCW columns per workgroup (fewer than the other column kernels: the
+// LDS per workgroup then allows 5 workgroups per CU, whose memory sweeps and
+// searches overlap); the kConvKL k-lanes stage t, s (FP64 state) and
+// prho (E) of each column into LDS (dynamic: 2*nk*kConvCW doubles + 2*nk*kConvCW
 // E), lane 0 runs the convective adjustment, then all lanes write back and
 // accumulate act, acs and acr (mod_int.f90:493-505) in R.
 template <typename R, typename E>
-__global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<E> g, const double* __restrict__ dpr, double* tc,
+__global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const double* __restrict__ dpr, double* tc,
                                                     double* sc, const E* __restrict__ prho1,
                                                     const E* __restrict__ prho2, const double* __restrict__ chc,
                                                     R* act, R* acs, R* acr) {
   extern __shared__ double conv_lds[];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
   double* lt = conv_lds;
-  double* ls = lt + nk * kCW;
-  E* l1 = reinterpret_cast<E*>(ls + nk * kCW);
-  E* l2 = l1 + nk * kCW;
-  const long p = (long)xcd_block() * kCW + tx;
+  double* ls = lt + nk * kConvCW;
+  E* l1 = reinterpret_cast<E*>(ls + nk * kConvCW);
+  E* l2 = l1 + nk * kConvCW;
+  const long p = (long)xcd_block() * kConvCW + tx;
   const bool ok = p < g.n2;
   const int nlev = ok ? g.lev_t[p] : 0;
   if (ok) {
-    for (int k = 1 + kl; k <= nk; k += kKL) {
+    for (int k = 1 + kl; k <= nk; k += kConvKL) {
       const long q = p + g.n2 * (k - 1);
-      const int o = (k - 1) * kCW + tx;
+      const int o = (k - 1) * kConvCW + tx;
       lt[o] = tc[q];
       ls[o] = sc[q];
       l1[o] = prho1[q];
@@ -873,15 +877,15 @@ __global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<E> g, const double* __r
     }
   }
   __syncthreads();
-  // the searches of the kCW columns are spread over the kKL waves (column
-  // tx + kCW/kKL * kl on lane tx < kCW/kKL of wave kl): a wave then waits for
-  // the longest of kCW/kKL searches instead of kCW
-  constexpr int kPer = kCW / kKL;
+  // the searches of the kConvCW columns are spread over the kConvKL k-lanes
+  // (column tx + kPer * kl on lane tx < kPer of k-lane kl), so every wave of
+  // the workgroup runs a few searches and waits only for the longest of them
+  constexpr int kPer = kConvCW / kConvKL;
   const int col = tx + kPer * kl;
-  const long pcol = (long)xcd_block() * kCW + col;
+  const long pcol = (long)xcd_block() * kConvCW + col;
   const int ncol = tx < kPer && pcol < g.n2 ? g.lev_t[pcol] : 0;
   if (ncol > 1) {
-    Column<E> c{lt + col, ls + col, l1 + col, l2 + col, dpr, kCW};
+    Column<E> c{lt + col, ls + col, l1 + col, l2 + col, dpr, kConvCW};
     const int nlev = ncol;
     int kcnt = 1;
     while (kcnt < nlev) {
@@ -912,9 +916,9 @@ __global__ void __launch_bounds__(kCW * kKL) k_conv(Grid<E> g, const double* __r
   __syncthreads();  // the column's new T/S are visible to the other k-lanes
   if (!ok) return;
   const E ch = (E)chc[p];
-  for (int k = 1 + kl; k <= nk; k += kKL) {
+  for (int k = 1 + kl; k <= nk; k += kConvKL) {
     const long q = p + g.n2 * (k - 1);
-    const int o = (k - 1) * kCW + tx;
+    const int o = (k - 1) * kConvCW + tx;
     const double t = lt[o], s = ls[o];
     tc[q] = t;
     sc[q] = s;
@@ -1207,13 +1211,14 @@ std::string step_post_t(Ctx& c) {
   }
   c.cur = 1 - c.cur;
   {
-    const size_t lds = 2 * (size_t)c.nk * kCW * (sizeof(double) + sizeof(E));
+    const size_t lds = 2 * (size_t)c.nk * kConvCW * (sizeof(double) + sizeof(E));
     if (lds > 65536) {
       hipError_t ea =
           hipFuncSetAttribute((const void*)k_conv<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (ea != hipSuccess) return hip_err(ea, "k_conv LDS size");
     }
-    hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), lds, c.stream, ge,
+    hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kConvCW - 1) / kConvCW)), dim3(kConvCW, kConvKL), lds,
+                       c.stream, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
                        P<R>(f.act), P<R>(f.acs), P<R>(f.acr));
     hipError_t e2 = hipGetLastError();
