@@ -169,27 +169,53 @@ __global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restric
 // mod_op.f90:946-981): bphi = vint(ter(adp) + pr*ter(alpha)),
 // bgraphi = vint(gra(adp)), each with the local mask and the mask cshift'ed
 // by -1 (row 1 wraps to row nj; the E/W wrap is overwritten by the swap).
+// kColCW columns per workgroup, kColKL k-lanes: the lanes evaluate the
+// per-level terms of the columns in parallel into LDS, one lane per column
+// then sums them in k order (the reference's order), and for the mode
+// coupling all lanes apply the result.
+constexpr int kColCW = 32, kColKL = 8;
+
 template <typename R>
-__global__ void k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict__ alpha, double* b0, double* b1,
-                       double* b2, double* b3, double* c0, double* c1, double* c2, double* c3) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  int ic = g.canon(i);
-  int js = (j == 1) ? g.nj : j - 1;
+__global__ void __launch_bounds__(kColCW * kColKL)
+    k_bphi(Grid<R> g, const R* __restrict__ adp, const R* __restrict__ alpha, double* b0, double* b1, double* b2,
+           double* b3, double* c0, double* c1, double* c2, double* c3) {
+  extern __shared__ unsigned char bphi_lds[];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  R* tv2 = reinterpret_cast<R*>(bphi_lds);  // [nk][kColCW] each
+  R* tv4 = tv2 + nk * kColCW;
+  R* tvx = tv4 + nk * kColCW;
+  R* tvy = tvx + nk * kColCW;
+  const long p = (long)xcd_block() * kColCW + tx;
+  const bool ok = p < g.n2;
+  const int i = ok ? (int)(p % g.ni) + 1 : 2, j = ok ? (int)(p / g.ni) + 1 : 1;
+  const int ic = g.canon(i);
+  const int js = (j == 1) ? g.nj : j - 1;
+  if (ok) {
+    for (int k = 1 + kl; k <= nk; k += kColKL) {
+      const R* ad = adp + g.n2 * (k - 1);
+      const R* al = alpha + g.n2 * (k - 1);
+      auto AD = [&](int m, int jj) -> R { return ad[g.ix(m, jj)]; };
+      auto AL = [&](int m, int jj) -> R { return al[g.ix(m, jj)]; };
+      R pr = g.pr2[k - 1], dpr = g.dpr2[k - 1];
+      R wk2 = ter_ew<1>(g, AD, ic, j) + pr * ter_ew<1>(g, AL, ic, j);
+      R wk4 = ter_ns<1>(g, AD, ic, j, (R)0) + pr * ter_ns<1>(g, AL, ic, j, (R)0);
+      R wx = grax<1, 2>(g, AD, ic, j);
+      R wy = gray<1, 4>(g, AD, ic, j);
+      const int o = (k - 1) * kColCW + tx;
+      tv2[o] = wk2 * dpr;
+      tv4[o] = wk4 * dpr;
+      tvx[o] = wx * dpr;
+      tvy[o] = wy * dpr;
+    }
+  }
+  __syncthreads();
+  if (!ok || kl != 0) return;
+  const int lu = g.lev_u[g.ix(ic, j)], lus = g.lev_u[g.ix(ic, js)], luw = g.lev_u[g.ix(ic - 1, j)];
   double sxn = 0, sxs = 0, sye = 0, syw = 0, gxn = 0, gxs = 0, gye = 0, gyw = 0;
-  for (int k = 1; k <= g.nk; ++k) {
-    const R* ad = adp + g.n2 * (k - 1);
-    const R* al = alpha + g.n2 * (k - 1);
-    auto AD = [&](int m, int jj) -> R { return ad[g.ix(m, jj)]; };
-    auto AL = [&](int m, int jj) -> R { return al[g.ix(m, jj)]; };
-    R pr = g.pr2[k - 1], dpr = g.dpr2[k - 1];
-    R wk2 = ter_ew<1>(g, AD, ic, j) + pr * ter_ew<1>(g, AL, ic, j);
-    R wk4 = ter_ns<1>(g, AD, ic, j, (R)0) + pr * ter_ns<1>(g, AL, ic, j, (R)0);
-    R wx = grax<1, 2>(g, AD, ic, j);
-    R wy = gray<1, 4>(g, AD, ic, j);
-    R mu = (R)g.msku(ic, j, k), mus = (R)g.msku(ic, js, k), muw = (R)g.msku(ic - 1, j, k);
-    R v2 = wk2 * dpr, v4 = wk4 * dpr, vx = wx * dpr, vy = wy * dpr;
+  for (int k = 1; k <= nk; ++k) {
+    const int o = (k - 1) * kColCW + tx;
+    const R v2 = tv2[o], v4 = tv4[o], vx = tvx[o], vy = tvy[o];
+    R mu = (R)(lu >= k ? 1 : 0), mus = (R)(lus >= k ? 1 : 0), muw = (R)(luw >= k ? 1 : 0);
     sxn += (double)(v2 * mu);
     sxs += (double)(v2 * mus);
     sye += (double)(v4 * mu);
@@ -579,24 +605,43 @@ __global__ void k_clin(ClinArgs<E> a) {
 
 // barotropic / baroclinic mode coupling (mod_int.f90:372-380), on the
 // FP64 state
-__global__ void k_clin_cpl(Grid<double> g, double* uc, double* vc, const double* __restrict__ buc,
-                           const double* __restrict__ bvc) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  double s1 = 0, s2 = 0;
-  for (int k = 1; k <= g.nk; ++k) {
-    long q = p + g.n2 * (k - 1);
-    double mu = (double)g.msku(i, j, k);
-    s1 += uc[q] * g.dpr2[k - 1] * mu;
-    s2 += vc[q] * g.dpr2[k - 1] * mu;
+__global__ void __launch_bounds__(kColCW * kColKL)
+    k_clin_cpl(Grid<double> g, double* uc, double* vc, const double* __restrict__ buc,
+               const double* __restrict__ bvc) {
+  extern __shared__ double cpl_lds[];  // [2][nk][kColCW]
+  __shared__ double m1s[kColCW], m2s[kColCW];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  double* t1 = cpl_lds;
+  double* t2 = t1 + nk * kColCW;
+  const long p = (long)xcd_block() * kColCW + tx;
+  const bool ok = p < g.n2;
+  const int i = ok ? (int)(p % g.ni) + 1 : 1, j = ok ? (int)(p / g.ni) + 1 : 1;
+  if (ok) {
+    for (int k = 1 + kl; k <= nk; k += kColKL) {
+      const long q = p + g.n2 * (k - 1);
+      const double mu = (double)g.msku(i, j, k);
+      const int o = (k - 1) * kColCW + tx;
+      t1[o] = uc[q] * g.dpr2[k - 1] * mu;
+      t2[o] = vc[q] * g.dpr2[k - 1] * mu;
+    }
   }
-  const double m1 = s1 / g.prh_u_r[p];
-  const double m2 = s2 / g.prh_u_r[p];
+  __syncthreads();
+  if (ok && kl == 0) {
+    double s1 = 0, s2 = 0;
+    for (int k = 1; k <= nk; ++k) {
+      s1 += t1[(k - 1) * kColCW + tx];
+      s2 += t2[(k - 1) * kColCW + tx];
+    }
+    m1s[tx] = s1 / g.prh_u_r[p];
+    m2s[tx] = s2 / g.prh_u_r[p];
+  }
+  __syncthreads();
+  if (!ok) return;
+  const double m1 = m1s[tx], m2 = m2s[tx];
   const double bu = buc[p], bv = bvc[p];
-  for (int k = 1; k <= g.nk; ++k) {
+  for (int k = 1 + kl; k <= nk; k += kColKL) {
     if (g.msku(i, j, k) > 0) {
-      long q = p + g.n2 * (k - 1);
+      const long q = p + g.n2 * (k - 1);
       uc[q] = uc[q] - m1 + bu;
       vc[q] = vc[q] - m2 + bv;
     }
@@ -1113,8 +1158,17 @@ std::string step_pre_t(Ctx& c) {
   // den_alpha, int_pgra
   LAUNCH(k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
   LAUNCH_COL(k_adp<E>, ge, P<E>(f.alpha), P<E>(f.adp));
-  LAUNCH(k_bphi<E>, n2, ge, P<E>(f.adp), P<E>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
-         h.bgra[1], h.bgra[2], h.bgra[3]);
+  {
+    const size_t lds = 4 * (size_t)c.nk * kColCW * sizeof(E);
+    hipError_t eb = hipSuccess;
+    if (lds > 65536)
+      eb = hipFuncSetAttribute((const void*)k_bphi<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (eb != hipSuccess) return hip_err(eb, "k_bphi LDS size");
+    hipLaunchKernelGGL(k_bphi<E>, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds, c.stream,
+                       ge, P<E>(f.adp), P<E>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
+                       h.bgra[1], h.bgra[2], h.bgra[3]);
+    if ((eb = hipGetLastError()) != hipSuccess) return hip_err(eb, "k_bphi");
+  }
   // int_readyc
   LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
   LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), (R*)nullptr, (const E*)nullptr, (const double*)nullptr);
@@ -1188,7 +1242,13 @@ std::string step_post_t(Ctx& c) {
     ca.bc = (double)c.dtbc;
     LAUNCH((k_clin<R, E>), n3, ca);
   }
-  LAUNCH(k_clin_cpl, n2, c.gd, U, V, h.buc, h.bvc);
+  {
+    const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(double);
+    hipLaunchKernelGGL(k_clin_cpl, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds,
+                       c.stream, c.gd, U, V, h.buc, h.bvc);
+    hipError_t ep = hipGetLastError();
+    if (ep != hipSuccess) return hip_err(ep, "k_clin_cpl");
+  }
   LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, h.schs, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
   c.n_uv += 1;
   // int_ssh
