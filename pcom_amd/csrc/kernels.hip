@@ -511,22 +511,36 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
 
 // column integral of the baroclinic tendency (mod_int.f90:136-149)
 template <typename R>
-__global__ void k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__ avc, const R* __restrict__ fx,
-                     const R* __restrict__ fy, const double* __restrict__ schr, const double* __restrict__ pax,
-                     const double* __restrict__ pay, double* ut, double* vt) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2) return;
-  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
-  const R sch = sch_ready<R>(g, schr, i, j);
-  const R px = (R)pax[p], py = (R)pay[p];
+__global__ void __launch_bounds__(kColCW * kColKL)
+    k_ut(Grid<R> g, const R* __restrict__ auc, const R* __restrict__ avc, const R* __restrict__ fx,
+         const R* __restrict__ fy, const double* __restrict__ schr, const double* __restrict__ pax,
+         const double* __restrict__ pay, double* ut, double* vt) {
+  extern __shared__ unsigned char ut_lds[];
+  const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  R* tx1 = reinterpret_cast<R*>(ut_lds);  // [nk][kColCW]
+  R* ty1 = tx1 + nk * kColCW;
+  const long p = (long)xcd_block() * kColCW + tx;
+  const bool ok = p < g.n2;
+  const int i = ok ? (int)(p % g.ni) + 1 : 1, j = ok ? (int)(p / g.ni) + 1 : 1;
+  if (ok) {
+    const R sch = sch_ready<R>(g, schr, i, j);
+    const R px = (R)pax[p], py = (R)pay[p];
+    for (int k = 1 + kl; k <= nk; k += kColKL) {
+      long q = p + g.n2 * (k - 1);
+      R x = -auc[q] - sch * (R)A0 * px + fx[q];
+      R y = -avc[q] - sch * (R)A0 * py + fy[q];
+      R mu = (R)g.msku(i, j, k);
+      const int o = (k - 1) * kColCW + tx;
+      tx1[o] = x * g.dpr2[k - 1] * mu;
+      ty1[o] = y * g.dpr2[k - 1] * mu;
+    }
+  }
+  __syncthreads();
+  if (!ok || kl != 0) return;
   R su = 0, sv = 0;
-  for (int k = 1; k <= g.nk; ++k) {
-    long q = p + g.n2 * (k - 1);
-    R x = -auc[q] - sch * (R)A0 * px + fx[q];
-    R y = -avc[q] - sch * (R)A0 * py + fy[q];
-    R mu = (R)g.msku(i, j, k);
-    su = su + x * g.dpr2[k - 1] * mu;
-    sv = sv + y * g.dpr2[k - 1] * mu;
+  for (int k = 1; k <= nk; ++k) {
+    su = su + tx1[(k - 1) * kColCW + tx];
+    sv = sv + ty1[(k - 1) * kColCW + tx];
   }
   ut[p] = (double)su / g.prh_u[p];
   vt[p] = (double)sv / g.prh_u[p];
@@ -649,35 +663,68 @@ __global__ void __launch_bounds__(kColCW * kColKL)
 }
 
 // unweighted velocity, its accumulator and smag (mod_int.f90:384-440)
+// The unweighted velocity u*sqrt(mch2)/mch2 is staged in LDS for a
+// kSmagTX x kSmagTY tile of canonical columns at one level plus a one-point
+// rim, so each point's five-point Smagorinsky stencil reads it instead of
+// recomputing it (one division per value instead of five).  Halo columns
+// take their accumulator term from their own values and am from their
+// canonical column, as the per-point composition does.
+constexpr int kSmagTX = 64, kSmagTY = 4;
+
 template <typename R>
-__global__ void k_smag(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
-                       const double* __restrict__ mch2, const double* __restrict__ schs, R* acu, R* acv, R* am,
-                       int dtbc) {
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2 * g.nk) return;
-  int i, j, k;
-  ijk(g, p, i, j, k);
+__global__ void __launch_bounds__(kSmagTX * kSmagTY)
+    k_smag(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
+           const double* __restrict__ mch2, const double* __restrict__ schs, R* acu, R* acv, R* am, int dtbc) {
+  const int G = g.ni - 2;
+  const int ntx = (G + kSmagTX - 1) / kSmagTX, nty = (g.nj + kSmagTY - 1) / kSmagTY;
+  const int b = (int)xcd_block();
+  const int k = b / (ntx * nty) + 1;
+  const int r = b % (ntx * nty);
+  const int c0 = 2 + (r % ntx) * kSmagTX, j0 = 1 + (r / ntx) * kSmagTY;
+  const int tx = threadIdx.x % kSmagTX, ty = threadIdx.x / kSmagTX;
+  const int ic = c0 + tx, j = j0 + ty;
   const long ko = g.n2 * (k - 1);
   auto S2 = [&](int m, int jj) -> R { return (R)schs[g.ix(m, jj)]; };
   auto U1 = [&](int m, int jj) -> R { return (R)uc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
   auto U2 = [&](int m, int jj) -> R { return (R)vc[ko + g.ix(m, jj)] * S2(m, jj) / (R)mch2[g.ix(m, jj)]; };
-  acu[p] = acu[p] + U1(i, j);
-  acv[p] = acv[p] + U2(i, j);
-  if (j >= 2 && j <= g.nj - 1) {
-    int ic = g.canon(i);
+  // [row][col]: raw columns c0-1 .. c0+kSmagTX, rows j0-1 .. j0+kSmagTY
+  __shared__ R su1[kSmagTY + 2][kSmagTX + 2], su2[kSmagTY + 2][kSmagTX + 2];
+  for (int t = threadIdx.x; t < (kSmagTY + 2) * (kSmagTX + 2); t += blockDim.x) {
+    const int rr = t / (kSmagTX + 2), cc = t % (kSmagTX + 2);
+    const int m = c0 - 1 + cc, jj = j0 - 1 + rr;
+    if (m <= g.ni && jj >= 1 && jj <= g.nj) {
+      su1[rr][cc] = U1(m, jj);
+      su2[rr][cc] = U2(m, jj);
+    }
+  }
+  __syncthreads();
+  if (ic > g.nim || j > g.nj) return;
+  const int rr = ty + 1, cc = tx + 1;
+  R amv = 0;
+  const bool rows = j >= 2 && j <= g.nj - 1;
+  if (rows) {
     R dx = (R)ARAD * (g.RH(4, ic, j) * g.DX1(4, ic, j) + g.RH(4, ic + 1, j) * g.DX1(4, ic + 1, j));
     R dy = (R)ARAD * (g.DX2(2, ic, j) + g.DX2(2, ic, j + 1));
     R tn = g.tn[3][g.ix(ic, j)];
-    R dt = (R)4 * (U1(ic + 1, j) - U1(ic - 1, j)) / dx - (R)4 * (U2(ic, j + 1) - U2(ic, j - 1)) / dy +
-           U2(ic, j) * tn / (R)ARAD;
-    R ds = (R)4 * (U2(ic + 1, j) - U2(ic - 1, j)) / dx + (R)4 * (U1(ic, j + 1) - U1(ic, j - 1)) / dy -
-           U1(ic, j) * tn / (R)ARAD;
-    R cc = (R)0.5 * (R)SMAG_C / (R)PI_F * ((R)0.5 * dx + (R)0.5 * dy);
-    R c = cc * cc;
+    R dt = (R)4 * (su1[rr][cc + 1] - su1[rr][cc - 1]) / dx - (R)4 * (su2[rr + 1][cc] - su2[rr - 1][cc]) / dy +
+           su2[rr][cc] * tn / (R)ARAD;
+    R ds = (R)4 * (su2[rr][cc + 1] - su2[rr][cc - 1]) / dx + (R)4 * (su1[rr + 1][cc] - su1[rr - 1][cc]) / dy -
+           su1[rr][cc] * tn / (R)ARAD;
+    R cs = (R)0.5 * (R)SMAG_C / (R)PI_F * ((R)0.5 * dx + (R)0.5 * dy);
+    R c = cs * cs;
     R wk = c * sqrt(dt * dt + ds * ds);
     R amax = (dx * dx + dy * dy) / (R)(4 * 8 * dtbc);
-    am[p] = (amax < wk) ? amax : wk;
+    amv = (amax < wk) ? amax : wk;
   }
+  auto cell = [&](int i, bool own) {
+    const long p = ko + g.ix(i, j);
+    acu[p] = acu[p] + (own ? su1[rr][cc] : U1(i, j));
+    acv[p] = acv[p] + (own ? su2[rr][cc] : U2(i, j));
+    if (rows) am[p] = amv;
+  };
+  cell(ic, true);
+  if (ic == g.nim) cell(1, false);
+  if (ic == 2) cell(g.ni, false);
 }
 
 // int_ssh (mod_int.f90:602-624): alpha from (tp, sp, mch), column SSH
@@ -1207,7 +1254,13 @@ std::string step_pre_t(Ctx& c) {
     hipError_t ef = hipGetLastError();
     if (ef != hipSuccess) return hip_err(ef, "k_fri_tiles");
   }
-  LAUNCH(k_ut<R>, n2, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
+  {
+    const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(R);
+    hipLaunchKernelGGL(k_ut<R>, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds,
+                       c.stream, g, au, av, P<R>(f.fx), P<R>(f.fy), h.schr, h.pax, h.pay, h.but, h.bvt);
+    hipError_t eu = hipGetLastError();
+    if (eu != hipSuccess) return hip_err(eu, "k_ut");
+  }
   return "";
 }
 
@@ -1249,7 +1302,14 @@ std::string step_post_t(Ctx& c) {
     hipError_t ep = hipGetLastError();
     if (ep != hipSuccess) return hip_err(ep, "k_clin_cpl");
   }
-  LAUNCH(k_smag<R>, n3, g, U, V, h.mch2, h.schs, P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
+  {
+    const int G = c.ni - 2;
+    const unsigned nb = (unsigned)(((G + kSmagTX - 1) / kSmagTX) * ((c.nj + kSmagTY - 1) / kSmagTY) * c.nk);
+    hipLaunchKernelGGL(k_smag<R>, dim3(nb), dim3(kSmagTX * kSmagTY), 0, c.stream, g, U, V, h.mch2, h.schs,
+                       P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
+    hipError_t es = hipGetLastError();
+    if (es != hipSuccess) return hip_err(es, "k_smag");
+  }
   c.n_uv += 1;
   // int_ssh
   LAUNCH_COL(k_ssh<E>, ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
