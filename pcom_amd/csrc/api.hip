@@ -280,8 +280,12 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   *out = reinterpret_cast<pcom_ctx*>(c);
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return fail(PCOM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess)
     return fail(PCOM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  for (hipEvent_t* ev : {&c->fork[0], &c->fork[1], &c->join[0], &c->join[1]})
+    if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
+      return fail(PCOM_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
 
   const bool fl = c->precision != PCOM_FP64;             // R = float
   const bool el = c->precision == PCOM_MIXED_EOS32;      // E = float
@@ -427,6 +431,9 @@ int pcom_destroy(pcom_ctx* ctx) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {c->fork[0], c->fork[1], c->join[0], c->join[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PCOM_OK;

@@ -46,6 +46,11 @@ struct Fields2 {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // a second stream for the kernels of a step that can run beside others
+  // (den_alpha + int_pgra beside int_readyc, int_ssh beside the mode
+  // coupling and smag), forked from and joined back into `stream` by events
+  hipStream_t side = nullptr;
+  hipEvent_t fork[2] = {nullptr, nullptr}, join[2] = {nullptr, nullptr};
   int precision = PCOM_FP64;
   int rst = 0;
   int dtbt = 60, dtbc = 3600;

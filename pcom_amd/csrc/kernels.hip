@@ -1171,6 +1171,31 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
+#define LAUNCH_ON(st, kern, n, ...)                                                 \
+  do {                                                                              \
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, st, __VA_ARGS__);   \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
+  } while (0)
+
+#define LAUNCH_COL_ON(st, kern, ...)                                                \
+  do {                                                                              \
+    hipLaunchKernelGGL(kern, dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), 0, st, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
+  } while (0)
+
+// work on c.side after everything queued so far on c.stream (fork), and
+// c.stream's later work after c.side's (join)
+inline hipError_t fork_side(Ctx& c, int w) {
+  hipError_t e = hipEventRecord(c.fork[w], c.stream);
+  return e != hipSuccess ? e : hipStreamWaitEvent(c.side, c.fork[w], 0);
+}
+inline hipError_t join_side(Ctx& c, int w) {
+  hipError_t e = hipEventRecord(c.join[w], c.side);
+  return e != hipSuccess ? e : hipStreamWaitEvent(c.stream, c.join[w], 0);
+}
+
 // main.f90:164-211: daily forcing, den_alpha, int_pgra, int_readyc (the
 // step up to the barotropic subcycle)
 template <typename R, typename E>
@@ -1202,16 +1227,18 @@ std::string step_pre_t(Ctx& c) {
     LAUNCH(k_pa, n2, c.gd, h.bnd[4], h.pax, h.pay);
     LAUNCH(k_prho<E>, n3, ge, T, S, h.chc, P<E>(f.prho1), P<E>(f.prho2));
   }
-  // den_alpha, int_pgra
-  LAUNCH(k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
-  LAUNCH_COL(k_adp<E>, ge, P<E>(f.alpha), P<E>(f.adp));
+  // den_alpha, int_pgra on the side stream: they read T, S, chc and write
+  // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes
+  if ((e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
+  LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
+  LAUNCH_COL_ON(c.side, k_adp<E>, ge, P<E>(f.alpha), P<E>(f.adp));
   {
     const size_t lds = 4 * (size_t)c.nk * kColCW * sizeof(E);
     hipError_t eb = hipSuccess;
     if (lds > 65536)
       eb = hipFuncSetAttribute((const void*)k_bphi<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (eb != hipSuccess) return hip_err(eb, "k_bphi LDS size");
-    hipLaunchKernelGGL(k_bphi<E>, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds, c.stream,
+    hipLaunchKernelGGL(k_bphi<E>, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds, c.side,
                        ge, P<E>(f.adp), P<E>(f.alpha), h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3], h.bgra[0],
                        h.bgra[1], h.bgra[2], h.bgra[3]);
     if ((eb = hipGetLastError()) != hipSuccess) return hip_err(eb, "k_bphi");
@@ -1261,6 +1288,7 @@ std::string step_pre_t(Ctx& c) {
     hipError_t eu = hipGetLastError();
     if (eu != hipSuccess) return hip_err(eu, "k_ut");
   }
+  if ((e = join_side(c, 0)) != hipSuccess) return hip_err(e, "join");
   return "";
 }
 
@@ -1295,6 +1323,13 @@ std::string step_post_t(Ctx& c) {
     ca.bc = (double)c.dtbc;
     LAUNCH((k_clin<R, E>), n3, ca);
   }
+  // int_ssh on the side stream once int_clin (which reads alpha) is done: it
+  // reads T, S, mch and writes alpha, ssh, acssh, none of which the mode
+  // coupling and smag touch
+  hipError_t e;
+  if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
+  LAUNCH_COL_ON(c.side, k_ssh<E>, ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
+  c.n_ssh += 1;
   {
     const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(double);
     hipLaunchKernelGGL(k_clin_cpl, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds,
@@ -1311,9 +1346,7 @@ std::string step_post_t(Ctx& c) {
     if (es != hipSuccess) return hip_err(es, "k_smag");
   }
   c.n_uv += 1;
-  // int_ssh
-  LAUNCH_COL(k_ssh<E>, ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
-  c.n_ssh += 1;
+  if ((e = join_side(c, 1)) != hipSuccess) return hip_err(e, "join");
   // int_ts
   LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), P<R>(f.acw), (const E*)P<E>(f.alpha),
              (const double*)h.mch2);
