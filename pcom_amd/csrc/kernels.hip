@@ -11,18 +11,21 @@
 // PCOM_MIXED_EOS32: R = E = float.  The barotropic subcycle is FP64 always.
 //
 // Thread mapping: point kernels use one thread per (i, j, k) with i fastest
-// (coalesced along the Fortran-contiguous dimension); column kernels use
-// one thread per (i, j) and walk k in the reference's summation order so
-// that every column sum is accumulated exactly as Fortran's sum() does.
-#include <hip/hip_cooperative_groups.h>
+// (coalesced along the Fortran-contiguous dimension).  The stencil kernels
+// whose operators are divergences of face fluxes (int_ts, op_fri, op_adv,
+// smag) run on 64 x 4 tiles of canonical columns per level and stage each
+// face value in LDS once, instead of once per cell beside it.  Column
+// kernels give each column several k-lanes that evaluate the per-level
+// terms in parallel; one lane then walks k in the reference's summation
+// order, so every column sum is accumulated exactly as Fortran's sum()
+// does.  A second stream runs den_alpha/int_pgra beside int_readyc and
+// int_ssh beside the mode coupling (step_pre_t, step_post_t).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <string>
 
 #include "context.h"
-
-namespace cg = cooperative_groups;
 
 namespace pcom {
 
