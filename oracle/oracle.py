@@ -48,6 +48,11 @@ def lib():
         L.orc_op_px2.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_op_div.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.orc_op_lap.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+        L.orc_op_adv.argtypes = [vp, dp, dp, dp]
+        L.orc_op_adv_ts.argtypes = [vp, dp, dp, dp]
+        L.orc_op_dif.argtypes = [vp, dp, dp, dp]
+        L.orc_op_fri.argtypes = [vp, dp]
+        L.orc_upwelling.argtypes = [vp]
         L.orc_metric.argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
         L.orc_metric.restype = C.c_int
         L.orc_vgrid.argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
@@ -217,6 +222,47 @@ class Oracle:
         out = self._out(1)
         lib().orc_op_lap(self.h, ga, gb, _dp(var), _dp(out))
         return out
+
+    # composite operators on the context's fields (set them with set())
+    def op_adv(self, var, sch):
+        """op_adv (mod_op.f90:224-278) of var on U points with the
+        context's uc, vc, wm, chc."""
+        var = np.asfortranarray(var, np.float64)
+        sch = np.asfortranarray(sch, np.float64)
+        out = self._out(self.nk)
+        lib().orc_op_adv(self.h, _dp(var), _dp(sch), _dp(out))
+        return out
+
+    def op_adv_ts(self, var, wm):
+        """op_adv_ts (mod_op.f90:280-325) of var on T points with the
+        context's uc, vc, chc and the given wm (nk+1 levels)."""
+        var = np.asfortranarray(var, np.float64)
+        wm = np.asfortranarray(wm, np.float64)
+        out = self._out(self.nk)
+        lib().orc_op_adv_ts(self.h, _dp(var), _dp(wm), _dp(out))
+        return out
+
+    def op_dif(self, ch, var):
+        """op_dif (mod_op.f90:186-222) of var on T points with the
+        context's kh."""
+        var = np.asfortranarray(var, np.float64)
+        ch = np.asfortranarray(ch, np.float64)
+        out = self._out(self.nk)
+        lib().orc_op_dif(self.h, _dp(ch), _dp(var), _dp(out))
+        return out
+
+    def op_fri(self, sch):
+        """op_fri (mod_op.f90:169-184): returns (fx, fy) from the context's
+        uc, vc, am, km, wind stress and chc."""
+        sch = np.asfortranarray(sch, np.float64)
+        lib().orc_op_fri(self.h, _dp(sch))
+        return self.get("fx"), self.get("fy")
+
+    def upwelling(self):
+        """upwelling (mod_int.f90:626-647): returns wm from the context's
+        uc, vc, chp and wm(1)."""
+        lib().orc_upwelling(self.h)
+        return self.get("wm")
 
 
 _LIB_R8 = None

@@ -1618,10 +1618,17 @@ void orc_op_div(orc *o, int ga, int gb, int gc, int nd3, const double *va, const
   divg(o, out, va, vb, nd3, ga, gb, gc);
 }
 void orc_op_lap(orc *o, int ga, int gb, const double *in, double *out) { lap(o, out, in, ga, gb); }
+void orc_op_adv(orc *o, const double *var, const double *sch, double *out) { op_adv(o, out, var, sch); }
+void orc_op_adv_ts(orc *o, const double *var, const double *wm, double *out) { op_adv_ts(o, out, var, wm); }
+void orc_op_dif(orc *o, const double *ch, const double *var, double *out) { op_dif(o, out, ch, var); }
+void orc_op_fri(orc *o, const double *sch) { op_fri(o, sch); }
+void orc_upwelling(orc *o) { upwelling(o); }
 
 int orc_metric(const orc *o, int g, int which, double *out, size_t n) {
   if (g < 1 || g > 4 || n != o->n2) return -1;
-  const double *src = which == 0 ? o->rh[g] : which == 1 ? o->tn[g] : which == 2 ? o->dx1[g] : o->dx2[g];
+  const double *src = which == 0 ? o->rh[g] : which == 1 ? o->tn[g] : which == 2 ? o->dx1[g]
+                    : which == 3 ? o->dx2[g] : which == 4 ? o->hposx + n * (size_t)(g - 1)
+                                                          : o->hposy + n * (size_t)(g - 1);
   memcpy(out, src, n * sizeof(double));
   return 0;
 }

@@ -60,7 +60,18 @@ void orc_op_px2(orc *o, int ga, int gb, int nd3, const double *in, double *out);
 void orc_op_div(orc *o, int ga, int gb, int gc, int nd3,
                 const double *va, const double *vb, double *out);
 void orc_op_lap(orc *o, int ga, int gb, const double *in, double *out);
-/* Horizontal metric arrays: which = 0 rh, 1 tn, 2 dx1, 3 dx2. */
+/* Composite operators on the context's fields (for invariant tests):
+ * op_adv of var (g3, nk) with the context's uc, vc, wm, chc and the given
+ * sch (n2); op_adv_ts of var (g1, nk) with the given wm (nk+1); op_dif of
+ * var (g1, nk) with ch (n2); op_fri into the context's fx, fy; upwelling
+ * into the context's wm. */
+void orc_op_adv(orc *o, const double *var, const double *sch, double *out);
+void orc_op_adv_ts(orc *o, const double *var, const double *wm, double *out);
+void orc_op_dif(orc *o, const double *ch, const double *var, double *out);
+void orc_op_fri(orc *o, const double *sch);
+void orc_upwelling(orc *o);
+/* Horizontal metric arrays: which = 0 rh, 1 tn, 2 dx1, 3 dx2, 4 longitude,
+ * 5 latitude (radians) of stagger g. */
 int orc_metric(const orc *o, int g, int which, double *out, size_t n);
 /* Vertical grid: which = 0 z, 1 pr, 2 dz, 3 dpr of gi1 (nk+1) or gi2 (nk). */
 int orc_vgrid(const orc *o, int gi, int which, double *out, size_t n);
