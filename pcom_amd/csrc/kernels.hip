@@ -308,6 +308,14 @@ __device__ __forceinline__ R sch_ready(const Grid<R>& g, const double* schr, int
 constexpr int kAdvTX = 64, kAdvTY = 4;
 
 template <typename R>
+struct AdvCellIn {
+  R wk[2][4], ch[4];  // wm at k, k+1 and chc at T(ic, j), (ic, j+1), (ic+1, j), (ic+1, j+1)
+  double var[2][3];   // uc, vc at k-1, k, k+1 (0 outside 1..nk)
+  int lev;
+  R rh3, dx13, dx23;
+};
+
+template <typename R>
 __global__ void __launch_bounds__(kAdvTX * kAdvTY)
     k_adv_tiles(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc, const R* __restrict__ wm,
                 const double* __restrict__ chc, const double* __restrict__ schr, R* auc, R* avc) {
@@ -347,8 +355,37 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
     fvv[row][f] = vv;
     for (int c = 0; c < 2; ++c) fm2[c][row][f] = (w ? (R)0.5 * (VK(c, m, jj - 1) + VK(c, m, jj)) : (R)0) * vv;
   };
+  // the inputs of cell (i, j) besides the LDS faces (the thread's own cell
+  // loads them before the face phase, so both rounds of loads overlap)
+  const bool rows = j >= 1 && j <= g.njm;  // ter_di's and p2's rows
+  auto cell_in = [&](int i) -> AdvCellIn<R> {
+    AdvCellIn<R> c;
+    const int m0 = g.canon(i);
+    for (int o = 0; o < 2; ++o) {
+      const R* w = wm + g.n2 * (k - 1 + o);
+      for (int e = 0; e < 4; ++e) {
+        const long qq = g.ix(m0 + (e >> 1), j + (e & 1));
+        c.wk[o][e] = rows ? w[qq] : (R)0;
+        if (o == 0) c.ch[e] = rows ? (R)chc[qq] : (R)1;
+      }
+    }
+    const long q = g.ix(i, j);
+    for (int o = 0; o < 3; ++o) {
+      const int kk = k - 1 + o;
+      const bool ok = kk >= 1 && kk <= g.nk;
+      c.var[0][o] = ok ? uc[q + g.n2 * (kk - 1)] : 0.0;
+      c.var[1][o] = ok ? vc[q + g.n2 * (kk - 1)] : 0.0;
+    }
+    c.lev = g.lev_u[q];
+    c.rh3 = g.RH(3, i, j);
+    c.dx13 = g.DX1(3, ic, j);
+    c.dx23 = g.DX2(3, ic, j);
+    return c;
+  };
   const bool in = ic <= g.nim && j <= g.nj;
+  AdvCellIn<R> own{};
   if (in) {
+    own = cell_in(ic);
     face_ew(ic, j, ty, tx);
     face_ns(ic, j, ty, tx);
     if (tx == kAdvTX - 1 || ic == g.nim) face_ew(g.canon(ic + 1), j, ty, tx + 1);
@@ -357,46 +394,46 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
   __syncthreads();
   if (!in) return;
 
-  auto cell = [&](int i) {
+  auto cell = [&](int i, const AdvCellIn<R>& in) {
     const long p = ko + g.ix(i, j);
-    // w = ter(wm/chc, g1 -> g3) on level kk of nkp
-    auto W = [&](int kk) -> R {
-      const R* w = wm + g.n2 * (kk - 1);
-      auto WW = [&](int m, int jj) -> R { return w[g.ix(m, jj)] / (R)chc[g.ix(m, jj)]; };
-      return ter_di<1>(g, WW, i, j, (R)0);
+    // w = ter(wm/chc, g1 -> g3) on level kk of nkp (ter_di: rows 1..njm)
+    auto W = [&](int o) -> R {
+      if (!rows) return (R)0;
+      return (in.wk[o][0] / in.ch[0] + in.wk[o][1] / in.ch[1] + in.wk[o][2] / in.ch[2] + in.wk[o][3] / in.ch[3]) *
+             (R)0.25;
     };
-    const R w_k = W(k), w_k1 = W(k + 1);
-    const R rh3 = g.RH(3, i, j);
+    const R w_k = W(0), w_k1 = W(1);
+    const R rh3 = in.rh3;
     const R dpr = g.dpr2[k - 1];
-    const bool rows = j >= 1 && j <= g.njm;  // p2's rows (g2 -> g3)
-    const R d1 = (fuu[ty][tx + 1] - fuu[ty][tx]) / g.DX1(3, ic, j);
-    const R d2 = (rows ? fvv[ty + 1][tx] - fvv[ty][tx] : (R)0) / g.DX2(3, ic, j);
+    const R d1 = (fuu[ty][tx + 1] - fuu[ty][tx]) / in.dx13;
+    const R d2 = (rows ? fvv[ty + 1][tx] - fvv[ty][tx] : (R)0) / in.dx23;
     const R d3 = (w_k - w_k1) / dpr;
-    const R mu = (R)g.msku(i, j, k);
+    const int lev = in.lev;
+    const R mu = (R)(lev >= k ? 1 : 0);
     R out[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const double* var = c == 0 ? uc : vc;
-      const R p1m = (fm1[c][ty][tx + 1] - fm1[c][ty][tx]) / g.DX1(3, ic, j);
-      const R p2m = (rows ? fm2[c][ty + 1][tx] - fm2[c][ty][tx] : (R)0) / g.DX2(3, ic, j);
+      const R p1m = (fm1[c][ty][tx + 1] - fm1[c][ty][tx]) / in.dx13;
+      const R p2m = (rows ? fm2[c][ty + 1][tx] - fm2[c][ty][tx] : (R)0) / in.dx23;
       R ans = (p1m + p2m) / ((R)ARAD * rh3);
       // vertical flux through the U-cell faces, zero unless both layers wet
       auto WB = [&](int kk) -> R {
         if (kk < 2 || kk > g.nk) return (R)0;
-        R a = ((R)var[g.ix(i, j, kk - 1)] + (R)var[g.ix(i, j, kk)]) * (R)0.5;
+        const int o = kk - k;
+        R a = ((R)in.var[c][o] + (R)in.var[c][o + 1]) * (R)0.5;
         a = a * (kk == k ? w_k : w_k1);
-        return a * (R)g.msku(i, j, kk - 1) * (R)g.msku(i, j, kk);
+        return a * (R)(lev >= kk - 1 ? 1 : 0) * (R)(lev >= kk ? 1 : 0);
       };
       ans = ans + (WB(k) - WB(k + 1)) / dpr;
-      ans = ans - (R)0.5 * (R)var[p] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
+      ans = ans - (R)0.5 * (R)in.var[c][1] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
       out[c] = ans * mu;
     }
     auc[p] = out[0];
     avc[p] = out[1];
   };
-  cell(ic);
-  if (ic == g.nim) cell(1);
-  if (ic == 2) cell(g.ni);
+  cell(ic, own);
+  if (ic == g.nim) cell(1, cell_in(1));
+  if (ic == 2) cell(g.ni, cell_in(g.ni));
 }
 
 // op_fri for both components (mod_op.f90:103-184) with the horizontal
@@ -410,6 +447,20 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
 // same operations in the same order.  Halo columns take their faces from their
 // canonical columns and everything else from themselves.
 constexpr int kFriTX = 64, kFriTY = 4;
+
+template <typename R>
+struct FriCellIn {
+  R sch, a;          // schr and am at the point
+  int lev;           // lev_u
+  double tau[2];     // taux, tauy
+  R cv1, cv2;
+  double vp[2][3];   // uc, vc at k-1, k, k+1 (0 outside 1..nk)
+  R km0, km1;        // km at k, k+1
+  double ue, uw, ve, vw;  // uc, vc at the east and west neighbours of level k
+  R sche, schw;
+  R dx13, dx23, rh3, rh4, d14, d14e;
+  double ub, vb;     // uc, vc at the bottom level (when the drag enters)
+};
 
 template <typename R>
 __global__ void __launch_bounds__(kFriTX * kFriTY)
@@ -450,8 +501,49 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
       fb[c][row][f] = chx * gy * g.RH(2, m, jj);
     }
   };
+  // the inputs of cell (i, j) besides the LDS faces (the thread's own cell
+  // loads them before the face phase, so both rounds of loads overlap)
+  auto cell_in = [&](int i) -> FriCellIn<R> {
+    FriCellIn<R> c;
+    const long q = g.ix(i, j);
+    c.sch = (R)schr[q];
+    c.a = am[ko + q];
+    c.lev = g.lev_u[q];
+    c.tau[0] = taux[q];
+    c.tau[1] = tauy[q];
+    c.cv1 = g.cv1[q];
+    c.cv2 = g.cv2[q];
+    for (int o = 0; o < 3; ++o) {
+      const int kk = k - 1 + o;
+      const bool ok = kk >= 1 && kk <= g.nk;
+      c.vp[0][o] = ok ? uc[q + g.n2 * (kk - 1)] : 0.0;
+      c.vp[1][o] = ok ? vc[q + g.n2 * (kk - 1)] : 0.0;
+    }
+    c.km0 = g.km[g.ix(i, j, k)];
+    c.km1 = g.km[g.ix(i, j, k + 1)];
+    const long qe = g.ix(ic + 1, j), qw = g.ix(ic - 1, j);
+    c.ue = uc[ko + qe];
+    c.uw = uc[ko + qw];
+    c.ve = vc[ko + qe];
+    c.vw = vc[ko + qw];
+    c.sche = (R)schr[qe];
+    c.schw = (R)schr[qw];
+    c.dx13 = g.DX1(3, ic, j);
+    c.dx23 = g.DX2(3, ic, j);
+    c.rh3 = g.RH(3, ic, j);
+    c.rh4 = g.RH(4, ic, j);
+    c.d14 = g.DX1(4, ic, j);
+    c.d14e = g.DX1(4, ic + 1, j);
+    // the bottom velocity, for the drag in the layer above the bottom face
+    const bool drag = c.lev > 0 && (c.lev == k - 1 || c.lev == k);
+    c.ub = drag ? uc[g.ix(i, j, c.lev)] : 0.0;
+    c.vb = drag ? vc[g.ix(i, j, c.lev)] : 0.0;
+    return c;
+  };
   const bool in = ic <= g.nim && j <= g.nj;
+  FriCellIn<R> own{};
   if (in) {
+    own = cell_in(ic);
     face_ew(ic, j, ty, tx);
     face_ns(ic, j, ty, tx);
     if (tx == kFriTX - 1 || ic == g.nim) face_ew(g.canon(ic + 1), j, ty, tx + 1);
@@ -460,56 +552,54 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
   __syncthreads();
   if (!in) return;
 
-  auto cell = [&](int i) {
+  auto cell = [&](int i, const FriCellIn<R>& in) {
     const long p = ko + g.ix(i, j);
-    const long q = g.ix(i, j);
-    const R sch = (R)schr[q];
-    const R a = am[p];
-    const int lev = g.lev_u[q];
+    const R sch = in.sch;
+    const R a = in.a;
+    const int lev = in.lev;
     R out[2];
     for (int c = 0; c < 2; ++c) {
       // fx from (u, uc, -v, taux), fy from (v, vc, u, tauy) (mod_op.f90:169-184)
-      const double* vp = c ? vc : uc;
-      const R tau = (R)(c ? tauy : taux)[q];
-      auto v3 = [&](int m, int jj, int kk) -> R {
-        return (R)(c ? vc : uc)[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)];
-      };
-      auto vcc = [&](int m, int jj, int kk) -> R {
-        // -v for fx, u for fy
-        return c ? (R)uc[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)]
-                 : -((R)vc[g.ix(m, jj, kk)] / (R)schr[g.ix(m, jj)]);
-      };
+      const R tau = (R)in.tau[c];
+      // u or v / sch at the bottom level of this column
+      auto v3b = [&]() -> R { return (R)(c ? in.vb : in.ub) / sch; };
+      // -v for fx, u for fy, at the bottom level and at the east and west
+      // neighbours of level k
+      auto vccb = [&]() -> R { return c ? (R)in.ub / sch : -((R)in.vb / sch); };
+      auto vcce = [&]() -> R { return c ? (R)in.ue / in.sche : -((R)in.ve / in.sche); };
+      auto vccw = [&]() -> R { return c ? (R)in.uw / in.schw : -((R)in.vw / in.schw); };
       // divg<4,2,3>(VA, VB) (mod_op.f90:813-848)
-      const R a1 = (fa[c][ty][tx + 1] - fa[c][ty][tx]) / g.DX1(3, ic, j);
+      const R a1 = (fa[c][ty][tx + 1] - fa[c][ty][tx]) / in.dx13;
       const R d = (j < 1 || j > g.njm) ? (R)0 : (fb[c][ty + 1][tx] - fb[c][ty][tx]);
-      const R b1 = d / g.DX2(3, ic, j);
-      const R wka = (a1 + b1) / ((R)ARAD * g.RH(3, ic, j));
+      const R b1 = d / in.dx23;
+      const R wka = (a1 + b1) / ((R)ARAD * in.rh3);
       R ans = a / sch * wka;
-      ans = ans + a * g.cv1[q] * (R)vp[g.ix(i, j, k)];
-      const R dx = (R)ARAD * g.RH(4, ic, j) * (g.DX1(4, ic, j) + g.DX1(4, ic + 1, j));
-      const R pxc = (vcc(ic + 1, j, k) - vcc(ic - 1, j, k)) / dx;
-      ans = ans + a * g.cv2[q] * sch * pxc;
+      ans = ans + a * in.cv1 * (R)in.vp[c][1];
+      const R dx = (R)ARAD * in.rh4 * (in.d14 + in.d14e);
+      const R pxc = (vcce() - vccw()) / dx;
+      ans = ans + a * in.cv2 * sch * pxc;
       auto WKP = [&](int kk) -> R {
         if (kk == 1) return (R)GRAV * tau / sch;
         if (lev > 0 && kk == lev + 1) {
-          R vv = v3(i, j, lev), cc = vcc(i, j, lev);
+          R vv = v3b(), cc = vccb();
           R mag = sqrt(vv * vv + cc * cc);
           return vv * (R)GRAV * (R)CDBOT * mag / sch;
         }
         if (kk > g.nk) return (R)0;
-        R pz = ((R)vp[g.ix(i, j, kk - 1)] - (R)vp[g.ix(i, j, kk)]) * (R)g.msku(i, j, kk - 1) *
-               (R)g.msku(i, j, kk) / g.dz1[kk - 1];
-        return (R)GRAV * (R)RHO0 * g.km[g.ix(i, j, kk)] * pz / (sch * sch);
+        const int o = kk - k;
+        R pz = ((R)in.vp[c][o] - (R)in.vp[c][o + 1]) * (R)(lev >= kk - 1 ? 1 : 0) * (R)(lev >= kk ? 1 : 0) /
+               g.dz1[kk - 1];
+        return (R)GRAV * (R)RHO0 * (kk == k ? in.km0 : in.km1) * pz / (sch * sch);
       };
       const R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
-      out[c] = ans * (R)g.msku(i, j, k) + vert;
+      out[c] = ans * (R)(lev >= k ? 1 : 0) + vert;
     }
     fx[p] = out[0];
     fy[p] = out[1];
   };
-  cell(ic);
-  if (ic == g.nim) cell(1);
-  if (ic == 2) cell(g.ni);
+  cell(ic, own);
+  if (ic == g.nim) cell(1, cell_in(1));
+  if (ic == 2) cell(g.ni, cell_in(g.ni));
 }
 
 // column integral of the baroclinic tendency (mod_int.f90:136-149)
@@ -795,6 +885,15 @@ struct TsArgs {
 constexpr int kTsTX = 64, kTsTY = 4;
 
 template <typename R>
+struct TsCellIn {
+  double var[2][3];  // T, S at k-1, k, k+1
+  R wm0, wm1, kh0, kh1;
+  int lev;
+  R rh1, dx1c, dx2c, rh1c;
+  double mch, bt, bs;
+};
+
+template <typename R>
 __global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
   const Grid<R>& g = a.g;
   const int G = g.ni - 2;
@@ -841,8 +940,36 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
       fb[v][row][col] = chy * (d / (R)ARAD) * g.RH(4, m, jj);
     }
   };
+  // the column inputs of cell (i, j): T/S at k-1, k, k+1 (0 outside 1..nk),
+  // wm and kh at k and k+1, the mask depth, metrics and forcing
+  auto cell_in = [&](int i) -> TsCellIn<R> {
+    TsCellIn<R> c;
+    const long q = g.ix(i, j);
+    for (int v = 0; v < 2; ++v) {
+      const double* varp = v ? a.s0 : a.t0;
+      for (int o = 0; o < 3; ++o) {
+        const int kk = k - 1 + o;
+        c.var[v][o] = (kk >= 1 && kk <= g.nk) ? varp[q + g.n2 * (kk - 1)] : 0.0;
+      }
+    }
+    c.wm0 = a.wm[g.ix(i, j, k)];
+    c.wm1 = a.wm[g.ix(i, j, k + 1)];
+    c.kh0 = g.kh[g.ix(i, j, k)];
+    c.kh1 = g.kh[g.ix(i, j, k + 1)];
+    c.lev = g.lev_t[q];
+    c.rh1 = g.RH(1, i, j);
+    c.dx1c = g.DX1(1, ic, j);
+    c.dx2c = g.DX2(1, ic, j);
+    c.rh1c = g.RH(1, ic, j);
+    c.mch = a.mch[q];
+    c.bt = k == 1 ? a.bndt[q] : 0.0;
+    c.bs = k == 1 ? a.bnds[q] : 0.0;
+    return c;
+  };
   const bool in = ic <= g.nim && j <= g.nj;
+  TsCellIn<R> own{};
   if (in) {
+    own = cell_in(ic);
     face_ew(ic, j, ty, tx + 1);
     face_ns(ic, j, ty + 1, tx);
     if (tx == 0) face_ew(c0 - 1 == 1 ? g.nim : c0 - 1, j, ty, 0);
@@ -851,58 +978,62 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
   __syncthreads();
   if (!in) return;
 
-  // one cell (i, j) from the faces of its canonical column ic
-  auto cell = [&](int i) {
+  // one cell (i, j) from the faces of its canonical column ic and the
+  // column inputs `in` (loaded by cell_in, for the thread's own cell before
+  // the face phase so that both rounds of loads are in flight together)
+  auto cell = [&](int i, const TsCellIn<R>& in) {
     const long p = ko + g.ix(i, j);
-    const long q = g.ix(i, j);
     R pt[2];
     for (int v = 0; v < 2; ++v) {
-      const double* varp = v ? a.s0 : a.t0;
       // op_adv_ts (mod_op.f90:280-325)
-      const R rh1 = g.RH(1, i, j);
+      const R rh1 = in.rh1;
       R ans = (R)0.5 * (fx[v][ty][tx] + fx[v][ty][tx + 1]) / ((R)ARAD * rh1);                      // ter_ew<2>
       ans = ans + ((j < 2 || j > g.nj) ? (R)0 : (R)0.5 * (fy[v][ty][tx] + fy[v][ty + 1][tx])) /    // ter_ns<4>
                       ((R)ARAD * rh1);
+      // var at kk - 1 minus var at kk (dx3_r3d_b), kk = k or k + 1
       auto DX3 = [&](int kk) -> R {
         if (kk < 2 || kk > g.nk) return (R)0;
-        return (R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)];
+        const int o = kk - k;
+        return (R)in.var[v][o] - (R)in.var[v][o + 1];
       };
-      R wka = (a.wm[g.ix(i, j, k)] * DX3(k) + a.wm[g.ix(i, j, k + 1)] * DX3(k + 1)) * (R)0.5;
+      R wka = (in.wm0 * DX3(k) + in.wm1 * DX3(k + 1)) * (R)0.5;
       wka = wka / g.dpr2[k - 1];
       pt[v] = -(ans + wka);
     }
     for (int v = 0; v < 2; ++v) {
-      const double* varp = v ? a.s0 : a.t0;
       // op_dif (mod_op.f90:186-222): divg<2,4,1> of (VA, VB*rh)
-      const R a1 = (fa[v][ty][tx + 1] - fa[v][ty][tx]) / g.DX1(1, ic, j);
+      const R a1 = (fa[v][ty][tx + 1] - fa[v][ty][tx]) / in.dx1c;
       const R d = (j < 2 || j > g.nj) ? (R)0 : (fb[v][ty + 1][tx] - fb[v][ty][tx]);
-      const R b1 = d / g.DX2(1, ic, j);
-      R ans = (R)AH_C * ((a1 + b1) / ((R)ARAD * g.RH(1, ic, j)));
+      const R b1 = d / in.dx2c;
+      R ans = (R)AH_C * ((a1 + b1) / ((R)ARAD * in.rh1c));
       auto WKP = [&](int kk) -> R {
         R w = (R)0;
-        if (kk >= 2 && kk <= g.nk)
-          w = ((R)varp[g.ix(i, j, kk - 1)] - (R)varp[g.ix(i, j, kk)]) * (R)g.mskt(i, j, kk - 1) *
-              (R)g.mskt(i, j, kk) / g.dz1[kk - 1];
-        return (R)GRAV * (R)RHO0 * g.kh[g.ix(i, j, kk)] * w;
+        if (kk >= 2 && kk <= g.nk) {
+          const int o = kk - k;
+          w = ((R)in.var[v][o] - (R)in.var[v][o + 1]) * (R)(in.lev >= kk - 1 ? 1 : 0) *
+              (R)(in.lev >= kk ? 1 : 0) / g.dz1[kk - 1];
+        }
+        return (R)GRAV * (R)RHO0 * (kk == k ? in.kh0 : in.kh1) * w;
       };
       pt[v] = pt[v] + (ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1]);
     }
-    const R mk = (R)g.mskt(i, j, k);
-    const R m = (R)a.mch[q];
+    const R mk = (R)(in.lev >= k ? 1 : 0);
+    const R m = (R)in.mch;
+    const double t0 = in.var[0][1], s0 = in.var[1][1];
     if (k == 1) {
-      pt[0] = pt[0] + (R)GAMMA_T * ((R)a.bndt[q] - (R)a.t0[p]) * mk / g.dpr2[0];
-      pt[1] = pt[1] + (R)GAMMA_S * ((R)a.bnds[q] - (R)a.s0[p]) * mk * m;
+      pt[0] = pt[0] + (R)GAMMA_T * ((R)in.bt - (R)t0) * mk / g.dpr2[0];
+      pt[1] = pt[1] + (R)GAMMA_S * ((R)in.bs - (R)s0) * mk * m;
     }
     // the forward step of the FP64 state (mod_int.f90:484-487)
-    double tn = a.t0[p] + (double)pt[0] * a.bc * (double)mk / (double)m;
-    double sn = a.s0[p] + (double)pt[1] * a.bc * (double)mk / (double)m;
-    if (g.mskt(i, j, k) > 0 && tn < TICE) tn = TICE;
+    double tn = t0 + (double)pt[0] * a.bc * (double)mk / (double)m;
+    double sn = s0 + (double)pt[1] * a.bc * (double)mk / (double)m;
+    if (in.lev >= k && tn < TICE) tn = TICE;
     a.t1[p] = tn;
     a.s1[p] = sn;
   };
-  cell(ic);
-  if (ic == g.nim) cell(1);
-  if (ic == 2) cell(g.ni);
+  if (in) cell(ic, own);
+  if (in && ic == g.nim) cell(1, cell_in(1));
+  if (in && ic == 2) cell(g.ni, cell_in(g.ni));
 }
 
 // convect / adjust / mix (mod_int.f90:509-600) on one column, in place.
