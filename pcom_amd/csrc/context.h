@@ -71,6 +71,7 @@ struct Ctx {
   bool trop_launches = false;                 // mode in use: one launch per exchange
   int trop_tiles_x = 0, trop_blocks = 0;
   size_t trop_lds = 0;
+  int trop_block = 0;  // subcycle workgroup size of the tiling
   unsigned long long* trop_flags = nullptr;  // per-tile exchange flags (zeroed at create)
   long trop_flag_cap = 0;                    // tiles the flag array holds
   int* trop_error = nullptr;                 // set by a timed-out neighbour wait

@@ -31,7 +31,12 @@ namespace pcom {
 
 namespace {
 
-constexpr int kTropBlock = 1024;  // one region point per thread: RW * RHt <= 1024
+// one region point per thread: RW * RHt <= the workgroup size, which is
+// kTropSmall for regions that fit it (four fewer idle waves per barrier and
+// 168 instead of 128 VGPRs per thread) and kTropBlock otherwise
+constexpr int kTropBlock = 1024;
+constexpr int kTropSmall = 768;
+inline int trop_block_for(long region) { return region <= kTropSmall ? kTropSmall : kTropBlock; }
 
 inline std::string hip_error_string(hipError_t e, const char* what) {
   return std::string(what) + ": " + hipGetErrorString(e);
@@ -99,7 +104,7 @@ enum ColTab { C_DX11, C_DX11I, C_DX14, C_DX14I, C_DX12, C_DX12I, C_DX13, C_DX13I
 // most kTropBlock points with sides of at least 6 is at most 170 long)
 constexpr int kTabStride = 176;
 
-size_t trop_lds_bytes(int, int, int) { return 16 * P_N * kTropBlock + 8 * (R_N + C_N) * kTabStride; }
+size_t trop_lds_bytes(int block) { return 16 * (size_t)P_N * block + 8 * (R_N + C_N) * kTabStride; }
 
 // Publish this tile's round and wait until every tile whose owned points
 // lie in this tile's region has published it.
@@ -165,7 +170,7 @@ __device__ __forceinline__ double xload(const double* p) {
 // result is independent of the tiling; points near the region edge read
 // clamped neighbours and are garbage, but no owned point depends on them
 // within S substeps.
-template <bool kLaunches>
+template <bool kLaunches, int kB>
 __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile,
                                                 const unsigned long long epoch) {
   extern __shared__ double2 lds2[];
@@ -186,15 +191,15 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const int RW = TW + HL + HR, RHt = TH + HL + HR;
   const int bx0 = x0 - HL, by0 = y0 - HL;
   const int NR = RW * RHt;
-  double* CHP = reinterpret_cast<double*>(lds2 + P_CH * kTropBlock);
-  double* PR = CHP + kTropBlock;
-  double2* UV = lds2 + P_UV * kTropBlock;
-  double2* SF = lds2 + P_SF * kTropBlock;
-  double* GX = reinterpret_cast<double*>(lds2 + P_GR * kTropBlock);
-  double* GY = GX + kTropBlock;
-  double2* WA = lds2 + P_WA * kTropBlock;
-  double2* WB = lds2 + P_WB * kTropBlock;
-  double* rowt = reinterpret_cast<double*>(lds2 + P_N * kTropBlock);  // [R_N][kTabStride]
+  double* CHP = reinterpret_cast<double*>(lds2 + P_CH * kB);
+  double* PR = CHP + kB;
+  double2* UV = lds2 + P_UV * kB;
+  double2* SF = lds2 + P_SF * kB;
+  double* GX = reinterpret_cast<double*>(lds2 + P_GR * kB);
+  double* GY = GX + kB;
+  double2* WA = lds2 + P_WA * kB;
+  double2* WB = lds2 + P_WB * kB;
+  double* rowt = reinterpret_cast<double*>(lds2 + P_N * kB);  // [R_N][kTabStride]
   double* colt = rowt + R_N * kTabStride;                             // [C_N][kTabStride]
   const double bt = a.bt;
   const int nstep = a.nstep;
@@ -502,12 +507,14 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   }
 }
 
-__global__ void __launch_bounds__(kTropBlock) k_trop_tiles(TropTileArgs a) { trop_tiles_body<false>(a, blockIdx.x, a.epoch); }
+template <int kB>
+__global__ void __launch_bounds__(kB) k_trop_tiles(TropTileArgs a) { trop_tiles_body<false, kB>(a, blockIdx.x, a.epoch); }
 
 // launch mode: tiles of a launch do not wait for each other, so neighbouring
 // tiles go to one XCD, whose L2 then serves their shared halo
-__global__ void __launch_bounds__(kTropBlock) k_trop_block(TropTileArgs a) {
-  trop_tiles_body<true>(a, (int)xcd_block(), 0);
+template <int kB>
+__global__ void __launch_bounds__(kB) k_trop_block(TropTileArgs a) {
+  trop_tiles_body<true, kB>(a, (int)xcd_block(), 0);
 }
 
 // The subcycles of several contexts (ensemble members) in one launch:
@@ -521,11 +528,26 @@ struct MultiLaunch {
   unsigned long long epoch[kMaxPack];
 };
 
-__global__ void __launch_bounds__(kTropBlock) k_trop_multi(const TropTileArgs* __restrict__ args, MultiLaunch ml) {
+template <int kB>
+__global__ void __launch_bounds__(kB) k_trop_multi(const TropTileArgs* __restrict__ args, MultiLaunch ml) {
   const int b = blockIdx.x;
   int m = 0;
   while (m + 1 < ml.n && b >= ml.start[m + 1]) ++m;
-  trop_tiles_body<false>(args[m], b - ml.start[m], ml.epoch[m]);
+  trop_tiles_body<false, kB>(args[m], b - ml.start[m], ml.epoch[m]);
+}
+
+// the kernels of one workgroup size
+struct TropKernels {
+  const void* tiles;
+  const void* block;
+  const void* multi;
+};
+TropKernels trop_kernels(int block) {
+  if (block == kTropSmall)
+    return {(const void*)k_trop_tiles<kTropSmall>, (const void*)k_trop_block<kTropSmall>,
+            (const void*)k_trop_multi<kTropSmall>};
+  return {(const void*)k_trop_tiles<kTropBlock>, (const void*)k_trop_block<kTropBlock>,
+          (const void*)k_trop_multi<kTropBlock>};
 }
 
 // vs at the first substep (mod_int.f90:198-207): seau*am(:,:,1)*lap(ub),
@@ -550,19 +572,21 @@ __global__ void k_vz(Grid<double> g, const R* __restrict__ am, const double* __r
 
 namespace {
 // co-resident workgroups of k_trop_tiles (and of k_trop_block, whichever
-// is fewer) with `lds` bytes of LDS each
-std::string trop_resident(const Ctx& c, size_t lds, long* resident) {
+// is fewer) of `block` threads with their LDS
+std::string trop_resident(const Ctx& c, int block, long* resident) {
+  const size_t lds = trop_lds_bytes(block);
   hipDeviceProp_t prop;
   hipError_t e = hipGetDeviceProperties(&prop, c.device);
   if (e != hipSuccess) return std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
   *resident = 0;
   if (lds > (size_t)prop.sharedMemPerBlock) return "";
   int per_cu = 1 << 30;
-  for (const void* k : {(const void*)k_trop_tiles, (const void*)k_trop_block}) {
+  const TropKernels tk = trop_kernels(block);
+  for (const void* k : {tk.tiles, tk.block}) {
     e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     int n = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kTropBlock, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, block, lds);
     if (e != hipSuccess) return std::string("occupancy of the subcycle kernels: ") + hipGetErrorString(e);
     per_cu = std::min(per_cu, n);
   }
@@ -588,17 +612,19 @@ constexpr long kExchangeWaves = 14, kLoadSubsteps = 2, kLaunchWaves = 14;
 std::string trop_candidates(const Ctx& c, bool launches, std::vector<TropCand>* out) {
   const int G = c.ni - 2;
   const long nstep = (long)c.dtbc / c.dtbt * 2;
-  long resident;
-  std::string err = trop_resident(c, trop_lds_bytes(0, 0, 0), &resident);
+  long res_small, res_big;
+  std::string err = trop_resident(c, kTropSmall, &res_small);
+  if (err.empty()) err = trop_resident(c, kTropBlock, &res_big);
   if (!err.empty()) return err;
   out->clear();
-  if (resident == 0) return "";
   for (int s = 1; s <= 4; ++s) {
     if (launches && s >= nstep) continue;  // a cycle takes at least two launches
     for (int tw = 4; tw <= 64; ++tw)
       for (int th = 4; th <= 64; ++th) {
         const long nr = (long)(tw + 5 * s) * (th + 5 * s);
         if (nr > kTropBlock) continue;
+        const long resident = trop_block_for(nr) == kTropSmall ? res_small : res_big;
+        if (resident == 0) continue;
         const long waves = (nr + 63) / 64;
         const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
         long cost;
@@ -632,10 +658,6 @@ std::string trop_configure(Ctx& c) {
           return "internal: horizontal metrics are not separable in (i, j)";
       }
   std::string err;
-  const size_t lds = trop_lds_bytes(0, 0, 0);
-  long resident;
-  if (!(err = trop_resident(c, lds, &resident)).empty()) return err;
-  if (resident == 0) return "the subcycle's " + std::to_string(lds) + " B of LDS do not fit this device";
   bool launches = c.trop_mode == PCOM_TROP_LAUNCHES;
   if (c.trop_tw > 0 || c.trop_th > 0 || c.trop_s > 0) {
     if (c.trop_tw <= 0) c.trop_tw = 24;
@@ -663,6 +685,11 @@ std::string trop_configure(Ctx& c) {
            std::to_string(kTropBlock) + " points";
   if (c.trop_tw + 5 * c.trop_s > kTabStride || c.trop_th + 5 * c.trop_s > kTabStride)
     return "barotropic tile sides plus halo must be at most " + std::to_string(kTabStride);
+  const int block = trop_block_for((long)(c.trop_tw + 5 * c.trop_s) * (c.trop_th + 5 * c.trop_s));
+  const size_t lds = trop_lds_bytes(block);
+  long resident;
+  if (!(err = trop_resident(c, block, &resident)).empty()) return err;
+  if (resident == 0) return "the subcycle's " + std::to_string(lds) + " B of LDS do not fit this device";
   long tiles_x = (G + c.trop_tw - 1) / c.trop_tw;
   long tiles_y = (c.nj + c.trop_th - 1) / c.trop_th;
   long need = tiles_x * tiles_y;
@@ -678,6 +705,7 @@ std::string trop_configure(Ctx& c) {
   c.trop_launches = launches;
   c.trop_tiles_x = (int)tiles_x;
   c.trop_blocks = (int)need;
+  c.trop_block = block;
   c.trop_lds = lds;
   return "";
 }
@@ -803,7 +831,10 @@ std::string launch_trop_kernel(Ctx& c) {
       a.nt1 = std::min(a.nstep, nt0 + c.trop_s - 1);
       a.src = (b + 1) & 1;
       a.dst = b & 1;
-      hipLaunchKernelGGL(k_trop_block, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
+      if (c.trop_block == kTropSmall)
+        hipLaunchKernelGGL(k_trop_block<kTropSmall>, dim3(c.trop_blocks), dim3(kTropSmall), c.trop_lds, c.stream, a);
+      else
+        hipLaunchKernelGGL(k_trop_block<kTropBlock>, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return std::string("k_trop_block: ") + hipGetErrorString(e);
     }
@@ -812,7 +843,10 @@ std::string launch_trop_kernel(Ctx& c) {
   a.epoch = ++c.trop_epoch;
   return chained_launch(c.device, c.stream, [&]() -> std::string {
     if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
-    hipLaunchKernelGGL(k_trop_tiles, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
+    if (c.trop_block == kTropSmall)
+      hipLaunchKernelGGL(k_trop_tiles<kTropSmall>, dim3(c.trop_blocks), dim3(kTropSmall), c.trop_lds, c.stream, a);
+    else
+      hipLaunchKernelGGL(k_trop_tiles<kTropBlock>, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
     return "";
@@ -841,7 +875,7 @@ struct Ensemble {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<std::vector<int>> packs;  // member indices
-  std::vector<size_t> pack_lds;
+  std::vector<int> pack_block;          // workgroup size of each pack's launch
   std::vector<int> pack_of;             // member -> pack (-1: runs alone)
   TropTileArgs* dargs = nullptr;        // device, in pack order
   std::vector<int> pack_first;          // first dargs index of each pack
@@ -850,14 +884,16 @@ struct Ensemble {
 
 namespace {
 
-std::string multi_capacity(int device, size_t lds, long* cap) {
+std::string multi_capacity(int device, int block, long* cap) {
   hipDeviceProp_t prop;
   hipError_t e = hipGetDeviceProperties(&prop, device);
   if (e != hipSuccess) return std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
-  e = hipFuncSetAttribute((const void*)k_trop_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const size_t lds = trop_lds_bytes(block);
+  const void* k = trop_kernels(block).multi;
+  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
   int per_cu = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trop_multi, kTropBlock, lds);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, lds);
   if (e != hipSuccess) return std::string("occupancy of k_trop_multi: ") + hipGetErrorString(e);
   *cap = (long)per_cu * prop.multiProcessorCount;
   return "";
@@ -872,7 +908,7 @@ std::string ensemble_plan(Ensemble& en) {
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return en.m[a]->trop_blocks > en.m[b]->trop_blocks; });
   en.packs.clear();
-  en.pack_lds.clear();
+  en.pack_block.clear();
   en.pack_of.assign(n, -1);
   std::vector<long> used;
   for (int q : order) {
@@ -880,13 +916,13 @@ std::string ensemble_plan(Ensemble& en) {
     bool placed = false;
     for (size_t p = 0; p < en.packs.size() && !placed; ++p) {
       if (en.packs[p].size() >= (size_t)kMaxPack) continue;
-      const size_t lds = std::max(en.pack_lds[p], c.trop_lds);
+      const int block = std::max(en.pack_block[p], c.trop_block);
       long cap;
-      std::string e = multi_capacity(en.device, lds, &cap);
+      std::string e = multi_capacity(en.device, block, &cap);
       if (!e.empty()) return e;
       if (used[p] + c.trop_blocks <= cap) {
         en.packs[p].push_back(q);
-        en.pack_lds[p] = lds;
+        en.pack_block[p] = block;
         used[p] += c.trop_blocks;
         en.pack_of[q] = (int)p;
         placed = true;
@@ -894,11 +930,11 @@ std::string ensemble_plan(Ensemble& en) {
     }
     if (!placed) {
       long cap;
-      std::string e = multi_capacity(en.device, c.trop_lds, &cap);
+      std::string e = multi_capacity(en.device, c.trop_block, &cap);
       if (!e.empty()) return e;
       if (c.trop_blocks > cap) return "an ensemble member's subcycle tiling does not fit the device";
       en.packs.push_back({q});
-      en.pack_lds.push_back(c.trop_lds);
+      en.pack_block.push_back(c.trop_block);
       used.push_back(c.trop_blocks);
       en.pack_of[q] = (int)en.packs.size() - 1;
     }
@@ -1007,12 +1043,16 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
       }
       ml.start[ml.n] = blocks;
       const TropTileArgs* args = en.dargs + en.pack_first[p];
-      const size_t lds = en.pack_lds[p];
+      const int block = en.pack_block[p];
+      const size_t lds = trop_lds_bytes(block);
       err = chained_launch(en.device, en.stream, [&]() -> std::string {
-        hipError_t ea = hipFuncSetAttribute((const void*)k_trop_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipError_t ea = hipFuncSetAttribute(trop_kernels(block).multi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)lds);
         if (ea != hipSuccess) return hip_error_string(ea, "hipFuncSetAttribute");
-        hipLaunchKernelGGL(k_trop_multi, dim3(blocks), dim3(kTropBlock), lds, en.stream, args, ml);
+        if (block == kTropSmall)
+          hipLaunchKernelGGL(k_trop_multi<kTropSmall>, dim3(blocks), dim3(kTropSmall), lds, en.stream, args, ml);
+        else
+          hipLaunchKernelGGL(k_trop_multi<kTropBlock>, dim3(blocks), dim3(kTropBlock), lds, en.stream, args, ml);
         ea = hipGetLastError();
         if (ea != hipSuccess) return hip_error_string(ea, "k_trop_multi");
         return "";
