@@ -320,10 +320,12 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
     k_adv_tiles(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc, const R* __restrict__ wm,
                 const double* __restrict__ chc, const double* __restrict__ schr, R* auc, R* avc) {
   const int G = g.ni - 2;
-  const int ntx = (G + kAdvTX - 1) / kAdvTX, nty = (g.nj + kAdvTY - 1) / kAdvTY;
+  const int ntx = (G + kAdvTX - 1) / kAdvTX;
   const int b = (int)xcd_block();
-  const int k = b / (ntx * nty) + 1;
-  const int r = b % (ntx * nty);
+  // levels fastest: a tile's levels run together on one XCD, whose L2 then
+  // serves the vertical neighbours each level reads
+  const int k = b % g.nk + 1;
+  const int r = b / g.nk;
   const int c0 = 2 + (r % ntx) * kAdvTX, j0 = 1 + (r / ntx) * kAdvTY;
   const int tx = threadIdx.x % kAdvTX, ty = threadIdx.x / kAdvTX;
   const int ic = c0 + tx, j = j0 + ty;
@@ -468,10 +470,12 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
                 const double* __restrict__ chc, const double* __restrict__ schr, const double* __restrict__ taux,
                 const double* __restrict__ tauy, R* fx, R* fy) {
   const int G = g.ni - 2;
-  const int ntx = (G + kFriTX - 1) / kFriTX, nty = (g.nj + kFriTY - 1) / kFriTY;
+  const int ntx = (G + kFriTX - 1) / kFriTX;
   const int b = (int)xcd_block();
-  const int k = b / (ntx * nty) + 1;
-  const int r = b % (ntx * nty);
+  // levels fastest: a tile's levels run together on one XCD, whose L2 then
+  // serves the vertical neighbours each level reads
+  const int k = b % g.nk + 1;
+  const int r = b / g.nk;
   const int c0 = 2 + (r % ntx) * kFriTX, j0 = 1 + (r / ntx) * kFriTY;
   const int tx = threadIdx.x % kFriTX, ty = threadIdx.x / kFriTX;
   const int ic = c0 + tx, j = j0 + ty;
@@ -769,10 +773,12 @@ __global__ void __launch_bounds__(kSmagTX * kSmagTY)
     k_smag(Grid<R> g, const double* __restrict__ uc, const double* __restrict__ vc,
            const double* __restrict__ mch2, const double* __restrict__ schs, R* acu, R* acv, R* am, int dtbc) {
   const int G = g.ni - 2;
-  const int ntx = (G + kSmagTX - 1) / kSmagTX, nty = (g.nj + kSmagTY - 1) / kSmagTY;
+  const int ntx = (G + kSmagTX - 1) / kSmagTX;
   const int b = (int)xcd_block();
-  const int k = b / (ntx * nty) + 1;
-  const int r = b % (ntx * nty);
+  // levels fastest: a tile's levels run together on one XCD, whose L2 then
+  // serves the vertical neighbours each level reads
+  const int k = b % g.nk + 1;
+  const int r = b / g.nk;
   const int c0 = 2 + (r % ntx) * kSmagTX, j0 = 1 + (r / ntx) * kSmagTY;
   const int tx = threadIdx.x % kSmagTX, ty = threadIdx.x / kSmagTX;
   const int ic = c0 + tx, j = j0 + ty;
@@ -897,10 +903,12 @@ template <typename R>
 __global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
   const Grid<R>& g = a.g;
   const int G = g.ni - 2;
-  const int ntx = (G + kTsTX - 1) / kTsTX, nty = (g.nj + kTsTY - 1) / kTsTY;
+  const int ntx = (G + kTsTX - 1) / kTsTX;
   const int b = (int)xcd_block();
-  const int k = b / (ntx * nty) + 1;
-  const int r = b % (ntx * nty);
+  // levels fastest: a tile's levels run together on one XCD, whose L2 then
+  // serves the vertical neighbours each level reads
+  const int k = b % g.nk + 1;
+  const int r = b / g.nk;
   const int c0 = 2 + (r % ntx) * kTsTX, j0 = 1 + (r / ntx) * kTsTY;
   const int tx = threadIdx.x % kTsTX, ty = threadIdx.x / kTsTX;
   const int ic = c0 + tx, j = j0 + ty;
