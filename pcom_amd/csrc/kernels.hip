@@ -899,8 +899,10 @@ struct TsCellIn {
   double mch, bt, bs;
 };
 
+// five waves per SIMD (92 VGPRs in FP64, no spills) hide more of the
+// kernel's memory latency than four with the compiler's default 124
 template <typename R>
-__global__ void __launch_bounds__(kTsTX * kTsTY) k_ts_tiles(TsArgs<R> a) {
+__global__ void __launch_bounds__(kTsTX * kTsTY) __attribute__((amdgpu_waves_per_eu(5))) k_ts_tiles(TsArgs<R> a) {
   const Grid<R>& g = a.g;
   const int G = g.ni - 2;
   const int ntx = (G + kTsTX - 1) / kTsTX;
