@@ -283,7 +283,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess)
     return fail(PCOM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  for (hipEvent_t* ev : {&c->fork[0], &c->fork[1], &c->join[0], &c->join[1]})
+  for (hipEvent_t* ev : {&c->fork[0], &c->fork[1], &c->fork[2], &c->join[0], &c->join[1], &c->join[2]})
     if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
       return fail(PCOM_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
 
@@ -431,7 +431,7 @@ int pcom_destroy(pcom_ctx* ctx) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {c->fork[0], c->fork[1], c->join[0], c->join[1]})
+  for (hipEvent_t e : {c->fork[0], c->fork[1], c->fork[2], c->join[0], c->join[1], c->join[2]})
     if (e) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -447,6 +447,9 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
     std::string e = launch_step(*c);
     if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);
   }
+  // the last step's convection is part of this call's work
+  std::string e = settle_side(*c);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);
   return PCOM_OK;
 }
 

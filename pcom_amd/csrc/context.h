@@ -50,7 +50,10 @@ struct Ctx {
   // (den_alpha + int_pgra beside int_readyc, int_ssh beside the mode
   // coupling and smag), forked from and joined back into `stream` by events
   hipStream_t side = nullptr;
-  hipEvent_t fork[2] = {nullptr, nullptr}, join[2] = {nullptr, nullptr};
+  hipEvent_t fork[3] = {nullptr, nullptr, nullptr}, join[3] = {nullptr, nullptr, nullptr};
+  // convect (the end of a step) runs on `side` and overlaps the next step's
+  // int_readyc; set while it has not been joined back into `stream`
+  bool side_pending = false;
   int precision = PCOM_FP64;
   int rst = 0;
   int dtbt = 60, dtbc = 3600;
@@ -90,6 +93,7 @@ struct Ctx {
 std::string launch_step(Ctx& c);                 // one baroclinic step
 std::string launch_step_pre(Ctx& c);             // up to int_trop
 std::string launch_step_post(Ctx& c);            // after int_trop
+std::string settle_side(Ctx& c);                 // c.stream waits for pending side-stream work
 std::string launch_global_sums(Ctx& c, double out[3]);
 std::string launch_check_finite(Ctx& c, int* bad);
 // time means into `stage` (>= glo_ni*glo_nj*nk + 2 doubles), copied to the

@@ -1357,8 +1357,11 @@ std::string step_pre_t(Ctx& c) {
 
   c.step_i += 1;
   if (c.timing) (void)hipEventRecord(c.ev[0], c.stream);
-  // daily forcing (main.f90:164-180)
+  // daily forcing (main.f90:164-180); den_prho reads T and S and writes the
+  // coefficients convect reads, so the previous step's convection must be done
   if (c.step_i == 1 || c.ct.d != c.pt.d) {
+    std::string es = settle_side(c);
+    if (!es.empty()) return es;
     int m1, m2;
     double fw;
     bnd_weights(c.ct, &m1, &m2, &fw);
@@ -1433,6 +1436,7 @@ std::string step_pre_t(Ctx& c) {
     if (eu != hipSuccess) return hip_err(eu, "k_ut");
   }
   if ((e = join_side(c, 0)) != hipSuccess) return hip_err(e, "join");
+  c.side_pending = false;  // the join follows the previous step's convection too
   return "";
 }
 
@@ -1514,8 +1518,13 @@ std::string step_post_t(Ctx& c) {
           hipFuncSetAttribute((const void*)k_conv<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (ea != hipSuccess) return hip_err(ea, "k_conv LDS size");
     }
+    // convect and the T/S/rho accumulators on the side stream: until the
+    // next step's int_readyc join nothing on c.stream reads or writes T, S,
+    // prho, chc or the accumulators, so it overlaps the next step's start
+    if ((e = fork_side(c, 2)) != hipSuccess) return hip_err(e, "fork");
+    c.side_pending = true;
     hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kConvCW - 1) / kConvCW)), dim3(kConvCW, kConvKL), lds,
-                       c.stream, ge,
+                       c.side, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
                        P<R>(f.act), P<R>(f.acs), P<R>(f.acr));
     hipError_t e2 = hipGetLastError();
@@ -1523,6 +1532,8 @@ std::string step_post_t(Ctx& c) {
   }
   c.n_ts += 1;
   if (c.timing) {
+    std::string es = settle_side(c);
+    if (!es.empty()) return es;
     (void)hipEventRecord(c.ev[3], c.stream);
     (void)hipEventSynchronize(c.ev[3]);
     float a = 0, b = 0, d = 0;
@@ -1610,6 +1621,14 @@ std::string launch_step_post(Ctx& c) {
   if (c.precision == PCOM_FP64) return step_post_t<double, double>(c);
   if (c.precision == PCOM_MIXED) return step_post_t<float, double>(c);
   return step_post_t<float, float>(c);
+}
+
+std::string settle_side(Ctx& c) {
+  if (!c.side_pending) return "";
+  hipError_t e = join_side(c, 2);
+  if (e != hipSuccess) return hip_err(e, "join");
+  c.side_pending = false;
+  return "";
 }
 
 std::string launch_step(Ctx& c) {

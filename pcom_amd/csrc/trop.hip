@@ -1069,6 +1069,8 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
       if (!(err = launch_step_post(c)).empty()) return err;
     }
   }
+  for (Ctx* c : en.m)
+    if (!(err = settle_side(*c)).empty()) return err;
   return "";
 }
 
