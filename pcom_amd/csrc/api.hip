@@ -427,6 +427,7 @@ int pcom_destroy(pcom_ctx* ctx) {
   Ctx* c = C(ctx);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);  // work left by a failed step
   nn_destroy(*c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t& e : c->ev)
