@@ -457,6 +457,7 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
 int pcom_sync(pcom_ctx* ctx) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_sync: NULL context");
   HIPCHK(hipStreamSynchronize(C(ctx)->stream), "hipStreamSynchronize");
+  HIPCHK(hipStreamSynchronize(C(ctx)->side), "hipStreamSynchronize");
   return check_async_error(*C(ctx));
 }
 
