@@ -36,7 +36,7 @@ TROP_FLOPS_PER_POINT = 184
 # HBM bytes per k_trop_tiles launch at 1 degree: rocprofv3 --pmc FETCH_SIZE
 # (x 2.000) + WRITE_SIZE (x 1.000), the factors being what the counters read
 # for a plain 1 GiB copy (profiles/k_trop_tiles_global1deg_pmc.md)
-TROP_TRAFFIC_BYTES = 496_400_000
+TROP_TRAFFIC_BYTES = 490_300_000
 
 
 def sypd(ms_per_step, dtbc):
