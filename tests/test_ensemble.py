@@ -45,6 +45,24 @@ def test_ensemble_matches_individual_runs(gpu_lib):
         assert not bad, f"{c.name}: " + "; ".join(bad)
 
 
+def test_pack_of_both_workgroup_sizes(gpu_lib):
+    """A pack whose members' regions need the 768- and the 1024-thread
+    subcycle kernels runs in the larger one; every member keeps its bits."""
+    from pcom_amd.lib import Ensemble, Model
+    cfgs = [_perturbed(configs.box(), 110), _perturbed(configs.box(), 111)]
+    tiles = [(8, 8, 1), (20, 14, 3)]  # regions of 169 and 1015 points
+    members = [Model(c, trop_tile=t) for c, t in zip(cfgs, tiles)]
+    ens = Ensemble(members)
+    ens.step(4)
+    npk, of = ens.packs()
+    assert npk == 1 and list(of) == [0, 0], (npk, of)
+    for c, t, m in zip(cfgs, tiles, members):
+        solo = Model(c, trop_tile=t)
+        solo.step(4)
+        bad = _same(m, solo)
+        assert not bad, f"{t}: " + "; ".join(bad)
+
+
 def test_autotuner_keeps_state_and_bits(gpu_lib):
     from pcom_amd.lib import Model
     cfg = configs.global_1deg()
