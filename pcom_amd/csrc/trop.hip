@@ -425,6 +425,14 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       // corrector fluxes prh*sch*ub (mod_int.f90:264-265)
       SF[q] = make_double2(prh_u * sch * un, prh_u * sch * vn);
     }
+    // an exchange follows this substep: the owned velocity goes out now, so
+    // that its stores complete while the corrector runs
+    const bool exch = !kLaunches && nt < nt1 && nt % a.S == 0;
+    if (exch && owned) {
+      const bool odd = round & 1;
+      xstore((odd ? a.xu[1] : a.xu[0]) + p, un);
+      xstore((odd ? a.xv[1] : a.xv[0]) + p, vn);
+    }
     __syncthreads();
     // C) corrector ch and the time-filter accumulators (mod_int.f90:263-273)
     double cn = 0;
@@ -439,17 +447,13 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
         acm2 = acm2 + cn;
       }
     }
-    if (!kLaunches && nt < nt1 && nt % a.S == 0) {
+    if (exch) {
       // uniform selects keep the buffer addresses in scalar registers
       const bool odd = round & 1;
       double* const xc = (odd ? a.xc[1] : a.xc[0]) + p;
       double* const xu = (odd ? a.xu[1] : a.xu[0]) + p;
       double* const xv = (odd ? a.xv[1] : a.xv[0]) + p;
-      if (owned) {
-        xstore(xc, cn);
-        xstore(xu, un);
-        xstore(xv, vn);
-      }
+      if (owned) xstore(xc, cn);
       tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
       if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
       // the halo from the neighbours (owned points already hold these values)
