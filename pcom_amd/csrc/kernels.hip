@@ -32,8 +32,10 @@ namespace pcom {
 namespace {
 
 constexpr int kBlock = 256;
-// column kernels: kCW columns x kKL k-lanes per workgroup; levels <= kNKMAX
-constexpr int kCW = 64, kKL = 4, kNKMAX = 64;
+// column kernels: workgroups of kColCW columns x kColKL k-lanes (the
+// column kernels are bound by load latency: more lanes per column keep
+// more loads in flight), with [nk][kColCW] LDS arrays
+constexpr int kColCW = 32, kColKL = 8;
 
 inline std::string hip_err(hipError_t e, const char* what) {
   if (e == hipSuccess) return "";
@@ -144,18 +146,19 @@ __global__ void k_alpha(Grid<R> g, const double* __restrict__ t, const double* _
 // t = alpha*dpr*msk; the suffix sum is accumulated in increasing k as
 // Fortran's sum() does, so FP64 results are the reference's bits.
 template <typename R>
-__global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
-  __shared__ R t[kNKMAX][kCW];
+__global__ void __launch_bounds__(kColCW * kColKL) k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
+  extern __shared__ unsigned char adp_lds[];
+  R(*t)[kColCW] = reinterpret_cast<R(*)[kColCW]>(adp_lds);  // [nk][kColCW]
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)xcd_block() * kCW + tx;
+  const long p = (long)xcd_block() * kColCW + tx;
   if (p < g.n2) {
     const int lev = g.lev_t[p];
-    for (int k = 1 + kl; k <= nk; k += kKL)
+    for (int k = 1 + kl; k <= nk; k += kColKL)
       t[k - 1][tx] = alpha[p + g.n2 * (k - 1)] * g.dpr2[k - 1] * (R)(lev >= k ? 1 : 0);
   }
   __syncthreads();
   if (p >= g.n2) return;
-  for (int k = 1 + kl; k <= nk; k += kKL) {
+  for (int k = 1 + kl; k <= nk; k += kColKL) {
     if constexpr (sizeof(R) == sizeof(double)) {
       R s = 0;
       for (int kk = k + 1; kk <= nk; ++kk) s += t[kk - 1][tx];
@@ -172,11 +175,9 @@ __global__ void __launch_bounds__(kCW * kKL) k_adp(Grid<R> g, const R* __restric
 // mod_op.f90:946-981): bphi = vint(ter(adp) + pr*ter(alpha)),
 // bgraphi = vint(gra(adp)), each with the local mask and the mask cshift'ed
 // by -1 (row 1 wraps to row nj; the E/W wrap is overwritten by the swap).
-// kColCW columns per workgroup, kColKL k-lanes: the lanes evaluate the
-// per-level terms of the columns in parallel into LDS, one lane per column
-// then sums them in k order (the reference's order), and for the mode
-// coupling all lanes apply the result.
-constexpr int kColCW = 32, kColKL = 8;
+// The lanes evaluate the per-level terms of the columns in parallel into
+// LDS, one lane per column then sums them in k order (the reference's
+// order).
 
 template <typename R>
 __global__ void __launch_bounds__(kColCW * kColKL)
@@ -242,20 +243,22 @@ __global__ void __launch_bounds__(kColCW * kColKL)
 // upwelling (mod_int.f90:626-647): wm from the continuity equation;
 // with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).
 template <typename R, typename E>
-__global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const double* __restrict__ uc,
-                                                   const double* __restrict__ vc, const double* __restrict__ sqp, R* wm,
-                                                   R* acw, const E* __restrict__ alpha,
-                                                   const double* __restrict__ mch2) {
-  // one LDS column array: the divergence per level, then in place the
-  // upwelling (half the LDS of two arrays, twice the workgroups per CU)
-  __shared__ R w3[kNKMAX][kCW];
+__global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double* __restrict__ uc,
+                                                         const double* __restrict__ vc,
+                                                         const double* __restrict__ sqp, R* wm, R* acw,
+                                                         const E* __restrict__ alpha,
+                                                         const double* __restrict__ mch2) {
+  // one LDS column array [nk][kColCW]: the divergence per level, then in
+  // place the upwelling
+  extern __shared__ unsigned char upw_lds[];
+  R(*w3)[kColCW] = reinterpret_cast<R(*)[kColCW]>(upw_lds);
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)xcd_block() * kCW + tx;
+  const long p = (long)xcd_block() * kColCW + tx;
   const bool ok = p < g.n2;
   const int i = ok ? (int)(p % g.ni) + 1 : 1, j = ok ? (int)(p / g.ni) + 1 : 1;
   auto SQ = [&](int m, int jj) -> R { return (R)sqp[g.ix(m, jj)]; };
   if (ok) {
-    for (int k = 1 + kl; k <= nk; k += kKL) {
+    for (int k = 1 + kl; k <= nk; k += kColKL) {
       const double* u = uc + g.n2 * (k - 1);
       const double* v = vc + g.n2 * (k - 1);
       auto VA = [&](int m, int jj) -> R { return SQ(m, jj) * (R)u[g.ix(m, jj)]; };
@@ -281,7 +284,7 @@ __global__ void __launch_bounds__(kCW * kKL) k_upw(Grid<R> g, const double* __re
   }
   __syncthreads();
   if (!ok) return;
-  for (int k = 1 + kl; k <= nk; k += kKL) {
+  for (int k = 1 + kl; k <= nk; k += kColKL) {
     long q = p + g.n2 * (k - 1);
     R w = w3[k - 1][tx];
     if (k >= 2) wm[q] = w;
@@ -828,17 +831,18 @@ __global__ void __launch_bounds__(kSmagTX * kSmagTY)
 
 // int_ssh (mod_int.f90:602-624): alpha from (tp, sp, mch), column SSH
 template <typename R>
-__global__ void __launch_bounds__(kCW * kKL) k_ssh(Grid<R> g, const double* __restrict__ tp,
-                                                   const double* __restrict__ sp,
-                                                   const double* __restrict__ mch, R* alpha, double* ssh,
-                                                   double* acssh) {
-  __shared__ R term[kNKMAX][kCW];
+__global__ void __launch_bounds__(kColCW * kColKL) k_ssh(Grid<R> g, const double* __restrict__ tp,
+                                                         const double* __restrict__ sp,
+                                                         const double* __restrict__ mch, R* alpha, double* ssh,
+                                                         double* acssh) {
+  extern __shared__ unsigned char ssh_lds[];
+  R(*term)[kColCW] = reinterpret_cast<R(*)[kColCW]>(ssh_lds);  // [nk][kColCW]
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
-  const long p = (long)xcd_block() * kCW + tx;
+  const long p = (long)xcd_block() * kColCW + tx;
   if (p < g.n2) {
     const R m = (R)mch[p];
     const int lev = g.lev_t[p];
-    for (int k = 1 + kl; k <= nk; k += kKL) {
+    for (int k = 1 + kl; k <= nk; k += kColKL) {
       long q = p + g.n2 * (k - 1);
       R al = alpha[q];
       if (lev >= k) {
@@ -1301,13 +1305,6 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
   return c.gf;
 }
 
-#define LAUNCH_COL(kern, ...)                                                       \
-  do {                                                                              \
-    hipLaunchKernelGGL(kern, dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), 0, c.stream, __VA_ARGS__); \
-    hipError_t e_ = hipGetLastError();                                              \
-    if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
-  } while (0)
-
 #define LAUNCH(kern, n, ...)                                                        \
   do {                                                                              \
     hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, __VA_ARGS__); \
@@ -1322,12 +1319,19 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
-#define LAUNCH_COL_ON(st, kern, ...)                                                \
+// a column kernel of kColCW columns x kColKL k-lanes with one [nk][kColCW]
+// LDS array of `elem`-byte entries
+#define LAUNCH_COLS_ON(st, kern, elem, ...)                                         \
   do {                                                                              \
-    hipLaunchKernelGGL(kern, dim3((unsigned)((c.n2 + kCW - 1) / kCW)), dim3(kCW, kKL), 0, st, __VA_ARGS__); \
+    hipLaunchKernelGGL(kern, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), \
+                       (size_t)c.nk * kColCW * (elem), st, __VA_ARGS__);            \
     hipError_t e_ = hipGetLastError();                                              \
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
+
+#define LAUNCH_UPW(acw, alpha, mch2)                                                \
+  LAUNCH_COLS_ON(c.stream, (k_upw<R, E>), sizeof(R), g, U, V, h.sqp, P<R>(f.wm), (R*)(acw), (const E*)(alpha), \
+                 (const double*)(mch2))
 
 // work on c.side after everything queued so far on c.stream (fork), and
 // c.stream's later work after c.side's (join)
@@ -1378,7 +1382,7 @@ std::string step_pre_t(Ctx& c) {
   // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes
   if ((e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
   LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
-  LAUNCH_COL_ON(c.side, k_adp<E>, ge, P<E>(f.alpha), P<E>(f.adp));
+  LAUNCH_COLS_ON(c.side, k_adp<E>, sizeof(E), ge, P<E>(f.alpha), P<E>(f.adp));
   {
     const size_t lds = 4 * (size_t)c.nk * kColCW * sizeof(E);
     hipError_t eb = hipSuccess;
@@ -1392,7 +1396,7 @@ std::string step_pre_t(Ctx& c) {
   }
   // int_readyc
   LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
-  LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), (R*)nullptr, (const E*)nullptr, (const double*)nullptr);
+  LAUNCH_UPW(nullptr, nullptr, nullptr);
   c.ab = (c.ab + 1) % 3;
   R* au = P<R>(f.AU[c.ab]);
   R* av = P<R>(f.AV[c.ab]);
@@ -1476,7 +1480,7 @@ std::string step_post_t(Ctx& c) {
   // coupling and smag touch
   hipError_t e;
   if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
-  LAUNCH_COL_ON(c.side, k_ssh<E>, ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
+  LAUNCH_COLS_ON(c.side, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
   c.n_ssh += 1;
   {
     const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(double);
@@ -1496,8 +1500,7 @@ std::string step_post_t(Ctx& c) {
   c.n_uv += 1;
   if ((e = join_side(c, 1)) != hipSuccess) return hip_err(e, "join");
   // int_ts
-  LAUNCH_COL((k_upw<R, E>), g, U, V, h.sqp, P<R>(f.wm), P<R>(f.acw), (const E*)P<E>(f.alpha),
-             (const double*)h.mch2);
+  LAUNCH_UPW(P<R>(f.acw), P<E>(f.alpha), h.mch2);
   c.n_w += 1;
   {
     TsArgs<R> a;
