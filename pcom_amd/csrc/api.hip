@@ -327,6 +327,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = make(*c, &f.alpha, n3, el, nullptr, (double)(1.0f / 1000.0f)))) return rc;
   if ((rc = make(*c, &f.adp, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.wm, n3p, fl, nullptr))) return rc;
+  if ((rc = make(*c, &f.wmr, n3p, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.prho1, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.prho2, n3, el, nullptr))) return rc;
   void** acc[] = {&f.act, &f.acs, &f.acr, &f.acu, &f.acv, &f.acw};
@@ -341,7 +342,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   for (double** a : ones)
     if ((rc = make2(*c, a, n2, nullptr, 1.0))) return rc;
   double** zeros[] = {&h.acch, &h.acssh, &h.ssh, &h.buc, &h.bvc, &h.bup, &h.bvp, &h.but, &h.bvt, &h.pax, &h.pay,
-                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1],
+                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.sqr, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1],
                       &h.xacc[0], &h.xacc[1], &h.xacc[2], &h.xacc[3]};
   for (double** a : zeros)
     if ((rc = make2(*c, a, n2, nullptr))) return rc;
@@ -428,6 +429,7 @@ int pcom_destroy(pcom_ctx* ctx) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);  // work left by a failed step
+  if (c->stream) trop_chain_forget(c->device, c->stream);
   nn_destroy(*c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t& e : c->ev)

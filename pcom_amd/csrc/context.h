@@ -24,6 +24,7 @@ struct Fields3 {
   void *AU[3], *AV[3];      // AB3 advection ring: auc = AU[ab], aup =
                             // AU[ab-1], aupp = AU[ab-2]
   void *fx, *fy, *am, *alpha, *adp, *wm, *prho1, *prho2;
+  void* wmr;  // int_readyc's upwelling; wm holds int_ts's (the two overlap)
   void *act, *acs, *acr, *acu, *acv, *acw;
   void *km, *kh;
 };
@@ -37,6 +38,7 @@ struct Fields2 {
   double *graphihx, *graphihy;
   double *vz1, *vz2, *am1;                      // subcycle statics
   double *schr, *sqp, *schs, *schm;             // sqrt(ter(ch)) factors on g3
+  double* sqr;  // int_readyc's sqp (int_ts's, in sqp, may still be in use)
   double *xc[2], *xu[2], *xv[2];                // subcycle tile exchange
   double *xacc[4];  // subcycle accumulators between launches (PCOM_TROP_LAUNCHES)
   double *red;                                  // reduction scratch
@@ -101,6 +103,7 @@ std::string launch_check_finite(Ctx& c, int* bad);
 std::string launch_output_means(Ctx& c, pcom_means* o, double* stage);
 std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
 std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
+void trop_chain_forget(int device, hipStream_t st);  // trop.hip: before a stream is destroyed
 std::string launch_vz(Ctx& c);                   // trop.hip: the subcycle's set-up kernel
 std::string ensemble_create(const std::vector<Ctx*>& members, Ensemble** out);
 std::string ensemble_step(Ensemble& en, int nsteps);

@@ -241,12 +241,14 @@ __global__ void __launch_bounds__(kColCW * kColKL)
 }
 
 // upwelling (mod_int.f90:626-647): wm from the continuity equation;
-// with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).
+// with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).  The
+// result goes to `wout`: levels 2..nk, and when wout is not wm also the
+// levels 1 and nk+1 the upwelling leaves as they are in wm.
 template <typename R, typename E>
 __global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double* __restrict__ uc,
                                                          const double* __restrict__ vc,
-                                                         const double* __restrict__ sqp, R* wm, R* acw,
-                                                         const E* __restrict__ alpha,
+                                                         const double* __restrict__ sqp, const R* wm, R* wout,
+                                                         R* acw, const E* __restrict__ alpha,
                                                          const double* __restrict__ mch2) {
   // one LDS column array [nk][kColCW]: the divergence per level, then in
   // place the upwelling
@@ -284,10 +286,12 @@ __global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double
   }
   __syncthreads();
   if (!ok) return;
+  const bool copy = wout != wm;
+  if (copy && kl == 0) wout[p + g.n2 * nk] = wm[p + g.n2 * nk];
   for (int k = 1 + kl; k <= nk; k += kColKL) {
     long q = p + g.n2 * (k - 1);
     R w = w3[k - 1][tx];
-    if (k >= 2) wm[q] = w;
+    if (k >= 2 || copy) wout[q] = w;
     if (acw) acw[q] = acw[q] + w * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
   }
 }
@@ -1329,9 +1333,9 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
-#define LAUNCH_UPW(acw, alpha, mch2)                                                \
-  LAUNCH_COLS_ON(c.stream, (k_upw<R, E>), sizeof(R), g, U, V, h.sqp, P<R>(f.wm), (R*)(acw), (const E*)(alpha), \
-                 (const double*)(mch2))
+#define LAUNCH_UPW(st, sq, wout, acw, alpha, mch2)                                  \
+  LAUNCH_COLS_ON(st, (k_upw<R, E>), sizeof(R), g, U, V, sq, P<R>(f.wm), (R*)(wout), (R*)(acw), \
+                 (const E*)(alpha), (const double*)(mch2))
 
 // work on c.side after everything queued so far on c.stream (fork), and
 // c.stream's later work after c.side's (join)
@@ -1379,8 +1383,11 @@ std::string step_pre_t(Ctx& c) {
     LAUNCH(k_prho<E>, n3, ge, T, S, h.chc, P<E>(f.prho1), P<E>(f.prho2));
   }
   // den_alpha, int_pgra on the side stream: they read T, S, chc and write
-  // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes
-  if ((e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
+  // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes.
+  // While the previous step's int_ts is queued there (side_pending), the
+  // side stream is already past everything of c.stream's that they read
+  // (it forked after the mode coupling); otherwise it forks here.
+  if (!c.side_pending && (e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
   LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
   LAUNCH_COLS_ON(c.side, k_adp<E>, sizeof(E), ge, P<E>(f.alpha), P<E>(f.adp));
   {
@@ -1394,9 +1401,10 @@ std::string step_pre_t(Ctx& c) {
                        h.bgra[1], h.bgra[2], h.bgra[3]);
     if ((eb = hipGetLastError()) != hipSuccess) return hip_err(eb, "k_bphi");
   }
-  // int_readyc
-  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
-  LAUNCH_UPW(nullptr, nullptr, nullptr);
+  // int_readyc (its sqp and upwelling in buffers of its own: the previous
+  // step's int_ts may still be reading sqp and wm on the side stream)
+  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqr, (double*)nullptr, (double*)nullptr);
+  LAUNCH_UPW(c.stream, h.sqr, f.wmr, nullptr, nullptr, nullptr);
   c.ab = (c.ab + 1) % 3;
   R* au = P<R>(f.AU[c.ab]);
   R* av = P<R>(f.AV[c.ab]);
@@ -1407,7 +1415,7 @@ std::string step_pre_t(Ctx& c) {
   {
     const int G = c.ni - 2;
     const unsigned nb = (unsigned)(((G + kAdvTX - 1) / kAdvTX) * ((c.nj + kAdvTY - 1) / kAdvTY) * c.nk);
-    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wm), h.chc,
+    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wmr), h.chc,
                        h.schr, au, av);
     hipError_t ea = hipGetLastError();
     if (ea != hipSuccess) return hip_err(ea, "k_adv_tiles");
@@ -1440,7 +1448,7 @@ std::string step_pre_t(Ctx& c) {
     if (eu != hipSuccess) return hip_err(eu, "k_ut");
   }
   if ((e = join_side(c, 0)) != hipSuccess) return hip_err(e, "join");
-  c.side_pending = false;  // the join follows the previous step's convection too
+  c.side_pending = false;  // the join follows the previous step's int_ts too
   return "";
 }
 
@@ -1475,13 +1483,6 @@ std::string step_post_t(Ctx& c) {
     ca.bc = (double)c.dtbc;
     LAUNCH((k_clin<R, E>), n3, ca);
   }
-  // int_ssh on the side stream once int_clin (which reads alpha) is done: it
-  // reads T, S, mch and writes alpha, ssh, acssh, none of which the mode
-  // coupling and smag touch
-  hipError_t e;
-  if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
-  LAUNCH_COLS_ON(c.side, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
-  c.n_ssh += 1;
   {
     const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(double);
     hipLaunchKernelGGL(k_clin_cpl, dim3((unsigned)((c.n2 + kColCW - 1) / kColCW)), dim3(kColCW, kColKL), lds,
@@ -1489,6 +1490,17 @@ std::string step_post_t(Ctx& c) {
     hipError_t ep = hipGetLastError();
     if (ep != hipSuccess) return hip_err(ep, "k_clin_cpl");
   }
+  // int_ssh, int_ts and convect on the side stream once the mode coupling
+  // is done.  Until the next step's int_readyc join nothing on c.stream
+  // reads or writes what they touch (alpha, ssh, acssh, wm, sqp, acw, T, S,
+  // prho, the T/S/rho accumulators), so they overlap smag and the next
+  // step's int_readyc; a daily forcing update (which writes bnd and prho)
+  // settles the side stream first.
+  hipError_t e;
+  if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
+  c.side_pending = true;
+  LAUNCH_COLS_ON(c.side, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
+  c.n_ssh += 1;
   {
     const int G = c.ni - 2;
     const unsigned nb = (unsigned)(((G + kSmagTX - 1) / kSmagTX) * ((c.nj + kSmagTY - 1) / kSmagTY) * c.nk);
@@ -1498,9 +1510,8 @@ std::string step_post_t(Ctx& c) {
     if (es != hipSuccess) return hip_err(es, "k_smag");
   }
   c.n_uv += 1;
-  if ((e = join_side(c, 1)) != hipSuccess) return hip_err(e, "join");
   // int_ts
-  LAUNCH_UPW(P<R>(f.acw), P<E>(f.alpha), h.mch2);
+  LAUNCH_UPW(c.side, h.sqp, f.wm, P<R>(f.acw), P<E>(f.alpha), h.mch2);
   c.n_w += 1;
   {
     TsArgs<R> a;
@@ -1509,7 +1520,7 @@ std::string step_post_t(Ctx& c) {
     a.t1 = P<double>(f.T[1 - c.cur]); a.s1 = P<double>(f.S[1 - c.cur]); a.bc = (double)c.dtbc;
     const int G = c.ni - 2;
     const unsigned nb = (unsigned)(((G + kTsTX - 1) / kTsTX) * ((c.nj + kTsTY - 1) / kTsTY) * c.nk);
-    hipLaunchKernelGGL(k_ts_tiles<R>, dim3(nb), dim3(kTsTX * kTsTY), 0, c.stream, a);
+    hipLaunchKernelGGL(k_ts_tiles<R>, dim3(nb), dim3(kTsTX * kTsTY), 0, c.side, a);
     hipError_t et = hipGetLastError();
     if (et != hipSuccess) return hip_err(et, "k_ts_tiles");
   }
@@ -1521,11 +1532,8 @@ std::string step_post_t(Ctx& c) {
           hipFuncSetAttribute((const void*)k_conv<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (ea != hipSuccess) return hip_err(ea, "k_conv LDS size");
     }
-    // convect and the T/S/rho accumulators on the side stream: until the
-    // next step's int_readyc join nothing on c.stream reads or writes T, S,
-    // prho, chc or the accumulators, so it overlaps the next step's start
-    if ((e = fork_side(c, 2)) != hipSuccess) return hip_err(e, "fork");
-    c.side_pending = true;
+    // convect and the T/S/rho accumulators (reads chc, which nothing
+    // writes before the next subcycle)
     hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kConvCW - 1) / kConvCW)), dim3(kConvCW, kConvKL), lds,
                        c.side, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,

@@ -717,9 +717,8 @@ std::string trop_configure(Ctx& c) {
 // process-wide order of subcycle launches per device (never destroyed: the
 // events outlive every context)
 struct TropChain {
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  int last = 0;
-  bool used = false;
+  hipEvent_t ev = nullptr;
+  hipStream_t last = nullptr;  // the stream of the latest launch
 };
 std::mutex g_trop_mutex;
 std::map<int, TropChain> g_trop_chain;
@@ -800,28 +799,36 @@ TropTileArgs trop_args(const Ctx& c) {
 // the other's tiles need, so launches of all contexts of this process on a
 // device are chained: each waits for the previous one.  (A cooperative
 // launch would add nothing but its per-process global-wave-sync resource,
-// which fails when another process on the device holds it.)
+// which fails when another process on the device holds it.)  Launches from
+// one stream are ordered by the stream; only a launch from another stream
+// than the latest one records that stream's progress (all it has queued,
+// which ends with or after the latest launch) and waits for it, so a single
+// context pays no event per step.
 template <typename F>
 std::string chained_launch(int device, hipStream_t st, F launch) {
   std::lock_guard<std::mutex> lock(g_trop_mutex);
   TropChain& ch = g_trop_chain[device];
-  if (!ch.ev[0]) {
-    for (hipEvent_t& ev : ch.ev) {
-      hipError_t ee = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (ch.last && ch.last != st) {
+    if (!ch.ev) {
+      hipError_t ee = hipEventCreateWithFlags(&ch.ev, hipEventDisableTiming);
       if (ee != hipSuccess) return std::string("hipEventCreate: ") + hipGetErrorString(ee);
     }
-  }
-  if (ch.used) {
-    hipError_t ew = hipStreamWaitEvent(st, ch.ev[ch.last], 0);
-    if (ew != hipSuccess) return std::string("hipStreamWaitEvent: ") + hipGetErrorString(ew);
+    hipError_t e = hipEventRecord(ch.ev, ch.last);
+    if (e != hipSuccess) return std::string("hipEventRecord: ") + hipGetErrorString(e);
+    if ((e = hipStreamWaitEvent(st, ch.ev, 0)) != hipSuccess)
+      return std::string("hipStreamWaitEvent: ") + hipGetErrorString(e);
   }
   std::string err = launch();
   if (!err.empty()) return err;
-  ch.last = 1 - ch.last;
-  hipError_t e = hipEventRecord(ch.ev[ch.last], st);
-  if (e != hipSuccess) return std::string("hipEventRecord: ") + hipGetErrorString(e);
-  ch.used = true;
+  ch.last = st;
   return "";
+}
+
+// a stream about to be destroyed leaves the chain
+void trop_chain_forget(int device, hipStream_t st) {
+  std::lock_guard<std::mutex> lock(g_trop_mutex);
+  auto it = g_trop_chain.find(device);
+  if (it != g_trop_chain.end() && it->second.last == st) it->second.last = nullptr;
 }
 
 std::string launch_trop_kernel(Ctx& c) {
@@ -996,6 +1003,7 @@ void ensemble_destroy(Ensemble* en) {
   if (!en) return;
   (void)hipSetDevice(en->device);
   if (en->stream) (void)hipStreamSynchronize(en->stream);
+  if (en->stream) trop_chain_forget(en->device, en->stream);
   for (hipEvent_t ev : en->ready) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : en->done) (void)hipEventDestroy(ev);
   if (en->dargs) (void)hipFree(en->dargs);
