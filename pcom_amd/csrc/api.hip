@@ -326,6 +326,8 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = make(*c, &f.am, n3, fl, nullptr, AM_C))) return rc;
   if ((rc = make(*c, &f.alpha, n3, el, nullptr, (double)(1.0f / 1000.0f)))) return rc;
   if ((rc = make(*c, &f.adp, n3, el, nullptr))) return rc;
+  if ((rc = make(*c, &f.alphn, n3, el, nullptr))) return rc;
+  if ((rc = make(*c, &f.adpn, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.wm, n3p, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.wmr, n3p, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.prho1, n3, el, nullptr))) return rc;
@@ -446,10 +448,13 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
   if (!ctx || nsteps < 0) return fail(PCOM_ERR_ARG, "pcom_step: bad argument");
   Ctx* c = C(ctx);
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  c->alpha_next = false;
   for (int s = 0; s < nsteps; ++s) {
+    c->more_steps = s + 1 < nsteps;
     std::string e = launch_step(*c);
     if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);
   }
+  c->more_steps = false;
   // the last step's convection is part of this call's work
   std::string e = settle_side(*c);
   if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);

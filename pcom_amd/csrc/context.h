@@ -25,6 +25,7 @@ struct Fields3 {
                             // AU[ab-1], aupp = AU[ab-2]
   void *fx, *fy, *am, *alpha, *adp, *wm, *prho1, *prho2;
   void* wmr;  // int_readyc's upwelling; wm holds int_ts's (the two overlap)
+  void *alphn, *adpn;  // the next step's den_alpha and int_pgra adp, formed by convect
   void *act, *acs, *acr, *acu, *acv, *acw;
   void *km, *kh;
 };
@@ -56,6 +57,12 @@ struct Ctx {
   // convect (the end of a step) runs on `side` and overlaps the next step's
   // int_readyc; set while it has not been joined back into `stream`
   bool side_pending = false;
+  // set by the caller's step loop while another step of the same call
+  // follows: convect then also forms that step's alpha and adp (alphn,
+  // adpn) and sets alpha_next, and the next step takes them instead of
+  // launching den_alpha and int_tobot
+  bool more_steps = false;
+  bool alpha_next = false;
   int precision = PCOM_FP64;
   int rst = 0;
   int dtbt = 60, dtbc = 3600;
@@ -102,7 +109,8 @@ std::string launch_check_finite(Ctx& c, int* bad);
 // host pointers of o, then accumulators and counts reset
 std::string launch_output_means(Ctx& c, pcom_means* o, double* stage);
 std::string trop_configure(Ctx& c);              // trop.hip: size the subcycle tiles
-std::string launch_trop(Ctx& c);                 // trop.hip: int_trop
+std::string launch_trop(Ctx& c);                 // trop.hip: int_trop (k_vz, then the subcycle)
+std::string launch_trop_kernel(Ctx& c);          // trop.hip: the subcycle alone
 void trop_chain_forget(int device, hipStream_t st);  // trop.hip: before a stream is destroyed
 std::string launch_vz(Ctx& c);                   // trop.hip: the subcycle's set-up kernel
 std::string ensemble_create(const std::vector<Ctx*>& members, Ensemble** out);

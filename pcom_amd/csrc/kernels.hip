@@ -144,7 +144,21 @@ __global__ void k_alpha(Grid<R> g, const double* __restrict__ t, const double* _
 
 // int_tobot (mod_op.f90:927-944): adp(k) = t(k)/2 + sum(t(k+1:nk)),
 // t = alpha*dpr*msk; the suffix sum is accumulated in increasing k as
-// Fortran's sum() does, so FP64 results are the reference's bits.
+// Fortran's sum() does, so FP64 results are the reference's bits.  t holds
+// the column's terms at a stride of `st` between levels.
+template <typename R>
+__device__ __forceinline__ R tobot(const R* t, int st, int k, int nk) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    R s = 0;
+    for (int kk = k + 1; kk <= nk; ++kk) s += t[st * (kk - 1)];
+    return t[st * (k - 1)] * (R)0.5 + s;
+  } else {
+    double s = 0;
+    for (int kk = k + 1; kk <= nk; ++kk) s += (double)t[st * (kk - 1)];
+    return (R)((double)t[st * (k - 1)] * 0.5 + s);
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kColCW * kColKL) k_adp(Grid<R> g, const R* __restrict__ alpha, R* adp) {
   extern __shared__ unsigned char adp_lds[];
@@ -158,17 +172,7 @@ __global__ void __launch_bounds__(kColCW * kColKL) k_adp(Grid<R> g, const R* __r
   }
   __syncthreads();
   if (p >= g.n2) return;
-  for (int k = 1 + kl; k <= nk; k += kColKL) {
-    if constexpr (sizeof(R) == sizeof(double)) {
-      R s = 0;
-      for (int kk = k + 1; kk <= nk; ++kk) s += t[kk - 1][tx];
-      adp[p + g.n2 * (k - 1)] = t[k - 1][tx] * (R)0.5 + s;
-    } else {
-      double s = 0;
-      for (int kk = k + 1; kk <= nk; ++kk) s += (double)t[kk - 1][tx];
-      adp[p + g.n2 * (k - 1)] = (R)((double)t[k - 1][tx] * 0.5 + s);
-    }
-  }
+  for (int k = 1 + kl; k <= nk; k += kColKL) adp[p + g.n2 * (k - 1)] = tobot(&t[0][tx], kColCW, k, nk);
 }
 
 // int_pgra's face integrals (mod_int.f90:72-86 with op_vint_ns / op_vint_ew,
@@ -1095,12 +1099,18 @@ constexpr int kConvCW = 32, kConvKL = 8;
 // searches overlap); the kConvKL k-lanes stage t, s (FP64 state) and
 // prho (E) of each column into LDS (dynamic: 2*nk*kConvCW doubles + 2*nk*kConvCW
 // E), lane 0 runs the convective adjustment, then all lanes write back and
-// accumulate act, acs and acr (mod_int.f90:493-505) in R.
+// accumulate act, acs and acr (mod_int.f90:493-505) in R.  The 1/rho that
+// acr takes is the next step's den_alpha (mod_den.f90, main.f90:205: the
+// same T, S and chc, which nothing changes in between); with `alphn` set
+// the kernel stores it there (alpha's own value at levels below the
+// bottom, which den_alpha leaves as they are) and int_tobot's adp of it in
+// `adpn`, in the prho1 LDS array the search no longer needs.
 template <typename R, typename E>
 __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const double* __restrict__ dpr, double* tc,
                                                     double* sc, const E* __restrict__ prho1,
                                                     const E* __restrict__ prho2, const double* __restrict__ chc,
-                                                    R* act, R* acs, R* acr) {
+                                                    R* act, R* acs, R* acr, const E* __restrict__ alpha,
+                                                    E* alphn, E* adpn) {
   extern __shared__ double conv_lds[];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
   double* lt = conv_lds;
@@ -1158,21 +1168,35 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
     }
   }
   __syncthreads();  // the column's new T/S are visible to the other k-lanes
-  if (!ok) return;
-  const E ch = (E)chc[p];
-  for (int k = 1 + kl; k <= nk; k += kConvKL) {
-    const long q = p + g.n2 * (k - 1);
-    const int o = (k - 1) * kConvCW + tx;
-    const double t = lt[o], s = ls[o];
-    tc[q] = t;
-    sc[q] = s;
-    act[q] = act[q] + (R)t;
-    acs[q] = acs[q] + (R)s;
-    if (nlev >= k) {
-      E wk = (E)1.0 / den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]);
-      acr[q] = acr[q] + (R)((E)1.0 / wk);
+  const bool fuse = alphn != nullptr;
+  if (ok) {
+    const E ch = (E)chc[p];
+    for (int k = 1 + kl; k <= nk; k += kConvKL) {
+      const long q = p + g.n2 * (k - 1);
+      const int o = (k - 1) * kConvCW + tx;
+      const double t = lt[o], s = ls[o];
+      tc[q] = t;
+      sc[q] = s;
+      act[q] = act[q] + (R)t;
+      acs[q] = acs[q] + (R)s;
+      if (nlev >= k) {
+        E wk = (E)1.0 / den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]);
+        acr[q] = acr[q] + (R)((E)1.0 / wk);
+        if (fuse) {
+          alphn[q] = wk;
+          l1[o] = wk * g.dpr2[k - 1] * (E)1;
+        }
+      } else if (fuse) {
+        const E a = alpha[q];
+        alphn[q] = a;
+        l1[o] = a * g.dpr2[k - 1] * (E)0;
+      }
     }
   }
+  if (!fuse) return;
+  __syncthreads();  // the column's int_tobot terms are in l1
+  if (!ok) return;
+  for (int k = 1 + kl; k <= nk; k += kConvKL) adpn[p + g.n2 * (k - 1)] = tobot(l1 + tx, kConvCW, k, nk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1388,8 +1412,14 @@ std::string step_pre_t(Ctx& c) {
   // side stream is already past everything of c.stream's that they read
   // (it forked after the mode coupling); otherwise it forks here.
   if (!c.side_pending && (e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
-  LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
-  LAUNCH_COLS_ON(c.side, k_adp<E>, sizeof(E), ge, P<E>(f.alpha), P<E>(f.adp));
+  if (c.alpha_next) {  // the previous step's convect formed them
+    std::swap(f.alpha, f.alphn);
+    std::swap(f.adp, f.adpn);
+    c.alpha_next = false;
+  } else {
+    LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
+    LAUNCH_COLS_ON(c.side, k_adp<E>, sizeof(E), ge, P<E>(f.alpha), P<E>(f.adp));
+  }
   {
     const size_t lds = 4 * (size_t)c.nk * kColCW * sizeof(E);
     hipError_t eb = hipSuccess;
@@ -1447,6 +1477,10 @@ std::string step_pre_t(Ctx& c) {
     hipError_t eu = hipGetLastError();
     if (eu != hipSuccess) return hip_err(eu, "k_ut");
   }
+  // the subcycle's set-up kernel needs nothing from the side stream: it
+  // runs while the join waits
+  std::string ev = launch_vz(c);
+  if (!ev.empty()) return ev;
   if ((e = join_side(c, 0)) != hipSuccess) return hip_err(e, "join");
   c.side_pending = false;  // the join follows the previous step's int_ts too
   return "";
@@ -1537,9 +1571,11 @@ std::string step_post_t(Ctx& c) {
     hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kConvCW - 1) / kConvCW)), dim3(kConvCW, kConvKL), lds,
                        c.side, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
-                       P<R>(f.act), P<R>(f.acs), P<R>(f.acr));
+                       P<R>(f.act), P<R>(f.acs), P<R>(f.acr), (const E*)P<E>(f.alpha),
+                       c.more_steps ? P<E>(f.alphn) : (E*)nullptr, P<E>(f.adpn));
     hipError_t e2 = hipGetLastError();
     if (e2 != hipSuccess) return hip_err(e2, "k_conv");
+    c.alpha_next = c.more_steps;
   }
   c.n_ts += 1;
   if (c.timing) {
@@ -1645,7 +1681,7 @@ std::string settle_side(Ctx& c) {
 std::string launch_step(Ctx& c) {
   std::string e = launch_step_pre(c);
   if (!e.empty()) return e;
-  e = c.nn ? nn_trop_step(c, launch_trop) : launch_trop(c);
+  e = c.nn ? nn_trop_step(c, launch_trop_kernel) : launch_trop_kernel(c);
   if (!e.empty()) return e;
   c.n_ch += 1;
   return launch_step_post(c);

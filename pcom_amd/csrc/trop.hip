@@ -1024,20 +1024,20 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
   (void)hipStreamSynchronize(en.stream);
   std::string err = ensemble_plan(en);
   if (!err.empty()) return err;
+  for (Ctx* c : en.m) c->alpha_next = false;
   for (int s = 0; s < nsteps; ++s) {
     // members: the step up to the subcycle
     for (size_t q = 0; q < en.m.size(); ++q) {
       Ctx& c = *en.m[q];
       if (!(err = launch_step_pre(c)).empty()) return err;
       if (c.nn) {  // a member with a surrogate runs its own barotropic part
-        if (!(err = nn_trop_step(c, launch_trop)).empty()) return err;
+        if (!(err = nn_trop_step(c, launch_trop_kernel)).empty()) return err;
         continue;
       }
       if (c.trop_launches) {  // so does one whose subcycle runs in launch mode
-        if (!(err = launch_trop(c)).empty()) return err;
+        if (!(err = launch_trop_kernel(c)).empty()) return err;
         continue;
       }
-      if (!(err = launch_vz(c)).empty()) return err;
       if ((e = hipEventRecord(en.ready[q], c.stream)) != hipSuccess) return hip_error_string(e, "hipEventRecord");
     }
     // packed subcycles
@@ -1078,7 +1078,9 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
       if (en.pack_of[q] >= 0 && (e = hipStreamWaitEvent(c.stream, en.done[en.pack_of[q]], 0)) != hipSuccess)
         return hip_error_string(e, "hipStreamWaitEvent");
       c.n_ch += 1;
+      c.more_steps = s + 1 < nsteps;
       if (!(err = launch_step_post(c)).empty()) return err;
+      c.more_steps = false;
     }
   }
   for (Ctx* c : en.m)
