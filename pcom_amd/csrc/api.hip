@@ -329,7 +329,6 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = make(*c, &f.alphn, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.adpn, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.wm, n3p, fl, nullptr))) return rc;
-  if ((rc = make(*c, &f.wmr, n3p, fl, nullptr))) return rc;
   if ((rc = make(*c, &f.prho1, n3, el, nullptr))) return rc;
   if ((rc = make(*c, &f.prho2, n3, el, nullptr))) return rc;
   void** acc[] = {&f.act, &f.acs, &f.acr, &f.acu, &f.acv, &f.acw};
@@ -344,7 +343,7 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   for (double** a : ones)
     if ((rc = make2(*c, a, n2, nullptr, 1.0))) return rc;
   double** zeros[] = {&h.acch, &h.acssh, &h.ssh, &h.buc, &h.bvc, &h.bup, &h.bvp, &h.but, &h.bvt, &h.pax, &h.pay,
-                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.sqr, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1],
+                      &h.vz1, &h.vz2, &h.am1, &h.schr, &h.sqp, &h.schs, &h.schm, &h.xc[0], &h.xc[1], &h.xu[0], &h.xu[1], &h.xv[0], &h.xv[1],
                       &h.xacc[0], &h.xacc[1], &h.xacc[2], &h.xacc[3]};
   for (double** a : zeros)
     if ((rc = make2(*c, a, n2, nullptr))) return rc;
@@ -448,7 +447,7 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
   if (!ctx || nsteps < 0) return fail(PCOM_ERR_ARG, "pcom_step: bad argument");
   Ctx* c = C(ctx);
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
-  c->alpha_next = false;
+  c->next_ready = false;
   for (int s = 0; s < nsteps; ++s) {
     c->more_steps = s + 1 < nsteps;
     std::string e = launch_step(*c);

@@ -24,7 +24,6 @@ struct Fields3 {
   void *AU[3], *AV[3];      // AB3 advection ring: auc = AU[ab], aup =
                             // AU[ab-1], aupp = AU[ab-2]
   void *fx, *fy, *am, *alpha, *adp, *wm, *prho1, *prho2;
-  void* wmr;  // int_readyc's upwelling; wm holds int_ts's (the two overlap)
   void *alphn, *adpn;  // the next step's den_alpha and int_pgra adp, formed by convect
   void *act, *acs, *acr, *acu, *acv, *acw;
   void *km, *kh;
@@ -39,7 +38,6 @@ struct Fields2 {
   double *graphihx, *graphihy;
   double *vz1, *vz2, *am1;                      // subcycle statics
   double *schr, *sqp, *schs, *schm;             // sqrt(ter(ch)) factors on g3
-  double* sqr;  // int_readyc's sqp (int_ts's, in sqp, may still be in use)
   double *xc[2], *xu[2], *xv[2];                // subcycle tile exchange
   double *xacc[4];  // subcycle accumulators between launches (PCOM_TROP_LAUNCHES)
   double *red;                                  // reduction scratch
@@ -58,11 +56,12 @@ struct Ctx {
   // int_readyc; set while it has not been joined back into `stream`
   bool side_pending = false;
   // set by the caller's step loop while another step of the same call
-  // follows: convect then also forms that step's alpha and adp (alphn,
-  // adpn) and sets alpha_next, and the next step takes them instead of
-  // launching den_alpha and int_tobot
+  // follows: the step then also forms what the next step's start would
+  // compute again from unchanged inputs (schr; the upwelling, which int_ts's
+  // already is; convect's den_alpha and adp, in alphn and adpn) and sets
+  // next_ready, and the next step starts from them
   bool more_steps = false;
-  bool alpha_next = false;
+  bool next_ready = false;
   int precision = PCOM_FP64;
   int rst = 0;
   int dtbt = 60, dtbc = 3600;

@@ -18,8 +18,8 @@
 // kernels give each column several k-lanes that evaluate the per-level
 // terms in parallel; one lane then walks k in the reference's summation
 // order, so every column sum is accumulated exactly as Fortran's sum()
-// does.  A second stream runs den_alpha/int_pgra beside int_readyc and
-// int_ssh beside the mode coupling (step_pre_t, step_post_t).
+// does.  A second stream runs int_ssh, int_ts, convect and int_pgra beside
+// smag and the next step's int_readyc (step_pre_t, step_post_t).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -244,16 +244,12 @@ __global__ void __launch_bounds__(kColCW * kColKL)
   c3[p] = gyw / phw;
 }
 
-// upwelling (mod_int.f90:626-647): wm from the continuity equation;
-// with `acw` set, also int_ts's w accumulator (mod_int.f90:458-461).  The
-// result goes to `wout`: levels 2..nk, and when wout is not wm also the
-// levels 1 and nk+1 the upwelling leaves as they are in wm.
-template <typename R, typename E>
+// upwelling (mod_int.f90:626-647): wm (levels 2..nk) from the continuity
+// equation
+template <typename R>
 __global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double* __restrict__ uc,
                                                          const double* __restrict__ vc,
-                                                         const double* __restrict__ sqp, const R* wm, R* wout,
-                                                         R* acw, const E* __restrict__ alpha,
-                                                         const double* __restrict__ mch2) {
+                                                         const double* __restrict__ sqp, R* wm) {
   // one LDS column array [nk][kColCW]: the divergence per level, then in
   // place the upwelling
   extern __shared__ unsigned char upw_lds[];
@@ -290,14 +286,7 @@ __global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double
   }
   __syncthreads();
   if (!ok) return;
-  const bool copy = wout != wm;
-  if (copy && kl == 0) wout[p + g.n2 * nk] = wm[p + g.n2 * nk];
-  for (int k = 1 + kl; k <= nk; k += kColKL) {
-    long q = p + g.n2 * (k - 1);
-    R w = w3[k - 1][tx];
-    if (k >= 2 || copy) wout[q] = w;
-    if (acw) acw[q] = acw[q] + w * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
-  }
+  for (int k = 2 + kl; k <= nk; k += kColKL) wm[p + g.n2 * (k - 1)] = w3[k - 1][tx];
 }
 
 // int_readyc's sch (mod_int.f90:107-109), precomputed by k_sch
@@ -1099,7 +1088,9 @@ constexpr int kConvCW = 32, kConvKL = 8;
 // searches overlap); the kConvKL k-lanes stage t, s (FP64 state) and
 // prho (E) of each column into LDS (dynamic: 2*nk*kConvCW doubles + 2*nk*kConvCW
 // E), lane 0 runs the convective adjustment, then all lanes write back and
-// accumulate act, acs and acr (mod_int.f90:493-505) in R.  The 1/rho that
+// accumulate act, acs and acr (mod_int.f90:493-505) in R, and int_ts's w
+// accumulator acw (mod_int.f90:458-461: wm * alpha / (g mch2) with
+// int_ssh's alpha; every point once, as convect visits them).  The 1/rho that
 // acr takes is the next step's den_alpha (mod_den.f90, main.f90:205: the
 // same T, S and chc, which nothing changes in between); with `alphn` set
 // the kernel stores it there (alpha's own value at levels below the
@@ -1109,7 +1100,8 @@ template <typename R, typename E>
 __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const double* __restrict__ dpr, double* tc,
                                                     double* sc, const E* __restrict__ prho1,
                                                     const E* __restrict__ prho2, const double* __restrict__ chc,
-                                                    R* act, R* acs, R* acr, const E* __restrict__ alpha,
+                                                    R* act, R* acs, R* acr, R* acw, const R* __restrict__ wm,
+                                                    const double* __restrict__ mch2, const E* __restrict__ alpha,
                                                     E* alphn, E* adpn) {
   extern __shared__ double conv_lds[];
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
@@ -1179,6 +1171,7 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
       sc[q] = s;
       act[q] = act[q] + (R)t;
       acs[q] = acs[q] + (R)s;
+      acw[q] = acw[q] + wm[q] * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
       if (nlev >= k) {
         E wk = (E)1.0 / den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]);
         acr[q] = acr[q] + (R)((E)1.0 / wk);
@@ -1357,9 +1350,7 @@ const Grid<float>& grid_of<float>(const Ctx& c) {
     if (e_ != hipSuccess) return hip_err(e_, #kern);                                \
   } while (0)
 
-#define LAUNCH_UPW(st, sq, wout, acw, alpha, mch2)                                  \
-  LAUNCH_COLS_ON(st, (k_upw<R, E>), sizeof(R), g, U, V, sq, P<R>(f.wm), (R*)(wout), (R*)(acw), \
-                 (const E*)(alpha), (const double*)(mch2))
+#define LAUNCH_UPW() LAUNCH_COLS_ON(c.stream, k_upw<R>, sizeof(R), g, U, V, h.sqp, P<R>(f.wm))
 
 // work on c.side after everything queued so far on c.stream (fork), and
 // c.stream's later work after c.side's (join)
@@ -1412,10 +1403,11 @@ std::string step_pre_t(Ctx& c) {
   // side stream is already past everything of c.stream's that they read
   // (it forked after the mode coupling); otherwise it forks here.
   if (!c.side_pending && (e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
-  if (c.alpha_next) {  // the previous step's convect formed them
+  const bool ready = c.next_ready;  // the previous step formed schr, wm, alpha, adp
+  c.next_ready = false;
+  if (ready) {
     std::swap(f.alpha, f.alphn);
     std::swap(f.adp, f.adpn);
-    c.alpha_next = false;
   } else {
     LAUNCH_ON(c.side, k_alpha<E>, n3, ge, T, S, h.chc, P<E>(f.alpha));
     LAUNCH_COLS_ON(c.side, k_adp<E>, sizeof(E), ge, P<E>(f.alpha), P<E>(f.adp));
@@ -1431,10 +1423,13 @@ std::string step_pre_t(Ctx& c) {
                        h.bgra[1], h.bgra[2], h.bgra[3]);
     if ((eb = hipGetLastError()) != hipSuccess) return hip_err(eb, "k_bphi");
   }
-  // int_readyc (its sqp and upwelling in buffers of its own: the previous
-  // step's int_ts may still be reading sqp and wm on the side stream)
-  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqr, (double*)nullptr, (double*)nullptr);
-  LAUNCH_UPW(c.stream, h.sqr, f.wmr, nullptr, nullptr, nullptr);
+  // int_readyc.  Its sqrt(ch) factors and upwelling come from the same
+  // chc, chp, mch, uc and vc as the previous step's int_ts ones, which that
+  // step formed (and its int_ts, on the side stream, may still read)
+  if (!ready) {
+    LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
+    LAUNCH_UPW();
+  }
   c.ab = (c.ab + 1) % 3;
   R* au = P<R>(f.AU[c.ab]);
   R* av = P<R>(f.AV[c.ab]);
@@ -1445,7 +1440,7 @@ std::string step_pre_t(Ctx& c) {
   {
     const int G = c.ni - 2;
     const unsigned nb = (unsigned)(((G + kAdvTX - 1) / kAdvTX) * ((c.nj + kAdvTY - 1) / kAdvTY) * c.nk);
-    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wmr), h.chc,
+    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wm), h.chc,
                        h.schr, au, av);
     hipError_t ea = hipGetLastError();
     if (ea != hipSuccess) return hip_err(ea, "k_adv_tiles");
@@ -1506,7 +1501,8 @@ std::string step_post_t(Ctx& c) {
   R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
   (void)n3;
   if (c.timing) (void)hipEventRecord(c.ev[2], c.stream);
-  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, (double*)nullptr, h.sqp, h.schs, h.schm);
+  // (schr too when another step follows: its int_readyc would form the same)
+  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, c.more_steps ? h.schr : (double*)nullptr, h.sqp, h.schs, h.schm);
   // int_clin
   {
     ClinArgs<E> ca;
@@ -1524,12 +1520,15 @@ std::string step_post_t(Ctx& c) {
     hipError_t ep = hipGetLastError();
     if (ep != hipSuccess) return hip_err(ep, "k_clin_cpl");
   }
-  // int_ssh, int_ts and convect on the side stream once the mode coupling
-  // is done.  Until the next step's int_readyc join nothing on c.stream
-  // reads or writes what they touch (alpha, ssh, acssh, wm, sqp, acw, T, S,
-  // prho, the T/S/rho accumulators), so they overlap smag and the next
-  // step's int_readyc; a daily forcing update (which writes bnd and prho)
-  // settles the side stream first.
+  // int_ts's upwelling (its w accumulator is convect's)
+  LAUNCH_UPW();
+  // int_ssh, int_ts and convect on the side stream once the upwelling is
+  // done.  Until the next step's int_readyc join nothing on c.stream writes
+  // what they read (T, S, uc, vc, wm, alpha, mch, mch2, chc, schs, bnd, prho)
+  // or touches what they write (alpha, ssh, acssh, T, S, the T/S/rho/w
+  // accumulators, alphn, adpn), so they overlap smag and the next step's
+  // int_readyc; a daily forcing update (which writes bnd and prho) settles
+  // the side stream first.
   hipError_t e;
   if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
   c.side_pending = true;
@@ -1545,7 +1544,6 @@ std::string step_post_t(Ctx& c) {
   }
   c.n_uv += 1;
   // int_ts
-  LAUNCH_UPW(c.side, h.sqp, f.wm, P<R>(f.acw), P<E>(f.alpha), h.mch2);
   c.n_w += 1;
   {
     TsArgs<R> a;
@@ -1571,11 +1569,11 @@ std::string step_post_t(Ctx& c) {
     hipLaunchKernelGGL((k_conv<R, E>), dim3((unsigned)((c.n2 + kConvCW - 1) / kConvCW)), dim3(kConvCW, kConvKL), lds,
                        c.side, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
-                       P<R>(f.act), P<R>(f.acs), P<R>(f.acr), (const E*)P<E>(f.alpha),
-                       c.more_steps ? P<E>(f.alphn) : (E*)nullptr, P<E>(f.adpn));
+                       P<R>(f.act), P<R>(f.acs), P<R>(f.acr), P<R>(f.acw), (const R*)P<R>(f.wm), h.mch2,
+                       (const E*)P<E>(f.alpha), c.more_steps ? P<E>(f.alphn) : (E*)nullptr, P<E>(f.adpn));
     hipError_t e2 = hipGetLastError();
     if (e2 != hipSuccess) return hip_err(e2, "k_conv");
-    c.alpha_next = c.more_steps;
+    c.next_ready = c.more_steps;
   }
   c.n_ts += 1;
   if (c.timing) {

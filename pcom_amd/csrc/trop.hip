@@ -1024,7 +1024,7 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
   (void)hipStreamSynchronize(en.stream);
   std::string err = ensemble_plan(en);
   if (!err.empty()) return err;
-  for (Ctx* c : en.m) c->alpha_next = false;
+  for (Ctx* c : en.m) c->next_ready = false;
   for (int s = 0; s < nsteps; ++s) {
     // members: the step up to the subcycle
     for (size_t q = 0; q < en.m.size(); ++q) {
