@@ -658,11 +658,16 @@ struct ClinArgs {
   double bc;
 };
 
+// Workgroups take kBlock points of one level, levels fastest: the nk
+// workgroups of a stretch of columns run together on one XCD, whose L2 then
+// serves the 2-D fields and the neighbouring rows every level reads.
 template <typename R, typename E>
 __global__ void k_clin(ClinArgs<E> a) {
   const Grid<E>& g = a.g;
-  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= g.n2 * g.nk) return;
+  const unsigned b = xcd_block();
+  const long p2 = (long)(b / g.nk) * blockDim.x + threadIdx.x;
+  if (p2 >= g.n2) return;
+  long p = p2 + g.n2 * (long)(b % g.nk);
   int i, j, k;
   ijk(g, p, i, j, k);
   const double c_tc = (double)(23.0f / 12.0f), c_tp = (double)(-(16.0f / 12.0f)), c_tpp = (double)(5.0f / 12.0f);
@@ -1511,7 +1516,9 @@ std::string step_post_t(Ctx& c) {
     ca.fx = f.fx; ca.fy = f.fy; ca.adp = P<E>(f.adp); ca.alpha = P<E>(f.alpha);
     ca.mch = h.mch; ca.schm = h.schm; ca.pax = h.pax; ca.pay = h.pay; ca.cor = c.gd.cor;
     ca.bc = (double)c.dtbc;
-    LAUNCH((k_clin<R, E>), n3, ca);
+    hipLaunchKernelGGL((k_clin<R, E>), dim3((unsigned)(nblocks(n2) * c.nk)), dim3(kBlock), 0, c.stream, ca);
+    hipError_t ec = hipGetLastError();
+    if (ec != hipSuccess) return hip_err(ec, "k_clin");
   }
   {
     const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(double);
