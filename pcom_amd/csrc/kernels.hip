@@ -1072,13 +1072,18 @@ struct Column {
   __device__ double rhob(int k) const {
     return (double)p1[st * k] * t[st * (k - 1)] + (double)p2[st * k] * s[st * (k - 1)];
   }
+  // the three sums of the reference's three loops in one pass: each is
+  // still accumulated over k = na..nb in increasing k
   __device__ void mix(int na, int nb) {
     double sumt = 0, sumd = 0, sums = 0;
-    for (int k = na; k <= nb; ++k) sumt += t[st * (k - 1)] * dpr[k - 1];
-    for (int k = na; k <= nb; ++k) sumd += dpr[k - 1];
-    double tv = sumt / sumd;
-    for (int k = na; k <= nb; ++k) sums += s[st * (k - 1)] * dpr[k - 1];
-    double sv = sums / sumd;
+    for (int k = na; k <= nb; ++k) {
+      const double d = dpr[k - 1];
+      sumt += t[st * (k - 1)] * d;
+      sumd += d;
+      sums += s[st * (k - 1)] * d;
+    }
+    const double tv = sumt / sumd;
+    const double sv = sums / sumd;
     for (int k = na; k <= nb; ++k) {
       t[st * (k - 1)] = tv;
       s[st * (k - 1)] = sv;
@@ -1109,7 +1114,10 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
                                                     const double* __restrict__ mch2, const E* __restrict__ alpha,
                                                     E* alphn, E* adpn) {
   extern __shared__ double conv_lds[];
+  __shared__ double ldpr[64];  // dpr per level (nk <= 64), read by the searches
   const int tx = threadIdx.x, kl = threadIdx.y, nk = g.nk;
+  if (kl == 0 && tx < nk) ldpr[tx] = dpr[tx];
+  if (kl == 1 && tx + kConvCW < nk) ldpr[tx + kConvCW] = dpr[tx + kConvCW];
   double* lt = conv_lds;
   double* ls = lt + nk * kConvCW;
   E* l1 = reinterpret_cast<E*>(ls + nk * kConvCW);
@@ -1136,7 +1144,7 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
   const long pcol = (long)xcd_block() * kConvCW + col;
   const int ncol = tx < kPer && pcol < g.n2 ? g.lev_t[pcol] : 0;
   if (ncol > 1) {
-    Column<E> c{lt + col, ls + col, l1 + col, l2 + col, dpr, kConvCW};
+    Column<E> c{lt + col, ls + col, l1 + col, l2 + col, ldpr, kConvCW};
     const int nlev = ncol;
     int kcnt = 1;
     while (kcnt < nlev) {
