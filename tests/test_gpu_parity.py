@@ -197,6 +197,39 @@ def test_box_many_steps_bitwise(gpu_lib):
     assert m.check_finite()
 
 
+def test_deep_convection_bitwise(gpu_lib):
+    """Box columns made statically unstable to seeded random depths (water
+    warmed by up to 8 K towards the bottom in 60% of the columns, salinity
+    noise of +-0.5 psu): convect then mixes layers that deepen downwards and
+    upwards many times over, and the result stays bit-identical to the
+    oracle after one step and after two more in one call (whose later steps
+    start from what the earlier ones formed)."""
+    import dataclasses
+    cfg = configs.box()
+    rng = np.random.default_rng(7)
+    shape = (cfg.glo_ni, cfg.glo_nj, cfg.nk)
+    ramp = np.linspace(0.0, 1.0, cfg.nk)[None, None, :]
+    warm = rng.uniform(0.0, 8.0, shape) * (rng.random(shape[:2] + (1,)) < 0.6) * ramp
+    pt = np.asfortranarray(np.asarray(cfg.pt) + warm)
+    sa = np.asfortranarray(np.asarray(cfg.sa) + rng.uniform(-0.5, 0.5, shape))
+    cfg = dataclasses.replace(cfg, pt=pt, sa=sa)
+    m = Model(cfg)
+    o = Oracle(cfg)
+    m.step(1)
+    o.step(1)
+    bad = _compare(m, o, STEP_FIELDS)
+    assert not bad, "\n".join(bad)
+    # the adjustment did mix deep layers: runs of 4 or more equal levels
+    t = m.get("tc")[1:-1, 1:-1, :]
+    run4 = (t[:, :, :-3] == t[:, :, 1:-2]) & (t[:, :, 1:-2] == t[:, :, 2:-1]) & (t[:, :, 2:-1] == t[:, :, 3:])
+    assert run4.any(axis=2).sum() > 100
+    m.step(2)
+    o.step(2)
+    bad = _compare(m, o, STEP_FIELDS)
+    assert not bad, "\n".join(bad)
+    assert m.check_finite()
+
+
 def test_global_1deg_bitwise_across_a_day(gpu_lib):
     """The full 1-degree workload (360x180x30): 25 steps, crossing the day
     boundary (daily forcing and den_prho), bit-identical to the oracle."""
