@@ -66,6 +66,17 @@ int check_async_error(Ctx& c) {
   return PCOM_OK;
 }
 
+// A step leaves int_ssh, int_ts, convect and the next step's int_pgra
+// queued on the side stream.  An entry point that reads or writes the state
+// first joins that work into c.stream; one that changes the state also drops
+// what the last step formed for the next (the next step then forms it again).
+int quiesce(Ctx& c, bool changes_state) {
+  std::string e = settle_side(c);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, e);
+  if (changes_state) c.next_ready = false;
+  return PCOM_OK;
+}
+
 int download(Ctx& c, double* dst, const void* src, size_t n, bool as_float) {
   HIPCHK(hipStreamSynchronize(c.stream), "sync before download");
   if (int rc = check_async_error(c)) return rc;
@@ -447,16 +458,12 @@ int pcom_step(pcom_ctx* ctx, int nsteps) {
   if (!ctx || nsteps < 0) return fail(PCOM_ERR_ARG, "pcom_step: bad argument");
   Ctx* c = C(ctx);
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
-  c->next_ready = false;
+  // the last step's side-stream work stays queued: the next step overlaps
+  // it, and every entry point that touches the state joins it first
   for (int s = 0; s < nsteps; ++s) {
-    c->more_steps = s + 1 < nsteps;
     std::string e = launch_step(*c);
     if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);
   }
-  c->more_steps = false;
-  // the last step's convection is part of this call's work
-  std::string e = settle_side(*c);
-  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_step: " + e);
   return PCOM_OK;
 }
 
@@ -485,6 +492,7 @@ size_t pcom_field_size(const pcom_ctx* ctx, int id) {
 int pcom_get_field(pcom_ctx* ctx, int id, double* host, size_t n) {
   if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_get_field: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   FieldRef r = field_ref(*c, id);
   if (!r.ptr && !r.host) return fail(PCOM_ERR_ARG, "pcom_get_field: unknown field id " + std::to_string(id));
   if (n != r.n) return fail(PCOM_ERR_ARG, "pcom_get_field: size " + std::to_string(n) + " != " + std::to_string(r.n));
@@ -499,6 +507,7 @@ int pcom_get_field(pcom_ctx* ctx, int id, double* host, size_t n) {
 int pcom_set_field(pcom_ctx* ctx, int id, const double* host, size_t n) {
   if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_set_field: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, true)) return rc;
   FieldRef r = field_ref(*c, id);
   if (!r.ptr) return fail(PCOM_ERR_ARG, "pcom_set_field: field id " + std::to_string(id) + " is not settable");
   if (n != r.n) return fail(PCOM_ERR_ARG, "pcom_set_field: size " + std::to_string(n) + " != " + std::to_string(r.n));
@@ -520,6 +529,7 @@ int pcom_get_counters(pcom_ctx* ctx, pcom_counters* k) {
 int pcom_set_clock(pcom_ctx* ctx, long long step, const int cdate[6], const int pdate[6]) {
   if (!ctx || !cdate || !pdate || step < 0) return fail(PCOM_ERR_ARG, "pcom_set_clock: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, true)) return rc;
   Calendar t[2];
   const int* d[2] = {cdate, pdate};
   for (int w = 0; w < 2; ++w) {
@@ -541,6 +551,7 @@ int pcom_set_clock(pcom_ctx* ctx, long long step, const int cdate[6], const int 
 int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
   if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_output_means: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if ((out->pt || out->sa || out->rho) && c->n_ts == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no T/S records");
   if ((out->u || out->v) && c->n_uv == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no u/v records");
   if (out->w && c->n_w == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no w records");
@@ -568,6 +579,8 @@ int pcom_ensemble_create(pcom_ctx* const* members, int count, pcom_ensemble** ou
       if (x == C(members[q])) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: a member is listed twice");
     m.push_back(C(members[q]));
   }
+  for (Ctx* x : m)
+    if (int rc = quiesce(*x, false)) return rc;
   Ensemble* en = nullptr;
   std::string e = ensemble_create(m, &en);
   if (!e.empty()) return fail(PCOM_ERR_ARG, "pcom_ensemble_create: " + e);
@@ -596,6 +609,7 @@ int pcom_ensemble_destroy(pcom_ensemble* ens) {
 int pcom_tune_subcycle(pcom_ctx* ctx, int trials, int tile[3], double* ms) {
   if (!ctx || trials < 1 || !tile || !ms) return fail(PCOM_ERR_ARG, "pcom_tune_subcycle: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, true)) return rc;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   if (int rc = check_async_error(*c)) return rc;
   std::string e = tune_subcycle(*c, trials, tile, ms);
@@ -606,6 +620,7 @@ int pcom_tune_subcycle(pcom_ctx* ctx, int trials, int tile[3], double* ms) {
 int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[4]) {
   if (!ctx || !tile) return fail(PCOM_ERR_ARG, "pcom_subcycle_tiling: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (set) {
     if (tile[0] < 0 || tile[1] < 0 || tile[2] < 0 || tile[0] > 256 || tile[1] > 256 || tile[2] > 16 ||
         tile[3] < PCOM_TROP_AUTO || tile[3] > PCOM_TROP_LAUNCHES)
@@ -637,6 +652,7 @@ int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[4]) {
 int pcom_nn_create(pcom_ctx* ctx, int capacity, unsigned long long seed) {
   if (!ctx || capacity < 0) return fail(PCOM_ERR_ARG, "pcom_nn_create: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   std::string e = nn_create(*c, capacity, seed);
   if (!e.empty()) {
@@ -649,6 +665,7 @@ int pcom_nn_create(pcom_ctx* ctx, int capacity, unsigned long long seed) {
 int pcom_nn_record(pcom_ctx* ctx, int on) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_nn_record: NULL context");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_record: no surrogate attached");
   nn_set_record(*c, on != 0);
   return PCOM_OK;
@@ -657,6 +674,7 @@ int pcom_nn_record(pcom_ctx* ctx, int on) {
 int pcom_nn_train(pcom_ctx* ctx, int iters, int batch, double lr, unsigned long long seed, double* loss) {
   if (!ctx || iters < 0 || batch < 1 || !(lr > 0)) return fail(PCOM_ERR_ARG, "pcom_nn_train: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_train: no surrogate attached");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   std::string e = nn_train(*c, iters, batch, lr, seed, loss);
@@ -667,6 +685,7 @@ int pcom_nn_train(pcom_ctx* ctx, int iters, int batch, double lr, unsigned long 
 int pcom_nn_use(pcom_ctx* ctx, int check_every, double tol) {
   if (!ctx || check_every < 0 || !(tol == tol)) return fail(PCOM_ERR_ARG, "pcom_nn_use: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_use: no surrogate attached");
   std::string e = nn_set_use(*c, check_every, tol);
   if (!e.empty()) return fail(PCOM_ERR_STATE, "pcom_nn_use: " + e);
@@ -676,6 +695,7 @@ int pcom_nn_use(pcom_ctx* ctx, int check_every, double tol) {
 int pcom_nn_eval(pcom_ctx* ctx, int first, int count, double rmse[8]) {
   if (!ctx || !rmse) return fail(PCOM_ERR_ARG, "pcom_nn_eval: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_eval: no surrogate attached");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   std::string e = nn_eval(*c, first, count, rmse);
@@ -698,6 +718,7 @@ size_t pcom_nn_param_count(const pcom_ctx* ctx) {
 int pcom_nn_get_params(pcom_ctx* ctx, float* host, size_t n) {
   if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_nn_get_params: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_get_params: no surrogate attached");
   if (n != pcom_nn_param_count(ctx)) return fail(PCOM_ERR_ARG, "pcom_nn_get_params: size mismatch");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
@@ -709,6 +730,7 @@ int pcom_nn_get_params(pcom_ctx* ctx, float* host, size_t n) {
 int pcom_nn_set_params(pcom_ctx* ctx, const float* host, size_t n) {
   if (!ctx || !host) return fail(PCOM_ERR_ARG, "pcom_nn_set_params: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_set_params: no surrogate attached");
   if (n != pcom_nn_param_count(ctx)) return fail(PCOM_ERR_ARG, "pcom_nn_set_params: size mismatch");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
@@ -722,6 +744,7 @@ int pcom_nn_forward_backward(pcom_ctx* ctx, int batch, const float* x, const flo
   if (!ctx || batch < 1 || !x || !target || !out || !grads || !loss)
     return fail(PCOM_ERR_ARG, "pcom_nn_forward_backward: bad argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_forward_backward: no surrogate attached");
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   std::string e = nn_forward_backward(*c, batch, x, target, out, grads, loss);
@@ -742,6 +765,7 @@ int pcom_get_metric(const pcom_ctx* ctx, int g, int which, double* out, size_t n
 int pcom_global_sums(pcom_ctx* ctx, double out[3]) {
   if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_global_sums: NULL argument");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   std::string e = launch_global_sums(*c, out);
   if (!e.empty()) return fail(PCOM_ERR_HIP, e);
@@ -751,6 +775,7 @@ int pcom_global_sums(pcom_ctx* ctx, double out[3]) {
 int pcom_check_finite(pcom_ctx* ctx) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_check_finite: NULL context");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   if (int rc = check_async_error(*c)) return rc;
   int bad = 0;
@@ -763,6 +788,7 @@ int pcom_check_finite(pcom_ctx* ctx) {
 int pcom_phase_times(pcom_ctx* ctx, int enable, double out_ms[3], int* nsteps) {
   if (!ctx) return fail(PCOM_ERR_ARG, "pcom_phase_times: NULL context");
   Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
   if (out_ms) {
     for (int q = 0; q < 3; ++q) out_ms[q] = c->t_steps ? c->t_ms[q] / c->t_steps : 0.0;
   }

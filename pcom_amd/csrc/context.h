@@ -55,12 +55,10 @@ struct Ctx {
   // convect (the end of a step) runs on `side` and overlaps the next step's
   // int_readyc; set while it has not been joined back into `stream`
   bool side_pending = false;
-  // set by the caller's step loop while another step of the same call
-  // follows: the step then also forms what the next step's start would
-  // compute again from unchanged inputs (schr; the upwelling, which int_ts's
-  // already is; convect's den_alpha and adp, in alphn and adpn) and sets
-  // next_ready, and the next step starts from them
-  bool more_steps = false;
+  // a step also forms what the next step's start would compute again from
+  // unchanged inputs (schr; the upwelling, which int_ts's already is;
+  // convect's den_alpha and adp, in alphn and adpn) and sets next_ready; the
+  // next step starts from them unless an entry point changed the state
   bool next_ready = false;
   int precision = PCOM_FP64;
   int rst = 0;

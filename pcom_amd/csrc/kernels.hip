@@ -1440,6 +1440,10 @@ std::string step_pre_t(Ctx& c) {
   // chc, chp, mch, uc and vc as the previous step's int_ts ones, which that
   // step formed (and its int_ts, on the side stream, may still read)
   if (!ready) {
+    if (c.side_pending) {  // nothing may overwrite wm and sqp under int_ts
+      std::string es = settle_side(c);
+      if (!es.empty()) return es;
+    }
     LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, (double*)nullptr, (double*)nullptr);
     LAUNCH_UPW();
   }
@@ -1514,8 +1518,8 @@ std::string step_post_t(Ctx& c) {
   R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
   (void)n3;
   if (c.timing) (void)hipEventRecord(c.ev[2], c.stream);
-  // (schr too when another step follows: its int_readyc would form the same)
-  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, c.more_steps ? h.schr : (double*)nullptr, h.sqp, h.schs, h.schm);
+  // (schr too: the next step's int_readyc would form the same)
+  LAUNCH(k_sch, n2, c.gd, h.chc, h.chp, h.mch, h.schr, h.sqp, h.schs, h.schm);
   // int_clin
   {
     ClinArgs<E> ca;
@@ -1585,10 +1589,10 @@ std::string step_post_t(Ctx& c) {
                        c.side, ge,
                        c.gd.dpr2, P<double>(f.T[c.cur]), P<double>(f.S[c.cur]), P<E>(f.prho1), P<E>(f.prho2), h.chc,
                        P<R>(f.act), P<R>(f.acs), P<R>(f.acr), P<R>(f.acw), (const R*)P<R>(f.wm), h.mch2,
-                       (const E*)P<E>(f.alpha), c.more_steps ? P<E>(f.alphn) : (E*)nullptr, P<E>(f.adpn));
+                       (const E*)P<E>(f.alpha), P<E>(f.alphn), P<E>(f.adpn));
     hipError_t e2 = hipGetLastError();
     if (e2 != hipSuccess) return hip_err(e2, "k_conv");
-    c.next_ready = c.more_steps;
+    c.next_ready = true;
   }
   c.n_ts += 1;
   if (c.timing) {

@@ -1024,7 +1024,6 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
   (void)hipStreamSynchronize(en.stream);
   std::string err = ensemble_plan(en);
   if (!err.empty()) return err;
-  for (Ctx* c : en.m) c->next_ready = false;
   for (int s = 0; s < nsteps; ++s) {
     // members: the step up to the subcycle
     for (size_t q = 0; q < en.m.size(); ++q) {
@@ -1078,9 +1077,7 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
       if (en.pack_of[q] >= 0 && (e = hipStreamWaitEvent(c.stream, en.done[en.pack_of[q]], 0)) != hipSuccess)
         return hip_error_string(e, "hipStreamWaitEvent");
       c.n_ch += 1;
-      c.more_steps = s + 1 < nsteps;
       if (!(err = launch_step_post(c)).empty()) return err;
-      c.more_steps = false;
     }
   }
   for (Ctx* c : en.m)
