@@ -230,6 +230,37 @@ def test_deep_convection_bitwise(gpu_lib):
     assert m.check_finite()
 
 
+def test_single_step_calls_and_set_field_bitwise(gpu_lib):
+    """One step per pcom_step call (the Fortran driver's loop), with field
+    reads between the calls (each joins the work a step leaves queued),
+    then pcom_set_field of T and uc (which must drop what the last step
+    prepared for the next) and more single steps: bit-identical to the
+    oracle throughout."""
+    cfg = configs.box()
+    m = Model(cfg)
+    o = Oracle(cfg)
+    names = ["tc", "sc", "uc", "vc", "wm", "alpha", "adp", "act", "acw", "acr", "chc", "ssh"]
+    for _ in range(4):
+        m.step(1)
+        o.step(1)
+        bad = _compare(m, o, names)
+        assert not bad, "\n".join(bad)
+    rng = np.random.default_rng(11)
+    t = o.get("tc")
+    t = t + rng.uniform(-0.5, 0.5, t.shape) * (t != 0)
+    u = o.get("uc") * 0.9
+    # the oracle keeps tc/tp and uc/up apart (the library holds one buffer
+    # for each pair, which the steps leave equal): set both names
+    for x in (m, o):
+        for name, v in (("tc", t), ("tp", t), ("uc", u), ("up", u)):
+            x.set(name, v)
+    for _ in range(3):
+        m.step(1)
+        o.step(1)
+    bad = _compare(m, o, STEP_FIELDS)
+    assert not bad, "\n".join(bad)
+
+
 def test_global_1deg_bitwise_across_a_day(gpu_lib):
     """The full 1-degree workload (360x180x30): 25 steps, crossing the day
     boundary (daily forcing and den_prho), bit-identical to the oracle."""
