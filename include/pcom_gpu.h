@@ -284,7 +284,7 @@ typedef struct pcom_nn_stats {
   long long recorded;        /* snapshots held in the store                */
   long long surrogate_steps; /* steps whose subcycle the network replaced  */
   long long plain_steps;     /* steps that ran the plain subcycle          */
-  long long checks;          /* plain steps that also judged a prediction  */
+  long long checks;          /* plain steps that also checked a prediction */
   long long fallbacks;       /* checks that turned the surrogate off       */
   int in_use;
   double last_check_rmse[4]; /* sea RMSE of mch2, mch (dimensionless ch),  */

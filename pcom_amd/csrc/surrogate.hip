@@ -2,7 +2,7 @@
 // mod_int.f90:153-296): a 3x3 convolutional network in BF16 with FP32
 // accumulation on the gfx950 matrix cores, trained on the device from
 // snapshots the plain subcycle produces, and used in pcom_step to skip most
-// subcycles.  The plain subcycle stays the fallback and the judge: every
+// subcycles.  The plain subcycle stays the fallback and the check: every
 // check_every-th step runs it and measures the network's error there.
 //
 // What the network maps (one baroclinic step, interior glo_nj x glo_ni
@@ -932,7 +932,7 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
     NN_LAUNCH(k_rec_in, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c),
               n->store_in + (long)slot * kDyn * hw);
   if (check) {
-    // the prediction the network would have made, judged by the plain cycle
+    // the prediction the network would have made, checked by the plain cycle
     NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
     std::string e = nn_predict(c);
     if (!e.empty()) return e;

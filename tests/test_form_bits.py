@@ -1,7 +1,7 @@
 """The mixed forms' bits, pinned.
 
 The FP64 form is pinned to the FP64 oracle bit for bit (test_gpu_parity.py).
-The mixed forms have no bitwise reference: their accuracy is judged by the
+The mixed forms have no bitwise reference: their accuracy is measured by the
 one-year RMSE against FP64.  Kernel rewrites that keep every operation and
 its order must keep these bits; this test records them (SHA-1 of nine fields
 after 30 steps of the 1-degree configuration, measured when the 1-year RMSE

@@ -1,7 +1,7 @@
 """Learned barotropic surrogate: the CPU network restatement pinned by a
 brute-force convolution, BF16 rounding cases and finite differences; the
 HIP forward/backward (BF16 MFMA) against it; training and use in the time
-stepper, judged by the plain subcycle."""
+stepper, checked by the plain subcycle."""
 import numpy as np
 import pytest
 
@@ -114,7 +114,7 @@ def test_gpu_forward_backward_matches_oracle(gpu_lib):
 @pytest.mark.gpu
 def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
     """Snapshots from plain box steps, a short training run, then steps
-    with the network judged by the plain subcycle every 4th step."""
+    with the network checked by the plain subcycle every 4th step."""
     from pcom_amd.lib import Model
     cfg = configs.box()
     m = Model(cfg)
