@@ -145,7 +145,9 @@ typedef struct pcom_counters {
 int pcom_create(const pcom_inputs *in, const pcom_opts *opts, pcom_ctx **out);
 
 /* Advance nsteps baroclinic steps: main.f90:158-227 without output,
- * including the daily int_bnd + den_prho on day change.                */
+ * including the daily int_bnd + den_prho on day change.  Returns with the
+ * work queued on the context's two streams; every entry point that reads
+ * or writes the state waits for it first.                              */
 int pcom_step(pcom_ctx *ctx, int nsteps);
 
 /* Dimensions of the halo'd arrays. */
@@ -206,7 +208,7 @@ int pcom_global_sums(pcom_ctx *ctx, double out[3]);
 /* Returns PCOM_ERR_NAN if any prognostic field holds a non-finite value. */
 int pcom_check_finite(pcom_ctx *ctx);
 
-/* Wait for all work queued on the context's stream. */
+/* Wait for all work queued on the context's streams. */
 int pcom_sync(pcom_ctx *ctx);
 
 /* Average device duration (ms) of each phase over the steps since the last
@@ -304,7 +306,7 @@ int pcom_nn_record(pcom_ctx *ctx, int on);
  * over the call; loss[iters] (mean squared error in normalised units) may
  * be NULL.                                                               */
 int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long long seed, double *loss);
-/* Use the network in pcom_step with a plain, judging subcycle every
+/* Use the network in pcom_step with a plain, checking subcycle every
  * check_every steps (0 turns the surrogate off).  At each check, tol > 0
  * bounds the sea RMSE of every output, tol < 0 bounds it by -tol times the
  * RMSE of persistence (no change over the step), tol = 0 never falls back;
