@@ -1393,8 +1393,9 @@ std::string step_pre_t(Ctx& c) {
 
   c.step_i += 1;
   if (c.timing) (void)hipEventRecord(c.ev[0], c.stream);
-  // daily forcing (main.f90:164-180); den_prho reads T and S and writes the
-  // coefficients convect reads, so the previous step's convection must be done
+  // daily forcing (main.f90:164-180): int_bnd writes the restoring fields
+  // int_ts reads, and den_prho reads T and S and writes the coefficients
+  // convect reads, so the previous step's int_ts and convect must be done
   if (c.step_i == 1 || c.ct.d != c.pt.d) {
     std::string es = settle_side(c);
     if (!es.empty()) return es;
@@ -1414,7 +1415,7 @@ std::string step_pre_t(Ctx& c) {
   // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes.
   // While the previous step's int_ts is queued there (side_pending), the
   // side stream is already past everything of c.stream's that they read
-  // (it forked after the mode coupling); otherwise it forks here.
+  // (it forked after int_ts's upwelling); otherwise it forks here.
   if (!c.side_pending && (e = fork_side(c, 0)) != hipSuccess) return hip_err(e, "fork");
   const bool ready = c.next_ready;  // the previous step formed schr, wm, alpha, adp
   c.next_ready = false;
