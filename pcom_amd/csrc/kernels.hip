@@ -1455,33 +1455,48 @@ std::string step_pre_t(Ctx& c) {
   R* avp = P<R>(f.AV[(c.ab + 2) % 3]);
   R* aupp = P<R>(f.AU[(c.ab + 1) % 3]);
   R* avpp = P<R>(f.AV[(c.ab + 1) % 3]);
-  {
-    const int G = c.ni - 2;
-    const unsigned nb = (unsigned)(((G + kAdvTX - 1) / kAdvTX) * ((c.nj + kAdvTY - 1) / kAdvTY) * c.nk);
-    hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wm), h.chc,
-                       h.schr, au, av);
-    hipError_t ea = hipGetLastError();
-    if (ea != hipSuccess) return hip_err(ea, "k_adv_tiles");
-  }
-  if (c.rst != 1) {  // AB3 bootstrap (mod_int.f90:122-132)
-    size_t bytes = n3 * sizeof(R);
-    if (c.step_i == 1) {
-      if ((e = hipMemcpyAsync(aup, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
-      if ((e = hipMemcpyAsync(aupp, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
-      if ((e = hipMemcpyAsync(avp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
-      if ((e = hipMemcpyAsync(avpp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
-    } else if (c.step_i == 2) {
-      if ((e = hipMemcpyAsync(aupp, aup, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
-      if ((e = hipMemcpyAsync(avpp, avp, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+  auto launch_adv = [&]() -> std::string {
+    {
+      const int G = c.ni - 2;
+      const unsigned nb = (unsigned)(((G + kAdvTX - 1) / kAdvTX) * ((c.nj + kAdvTY - 1) / kAdvTY) * c.nk);
+      hipLaunchKernelGGL(k_adv_tiles<R>, dim3(nb), dim3(kAdvTX * kAdvTY), 0, c.stream, g, U, V, P<R>(f.wm), h.chc,
+                         h.schr, au, av);
+      hipError_t ea = hipGetLastError();
+      if (ea != hipSuccess) return hip_err(ea, "k_adv_tiles");
     }
-  }
-  {
-    const int G = c.ni - 2;
-    const unsigned nb = (unsigned)(((G + kFriTX - 1) / kFriTX) * ((c.nj + kFriTY - 1) / kFriTY) * c.nk);
-    hipLaunchKernelGGL(k_fri_tiles<R>, dim3(nb), dim3(kFriTX * kFriTY), 0, c.stream, g, U, V, P<R>(f.am), h.chc,
-                       h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
-    hipError_t ef = hipGetLastError();
-    if (ef != hipSuccess) return hip_err(ef, "k_fri_tiles");
+    if (c.rst != 1) {  // AB3 bootstrap (mod_int.f90:122-132)
+      size_t bytes = n3 * sizeof(R);
+      if (c.step_i == 1) {
+        if ((e = hipMemcpyAsync(aup, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+        if ((e = hipMemcpyAsync(aupp, au, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+        if ((e = hipMemcpyAsync(avp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+        if ((e = hipMemcpyAsync(avpp, av, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      } else if (c.step_i == 2) {
+        if ((e = hipMemcpyAsync(aupp, aup, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+        if ((e = hipMemcpyAsync(avpp, avp, bytes, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return hip_err(e, "copy");
+      }
+    }
+    return "";
+  };
+  auto launch_fri = [&]() -> std::string {
+    {
+      const int G = c.ni - 2;
+      const unsigned nb = (unsigned)(((G + kFriTX - 1) / kFriTX) * ((c.nj + kFriTY - 1) / kFriTY) * c.nk);
+      hipLaunchKernelGGL(k_fri_tiles<R>, dim3(nb), dim3(kFriTX * kFriTY), 0, c.stream, g, U, V, P<R>(f.am), h.chc,
+                         h.schr, h.bnd[0], h.bnd[1], P<R>(f.fx), P<R>(f.fy));
+      hipError_t ef = hipGetLastError();
+      if (ef != hipSuccess) return hip_err(ef, "k_fri_tiles");
+    }
+    return "";
+  };
+  // op_adv and op_fri are independent; which goes first decides what each
+  // shares the device with while the other stream runs int_ts and convect
+  // (measured: FP64 adv first, FP32 stencils fri first, 1.2% per step)
+  std::string el;
+  if constexpr (sizeof(R) == sizeof(float)) {
+    if (!(el = launch_fri()).empty() || !(el = launch_adv()).empty()) return el;
+  } else {
+    if (!(el = launch_adv()).empty() || !(el = launch_fri()).empty()) return el;
   }
   {
     const size_t lds = 2 * (size_t)c.nk * kColCW * sizeof(R);
