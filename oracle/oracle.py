@@ -53,6 +53,7 @@ def lib():
         L.orc_op_dif.argtypes = [vp, dp, dp, dp]
         L.orc_op_fri.argtypes = [vp, dp]
         L.orc_upwelling.argtypes = [vp]
+        L.orc_smag.argtypes = [vp, dp, dp]
         L.orc_metric.argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
         L.orc_metric.restype = C.c_int
         L.orc_vgrid.argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
@@ -257,6 +258,14 @@ class Oracle:
         sch = np.asfortranarray(sch, np.float64)
         lib().orc_op_fri(self.h, _dp(sch))
         return self.get("fx"), self.get("fy")
+
+    def smag(self, u, v):
+        """smag (mod_int.f90:403-440): returns am from unweighted u, v (g3);
+        am keeps its previous values where smag does not write it."""
+        u = np.asfortranarray(u, np.float64)
+        v = np.asfortranarray(v, np.float64)
+        lib().orc_smag(self.h, _dp(u), _dp(v))
+        return self.get("am")
 
     def upwelling(self):
         """upwelling (mod_int.f90:626-647): returns wm from the context's

@@ -1623,6 +1623,7 @@ void orc_op_adv_ts(orc *o, const double *var, const double *wm, double *out) { o
 void orc_op_dif(orc *o, const double *ch, const double *var, double *out) { op_dif(o, out, ch, var); }
 void orc_op_fri(orc *o, const double *sch) { op_fri(o, sch); }
 void orc_upwelling(orc *o) { upwelling(o); }
+void orc_smag(orc *o, const double *u, const double *v) { smag(o, u, v); }
 
 int orc_metric(const orc *o, int g, int which, double *out, size_t n) {
   if (g < 1 || g > 4 || n != o->n2) return -1;

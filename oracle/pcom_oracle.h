@@ -70,6 +70,9 @@ void orc_op_adv_ts(orc *o, const double *var, const double *wm, double *out);
 void orc_op_dif(orc *o, const double *ch, const double *var, double *out);
 void orc_op_fri(orc *o, const double *sch);
 void orc_upwelling(orc *o);
+/* smag (mod_int.f90:403-440) of unweighted velocities u, v (g3, nk) into
+ * the context's am. */
+void orc_smag(orc *o, const double *u, const double *v);
 /* Horizontal metric arrays: which = 0 rh, 1 tn, 2 dx1, 3 dx2, 4 longitude,
  * 5 latitude (radians) of stagger g. */
 int orc_metric(const orc *o, int g, int which, double *out, size_t n);

@@ -5,8 +5,11 @@
 write fields.  The HIP library is required; there is no CPU fallback, and
 a missing or unloadable library raises at import of ``lib()``.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -15,6 +18,10 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpcom_gpu.so")
 _LIB = None
+# contexts and ensembles not closed yet: closed by _close_all at interpreter
+# exit, while the HIP runtime is still fully up (ensembles before members)
+_LIVE_ENSEMBLES = weakref.WeakSet()
+_LIVE_MODELS = weakref.WeakSet()
 
 # every symbol include/pcom_gpu.h declares
 EXPORTS = [
@@ -91,7 +98,20 @@ def lib():
             if name not in ("pcom_field_size", "pcom_last_error", "pcom_nn_param_count"):
                 getattr(L, name).restype = C.c_int
         _LIB = L
+        atexit.register(_close_all)
     return _LIB
+
+
+def _close_all():
+    """Destroy every live ensemble, then every live context, in the order
+    pcom_destroy needs (its streams drained, its device memory freed) and
+    before Python's finalisation and the C runtime's exit handlers run:
+    nothing of the library is left for object finalisers at shutdown or for
+    the HIP runtime's own teardown (or a profiler's) to race with."""
+    for e in list(_LIVE_ENSEMBLES):
+        e.close()
+    for m in list(_LIVE_MODELS):
+        m.close()
 
 
 def _check(rc):
@@ -112,6 +132,7 @@ class Ensemble:
         h = C.c_void_p()
         _check(lib().pcom_ensemble_create(arr, len(self.members), C.byref(h)))
         self.h = h
+        _LIVE_ENSEMBLES.add(self)
 
     def step(self, n=1):
         _check(lib().pcom_ensemble_step(self.h, int(n)))
@@ -126,9 +147,13 @@ class Ensemble:
         if getattr(self, "h", None) is not None and _LIB is not None:
             _LIB.pcom_ensemble_destroy(self.h)
             self.h = None
+        _LIVE_ENSEMBLES.discard(self)
 
     def __del__(self):
-        self.close()
+        # at interpreter shutdown _close_all has already run; a finaliser
+        # that reaches here then must not call into the HIP runtime
+        if not sys.is_finalizing():
+            self.close()
 
 
 def den_rho(t, s, p):
@@ -154,6 +179,7 @@ class Model:
         h = C.c_void_p()
         _check(lib().pcom_create(C.byref(self._inp), C.byref(self._opt), C.byref(h)))
         self.h = h
+        _LIVE_MODELS.add(self)
         ni, nj, nk = C.c_int(), C.c_int(), C.c_int()
         _check(lib().pcom_get_dims(self.h, C.byref(ni), C.byref(nj), C.byref(nk)))
         self.ni, self.nj, self.nk = ni.value, nj.value, nk.value
@@ -162,9 +188,11 @@ class Model:
         if getattr(self, "h", None) is not None and _LIB is not None:
             _LIB.pcom_destroy(self.h)
             self.h = None
+        _LIVE_MODELS.discard(self)
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():
+            self.close()
 
     def step(self, n=1):
         _check(lib().pcom_step(self.h, int(n)))

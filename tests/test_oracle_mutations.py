@@ -2,7 +2,8 @@
 (a dropped or flipped term, a wrong stagger's metric factor, an index off
 by one, a transposed operand), planted in a copy of the C oracle built on
 the CPU, must make the named test of tests/test_oracle_compositions.py
-fail.  The repository's oracle itself is never modified."""
+(or of the file a fourth entry names) fail.  The repository's oracle
+itself is never modified."""
 import os
 import subprocess
 import sys
@@ -12,8 +13,75 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "pcom_oracle.c")
 
-# name: (statement in oracle/pcom_oracle.c, mistaken statement, test that must fail)
+PINS = "test_oracle_pins.py"
+
+# name: (statement in oracle/pcom_oracle.c, mistaken statement, test that
+# must fail[, test file]); the first occurrence of the statement is changed
 MUTANTS = {
+    # smag
+    "smag_shear_metric_sign": ("u[I3(i, j, k)] * tn / ARAD;", "u[I3(i, j, k)] * tn / ARAD * -1.0;",
+                               "smag_closed_form", PINS),
+    "smag_factor_of_four": ("double ds = 4 * (v[I3(i + 1, j, k)] - v[I3(i - 1, j, k)]) / dx +",
+                            "double ds = 2 * (v[I3(i + 1, j, k)] - v[I3(i - 1, j, k)]) / dx +",
+                            "smag_closed_form", PINS),
+    "smag_tn_of_T_row": ("double tn = o->tn[3][I2(i, j)];", "double tn = o->tn[1][I2(i, j)];",
+                         "smag_closed_form", PINS),
+    "smag_cap_divisor": ("(double)(4 * 8 * o->dtbc)", "(double)(4 * 4 * o->dtbc)", "smag_cap", PINS),
+    "smag_last_row": ("for (int j = 2; j <= NJ - 1; ++j)", "for (int j = 2; j <= NJ - 2; ++j)",
+                      "smag_cap", PINS),
+    # den_prho
+    "prho_interface_pressure": ("double p = o->chc[I2(i, j)] * o->pr2[k - 1];",
+                                "double p = o->chc[I2(i, j)] * o->pr1[k - 1];", "den_prho", PINS),
+    "prho_no_chc": ("double p = o->chc[I2(i, j)] * o->pr2[k - 1];", "double p = o->pr2[k - 1];",
+                    "den_prho", PINS),
+    "prho_one_sided_divisor": ("o->prho1[I3(i, j, k)] = (ra - rb) / (2 * DELTAT);",
+                               "o->prho1[I3(i, j, k)] = (ra - rb) / DELTAT;", "den_prho", PINS),
+    "prho_components_swapped": ("o->prho2[I3(i, j, k)] = (ra - rb) / (2 * DELTAS);",
+                                "o->prho1[I3(i, j, k)] = (ra - rb) / (2 * DELTAS);", "den_prho", PINS),
+    # den_alpha
+    "alpha_interface_pressure": ("ch[I2(i, j)] * o->pr2[k - 1]);", "ch[I2(i, j)] * o->pr1[k - 1]);",
+                                 "den_alpha", PINS),
+    "alpha_no_ch": ("ch[I2(i, j)] * o->pr2[k - 1]);", "o->pr2[k - 1]);", "den_alpha", PINS),
+    # int_bnd
+    "bnd_century_leap": ("if (y % 100 == 0) { if (y % 400 == 0) d = 366; }", "if (y % 100 == 0) { d = 366; }",
+                         "int_bnd_interpolates", PINS),
+    "bnd_january_wrap": ("m2 = m; m1 = m - 1; if (m == 1) m1 = 12;", "m2 = m; m1 = m - 1; if (m == 1) m1 = 1;",
+                         "int_bnd_interpolates", PINS),
+    "bnd_length_of_later_month": ("days = days_month(y, m1);", "days = days_month(y, m2);",
+                                  "int_bnd_interpolates", PINS),
+    "bnd_weight_in_double": ("double fd = (double)f;",
+                             "double fd = (d < 15) ? ((double)days - 15.0 + (double)d) / (double)days "
+                             ": ((double)d - 15.0) / (double)days; (void)f;",
+                             "int_bnd_interpolates", PINS),
+    # int_readyc
+    "readyc_pa_force_sign": ("double x = -o->auc[q] - sch[p] * A0 * o->pax[p] + o->fx[q];",
+                             "double x = -o->auc[q] + sch[p] * A0 * o->pax[p] + o->fx[q];",
+                             "pressure_force_at_rest", PINS),
+    "readyc_pa_force_no_sch": ("double x = -o->auc[q] - sch[p] * A0 * o->pax[p] + o->fx[q];",
+                               "double x = -o->auc[q] - A0 * o->pax[p] + o->fx[q];",
+                               "pressure_force_at_rest", PINS),
+    "readyc_pa_gradient_staggers": ("gra(o, wkx, wky, o->bnd[4], 1, 1, 2, 4);",
+                                    "gra(o, wkx, wky, o->bnd[4], 1, 1, 4, 2);",
+                                    "pressure_force_at_rest", PINS),
+    "readyc_bootstrap_on_restart": ("if (o->rst != 1) {", "if (o->rst != 2) {", "adams_bashforth_history", PINS),
+    "readyc_bootstrap_second_step": ("copy3(o, o->aupp, o->aup); copy3(o, o->avpp, o->avp);",
+                                     "copy3(o, o->aupp, o->auc); copy3(o, o->avpp, o->avc);",
+                                     "adams_bashforth_history", PINS),
+    "readyc_shift_order": ("copy3(o, o->aupp, o->aup);\n  copy3(o, o->avpp, o->avp);\n  copy3(o, o->aup, o->auc);\n"
+                           "  copy3(o, o->avp, o->avc);",
+                           "copy3(o, o->aup, o->auc);\n  copy3(o, o->avp, o->avc);\n  copy3(o, o->aupp, o->aup);\n"
+                           "  copy3(o, o->avpp, o->avp);",
+                           "adams_bashforth_history", PINS),
+    # int_ts accumulators
+    "ts_acr_pressure_of_mch": ("den_alpha(o, wk, o->tp, o->sp, o->chc, o->msk_t);",
+                               "den_alpha(o, wk, o->tp, o->sp, o->mch, o->msk_t);", "density_accumulator", PINS),
+    "ts_acr_specific_volume": ("o->acr[q] = o->acr[q] + 1.0 / wk[q];", "o->acr[q] = o->acr[q] + wk[q];",
+                               "density_accumulator", PINS),
+    "ts_acw_divisor": ("o->wm[q] * o->alpha[q] / (GRAV * o->mch2[p]);", "o->wm[q] * o->alpha[q] / (GRAV * o->mch[p]);",
+                       "vertical_velocity_accumulator", PINS),
+    "ts_acw_interface": ("o->acw[q] = o->acw[q] + o->wm[q] * o->alpha[q]",
+                         "o->acw[q] = o->acw[q] + o->wm[q + n2] * o->alpha[q]",
+                         "vertical_velocity_accumulator", PINS),
     # op_adv
     "adv_half_correction": ("ans[q] = ans[q] - 0.5 * var[q] * ((m1[q] + m2[q])",
                             "ans[q] = ans[q] - 0.25 * var[q] * ((m1[q] + m2[q])", "skew_symmetric"),
@@ -110,7 +178,8 @@ sys.exit(pytest.main(["-q", "-p", "no:cacheprovider", "--tb=no", "-k", {k!r},
 
 @pytest.mark.parametrize("name", sorted(MUTANTS))
 def test_planted_mistake_is_caught(name, tmp_path):
-    good, bad, test = MUTANTS[name]
+    good, bad, test = MUTANTS[name][:3]
+    tfile = MUTANTS[name][3] if len(MUTANTS[name]) > 3 else "test_oracle_compositions.py"
     src = open(SRC).read()
     assert src.count(good) >= 1, f"{name}: statement no longer in the oracle"
     (tmp_path / "pcom_oracle.c").write_text(src.replace(good, bad, 1))
@@ -118,7 +187,7 @@ def test_planted_mistake_is_caught(name, tmp_path):
     subprocess.check_call(["gcc", "-O1", "-fPIC", "-ffp-contract=off", "-std=c11",
                            "-I", os.path.join(ROOT, "oracle"), "-shared", "-o", str(lib),
                            str(tmp_path / "pcom_oracle.c"), "-lm"])
-    tests = os.path.join(ROOT, "tests", "test_oracle_compositions.py")
+    tests = os.path.join(ROOT, "tests", tfile)
     code = _RUN.format(root=ROOT, here=str(tmp_path), k=test, tests=tests)
     p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=300)
     assert "passed" in p.stdout or "failed" in p.stdout, p.stdout + p.stderr
