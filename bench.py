@@ -33,10 +33,12 @@ YEAR_S = 31536000.0
 # sch and gradients 23, momentum with pressure gradient, Coriolis and
 # corrector fluxes 104, corrector div, ch and accumulators 26.
 TROP_FLOPS_PER_POINT = 184
-# HBM bytes per k_trop_tiles launch at 1 degree: rocprofv3 --pmc FETCH_SIZE
-# (x 2.000) + WRITE_SIZE (x 1.000), the factors being what the counters read
-# for a plain 1 GiB copy (profiles/k_trop_tiles_global1deg_pmc.md)
+# HBM bytes per k_trop_tiles launch at 1 degree, from two rocprofv3 --pmc
+# passes over this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
+# factors being what the counters read for a plain 1 GiB copy); the passes
+# and their commands are in the profile named here
 TROP_TRAFFIC_BYTES = 490_300_000
+TROP_TRAFFIC_SOURCE = "profiles/k_trop_tiles_global1deg_pmc.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
 
 
 def sypd(ms_per_step, dtbc):
@@ -53,6 +55,7 @@ def step_bytes(cfg, s3):
 
 
 SUR_ITERS = 1000
+SUR_RECORD = 48
 SUR_TOL = -1.0  # a check fails when the network's error exceeds persistence's
 
 
@@ -121,6 +124,45 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes_read": a.numel() * 8, "calibrate_copy_bytes_written": b.numel() * 8}))
 
 
+def accuracy(cfg, forms, days):
+    """SST / SSH RMSE (and every prognostic field's) of each form against
+    the FP64 form after `days` simulated days from the initial state, run
+    side by side (the FP64 form is bit-identical to the FP64 oracle).  A
+    surrogate form records its first days, trains, then runs checked."""
+    from pcom_amd import abi, rmse
+    from pcom_amd.lib import Model
+    spd = 86400 // cfg.dtbc
+    ref = Model(cfg, precision=abi.PCOM_FP64)
+    ref.step(days * spd)
+    res = {"days": days, "gates": {"sst_K": 1e-3, "ssh_m": 1e-3}, "forms": {}}
+    for name in forms:
+        if name == "fp64":
+            continue
+        base, sur = name.split("+")[0], name.endswith("+surrogate")
+        m = Model(cfg, precision=abi.FORMS[base])
+        if sur:
+            rec_steps = min(SUR_RECORD, days * spd // 2)
+            m.nn_create(capacity=rec_steps, seed=1)
+            m.nn_record(True)
+            m.step(rec_steps)
+            m.nn_record(False)
+            m.nn_train(SUR_ITERS, batch=8, lr=1e-3, seed=2)
+            m.nn_use(10, tol=SUR_TOL)
+            m.step(days * spd - rec_steps)
+        else:
+            m.step(days * spd)
+        r = {k: float(f"{v:.4g}") for k, v in rmse.rmse_report(m, ref, cfg).items()}
+        r["finite"] = bool(m.check_finite())
+        if sur:
+            st = m.nn_stats()
+            r["surrogate_steps"] = st.surrogate_steps
+            r["plain_steps"] = st.plain_steps
+        res["forms"][name] = r
+        m.close()
+    ref.close()
+    return res
+
+
 def cpu_baseline(cfg, nsteps):
     from oracle.oracle import Oracle
     o = Oracle(cfg)
@@ -143,6 +185,8 @@ def main():
                     help="form reported as `value`; the others are measured alongside")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--config", default="global", choices=["global", "box", "eddy"])
+    ap.add_argument("--rmse-days", type=int, default=30,
+                    help="simulated days of the side-by-side accuracy run of each form against FP64 (0 = skip)")
     ap.add_argument("--forms", default="fp64,mixed,mixed_eos32,mixed+surrogate",
                     help="forms to measure (comma-separated; must include --form)")
     ap.add_argument("--calibrate-copy", action="store_true", help=calibrate_copy.__doc__.split(":")[0])
@@ -193,14 +237,19 @@ def main():
                                f"dtbc={cfg.dtbc} s, {nstep} barotropic substeps/step",
                    "form": args.form, "parallelism": "single GPU, whole domain"},
         "forms": forms,
-        "accuracy": "1-year SST/SSH RMSE of each form against FP64: profiles/rmse_1yr_global1deg.md",
-        "roofline": {"bound": "mfma",
+        "accuracy_1yr": "1-year SST/SSH RMSE of each form against FP64: profiles/rmse_1yr_global1deg.md",
+        "roofline": {"bound": "fp64-valu",
+                     "limiter": "latency: five barrier-separated stages per substep on a cache-resident "
+                                "2-D working set (waves wait for most cycles; see the PMC profile)",
+                     "us_per_substep": round(1000.0 * trop_ms / nstep, 4),
                      "kernel": ("k_trop_tiles (barotropic subcycle, FP64, one persistent launch per step)"
                                 if rep["subcycle_mode"] == abi.PCOM_TROP_PERSISTENT else
                                 f"k_trop_block (barotropic subcycle, FP64, {rep['subcycle_launches']} launches "
                                 "per step, timed together)"),
                      "achieved": round(trop_tflops, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5), "traffic": TROP_TRAFFIC_BYTES if cfg.name == "global" else None,
+                     "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5),
+                     "traffic": TROP_TRAFFIC_BYTES if cfg.name == "global" else None,
+                     "traffic_source": TROP_TRAFFIC_SOURCE if cfg.name == "global" else None,
                      "flops_per_launch": flops, "launch_ms": round(trop_ms, 4),
                      "definition": f"{TROP_FLOPS_PER_POINT} FP64 flops per point per substep x "
                                    f"{cfg.glo_ni}x{cfg.glo_nj} points x {nstep} substeps / event-timed "
@@ -209,6 +258,8 @@ def main():
                           "frac": round(step_gbs / PEAK_HBM_GBS, 5), "bytes_per_step": b,
                           "definition": "algorithmic bytes per baroclinic step (N3*31*s + N2*50*8) / wall time"},
     }
+    if args.rmse_days > 0:
+        out["accuracy"] = accuracy(cfg, wanted, args.rmse_days)
     if rank == 0 and args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     if rank == 0:

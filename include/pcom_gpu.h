@@ -172,6 +172,37 @@ int pcom_get_counters(pcom_ctx *ctx, pcom_counters *c);
  * was created with restart_run = 1.                                     */
 int pcom_set_clock(pcom_ctx *ctx, long long step, const int cdate[6], const int pdate[6]);
 
+/* Forcing records the caller reads from its daily or 6-hourly files
+ * (main.f90:164-202: namelist ff_ws -> taux, tauy; ff_pa -> pa; ff_bct ->
+ * bct).  Each is a global interior array (glo_ni, glo_nj), column-major,
+ * as mympi_input reads one record (mod_mympi.f90:638-657); NULL keeps the
+ * value int_bnd interpolated.                                            */
+typedef struct pcom_forcing {
+  const double *taux, *tauy;  /* wind stress, N m-2          (bnd%taux, bnd%tauy) */
+  const double *pa;           /* atmospheric pressure         (bnd%pa)            */
+  const double *bct;          /* restoring temperature, degC  (bnd%t)             */
+} pcom_forcing;
+
+/* Which forcing refreshes the next pcom_step performs, by the reference's
+ * conditions for its step i (main.f90:164, 184-185): due[0] = 1 when i == 1
+ * or the day changed (int_bnd, then the daily records, then den_prho);
+ * due[1] = 1 when i == 1 or the hour changed to a multiple of 6 (the
+ * 6-hourly records of out_per = '6hour', then den_prho).                 */
+int pcom_forcing_due(pcom_ctx *ctx, int due[2]);
+
+/* Apply forcing records for the next step, before it runs, in the order of
+ * main.f90:164-202.  interpolate = 1 (the daily refresh): int_bnd at the
+ * current date, then the non-NULL records of f replace its fields, then
+ * den_prho; the next step does not repeat its own daily refresh.
+ * interpolate = 0 (the 6-hourly refresh): the records replace the current
+ * fields, then den_prho; it fails with PCOM_ERR_STATE while the next
+ * step's daily refresh is due and not yet applied, which must come first.
+ * The pressure gradient of int_readyc follows every change of bnd pa
+ * (also through pcom_set_field).  Not calling this leaves pcom_step to
+ * refresh the daily forcing from the monthly fields, as the reference does
+ * without forcing files.                                                 */
+int pcom_set_forcing(pcom_ctx *ctx, const pcom_forcing *f, int interpolate);
+
 /* missing_float of mod_param.f90:62 (a float32 constant) */
 #define PCOM_MISSING_FLOAT (-9.99999e30f)
 

@@ -42,6 +42,9 @@ def lib():
         L.orc_set_step_index.argtypes = [vp, C.c_longlong]
         L.orc_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
+        L.orc_set_forcing.argtypes = [vp, C.POINTER(abi.PcomForcing), C.c_int]
+        L.orc_set_forcing.restype = C.c_int
+        L.orc_forcing_due.argtypes = [vp, C.POINTER(C.c_int)]
         L.orc_output_means.restype = C.c_int
         L.orc_op_ter.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp]
         L.orc_op_px1.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
@@ -157,6 +160,16 @@ class Oracle:
         c = (C.c_int * 6)(*cdate)
         p = (C.c_int * 6)(*pdate)
         lib().orc_set_clock(self.h, C.c_longlong(step), c, p)
+
+    def forcing_due(self):
+        d = (C.c_int * 2)()
+        lib().orc_forcing_due(self.h, d)
+        return bool(d[0]), bool(d[1])
+
+    def set_forcing(self, interpolate, taux=None, tauy=None, pa=None, bct=None):
+        f = abi.forcing(taux, tauy, pa, bct)
+        if lib().orc_set_forcing(self.h, C.byref(f), int(bool(interpolate))) != 0:
+            raise ValueError("orc_set_forcing: the daily refresh of the next step comes first")
 
     def output_means(self, names=abi.MEAN_NAMES):
         g = (self.ni - 2, self.nj - 2)

@@ -93,6 +93,7 @@ struct orc {
   int n_ts, n_uv, n_w, n_ch, n_ssh;
   long long step_i;            /* tctr%i */
   long long nsteps;
+  long long forcing_taken;     /* step whose daily refresh the caller applied */
   otime ct, pt;
 };
 
@@ -1099,11 +1100,43 @@ void orc_advance_time(orc *o) {
 
 void orc_set_step_index(orc *o, long long i) { o->step_i = i; }
 
+/* main.f90:164-202 with forcing files: the records are read after int_bnd
+ * (daily) or on their own (6-hourly), divided into the halo'd fields as
+ * mympi_input does (mod_mympi.f90:638-657, 898-928: interior, then the E/W
+ * swap; wall rows keep their values), then den_prho. */
+static void put_record(orc *o, double *bnd, const double *rec) {
+  for (int j = 2; j <= NJ - 1; ++j)
+    for (int i = 2; i <= NI - 1; ++i)
+      bnd[I2(i, j)] = rec[(size_t)(i - 2) + (size_t)o->glo_ni * (size_t)(j - 2)];
+  swp(o, bnd, 1);
+}
+
+void orc_forcing_due(const orc *o, int due[2]) {
+  int first = o->step_i + 1 == 1;
+  due[0] = first || o->ct.d != o->pt.d;
+  due[1] = first || (o->ct.h != o->pt.h && o->ct.h % 6 == 0);
+}
+
+int orc_set_forcing(orc *o, const pcom_forcing *f, int interpolate) {
+  long long next = o->step_i + 1;
+  int daily_due = next == 1 || o->ct.d != o->pt.d;
+  if (!interpolate && daily_due && o->forcing_taken != next) return -1;
+  if (interpolate) orc_int_bnd(o);
+  if (f->taux) put_record(o, o->bnd[0], f->taux);
+  if (f->tauy) put_record(o, o->bnd[1], f->tauy);
+  if (f->bct) put_record(o, o->bnd[2], f->bct);
+  if (f->pa) put_record(o, o->bnd[4], f->pa);
+  orc_den_prho(o);
+  if (interpolate) o->forcing_taken = next;
+  return 0;
+}
+
 /* main.f90:128-141: a restart run continues from tctr%ist = step + 1 with
  * tctr%ct and tctr%pt from the restart file. */
 void orc_set_clock(orc *o, long long step, const int c[6], const int p[6]) {
   o->nsteps = step;
   o->step_i = step;
+  o->forcing_taken = 0;
   otime *t[2] = {&o->ct, &o->pt};
   const int *d[2] = {c, p};
   for (int w = 0; w < 2; ++w) {
@@ -1204,7 +1237,7 @@ void orc_step(orc *o, int n) {
   for (int s = 0; s < n; ++s) {
     o->nsteps += 1;
     o->step_i = o->nsteps;
-    if (o->step_i == 1 || o->ct.d != o->pt.d) {
+    if ((o->step_i == 1 || o->ct.d != o->pt.d) && o->forcing_taken != o->step_i) {
       orc_int_bnd(o);
       orc_den_prho(o);
     }

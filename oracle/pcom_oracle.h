@@ -49,6 +49,11 @@ void orc_advance_time(orc *o);
 void orc_set_step_index(orc *o, long long i);
 /* Restart clock, as pcom_set_clock (main.f90:128-141). */
 void orc_set_clock(orc *o, long long step, const int cdate[6], const int pdate[6]);
+/* Caller forcing records for the next step, as pcom_set_forcing;
+ * -1 when interpolate = 0 comes before a due daily refresh. */
+int orc_set_forcing(orc *o, const pcom_forcing *f, int interpolate);
+/* The refreshes the next step performs, as pcom_forcing_due. */
+void orc_forcing_due(const orc *o, int due[2]);
 /* output() time means, as pcom_output_means; -1 when a count is zero. */
 int orc_output_means(orc *o, pcom_means *m);
 

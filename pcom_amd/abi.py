@@ -55,6 +55,24 @@ class PcomCounters(C.Structure):
     ]
 
 
+class PcomForcing(C.Structure):
+    _fields_ = [("taux", _DP), ("tauy", _DP), ("pa", _DP), ("bct", _DP)]
+
+
+def forcing(taux=None, tauy=None, pa=None, bct=None):
+    """A PcomForcing of (glo_ni, glo_nj) records (None = keep int_bnd's
+    value); the arrays are kept alive on the returned struct."""
+    f = PcomForcing()
+    keep = []
+    for name, a in (("taux", taux), ("tauy", tauy), ("pa", pa), ("bct", bct)):
+        if a is not None:
+            a = __import__("numpy").asfortranarray(a, dtype="float64")
+            keep.append(a)
+            setattr(f, name, a.ctypes.data_as(_DP))
+    f._keep = keep
+    return f
+
+
 PCOM_MISSING_FLOAT = float(__import__("numpy").float32(-9.99999e30))
 
 MEAN_NAMES = ["pt", "sa", "rho", "u", "v", "w", "ch", "ssh"]

@@ -140,6 +140,24 @@ def box(glo_ni=60, glo_nj=40, nk=10, dz=400.0):
     )
 
 
+def box_pa(seed=11):
+    """The box basin with a seeded atmospheric pressure: a smooth monthly
+    field of a few hundred Pa (bnd%pa drives int_readyc's pressure force,
+    mod_int.f90:100-103, 140-145), for tests of the pressure gradient, the
+    forcing hook and restarts; everything else as ``box``."""
+    cfg = box()
+    rng = np.random.default_rng(seed)
+    ni, nj = cfg.glo_ni, cfg.glo_nj
+    x = np.arange(ni)[:, None, None] / ni
+    y = np.arange(nj)[None, :, None] / nj
+    m = np.arange(12)[None, None, :]
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=12)[None, None, :]
+    pa = 400.0 * np.sin(2.0 * np.pi * x + ph) * np.cos(np.pi * y) + 150.0 * np.cos(2.0 * np.pi * m / 12.0) * y
+    cfg.pa = _f(pa)
+    cfg.name = "box_pa"
+    return cfg
+
+
 def _smooth(field2d, sigma):
     from scipy.ndimage import gaussian_filter
     return gaussian_filter(field2d, sigma=sigma, mode=("wrap", "reflect"))
@@ -235,7 +253,7 @@ def eddy_025deg():
     return cfg
 
 
-CONFIGS = {"box": box, "global": global_1deg, "small": small, "eddy": eddy_025deg}
+CONFIGS = {"box": box, "box_pa": box_pa, "global": global_1deg, "small": small, "eddy": eddy_025deg}
 
 
 if __name__ == "__main__":

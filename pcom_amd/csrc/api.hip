@@ -73,7 +73,10 @@ int check_async_error(Ctx& c) {
 int quiesce(Ctx& c, bool changes_state) {
   std::string e = settle_side(c);
   if (!e.empty()) return fail(PCOM_ERR_HIP, e);
-  if (changes_state) c.next_ready = false;
+  if (changes_state) {
+    c.next_ready = false;
+    c.pa_stale = true;
+  }
   return PCOM_OK;
 }
 
@@ -369,6 +372,11 @@ int create_impl(const pcom_inputs* in, const pcom_opts* opts, pcom_ctx** out) {
   if ((rc = make2(*c, &h.graphihx, n2, hs.graphihx.data()))) return rc;
   if ((rc = make2(*c, &h.graphihy, n2, hs.graphihy.data()))) return rc;
   {
+    void* p;
+    if ((rc = dev_alloc(*c, &p, 4 * (size_t)(c->ni - 2) * (c->nj - 2), sizeof(double)))) return rc;
+    c->fstage = static_cast<double*>(p);
+  }
+  {
     size_t nb = (n2 + 255) / 256;
     if ((rc = make2(*c, &h.red, n2 + 3 * nb + 8, nullptr))) return rc;
     void* p;
@@ -543,8 +551,49 @@ int pcom_set_clock(pcom_ctx* ctx, long long step, const int cdate[6], const int 
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   HIPCHK(hipStreamSynchronize(c->stream), "sync");
   c->step_i = step;
+  c->forcing_taken = 0;
   c->ct = t[0];
   c->pt = t[1];
+  return PCOM_OK;
+}
+
+// the reference's forcing-refresh conditions for step tctr%i = step_i + 1
+// (main.f90:164, 184-185)
+static void forcing_due(const Ctx& c, int due[2]) {
+  const bool first = c.step_i + 1 == 1;
+  due[0] = first || c.ct.d != c.pt.d;
+  due[1] = first || (c.ct.h != c.pt.h && c.ct.h % 6 == 0);
+}
+
+int pcom_forcing_due(pcom_ctx* ctx, int due[2]) {
+  if (!ctx || !due) return fail(PCOM_ERR_ARG, "pcom_forcing_due: NULL argument");
+  forcing_due(*C(ctx), due);
+  return PCOM_OK;
+}
+
+int pcom_set_forcing(pcom_ctx* ctx, const pcom_forcing* f, int interpolate) {
+  if (!ctx || !f || (interpolate != 0 && interpolate != 1))
+    return fail(PCOM_ERR_ARG, "pcom_set_forcing: bad argument");
+  Ctx* c = C(ctx);
+  int due[2];
+  forcing_due(*c, due);
+  if (!interpolate && due[0] && c->forcing_taken != c->step_i + 1)
+    return fail(PCOM_ERR_STATE, "pcom_set_forcing: the next step's daily refresh (interpolate = 1) comes first");
+  if (int rc = quiesce(*c, true)) return rc;
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  const size_t g2 = (size_t)(c->ni - 2) * (c->nj - 2);
+  // record q of the staging area -> bnd field (taux 0, tauy 1, t 2, pa 4)
+  const double* src[4] = {f->taux, f->tauy, f->bct, f->pa};
+  const int field[4] = {0, 1, 2, 4};
+  const double* rec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int q = 0; q < 4; ++q)
+    if (src[q]) {
+      if (int rc = upload(*c, c->fstage + q * g2, src[q], g2, false)) return rc;
+      rec[field[q]] = c->fstage + q * g2;
+    }
+  std::string e = launch_set_forcing(*c, rec, interpolate == 1);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_set_forcing: " + e);
+  if (interpolate) c->forcing_taken = c->step_i + 1;
   return PCOM_OK;
 }
 

@@ -60,6 +60,14 @@ struct Ctx {
   // convect's den_alpha and adp, in alphn and adpn) and sets next_ready; the
   // next step starts from them unless an entry point changed the state
   bool next_ready = false;
+  // int_readyc's pressure gradient (pax, pay) must be formed again from
+  // bnd pa before the next step: set at create and by every entry point
+  // that changes the state; the daily refresh forms it too
+  bool pa_stale = true;
+  // the step index whose daily forcing refresh the caller applied
+  // (pcom_set_forcing with interpolate = 1), which that step then skips
+  long long forcing_taken = 0;
+  double* fstage = nullptr;  // (glo_ni, glo_nj) staging of caller forcing records
   int precision = PCOM_FP64;
   int rst = 0;
   int dtbt = 60, dtbc = 3600;
@@ -101,6 +109,9 @@ std::string launch_step_pre(Ctx& c);             // up to int_trop
 std::string launch_step_post(Ctx& c);            // after int_trop
 std::string settle_side(Ctx& c);                 // c.stream waits for pending side-stream work
 std::string launch_global_sums(Ctx& c, double out[3]);
+// int_bnd at the current date (when interpolate), caller records over it
+// (rec[v]: device (glo_ni, glo_nj) arrays for bnd v, or null), den_prho
+std::string launch_set_forcing(Ctx& c, const double* const rec[6], bool interpolate);
 std::string launch_check_finite(Ctx& c, int* bad);
 // time means into `stage` (>= glo_ni*glo_nj*nk + 2 doubles), copied to the
 // host pointers of o, then accumulators and counts reset

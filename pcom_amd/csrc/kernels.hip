@@ -70,8 +70,25 @@ __global__ void k_bnd(long n2, Six s, int m1, int m2, double f) {
   }
 }
 
+// one global record of a forcing file into a halo'd bnd field, as
+// mympi_input does (mod_mympi.f90:638-657, 898-928): interior points from
+// the record, the wall rows 1 and nj kept, then the E/W swap (halo columns
+// of the wall rows copy interior columns of those rows, which no thread
+// writes)
+__global__ void k_put_forcing(Grid<double> g, const double* __restrict__ rec, double* bnd) {
+  long p = (long)xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const int ic = i == 1 ? g.ni - 1 : (i == g.ni ? 2 : i);
+  if (j >= 2 && j <= g.nj - 1)
+    bnd[p] = rec[(long)(ic - 2) + (long)(g.ni - 2) * (j - 2)];
+  else if (ic != i)
+    bnd[p] = bnd[g.ix(ic, j)];
+}
+
 // atmospheric pressure gradient (mod_int.f90:100-103): gra_r2d_b(pa, g1)
-// then ter to g3.  bnd%pa changes only with int_bnd, so this runs daily.
+// then ter to g3.  The reference forms it every step from bnd%pa, which
+// changes only with the daily forcing or a caller's set: it runs then.
 __global__ void k_pa(Grid<double> g, const double* __restrict__ pa, double* pax, double* pay) {
   long p = (long)xcd_block() * blockDim.x + threadIdx.x;
   if (p >= g.n2) return;
@@ -1396,7 +1413,10 @@ std::string step_pre_t(Ctx& c) {
   // daily forcing (main.f90:164-180): int_bnd writes the restoring fields
   // int_ts reads, and den_prho reads T and S and writes the coefficients
   // convect reads, so the previous step's int_ts and convect must be done
-  if (c.step_i == 1 || c.ct.d != c.pt.d) {
+  // (skipped when the caller applied this step's refresh with its own
+  // forcing records, pcom_set_forcing)
+  const bool daily = (c.step_i == 1 || c.ct.d != c.pt.d) && c.forcing_taken != c.step_i;
+  if (daily) {
     std::string es = settle_side(c);
     if (!es.empty()) return es;
     int m1, m2;
@@ -1408,8 +1428,11 @@ std::string step_pre_t(Ctx& c) {
       s6.b[v] = h.bnd[v];
     }
     LAUNCH(k_bnd, n2, (long)n2, s6, m1, m2, fw);
-    LAUNCH(k_pa, n2, c.gd, h.bnd[4], h.pax, h.pay);
     LAUNCH(k_prho<E>, n3, ge, T, S, h.chc, P<E>(f.prho1), P<E>(f.prho2));
+  }
+  if (daily || c.pa_stale) {
+    LAUNCH(k_pa, n2, c.gd, h.bnd[4], h.pax, h.pay);
+    c.pa_stale = false;
   }
   // den_alpha, int_pgra on the side stream: they read T, S, chc and write
   // alpha, adp, bphi, bgraphi, none of which int_readyc reads or writes.
@@ -1709,6 +1732,36 @@ std::string settle_side(Ctx& c) {
   if (e != hipSuccess) return hip_err(e, "join");
   c.side_pending = false;
   return "";
+}
+
+namespace {
+template <typename E>
+std::string set_forcing_t(Ctx& c, const double* const rec[6], bool interpolate) {
+  Fields2& h = c.f2;
+  const size_t n2 = c.n2, n3 = c.n3;
+  if (interpolate) {
+    int m1, m2;
+    double fw;
+    bnd_weights(c.ct, &m1, &m2, &fw);
+    Six s6;
+    for (int v = 0; v < 6; ++v) {
+      s6.a[v] = h.frc[v];
+      s6.b[v] = h.bnd[v];
+    }
+    LAUNCH(k_bnd, n2, (long)n2, s6, m1, m2, fw);
+  }
+  for (int v = 0; v < 6; ++v)
+    if (rec[v]) LAUNCH(k_put_forcing, n2, c.gd, rec[v], h.bnd[v]);
+  LAUNCH(k_prho<E>, n3, grid_of<E>(c), P<double>(c.f3.T[c.cur]), P<double>(c.f3.S[c.cur]), h.chc,
+         P<E>(c.f3.prho1), P<E>(c.f3.prho2));
+  c.pa_stale = true;
+  return "";
+}
+}  // namespace
+
+std::string launch_set_forcing(Ctx& c, const double* const rec[6], bool interpolate) {
+  if (c.precision == PCOM_MIXED_EOS32) return set_forcing_t<float>(c, rec, interpolate);
+  return set_forcing_t<double>(c, rec, interpolate);
 }
 
 std::string launch_step(Ctx& c) {

@@ -29,7 +29,7 @@ EXPORTS = [
     "pcom_set_field", "pcom_get_counters", "pcom_global_sums", "pcom_check_finite", "pcom_sync",
     "pcom_phase_times", "pcom_destroy", "pcom_last_error", "pcom_op_ter", "pcom_op_px1",
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
-    "pcom_set_clock", "pcom_output_means",
+    "pcom_set_clock", "pcom_output_means", "pcom_forcing_due", "pcom_set_forcing",
     "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
@@ -75,6 +75,8 @@ def lib():
         L.pcom_get_metric.argtypes = [vp, C.c_int, C.c_int, dp, sz]
         L.pcom_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pcom_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
+        L.pcom_forcing_due.argtypes = [vp, C.POINTER(C.c_int)]
+        L.pcom_set_forcing.argtypes = [vp, C.POINTER(abi.PcomForcing), C.c_int]
         fp = C.POINTER(C.c_float)
         L.pcom_nn_create.argtypes = [vp, C.c_int, C.c_ulonglong]
         L.pcom_nn_record.argtypes = [vp, C.c_int]
@@ -226,6 +228,20 @@ class Model:
         c = (C.c_int * 6)(*cdate)
         p = (C.c_int * 6)(*pdate)
         _check(lib().pcom_set_clock(self.h, int(step), c, p))
+
+    def forcing_due(self):
+        """(daily, six_hourly): the forcing refreshes the next step performs
+        (main.f90:164, 184-185)."""
+        d = (C.c_int * 2)()
+        _check(lib().pcom_forcing_due(self.h, d))
+        return bool(d[0]), bool(d[1])
+
+    def set_forcing(self, interpolate, taux=None, tauy=None, pa=None, bct=None):
+        """Forcing records (glo_ni, glo_nj) for the next step (main.f90:
+        164-202): interpolate=True is the daily refresh (int_bnd, then the
+        records), False the 6-hourly one (records only); den_prho follows."""
+        f = abi.forcing(taux, tauy, pa, bct)
+        _check(lib().pcom_set_forcing(self.h, C.byref(f), int(bool(interpolate))))
 
     def output_means(self, names=abi.MEAN_NAMES):
         """Time means of the output interval (main.f90:726-760); resets the

@@ -40,6 +40,7 @@ def test_struct_layouts():
     assert C.sizeof(abi.PcomInputs) == 16 + 15 * 8
     assert C.sizeof(abi.PcomOpts) == 13 * 4  # 13 ints, trop_mode last
     assert C.sizeof(abi.PcomCounters) == 8 + 11 * 4 + 4
+    assert C.sizeof(abi.PcomForcing) == 4 * 8
 
 
 def test_no_gpu_error_is_reported_not_aborted():

@@ -49,7 +49,7 @@ def _interior(a):
     return a[1:-1, 1:-1]
 
 
-@pytest.mark.parametrize("cfgname,n1,n2", [("small", 3, 2), ("box", 20, 10)])
+@pytest.mark.parametrize("cfgname,n1,n2", [("small", 3, 2), ("box", 20, 10), ("box_pa", 18, 12)])
 def test_oracle_restart_continues_exactly(tmp_path, cfgname, n1, n2):
     """Writing the 29 restart fields after n1 steps and continuing in a new
     restart-run context for n2 steps gives the uninterrupted run's state
