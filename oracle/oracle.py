@@ -69,12 +69,21 @@ def lib():
         L.orc_constants.restype = C.c_int
         L.orc_constants.argtypes = [C.POINTER(C.POINTER(C.c_char_p)), C.POINTER(dp)]
         L.orc_convect_column.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.orc_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         _LIB = L
     return _LIB
 
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def threads(n=None):
+    """Host threads the oracle's loops run on (OpenMP); sets them if n."""
+    if n:
+        lib().orc_set_threads(int(n))
+    return lib().orc_threads()
 
 
 def den_rho(t, s, p):

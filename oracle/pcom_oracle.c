@@ -50,6 +50,15 @@ static const double k_const_values[] = {
   RHO0, A0, GRAV, ARAD, PI_F, OMEGA, TORAD, GAMMA_T, GAMMA_S, GAMMA_B,
   DELTAT, DELTAS, CDBOT, TICE, KM_C, AM_C, SMAG_C, AH_C};
 
+#ifdef _OPENMP
+#include <omp.h>
+int orc_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+#else
+int orc_threads(void) { return 1; }
+void orc_set_threads(int n) { (void)n; }
+#endif
+
 int orc_constants(const char ***names, const double **values) {
   *names = k_const_names;
   *values = k_const_values;
@@ -105,7 +114,10 @@ struct orc {
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 static int *ialloc(size_t n) { return (int *)calloc(n ? n : 1, sizeof(int)); }
-static void dfill(double *a, size_t n, double v) { for (size_t p = 0; p < n; ++p) a[p] = v; }
+static void dfill(double *a, size_t n, double v) {
+#pragma omp parallel for schedule(static)
+  for (size_t p = 0; p < n; ++p) a[p] = v;
+}
 
 /* ---- halo swap, one rank: mod_mympi.f90:669-746 ---------------------- */
 static void swp(const orc *o, double *a, int nd3) {
@@ -163,12 +175,14 @@ double orc_den_rho(double t, double s, double p0) {
 
 void orc_den_rho_n(size_t n, const double *t, const double *s,
                    const double *p, double *rho) {
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; ++q) rho[q] = orc_den_rho(t[q], s[q], p[q]);
 }
 
 /* mod_den.f90:33-55 */
 static void den_alpha(orc *o, double *alpha, const double *t, const double *s,
                       const double *ch, const int *mask) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK; ++k)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 1; i <= NI; ++i)
@@ -179,6 +193,7 @@ static void den_alpha(orc *o, double *alpha, const double *t, const double *s,
 
 /* mod_den.f90:120-153 */
 void orc_den_prho(orc *o) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK; ++k)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 1; i <= NI; ++i)
@@ -203,16 +218,19 @@ static void ter(orc *o, double *ans, const double *var, int nd3, int ga, int gb,
   dfill(ans, o->n2 * (size_t)nd3, dft);
   int oi = GOI[ga], oj = GOJ[ga];
   if (gb == GEW[ga]) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= nd3; ++k)
       for (int j = 1; j <= NJ; ++j)
         for (int i = 1 + oi; i <= o->nim + oi; ++i)
           ans[I3(i, j, k)] = 0.5 * (var[I3(i - oi, j, k)] + var[I3(i - oi + 1, j, k)]);
   } else if (gb == GNS[ga]) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= nd3; ++k)
       for (int j = 1 + oj; j <= o->njm + oj; ++j)
         for (int i = 1; i <= NI; ++i)
           ans[I3(i, j, k)] = 0.5 * (var[I3(i, j - oj, k)] + var[I3(i, j - oj + 1, k)]);
   } else if (gb == GDI[ga]) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= nd3; ++k)
       for (int j = 1 + oj; j <= o->njm + oj; ++j)
         for (int i = 1 + oi; i <= o->nim + oi; ++i)
@@ -236,6 +254,7 @@ static void p1(orc *o, double *ans, const double *var, int nd3, int ga, int gb, 
   }
   dfill(ans, n, 0.0);
   int oi = GOI[ga];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nd3; ++k)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 1 + oi; i <= o->nim + oi; ++i) {
@@ -244,6 +263,7 @@ static void p1(orc *o, double *ans, const double *var, int nd3, int ga, int gb, 
         ans[I3(i, j, k)] = d;
       }
   const double *dx = o->dx1[gb];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nd3; ++k)
     for (size_t p = 0; p < o->n2; ++p) ans[p + o->n2 * (size_t)(k - 1)] /= dx[p];
   swp(o, ans, nd3);
@@ -262,6 +282,7 @@ static void p2(orc *o, double *ans, const double *var, int nd3, int ga, int gb, 
   }
   dfill(ans, n, 0.0);
   int oj = GOJ[ga];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nd3; ++k)
     for (int j = 1 + oj; j <= o->njm + oj; ++j)
       for (int i = 1; i <= NI; ++i) {
@@ -270,6 +291,7 @@ static void p2(orc *o, double *ans, const double *var, int nd3, int ga, int gb, 
         ans[I3(i, j, k)] = d;
       }
   const double *dy = o->dx2[gb];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nd3; ++k)
     for (size_t p = 0; p < o->n2; ++p) ans[p + o->n2 * (size_t)(k - 1)] /= dy[p];
   swp(o, ans, nd3);
@@ -278,6 +300,7 @@ static void p2(orc *o, double *ans, const double *var, int nd3, int ga, int gb, 
 
 /* p3_r3d, mod_op.f90:715-735: var (nkp) -> ans (nk) */
 static void p3(orc *o, double *ans, const double *var) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK; ++k)
     for (size_t p = 0; p < o->n2; ++p)
       ans[p + o->n2 * (k - 1)] = (var[p + o->n2 * (k - 1)] - var[p + o->n2 * k]) / o->dpr2[k - 1];
@@ -286,6 +309,7 @@ static void p3(orc *o, double *ans, const double *var) {
 /* pz_r3d, mod_op.f90:762-790: var (nk) -> ans (nkp), height derivative */
 static void pz(orc *o, double *ans, const double *var, const int *msk) {
   dfill(ans, o->n3p, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 2; k <= NK; ++k)
     for (size_t p = 0; p < o->n2; ++p)
       ans[p + o->n2 * (k - 1)] = (var[p + o->n2 * (k - 2)] - var[p + o->n2 * (k - 1)]) *
@@ -296,6 +320,7 @@ static void pz(orc *o, double *ans, const double *var, const int *msk) {
 /* dx3_r3d_b, mod_op.f90:792-811 */
 static void dx3(orc *o, double *ans, const double *var) {
   dfill(ans, o->n3p, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 2; k <= NK; ++k)
     for (size_t p = 0; p < o->n2; ++p)
       ans[p + o->n2 * (k - 1)] = var[p + o->n2 * (k - 2)] - var[p + o->n2 * (k - 1)];
@@ -305,10 +330,12 @@ static void dx3(orc *o, double *ans, const double *var) {
 static void vter(orc *o, double *ans, const double *var, int from_gi) {
   if (from_gi == 2) {
     dfill(ans, o->n3p, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 2; k <= NK; ++k)
       for (size_t p = 0; p < o->n2; ++p)
         ans[p + o->n2 * (k - 1)] = (var[p + o->n2 * (k - 2)] + var[p + o->n2 * (k - 1)]) * 0.5;
   } else {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= NK; ++k)
       for (size_t p = 0; p < o->n2; ++p)
         ans[p + o->n2 * (k - 1)] = (var[p + o->n2 * (k - 1)] + var[p + o->n2 * k]) * 0.5;
@@ -318,6 +345,7 @@ static void vter(orc *o, double *ans, const double *var, int from_gi) {
 /* px_r3d_center, mod_op.f90:529-559 (hg = guj = g3, hg%ew = g4) */
 static void px_center(orc *o, double *ans, const double *var, int nd3) {
   dfill(ans, o->n2 * (size_t)nd3, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nd3; ++k)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 2; i <= NI - 1; ++i) {
@@ -331,10 +359,12 @@ static void px_center(orc *o, double *ans, const double *var, int nd3) {
 static void divg(orc *o, double *ans, const double *va, const double *vb, int nd3, int ga, int gb, int gc) {
   size_t n = o->n2 * (size_t)nd3;
   double *wa = dalloc(n), *wb = dalloc(n), *vr = dalloc(n);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nd3; ++k)
     for (size_t p = 0; p < o->n2; ++p) vr[p + o->n2 * k] = vb[p + o->n2 * k] * o->rh[gb][p];
   p1(o, wa, va, nd3, ga, gc, NULL);
   p2(o, wb, vr, nd3, gb, gc, NULL);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nd3; ++k)
     for (size_t p = 0; p < o->n2; ++p) {
       size_t q = p + o->n2 * k;
@@ -346,9 +376,11 @@ static void divg(orc *o, double *ans, const double *va, const double *vb, int nd
 /* gra_r2d, gra_r2d_b, gra_r3d, gra_r3d_b, mod_op.f90:850-902 */
 static void gra(orc *o, double *ax, double *ay, const double *var, int nd3, int ga, int gb, int gc) {
   p1(o, ax, var, nd3, ga, gb, NULL);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nd3; ++k)
     for (size_t p = 0; p < o->n2; ++p) ax[p + o->n2 * k] = ax[p + o->n2 * k] / (ARAD * o->rh[gb][p]);
   p2(o, ay, var, nd3, ga, gc, NULL);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->n2 * (size_t)nd3; ++q) ay[q] = ay[q] / ARAD;
 }
 
@@ -357,13 +389,18 @@ static void lap(orc *o, double *ans, const double *var, int ga, int gb) {
   double *wa = dalloc(o->n2), *wb = dalloc(o->n2);
   int ge = GEW[gb], gn = GNS[gb];
   p1(o, wa, var, 1, ga, ge, NULL);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) wa[p] = wa[p] / o->rh[ge][p];
   p1(o, wb, wa, 1, ge, gb, NULL);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) ans[p] = wb[p];
   p2(o, wa, var, 1, ga, gn, NULL);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) wa[p] = wa[p] * o->rh[gn][p];
   p2(o, wb, wa, 1, gn, gb, NULL);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) ans[p] = ans[p] + wb[p];
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) ans[p] = ans[p] / (ARAD * ARAD * o->rh[gb][p]);
   free(wa); free(wb);
 }
@@ -371,17 +408,20 @@ static void lap(orc *o, double *ans, const double *var, int ga, int gb) {
 /* int_tobot, mod_op.f90:927-944 (grd = g12: dpr = gi2%dpr) */
 static void int_tobot(orc *o, double *ans, const double *var, const int *msk) {
   double *t = dalloc(o->n3);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK; ++k)
     for (size_t p = 0; p < o->n2; ++p) {
       size_t q = p + o->n2 * (k - 1);
       t[q] = var[q] * o->dpr2[k - 1] * (double)msk[q];
     }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK - 1; ++k)
     for (size_t p = 0; p < o->n2; ++p) {
       double s = 0.0;
       for (int kk = k + 1; kk <= NK; ++kk) s += t[p + o->n2 * (kk - 1)];
       ans[p + o->n2 * (k - 1)] = t[p + o->n2 * (k - 1)] * 0.5 + s;
     }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) ans[p + o->n2 * (NK - 1)] = t[p + o->n2 * (NK - 1)] * 0.5;
   free(t);
 }
@@ -389,6 +429,7 @@ static void int_tobot(orc *o, double *ans, const double *var, const int *msk) {
 /* op_vint_ns (dir = 2) / op_vint_ew (dir = 1), mod_op.f90:946-981:
  * ra uses g32 msk/prh, rb the same cshift'ed by -1 along dir. */
 static void vint_face(orc *o, const double *var, double *ra, double *rb, int dir) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int j = 1; j <= NJ; ++j)
     for (int i = 1; i <= NI; ++i) {
       int is = i, js = j;
@@ -409,6 +450,7 @@ static void vint_face(orc *o, const double *var, double *ra, double *rb, int dir
 
 /* vint_r3d, mod_op.f90:983-995 */
 static void vint(orc *o, double *ans, const double *var, const int *msk, const double *prh) {
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) {
     double s = 0.0;
     for (int k = 1; k <= NK; ++k) {
@@ -428,35 +470,42 @@ static void fri_r3d(orc *o, double *ans, const double *sch, const double *v,
   gra(o, mx, my, v, NK, 3, 4, 2);
   ter(o, cha, o->chc, 1, 1, 4, 0.0);
   ter(o, chb, o->chc, 1, 1, 2, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       va[p + n2 * k] = cha[p] * mx[p + n2 * k];
       vb[p + n2 * k] = chb[p] * my[p + n2 * k];
     }
   divg(o, wka, va, vb, NK, 4, 2, 3);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = o->am[q] / sch[p] * wka[q];
     }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = ans[q] + o->am[q] * o->cv1[p] * vp[q];
     }
   px_center(o, wka, vc, NK);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = ans[q] + o->am[q] * o->cv2[p] * sch[p] * wka[q];
     }
   pz(o, wkp, vp, o->msk_u);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;   /* km%v(:,:,1:nk) paired element-wise */
       wkp[q] = GRAV * RHO0 * o->km[q] * wkp[q] / (sch[p] * sch[p]);
     }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) wkp[p] = GRAV * tau[p] / sch[p];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int j = 1; j <= NJ; ++j)
     for (int i = 1; i <= NI; ++i) {
       int k = o->lev_u[I2(i, j)];
@@ -467,6 +516,7 @@ static void fri_r3d(orc *o, double *ans, const double *sch, const double *v,
       }
     }
   p3(o, wka, wkp);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = ans[q] * (double)o->msk_u[q] + wka[q];
   free(mx); free(my); free(wka); free(wkp); free(cha); free(chb); free(va); free(vb);
 }
@@ -475,6 +525,7 @@ static void fri_r3d(orc *o, double *ans, const double *sch, const double *v,
 static void op_fri(orc *o, const double *sch) {
   size_t n3 = o->n3, n2 = o->n2;
   double *u = dalloc(n3), *v = dalloc(n3), *mv = dalloc(n3);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -493,22 +544,28 @@ static void op_dif(orc *o, double *ans, const double *ch, const double *var) {
   double *mx = dalloc(n3), *my = dalloc(n3), *wk = dalloc(n3), *wkp = dalloc(o->n3p);
   double *cha = dalloc(n2), *chb = dalloc(n2), *va = dalloc(n3), *vb = dalloc(n3);
   p1(o, mx, var, NK, 1, 2, o->msk_t);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) mx[p + n2 * k] = mx[p + n2 * k] / (ARAD * o->rh[2][p]);
   p2(o, my, var, NK, 1, 4, o->msk_t);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) my[q] = my[q] / ARAD;
   ter(o, cha, ch, 1, 1, 2, 0.0);
   ter(o, chb, ch, 1, 1, 4, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       va[p + n2 * k] = cha[p] * mx[p + n2 * k];
       vb[p + n2 * k] = chb[p] * my[p + n2 * k];
     }
   divg(o, wk, va, vb, NK, 2, 4, 1);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = AH_C * wk[q];
   pz(o, wkp, var, o->msk_t);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->n3p; ++q) wkp[q] = GRAV * RHO0 * o->kh[q] * wkp[q];
   p3(o, wk, wkp);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = ans[q] + wk[q];
   free(mx); free(my); free(wk); free(wkp); free(cha); free(chb); free(va); free(vb);
 }
@@ -519,9 +576,11 @@ static void op_adv(orc *o, double *ans, const double *var, const double *sch) {
   double *wka = dalloc(n3), *wkb = dalloc(n3), *u = dalloc(n3), *v = dalloc(n3);
   double *m1 = dalloc(n3), *m2 = dalloc(n3);
   double *wkap = dalloc(n3p), *wkbp = dalloc(n3p), *w = dalloc(n3p);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) wka[p + n2 * k] = o->uc[p + n2 * k] / sch[p];
   ter(o, u, wka, NK, 3, 4, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -529,36 +588,45 @@ static void op_adv(orc *o, double *ans, const double *var, const double *sch) {
       wka[q] = wka[q] * o->rh[3][p];
     }
   ter(o, v, wka, NK, 3, 2, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < o->nkp; ++k)
     for (size_t p = 0; p < n2; ++p) wkap[p + n2 * k] = o->wm[p + n2 * k] / o->chc[p];
   ter(o, w, wkap, o->nkp, 1, 3, 0.0);
   ter(o, wka, var, NK, 3, 4, 0.0);
   ter(o, wkb, var, NK, 3, 2, 0.0);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) { m1[q] = wka[q] * u[q]; m2[q] = wkb[q] * v[q]; }
   p1(o, wka, m1, NK, 4, 3, NULL);
   p2(o, wkb, m2, NK, 2, 3, NULL);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = (wka[q] + wkb[q]) / (ARAD * o->rh[3][p]);
     }
   vter(o, wkap, var, 2);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3p; ++q) wkap[q] = wkap[q] * w[q];
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) { wkbp[p] = 0.0; wkbp[p + n2 * NK] = 0.0; }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 2; k <= NK; ++k)
     for (size_t p = 0; p < n2; ++p)
       wkbp[p + n2 * (k - 1)] = wkap[p + n2 * (k - 1)] * (double)o->msk_u[p + n2 * (k - 2)] *
                                (double)o->msk_u[p + n2 * (k - 1)];
   p3(o, wka, wkbp);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = ans[q] + wka[q];
   p1(o, m1, u, NK, 4, 3, NULL);
   p2(o, m2, v, NK, 2, 3, NULL);
   p3(o, wka, w);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = ans[q] - 0.5 * var[q] * ((m1[q] + m2[q]) / (ARAD * o->rh[3][p]) + wka[q]);
     }
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = ans[q] * (double)o->msk_u[q];
   free(wka); free(wkb); free(u); free(v); free(m1); free(m2); free(wkap); free(wkbp); free(w);
 }
@@ -569,7 +637,9 @@ static void op_adv_ts(orc *o, double *ans, const double *var, const double *wm) 
   double *sch = dalloc(n2), *um1 = dalloc(n3), *um2 = dalloc(n3), *m1 = dalloc(n3), *m2 = dalloc(n3);
   double *wka = dalloc(n3), *wkb = dalloc(n3), *tmp = dalloc(n3), *wkp = dalloc(n3p);
   ter(o, sch, o->chc, 1, 1, 3, 1.0);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) sch[p] = sqrt(sch[p]);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -579,23 +649,30 @@ static void op_adv_ts(orc *o, double *ans, const double *var, const double *wm) 
   p1(o, m1, var, NK, 1, 2, NULL);
   p2(o, m2, var, NK, 1, 4, NULL);
   ter(o, wka, um1, NK, 3, 2, 0.0);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) tmp[q] = wka[q] * m1[q];
   ter(o, wkb, tmp, NK, 2, 1, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) ans[p + n2 * k] = wkb[p + n2 * k] / (ARAD * o->rh[1][p]);
   ter(o, wka, um2, NK, 3, 4, 0.0);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) tmp[q] = wka[q] * m2[q];
   ter(o, wkb, tmp, NK, 4, 1, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       ans[q] = ans[q] + wkb[q] / (ARAD * o->rh[1][p]);
     }
   dx3(o, wkp, var);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3p; ++q) wkp[q] = wm[q] * wkp[q];
   vter(o, wka, wkp, 1);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) wka[p + n2 * k] = wka[p + n2 * k] / o->dpr2[k];
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) ans[q] = ans[q] + wka[q];
   free(sch); free(um1); free(um2); free(m1); free(m2); free(wka); free(wkb); free(tmp); free(wkp);
 }
@@ -609,6 +686,7 @@ void orc_int_pgra(orc *o) {
   int_tobot(o, o->adp, o->alpha, o->msk_t);
   ter(o, wkb, o->adp, NK, 1, 2, 0.0);
   ter(o, wkc, o->alpha, NK, 1, 2, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -617,6 +695,7 @@ void orc_int_pgra(orc *o) {
   vint_face(o, wk, o->bphi[0], o->bphi[1], 2);
   ter(o, wkb, o->adp, NK, 1, 4, 0.0);
   ter(o, wkc, o->alpha, NK, 1, 4, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -634,6 +713,7 @@ static void upwelling(orc *o) {
   size_t n3 = o->n3, n2 = o->n2;
   double *sch = dalloc(n2), *va = dalloc(n3), *vb = dalloc(n3), *wk3 = dalloc(n3), *wk2 = dalloc(n2);
   ter(o, sch, o->chp, 1, 1, 3, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -643,8 +723,10 @@ static void upwelling(orc *o) {
     }
   divg(o, wk3, va, vb, NK, 3, 3, 1);
   vint(o, wk2, wk3, o->msk_t, o->prh_t);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) wk2[p] = -wk2[p];
   for (int k = 2; k <= NK; ++k)
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * (k - 1), qm = p + n2 * (k - 2);
       o->wm[q] = o->wm[qm] + o->dpr2[k - 2] * (wk2[p] + wk3[qm]);
@@ -663,7 +745,9 @@ void orc_int_readyc(orc *o) {
   ter(o, o->pax, wkx, 1, 2, 3, 0.0);
   ter(o, o->pay, wky, 1, 4, 3, 0.0);
   ter(o, sch, o->chc, 1, 1, 3, 1.0);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) sch[p] = sqrt(sch[p]);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) if (o->lev_u[p] == 0) sch[p] = 1.0;
   upwelling(o);
   copy3(o, o->aupp, o->aup);
@@ -684,6 +768,7 @@ void orc_int_readyc(orc *o) {
   dfill(o->but, n2, 0.0);
   dfill(o->bvt, n2, 0.0);
   for (int k = 0; k < NK; ++k)
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       double x = -o->auc[q] - sch[p] * A0 * o->pax[p] + o->fx[q];
@@ -691,6 +776,7 @@ void orc_int_readyc(orc *o) {
       o->but[p] = o->but[p] + x * o->dpr2[k] * (double)o->msk_u[q];
       o->bvt[p] = o->bvt[p] + y * o->dpr2[k] * (double)o->msk_u[q];
     }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) {
     o->but[p] = o->but[p] / o->prh_u[p];
     o->bvt[p] = o->bvt[p] / o->prh_u[p];
@@ -711,50 +797,62 @@ void orc_int_trop(orc *o) {
   double *pu1 = dalloc(n2), *pu2 = dalloc(n2), *wm1 = dalloc(n2), *wm2 = dalloc(n2);
   const int *seau = o->msk_u;  /* gu%msk(:,:,1) */
   double bt = (double)o->dtbt;
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) { upb1[p] = o->buc[p]; upb2[p] = o->bvc[p]; }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) { o->mch[p] = o->chc[p]; o->mch2[p] = o->chc[p]; }
   int nstep = o->dtbc / o->dtbt * 2;
   for (int nt = 1; nt <= nstep; ++nt) {
     dfill(sch, n2, 1.0);
     ter(o, wk, o->chp, 1, 1, 3, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) if (o->lev_u[p] > 0) sch[p] = sqrt(wk[p]);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       s1[p] = o->prh_u[p] * sch[p] * o->bup[p];
       s2[p] = o->prh_u[p] * sch[p] * o->bvp[p];
     }
     divg(o, wk, s1, s2, 1, 3, 3, 1);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       wk[p] = o->chp[p] - wk[p] * bt * GAMMA_B / o->prh_t[p];
       if (o->lev_t[p] > 0) o->chc[p] = wk[p];
     }
     ter(o, wk, o->chc, 1, 1, 3, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) if (o->lev_u[p] > 0) sch[p] = sqrt(wk[p]);
     lap(o, l1, o->bup, 3, 3);
     lap(o, l2, o->bvp, 3, 3);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       vs1[p] = (double)seau[p] * o->am[p] * l1[p];
       vs2[p] = (double)seau[p] * o->am[p] * l2[p];
     }
     if (nt == 1) for (size_t p = 0; p < n2; ++p) { vz1[p] = vs1[p]; vz2[p] = vs2[p]; }
     gra(o, gx, gy, o->chc, 1, 1, 2, 4);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= o->njm; ++j)
       for (int i = 1; i <= NI; ++i)
         gi1[I2(i, j)] = 0.5 * (gx[I2(i, j)] * o->bphi[0][I2(i, j)] + gx[I2(i, j + 1)] * o->bphi[1][I2(i, j + 1)]);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 1; i <= o->nim; ++i)
         gi2[I2(i, j)] = 0.5 * (gy[I2(i, j)] * o->bphi[2][I2(i, j)] + gy[I2(i + 1, j)] * o->bphi[3][I2(i + 1, j)]);
     swp(o, gi1, 1);
     swp(o, gi2, 1);
     ter(o, wk, o->chc, 1, 1, 2, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= o->njm; ++j)
       for (int i = 1; i <= NI; ++i)
         ci1[I2(i, j)] = 0.5 * (wk[I2(i, j)] * o->bgra[0][I2(i, j)] + wk[I2(i, j + 1)] * o->bgra[1][I2(i, j + 1)]);
     ter(o, wk, o->chc, 1, 1, 4, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= NJ; ++j)
       for (int i = 1; i <= o->nim; ++i)
         ci2[I2(i, j)] = 0.5 * (wk[I2(i, j)] * o->bgra[2][I2(i, j)] + wk[I2(i + 1, j)] * o->bgra[3][I2(i + 1, j)]);
     swp(o, ci1, 1);
     swp(o, ci2, 1);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       double su = (double)seau[p];
       pg1[p] = (gi1[p] + ci1[p] + o->graphihx[p]) * su * sch[p];
@@ -775,11 +873,13 @@ void orc_int_trop(orc *o) {
       o->bup[p] = o->buc[p];
       o->bvp[p] = o->bvc[p];
     }
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       s1[p] = o->prh_u[p] * sch[p] * o->buc[p];
       s2[p] = o->prh_u[p] * sch[p] * o->bvc[p];
     }
     divg(o, wk, s1, s2, 1, 3, 3, 1);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       o->chc[p] = o->chp[p] - (double)o->msk_t[p] * wk[p] * bt / o->prh_t[p];
       if (nt <= nstep / 2) o->mch[p] = o->mch[p] + o->chc[p];
@@ -789,6 +889,7 @@ void orc_int_trop(orc *o) {
       o->mch2[p] = o->mch2[p] + o->chc[p];
     }
   }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) {
     o->buc[p] = upb1[p] / (double)(nstep + 1);
     o->bvc[p] = upb2[p] / (double)(nstep + 1);
@@ -808,6 +909,7 @@ void orc_int_trop(orc *o) {
 
 /* smag, mod_int.f90:403-440 */
 static void smag(orc *o, const double *u, const double *v) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= NK; ++k)
     for (int j = 2; j <= NJ - 1; ++j)
       for (int i = 2; i <= NI - 1; ++i) {
@@ -840,20 +942,27 @@ void orc_int_clin(orc *o) {
   double *u1 = dalloc(n3), *u2 = dalloc(n3);
   dfill(sch, n2, 1.0);
   ter(o, wk, o->mch, 1, 1, 3, 1.0);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) if (o->lev_u[p] > 0) sch[p] = sqrt(wk[p]);
   for (int k = 1; k <= NK; ++k) {
     size_t ko = n2 * (size_t)(k - 1);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) wk[p] = o->phih_t[p] + o->mch[p] * o->adp[ko + p];
     gra(o, pg1, pg2, wk, 1, 1, 2, 4);
     gra(o, wm1, wm2, o->mch, 1, 1, 2, 4);
     ter(o, wk, o->alpha + ko, 1, 1, 2, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) pg1[p] = pg1[p] + wk[p] * o->pr2[k - 1] * wm1[p];
     ter(o, wk, o->alpha + ko, 1, 1, 4, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) pg2[p] = pg2[p] + wk[p] * o->pr2[k - 1] * wm2[p];
     ter(o, wk, pg1, 1, 2, 3, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) pu1[p] = -((wk[p] + A0 * o->pax[p]) * sch[p]);
     ter(o, wk, pg2, 1, 4, 3, 0.0);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) pu2[p] = -((wk[p] + A0 * o->pay[p]) * sch[p]);
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = ko + p;
       pu1[p] = pu1[p] - (c_tc * o->auc[q] + c_tp * o->aup[q] + c_tpp * o->aupp[q]);
@@ -875,6 +984,7 @@ void orc_int_clin(orc *o) {
   }
   vint(o, wm1, o->uc, o->msk_u, o->prh_u);
   vint(o, wm2, o->vc, o->msk_u, o->prh_u);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
@@ -886,13 +996,16 @@ void orc_int_clin(orc *o) {
   copy3(o, o->up, o->uc);
   copy3(o, o->vp, o->vc);
   ter(o, wk, o->mch2, 1, 1, 3, 1.0);
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) sch[p] = sqrt(wk[p]);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       u1[q] = o->uc[q] * sch[p] / o->mch2[p];
       u2[q] = o->vc[q] * sch[p] / o->mch2[p];
     }
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) { o->acu[q] = o->acu[q] + u1[q]; o->acv[q] = o->acv[q] + u2[q]; }
   o->n_uv += 1;
   smag(o, u1, u2);
@@ -903,6 +1016,7 @@ void orc_int_clin(orc *o) {
 /* int_ssh, mod_int.f90:602-624 */
 void orc_int_ssh(orc *o) {
   den_alpha(o, o->alpha, o->tp, o->sp, o->mch, o->msk_t);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int j = 1; j <= NJ; ++j)
     for (int i = 1; i <= NI; ++i) {
       double s = 0.0;
@@ -910,7 +1024,9 @@ void orc_int_ssh(orc *o) {
         s += o->alpha[I3(i, j, k)] * o->dpr2[k - 1] * o->mch[I2(i, j)] * (double)o->msk_t[I3(i, j, k)];
       o->ssh[I2(i, j)] = s / GRAV;
     }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) o->ssh[p] = o->phih_t[p] / GRAV + o->ssh[p];
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < o->n2; ++p) o->acssh[p] = o->acssh[p] + o->ssh[p];
   o->n_ssh += 1;
 }
@@ -983,10 +1099,14 @@ void orc_convect_column(int nk, int nlev, const double *dpr, const double *prho_
 /* convect, mod_int.f90:509-542 */
 static void convect(orc *o) {
   int nk = NK;
+#pragma omp parallel
+  {
+  /* columns are independent: each thread works in its own column buffers */
   double *buf = dalloc(7 * (size_t)(nk + 1));
   double *d = buf, *a1 = d + nk + 1, *a2 = a1 + nk + 1, *tt = a2 + nk + 1, *ss = tt + nk + 1;
   double *ra = ss + nk + 1, *rb = ra + nk + 1;
   for (int k = 1; k <= nk; ++k) d[k] = o->dpr2[k - 1];
+#pragma omp for collapse(2) schedule(dynamic, 64)
   for (int i = 1; i <= NI; ++i)
     for (int j = 1; j <= NJ; ++j) {
       int nlev = o->lev_t[I2(i, j)];
@@ -1000,6 +1120,7 @@ static void convect(orc *o) {
       }
     }
   free(buf);
+  }
 }
 
 /* int_ts, mod_int.f90:442-507.  The density accumulator reads a local
@@ -1018,29 +1139,36 @@ void orc_int_ts(orc *o) {
   o->n_w += 1;
   op_adv_ts(o, w1, o->tc, o->wm);
   op_adv_ts(o, w2, o->sc, o->wm);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) { pt1[q] = -w1[q]; pt2[q] = -w2[q]; }
   op_dif(o, w1, o->mch, o->tp);
   op_dif(o, w2, o->mch, o->sp);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) { pt1[q] = pt1[q] + w1[q]; pt2[q] = pt2[q] + w2[q]; }
+#pragma omp parallel for schedule(static)
   for (size_t p = 0; p < n2; ++p) {
     double pb = GAMMA_T * (o->bnd[2][p] - o->tc[p]) * (double)o->msk_t[p] / o->dpr2[0];
     pt1[p] = pt1[p] + pb;
     pb = GAMMA_S * (o->bnd[3][p] - o->sc[p]) * (double)o->msk_t[p] * o->mch[p];
     pt2[p] = pt2[p] + pb;
   }
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < NK; ++k)
     for (size_t p = 0; p < n2; ++p) {
       size_t q = p + n2 * k;
       o->tc[q] = o->tp[q] + pt1[q] * bc * (double)o->msk_t[q] / o->mch[p];
       o->sc[q] = o->sp[q] + pt2[q] * bc * (double)o->msk_t[q] / o->mch[p];
     }
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q)
     if (o->msk_t[q] > 0 && o->tc[q] < TICE) o->tc[q] = TICE;
   convect(o);
   copy3(o, o->tp, o->tc);
   copy3(o, o->sp, o->sc);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q) { o->act[q] = o->act[q] + o->tc[q]; o->acs[q] = o->acs[q] + o->sc[q]; }
   den_alpha(o, wk, o->tp, o->sp, o->chc, o->msk_t);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n3; ++q)
     if (o->msk_t[q] == 1) o->acr[q] = o->acr[q] + 1.0 / wk[q];
   o->n_ts += 1;
@@ -1087,6 +1215,7 @@ void orc_int_bnd(orc *o) {
   }
   double fd = (double)f;
   for (int v = 0; v < 6; ++v)
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < o->n2; ++p) {
       double a1 = o->frc[v][p + o->n2 * (size_t)(m1 - 1)], a2 = o->frc[v][p + o->n2 * (size_t)(m2 - 1)];
       o->bnd[v][p] = (a2 - a1) * fd + a1;

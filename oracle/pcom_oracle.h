@@ -86,6 +86,10 @@ int orc_vgrid(const orc *o, int gi, int which, double *out, size_t n);
 /* Integer grid: which = 0 g12 msk (3D), 1 g32 msk (3D), 2 g12 lev, 3 g32 lev. */
 int orc_igrid(const orc *o, int which, int *out, size_t n);
 
+/* Host threads of the OpenMP loops (1 in a serial build); set n > 0. */
+int orc_threads(void);
+void orc_set_threads(int n);
+
 /* mod_den.f90:57-118 */
 double orc_den_rho(double t, double s, double p0);
 void orc_den_rho_n(size_t n, const double *t, const double *s,

@@ -27,9 +27,12 @@ import numpy as np
 from . import abi, configs
 from .lib import Model
 
-FIELDS_T = ["tc", "sc"]
-FIELDS_U = ["uc", "vc"]
-FIELDS_2D = ["chc", "ssh", "buc", "bvc"]
+# the prognostic state a restart file holds (mod_io.f90:80-113), less the
+# copies a step leaves equal (tp = tc, up = uc, chp = chc, bup = buc) and
+# the forcing, plus the diagnosed SSH and the step-mean free surface
+FIELDS_T = ["tc", "sc", "prho_t", "prho_s"]
+FIELDS_U = ["uc", "vc", "auc", "avc", "aup", "avp", "aupp", "avpp", "am"]
+FIELDS_2D = ["chc", "mch", "mch2", "ssh", "buc", "bvc"]
 
 
 def weights(m, cfg):
@@ -60,7 +63,7 @@ def rmse_report(ma, mb, cfg):
         out[name] = float(np.sqrt(np.sum(w * (a - b) ** 2) / np.sum(w)))
     for name in FIELDS_2D:
         a, b = ma.get(name), mb.get(name)
-        msk = sea1 if name in ("chc", "ssh") else wet_u
+        msk = sea1 if name in ("chc", "mch", "mch2", "ssh") else wet_u
         d = (a - b)[msk]
         out[name] = float(np.sqrt(np.sum(area[msk] * d * d) / np.sum(area[msk])))
     return out

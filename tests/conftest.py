@@ -11,6 +11,10 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); runs the HIP library")
+    # parallel test workers each running the OpenMP oracle on every core
+    # would oversubscribe the host: one thread per worker unless asked
+    if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ:
+        os.environ["OMP_NUM_THREADS"] = "1"
     # the oracle is plain C: build it if this checkout has not yet
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(lib):

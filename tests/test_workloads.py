@@ -1,0 +1,197 @@
+"""The five workloads at their stated sizes on the GPU:
+
+* config 1, the box basin: 100 steps bit for bit against the oracle;
+* config 2, the 1-degree grid: the mixed forms' error against FP64 over
+  ten days, field by field (FP64 itself is bit-identical to the oracle,
+  tests/test_gpu_parity.py);
+* config 3, 24 members at 2 degrees plus 8 at 1 degree stepped as one
+  ensemble: a 2-degree and a 1-degree member (FP64) bit for bit against the
+  oracle, mixed members bit for bit against themselves stepped alone;
+* config 4, the surrogate network at 360 x 180: forward and backward
+  against the CPU restatement, and a record -> train -> checked-use cycle;
+* config 5, the 0.25-degree 1440 x 720 x 50 grid: one FP64 step bit for
+  bit against the oracle, two FP64 and two mixed steps conserving mass and
+  staying finite, and the same bits under another launch tiling.
+"""
+import numpy as np
+import pytest
+
+from oracle import surrogate_ref as ref
+from oracle.oracle import Oracle
+from pcom_amd import abi, configs, ensemble, rmse
+from test_gpu_parity import STEP_FIELDS, _compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_config1_box_100_steps_bitwise(gpu_lib):
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    _say("config 1: box 60x40x10, 100 steps")
+    m, o = Model(cfg), Oracle(cfg)
+    m.step(100)
+    o.step(100)
+    bad = _compare(m, o, STEP_FIELDS)
+    assert not bad, "\n".join(bad)
+    c = m.counters()
+    assert (c.step, c.day, c.hour) == (100, 5, 4)
+
+
+# ten days of the 1-degree grid: bounds at about ten times what the forms
+# reach there (SST/SSH in K and m, velocities m/s, ch dimensionless)
+MIXED_BOUNDS = {
+    "mixed": {"tc": 2e-4, "sst": 2e-4, "sc": 5e-5, "uc": 5e-5, "vc": 5e-5, "chc": 5e-9, "mch": 5e-9,
+              "ssh": 2e-5, "buc": 5e-6, "bvc": 5e-6, "am": 2.0, "auc": 5e-10},
+    "mixed_eos32": {"tc": 5e-2, "sst": 2e-2, "sc": 1e-2, "uc": 2e-2, "vc": 2e-2, "chc": 5e-7, "mch": 5e-7,
+                    "ssh": 1e-3, "buc": 2e-3, "bvc": 2e-3, "am": 500.0, "auc": 1e-7},
+}
+
+
+def test_config2_mixed_forms_error_per_field(gpu_lib):
+    """Every prognostic field of the mixed forms against FP64 after ten
+    days (240 steps) at 1 degree, area-weighted over sea."""
+    from pcom_amd.lib import Model
+    cfg = configs.global_1deg()
+    _say("config 2: mixed forms vs fp64, 240 steps")
+    ref64 = Model(cfg)
+    ref64.step(240)
+    for form, bounds in MIXED_BOUNDS.items():
+        m = Model(cfg, precision=abi.FORMS[form])
+        m.step(240)
+        assert m.check_finite()
+        err = rmse.rmse_report(m, ref64, cfg)
+        _say(f"{form}: " + ", ".join(f"{k}={v:.3g}" for k, v in err.items()))
+        over = {k: (err[k], b) for k, b in bounds.items() if not err[k] <= b}
+        assert not over, f"{form}: {over}"
+        assert err["sst"] > 0.0      # the form differs from FP64 at all
+        m.close()
+
+
+def test_config3_ensemble_members_bitwise(gpu_lib):
+    """The 24 + 8 member ensemble in packed launches: member 0 (2 degrees)
+    and member 24 (1 degree) run FP64 and match the oracle; members 5 and
+    30 (mixed) match themselves stepped alone."""
+    from pcom_amd.lib import Ensemble, Model
+    cfgs = ensemble.member_configs()
+    assert len(cfgs) == 32 and sum(d == 2 for d, _ in cfgs) == 24
+    fp64 = {0, 24}
+    _say("config 3: creating 32 members")
+    members = [Model(c, precision=abi.PCOM_FP64 if i in fp64 else abi.PCOM_MIXED) for i, (_, c) in enumerate(cfgs)]
+    ens = Ensemble(members)
+    _say("config 3: 3 ensemble steps")
+    ens.step(3)
+    npk, of = ens.packs()
+    assert npk < 32 and all(p >= 0 for p in of), (npk, of)
+    names = ["tc", "sc", "uc", "vc", "chc", "mch", "mch2", "buc", "bvc", "am", "ssh", "acch", "wm", "auc"]
+    for i in sorted(fp64):
+        o = Oracle(cfgs[i][1])
+        o.step(3)
+        bad = _compare(members[i], o, names)
+        assert not bad, f"member {i}: " + "\n".join(bad)
+    for i in (5, 30):
+        solo = Model(cfgs[i][1], precision=abi.PCOM_MIXED)
+        solo.step(3)
+        bad = _compare(members[i], solo, names)
+        assert not bad, f"member {i}: " + "\n".join(bad)
+        solo.close()
+    ens.close()
+
+
+def test_config4_network_at_1deg_matches_oracle(gpu_lib):
+    """Forward and backward of the network on one 360 x 180 snapshot (the
+    size config 4 trains on) against the CPU restatement with the device's
+    BF16 roundings."""
+    from pcom_amd.lib import Model
+    from test_surrogate import _inputs, _random_params
+    cfg = configs.global_1deg()
+    m = Model(cfg, precision=abi.PCOM_MIXED)
+    m.nn_create(capacity=0, seed=5)
+    rng = np.random.default_rng(12)
+    p = _random_params(rng).astype(np.float32)
+    m.nn_params(p)
+    x, t = _inputs(cfg, rng, 1)
+    _say("config 4: forward/backward at 360x180, batch 1")
+    loss, out, g = m.nn_forward_backward(x, t)
+    rl, rout, rg = ref.loss_and_grads(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rnd=True)
+    assert abs(loss - rl) <= 2e-3 * rl
+    assert np.sqrt(np.mean((out - rout) ** 2)) <= 1e-2 * np.sqrt(np.mean(rout ** 2))
+    off = 0
+    for li, (ci, co) in enumerate(ref.DIMS):
+        n = 9 * ci * co + co
+        gg, rr = g[off:off + n], rg[off:off + n]
+        assert np.linalg.norm(gg - rr) <= 3e-2 * np.linalg.norm(rr), f"layer {li}"
+        off += n
+
+
+def test_config4_record_train_use_at_1deg(gpu_lib):
+    """At 1 degree: record plain subcycles, train on the device, then step
+    with the network and a plain checking subcycle every 10th step; the
+    network learns (loss falls), the run stays finite and conserves the
+    mass of the plain run."""
+    from pcom_amd.lib import Model
+    cfg = configs.global_1deg()
+    m = Model(cfg, precision=abi.PCOM_MIXED)
+    plain = Model(cfg, precision=abi.PCOM_MIXED)
+    _say("config 4: record 96 steps, train 600 iterations, use 20 steps")
+    m.step(24)
+    plain.step(24)
+    m.nn_create(capacity=96, seed=3)
+    m.nn_record(True)
+    m.step(96)
+    plain.step(96)
+    m.nn_record(False)
+    loss = m.nn_train(600, batch=4, lr=1e-3, seed=4)
+    assert np.all(np.isfinite(loss)) and loss[-50:].mean() < 0.7 * loss[:10].mean()
+    m.nn_use(10, tol=0.0)
+    m.step(20)
+    plain.step(20)
+    st = m.nn_stats()
+    assert (st.surrogate_steps, st.checks) == (18, 2)
+    assert m.check_finite()
+    s_nn, s_pl = m.global_sums()[2], plain.global_sums()[2]
+    assert abs(s_nn - s_pl) <= 1e-9 * abs(s_pl)
+
+
+def _mass(m, cfg):
+    area, lev_t, inner, _ = rmse.weights(m, cfg)
+    sea = (lev_t > 0) & inner
+    return float(np.sum((m.get("prh_t") * m.get("chc") * area)[sea]))
+
+
+def test_config5_quarter_degree_50_levels(gpu_lib):
+    """1440 x 720 x 50, dtbt 15 s, dtbc 900 s (launch-mode subcycle): the
+    first FP64 step bit for bit against the oracle; two more FP64 steps and
+    two mixed steps conserve the bottom-pressure mass to 1e-12 and stay
+    finite; another launch tiling gives the same bits."""
+    from pcom_amd.lib import Model
+    cfg = configs.eddy_025deg()
+    assert (cfg.glo_ni, cfg.glo_nj, cfg.nk) == (1440, 720, 50)
+    _say("config 5: fp64 model, 1 step")
+    m = Model(cfg)
+    assert m.subcycle_mode() == abi.PCOM_TROP_LAUNCHES
+    m0 = _mass(m, cfg)
+    m.step(1)
+    _say("config 5: oracle, 1 step")
+    o = Oracle(cfg)
+    o.step(1)
+    bad = _compare(m, o, ["tc", "sc", "uc", "vc", "chc", "mch", "mch2", "buc", "bvc", "am", "ssh", "wm", "adp",
+                          "bphi_xn", "bgra_ye", "but"])
+    assert not bad, "\n".join(bad)
+    del o
+    m.step(2)
+    assert m.check_finite()
+    assert abs(_mass(m, cfg) - m0) <= 1e-12 * abs(m0)
+    _say("config 5: other tiling, mixed form")
+    b = Model(cfg, trop_tile=(12, 10, 1))
+    b.step(3)
+    for n in ("chc", "mch", "buc", "bvc", "tc", "uc"):
+        assert np.array_equal(m.get(n), b.get(n)), n
+    b.close()
+    x = Model(cfg, precision=abi.PCOM_MIXED)
+    x.step(2)
+    assert x.check_finite()
+    assert abs(_mass(x, cfg) - m0) <= 1e-12 * abs(m0)
