@@ -464,7 +464,8 @@ __global__ void k_pack_live(int ni, int H, int W, LiveIn in, NnNorm nm, const fl
 }
 
 // prediction -> increments (FP64) with per-block sums over sea of
-// area*d(mch2), area*d(mch) and area (rh of the T points)
+// w*d(mch2), w*d(mch) and w, w = area * prh of the T points: the barotropic
+// continuity equation (mod_int.f90:189-193, 264-267) conserves sum(w chc)
 __global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const float* stat, const double* area,
                        double* inc, double* part) {
   __shared__ double s[3][256];
@@ -674,6 +675,7 @@ struct Nn {
   double* dpart = nullptr;     // loss partials, moments, prediction sums
   double* inc = nullptr;       // [4][HW] predicted increments
   double* base = nullptr;      // [4][HW] state before a checked subcycle
+  double* massw = nullptr;     // [n2] area * prh of the T points (mass weight)
   int* idx = nullptr;          // minibatch indices
   std::vector<void*> allocs;
 };
@@ -843,6 +845,13 @@ std::string nn_create(Ctx& c, int capacity, unsigned long long seed) {
     stat[hw + q] = c.hs.lev_u[k] > 0 ? 1.f : 0.f;
     stat[2 * hw + q] = (float)(c.hs.cor[k] / n->norm.cor_scale);
   }
+  {
+    std::vector<double> w(c.n2);
+    for (size_t k = 0; k < c.n2; ++k) w[k] = c.hs.rh[1][k] * c.hs.dx1[1][k] * c.hs.dx2[1][k] * c.hs.prh_t[k];
+    if (!(e = nn_make(*n, &n->massw, c.n2)).empty()) return e;
+    if (hipMemcpy(n->massw, w.data(), c.n2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return "upload surrogate mass weights";
+  }
   if (!(e = nn_make(*n, &n->stat, (size_t)3 * hw)).empty()) return e;
   if (hipMemcpy(n->stat, stat.data(), stat.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return "upload surrogate statics";
@@ -900,7 +909,7 @@ std::string nn_predict(Ctx& c) {
             n->act[0]);
   std::string e = nn_forward(*n, st, 1);
   if (!e.empty()) return e;
-  NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->stat, c.gd.rh[1], n->inc,
+  NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->stat, n->massw, n->inc,
             n->dpart);
   NN_LAUNCH(k_pred_mean, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc);
   NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
