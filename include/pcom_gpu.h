@@ -225,6 +225,18 @@ typedef struct pcom_means {
  * PCOM_ERR_STATE when a count is zero (no step since the last output).  */
 int pcom_output_means(pcom_ctx *ctx, pcom_means *out);
 
+/* pcom_output_means that allocates nothing: the means are staged in a
+ * static device buffer of the library (16 MiB, held by each GPU the library
+ * runs on until the process ends) instead of one allocated at the first
+ * output.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Limit: glo_ni*glo_nj*nk + 2 <=
+ * 2097152 doubles (1 degree x 30 levels fits); a larger grid fails with
+ * PCOM_ERR_ARG.  All calls on one GPU share the buffer, which keeps the
+ * last call's contents: calls that overlap on one GPU (from several host
+ * threads) race.  Same results and accumulator reset as pcom_output_means. */
+int pcom_output_means_noalloc(pcom_ctx *ctx, pcom_means *out);
+
 /* Horizontal metric of stagger g (1..4) from the start-up chain
  * (main.f90:408-497): which = 0 rh = cos(lat), 1 tn = tan(lat),
  * 2 dx%x(1), 3 dx%x(2) (radians); halo'd 2-D, column-major.             */

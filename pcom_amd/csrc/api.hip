@@ -7,6 +7,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -597,15 +598,22 @@ int pcom_set_forcing(pcom_ctx* ctx, const pcom_forcing* f, int interpolate) {
   return PCOM_OK;
 }
 
+// the record counts an output needs (main.f90:726-760)
+static int means_ready(const Ctx& c, const pcom_means* out, const char* who) {
+  const std::string w(who);
+  if ((out->pt || out->sa || out->rho) && c.n_ts == 0) return fail(PCOM_ERR_STATE, w + ": no T/S records");
+  if ((out->u || out->v) && c.n_uv == 0) return fail(PCOM_ERR_STATE, w + ": no u/v records");
+  if (out->w && c.n_w == 0) return fail(PCOM_ERR_STATE, w + ": no w records");
+  if (out->ch && c.n_ch == 0) return fail(PCOM_ERR_STATE, w + ": no ch records");
+  if (out->ssh && c.n_ssh == 0) return fail(PCOM_ERR_STATE, w + ": no ssh records");
+  return PCOM_OK;
+}
+
 int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
   if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_output_means: NULL argument");
   Ctx* c = C(ctx);
   if (int rc = quiesce(*c, false)) return rc;
-  if ((out->pt || out->sa || out->rho) && c->n_ts == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no T/S records");
-  if ((out->u || out->v) && c->n_uv == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no u/v records");
-  if (out->w && c->n_w == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no w records");
-  if (out->ch && c->n_ch == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no ch records");
-  if (out->ssh && c->n_ssh == 0) return fail(PCOM_ERR_STATE, "pcom_output_means: no ssh records");
+  if (int rc = means_ready(*c, out, "pcom_output_means")) return rc;
   HIPCHK(hipSetDevice(c->device), "hipSetDevice");
   if (int rc = check_async_error(*c)) return rc;
   if (!c->stage) {
@@ -616,6 +624,56 @@ int pcom_output_means(pcom_ctx* ctx, pcom_means* out) {
   }
   std::string e = launch_output_means(*c, out, c->stage);
   if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_output_means: " + e);
+  return PCOM_OK;
+}
+
+}  // extern "C"
+
+// Staging of pcom_output_means_noalloc: one static device array per GPU
+// (each device holds its own copy of it for the life of the process), so
+// that entry point allocates nothing.  Host code never uses the array's
+// name as an address: the device address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and kept here.
+constexpr size_t kScratchDoubles = (size_t)1 << 21;  // 16 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_of_device(int device, double** out) {
+  if (device < 0 || device >= kScratchDevices) return fail(PCOM_ERR_ARG, "scratch: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipSetDevice(device), "hipSetDevice");
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(PCOM_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return PCOM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pcom_output_means_noalloc(pcom_ctx* ctx, pcom_means* out) {
+  if (!ctx || !out) return fail(PCOM_ERR_ARG, "pcom_output_means_noalloc: NULL argument");
+  Ctx* c = C(ctx);
+  const size_t need = (size_t)(c->ni - 2) * (c->nj - 2) * c->nk + 2;
+  if (need > kScratchDoubles)
+    return fail(PCOM_ERR_ARG, "pcom_output_means_noalloc: the grid needs " + std::to_string(need) +
+                                  " doubles of staging, above the static buffer's " + std::to_string(kScratchDoubles) +
+                                  " (glo_ni*glo_nj*nk + 2); use pcom_output_means");
+  if (int rc = quiesce(*c, false)) return rc;
+  if (int rc = means_ready(*c, out, "pcom_output_means_noalloc")) return rc;
+  HIPCHK(hipSetDevice(c->device), "hipSetDevice");
+  if (int rc = check_async_error(*c)) return rc;
+  double* stage = nullptr;
+  if (int rc = scratch_of_device(c->device, &stage)) return rc;
+  std::string e = launch_output_means(*c, out, stage);
+  if (!e.empty()) return fail(PCOM_ERR_HIP, "pcom_output_means_noalloc: " + e);
   return PCOM_OK;
 }
 

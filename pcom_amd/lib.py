@@ -30,6 +30,7 @@ EXPORTS = [
     "pcom_phase_times", "pcom_destroy", "pcom_last_error", "pcom_op_ter", "pcom_op_px1",
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
     "pcom_set_clock", "pcom_output_means", "pcom_forcing_due", "pcom_set_forcing",
+    "pcom_output_means_noalloc",
     "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
@@ -75,6 +76,7 @@ def lib():
         L.pcom_get_metric.argtypes = [vp, C.c_int, C.c_int, dp, sz]
         L.pcom_set_clock.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pcom_output_means.argtypes = [vp, C.POINTER(abi.PcomMeans)]
+        L.pcom_output_means_noalloc.argtypes = [vp, C.POINTER(abi.PcomMeans)]
         L.pcom_forcing_due.argtypes = [vp, C.POINTER(C.c_int)]
         L.pcom_set_forcing.argtypes = [vp, C.POINTER(abi.PcomForcing), C.c_int]
         fp = C.POINTER(C.c_float)
@@ -243,16 +245,19 @@ class Model:
         f = abi.forcing(taux, tauy, pa, bct)
         _check(lib().pcom_set_forcing(self.h, C.byref(f), int(bool(interpolate))))
 
-    def output_means(self, names=abi.MEAN_NAMES):
+    def output_means(self, names=abi.MEAN_NAMES, noalloc=False):
         """Time means of the output interval (main.f90:726-760); resets the
-        accumulators.  Returns {name: array (glo_ni, glo_nj[, nk])}."""
+        accumulators.  Returns {name: array (glo_ni, glo_nj[, nk])}.
+        noalloc=True stages them in the library's static device buffer
+        (pcom_output_means_noalloc)."""
         g = (self.ni - 2, self.nj - 2)
         out, m = {}, abi.PcomMeans()
         for n in names:
             shp = g if n in ("ch", "ssh") else g + (self.nk,)
             out[n] = np.empty(shp, order="F")
             setattr(m, n, _dp(out[n]))
-        _check(lib().pcom_output_means(self.h, C.byref(m)))
+        fn = lib().pcom_output_means_noalloc if noalloc else lib().pcom_output_means
+        _check(fn(self.h, C.byref(m)))
         return out
 
     def tune_subcycle(self, trials=8):
