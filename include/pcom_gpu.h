@@ -344,6 +344,14 @@ typedef struct pcom_nn_stats {
 int pcom_nn_create(pcom_ctx *ctx, int capacity, unsigned long long seed);
 /* Record (inputs, outputs) of every plain subcycle from now on (1) or not. */
 int pcom_nn_record(pcom_ctx *ctx, int on);
+/* While recording, also record, before a `fraction` of the steps, two
+ * extra snapshots: the state perturbed by seeded noise of random smoothness
+ * and sea RMS up to amp_ch (ch) and amp_u (m/s, u and v) with its plain
+ * subcycle, and the state that subcycle leaves with its own; then the
+ * state is restored and the step itself runs unperturbed.
+ * Such snapshots teach the network how the plain cycle damps unbalanced
+ * states, which its own errors create when it is in use (fraction 0: off). */
+int pcom_nn_augment(pcom_ctx *ctx, double fraction, double amp_ch, double amp_u, unsigned long long seed);
 /* Adam training on the recorded snapshots: `iters` minibatches of `batch`
  * snapshots drawn with `seed`, learning rate lr decayed to 0 by a cosine
  * over the call; loss[iters] (mean squared error in normalised units) may
@@ -364,9 +372,9 @@ int pcom_nn_eval(pcom_ctx *ctx, int first, int count, double rmse[8]);
 int pcom_nn_get_stats(pcom_ctx *ctx, pcom_nn_stats *s);
 /* Parameters: per layer the FP32 master weights [cout][3][3][cin] (channels
  * padded: input 32 (16 used), hidden 64, output 32 (4 used)) and biases
- * [cout], then 31 normalisation values (means and sds of the 13 recorded
- * inputs, output scales 4, Coriolis scale).  Setting them marks the
- * network as trained.                                                    */
+ * [cout], then 30 normalisation values (offsets and scales of the 12
+ * recorded inputs, output scales 4, Coriolis and bottom-pressure scales).
+ * Setting them marks the network as trained.                             */
 size_t pcom_nn_param_count(const pcom_ctx *ctx);
 int pcom_nn_get_params(pcom_ctx *ctx, float *host, size_t n);
 int pcom_nn_set_params(pcom_ctx *ctx, const float *host, size_t n);

@@ -3,14 +3,15 @@
 TEST INFRASTRUCTURE ONLY: the checker the HIP kernels of
 pcom_amd/csrc/surrogate.hip are compared against; only tests/ load it.
 
-The network has six 3x3 convolutions, zonally periodic with zero padding
-at the walls, on NHWC tensors.  Its channels are 32 (16 used) -> 64 -> 64
+The network has six 3x3 convolutions with their taps DIL[l] pixels apart
+(1, 2, 4, 8, 4, 1), zonally periodic with zero padding at the walls, on
+NHWC tensors.  Its channels are 32 (16 used) -> 64 -> 64
 -> 64 -> 64 -> 64 -> 32 (4 used), with ReLU after all but the last layer.
 The loss is the masked mean-squared error over the 4 used outputs: channel
 c < 2 is masked by input channel 5 (T-point sea) and the others by input
 channel 6 (U-point sea), divided by pixels x 4.  Parameters are laid out as
 pcom_nn_get_params gives them: per layer W [cout][3][3][cin] then b
-[cout], then 31 normalisation values.
+[cout], then 30 normalisation values.
 
 With rnd=True the same BF16 roundings as the device are applied at the
 same points:
@@ -26,7 +27,8 @@ import numpy as np
 
 IN_C, HID, OUT_C, IN_USED, OUT_USED = 32, 64, 32, 16, 4
 DIMS = [(IN_C, HID), (HID, HID), (HID, HID), (HID, HID), (HID, HID), (HID, OUT_C)]
-N_NORM = 31
+DIL = [1, 2, 4, 8, 4, 1]
+N_NORM = 30
 
 
 def param_count():
@@ -57,6 +59,8 @@ def shift(x, oy, ox):
     _, h, _, _ = x.shape
     r = np.roll(x, -ox, axis=2)
     out = np.zeros_like(r)
+    if abs(oy) >= h:
+        return out
     if oy >= 0:
         out[:, :h - oy] = r[:, oy:]
     else:
@@ -64,32 +68,32 @@ def shift(x, oy, ox):
     return out
 
 
-def conv(x, w, b):
-    """out[p][co] = b[co] + sum_{dy,dx,ci} w[co,dy,dx,ci] x[p + (dy-1, dx-1)][ci]"""
+def conv(x, w, b, dil=1):
+    """out[p][co] = b[co] + sum_{dy,dx,ci} w[co,dy,dx,ci] x[p + dil (dy-1, dx-1)][ci]"""
     out = np.zeros(x.shape[:3] + (w.shape[0],)) + b
     for dy in range(3):
         for dx in range(3):
-            out += shift(x, dy - 1, dx - 1) @ w[:, dy, dx, :].T
+            out += shift(x, (dy - 1) * dil, (dx - 1) * dil) @ w[:, dy, dx, :].T
     return out
 
 
-def conv_data_grad(d, w):
-    """din[p][ci] = sum_{dy,dx,co} w[co,dy,dx,ci] d[p - (dy-1, dx-1)][co]"""
+def conv_data_grad(d, w, dil=1):
+    """din[p][ci] = sum_{dy,dx,co} w[co,dy,dx,ci] d[p - dil (dy-1, dx-1)][co]"""
     out = np.zeros(d.shape[:3] + (w.shape[3],))
     for dy in range(3):
         for dx in range(3):
-            out += shift(d, 1 - dy, 1 - dx) @ w[:, dy, dx, :]
+            out += shift(d, (1 - dy) * dil, (1 - dx) * dil) @ w[:, dy, dx, :]
     return out
 
 
-def conv_weight_grad(x, d):
-    """gw[co,dy,dx,ci] = sum_p d[p][co] x[p + (dy-1, dx-1)][ci]"""
+def conv_weight_grad(x, d, dil=1):
+    """gw[co,dy,dx,ci] = sum_p d[p][co] x[p + dil (dy-1, dx-1)][ci]"""
     co, ci = d.shape[3], x.shape[3]
     g = np.zeros((co, 3, 3, ci))
     dm = d.reshape(-1, co)
     for dy in range(3):
         for dx in range(3):
-            g[:, dy, dx, :] = dm.T @ shift(x, dy - 1, dx - 1).reshape(-1, ci)
+            g[:, dy, dx, :] = dm.T @ shift(x, (dy - 1) * dil, (dx - 1) * dil).reshape(-1, ci)
     return g
 
 
@@ -105,7 +109,7 @@ def forward(params, x8, rnd=True):
     acts = [_r(x, rnd)]
     out = None
     for li, (w, b) in enumerate(layers):
-        z = conv(acts[-1], _r(w, rnd), np.asarray(b, np.float32 if rnd else np.float64))
+        z = conv(acts[-1], _r(w, rnd), np.asarray(b, np.float32 if rnd else np.float64), DIL[li])
         if li < len(layers) - 1:
             acts.append(_r(np.maximum(z, 0.0), rnd))
         else:
@@ -134,10 +138,10 @@ def loss_and_grads(params, x8, target, rnd=True):
     grads = [None] * len(layers)
     for li in range(len(layers) - 1, -1, -1):
         w, _ = layers[li]
-        grads[li] = (conv_weight_grad(acts[li], dpre), dpre.reshape(-1, dpre.shape[-1]).sum(axis=0))
+        grads[li] = (conv_weight_grad(acts[li], dpre, DIL[li]), dpre.reshape(-1, dpre.shape[-1]).sum(axis=0))
         if li == 0:
             break
-        din = _r(conv_data_grad(dpre, _r(w, rnd)), rnd)
+        din = _r(conv_data_grad(dpre, _r(w, rnd), DIL[li]), rnd)
         dpre = din * (acts[li] > 0)
     flat = np.concatenate([np.concatenate([g.ravel(), gb]) for g, gb in grads] + [np.zeros(N_NORM)])
     return loss, out[..., :OUT_USED], flat

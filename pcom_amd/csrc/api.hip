@@ -778,6 +778,16 @@ int pcom_nn_record(pcom_ctx* ctx, int on) {
   return PCOM_OK;
 }
 
+int pcom_nn_augment(pcom_ctx* ctx, double fraction, double amp_ch, double amp_u, unsigned long long seed) {
+  if (!ctx) return fail(PCOM_ERR_ARG, "pcom_nn_augment: NULL context");
+  Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_augment: no surrogate attached");
+  std::string e = nn_set_augment(*c, fraction, amp_ch, amp_u, seed);
+  if (!e.empty()) return fail(PCOM_ERR_ARG, "pcom_nn_augment: " + e);
+  return PCOM_OK;
+}
+
 int pcom_nn_train(pcom_ctx* ctx, int iters, int batch, double lr, unsigned long long seed, double* loss) {
   if (!ctx || iters < 0 || batch < 1 || !(lr > 0)) return fail(PCOM_ERR_ARG, "pcom_nn_train: bad argument");
   Ctx* c = C(ctx);

@@ -130,6 +130,7 @@ std::string tune_subcycle(Ctx& c, int trials, int out[3], double* best_ms);
 std::string nn_create(Ctx& c, int capacity, unsigned long long seed);
 void nn_destroy(Ctx& c);
 void nn_set_record(Ctx& c, bool on);
+std::string nn_set_augment(Ctx& c, double fraction, double amp_ch, double amp_u, unsigned long long seed);
 std::string nn_set_use(Ctx& c, int every, double tol);
 std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&));
 std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long seed, double* loss);

@@ -6,20 +6,31 @@
 // check_every-th step runs it and measures the network's error there.
 //
 // What the network maps (one baroclinic step, interior glo_nj x glo_ni
-// points, zonally periodic, zero padding at the walls):
-//   inputs (16 channels): chc-1, buc, bvc, but, bvt (the state and the
-//           column-integrated baroclinic forcing int_readyc hands to
-//           int_trop), T-point and U-point sea masks, Coriolis parameter,
-//           and the face integrals bphi and bgraphi of int_pgra (xn, xs,
-//           ye, yw each), through which the density field enters the
-//           barotropic pressure gradient (mod_int.f90:64-88, 224-248)
+// points, zonally periodic, zero padding at the walls).  Over one step the
+// subcycle is, up to the small variation of its sqrt(ch) factors, a linear
+// recurrence x_{n+1} = A x_n + b of its state x = (ch, u, v), A and b fixed
+// for the step (b holds the forcing ut, vt, graphih and the offset of the
+// initial viscosity vs1, mod_int.f90:197-248).  Its step means then differ
+// from x_0 by a fixed linear map of the first substep's increment
+// r = x_1 - x_0 alone: mean_n(x_n) - x_0 = mean_n sum_{k<n} A^k r.  So:
+//   inputs (16 channels): r of ch, u, v (computed here exactly as the
+//           first substep does it, in FP64) divided by a per-snapshot
+//           scale s (the sea RMS of r_u, r_v), which makes the map
+//           independent of how far the state is from balance; and what A
+//           depends on, each normalised by its mean and standard deviation
+//           over the recorded snapshots: the surface viscosity am, the
+//           face integrals bphi and bgraphi of int_pgra (xn, xs, ye, yw),
+//           the bottom pressure prh, the T- and U-point sea masks and the
+//           Coriolis parameter
 //   outputs (4 channels): mch2-chc, mch-chc, ubt-buc, vbt-bvc (the
 //           step-mean free surface of the full and half cycle and the
-//           step-mean barotropic velocity, as increments)
-// each channel normalised by its mean and standard deviation over the
-// recorded snapshots.  Applying a prediction writes what int_trop's end
-// writes (mod_int.f90:276-287) with the area-weighted mean of the free
-// surface increment removed over sea, so the prediction conserves mass.
+//           step-mean barotropic velocity, as increments), divided by s
+// The gravity waves of a 2*dtbc cycle travel some 15 cells at 1 degree, so
+// the six 3x3 layers are dilated (1, 2, 4, 8, 4, 1): a receptive field of
+// +-20 cells.  Applying a prediction writes what int_trop's end writes
+// (mod_int.f90:276-287) with the mean of the free-surface increment over
+// sea, weighted by area and bottom pressure, removed: the plain cycle's
+// continuity equation conserves that mass.
 //
 // Tensors are NHWC with channels padded to multiples of 32 (the K width of
 // v_mfma_f32_16x16x32_bf16).  Lane layout of that instruction (checked on
@@ -46,11 +57,15 @@ typedef unsigned short bf16_t;
 constexpr int kInC = 32;    // padded input channels (16 used)
 constexpr int kHid = 64;    // hidden channels
 constexpr int kOutC = 32;   // padded output channels (4 used)
-// input channels: 0-4 and 8-15 recorded per snapshot (kDyn of them), 5-7
-// static (sea masks, Coriolis)
-constexpr int kInUsed = 16, kOutUsed = 4, kDyn = 13;
-__host__ __device__ constexpr int dyn_channel(int d) { return d < 5 ? d : d + 3; }
+// recorded per snapshot (kDyn raw channels): r_ch, r_u, r_v (kScaled of
+// them, divided by the snapshot's scale), am, bphi xn xs ye yw, bgraphi xn
+// xs ye yw; network input channels 0-3 and 8-15.  Static: prh 4, sea T 5,
+// sea U 6, Coriolis 7.
+constexpr int kInUsed = 16, kOutUsed = 4, kDyn = 12, kScaled = 3, kStat = 4;
+__host__ __device__ constexpr int dyn_channel(int d) { return d < 4 ? d : d + 4; }
 constexpr int kLayers = 6;  // 32->64, 4 x 64->64, 64->32 (last linear)
+// dilation of each layer's 3x3 taps
+__host__ __device__ constexpr int layer_dil(int l) { return l == 0 || l == 5 ? 1 : (l == 1 ? 2 : (l == 3 ? 8 : 4)); }
 
 __host__ __device__ inline bf16_t f2bf(float f) {
   unsigned u;
@@ -67,8 +82,13 @@ __host__ __device__ inline float bf2f(bf16_t h) {
 }
 
 struct Geo {
-  int n, H, W;
+  int n, H, W, dil;  // images, rows, columns, tap spacing
   __host__ __device__ long npix() const { return (long)n * H * W; }
+  // column x + o, zonally periodic (any offset)
+  __device__ __forceinline__ int wrap(int x) const {
+    int r = x % W;
+    return r < 0 ? r + W : r;
+  }
 };
 
 __device__ __forceinline__ bf8 load8(const bf16_t* p) {
@@ -78,7 +98,8 @@ __device__ __forceinline__ bf8 load8(const bf16_t* p) {
 __device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uint4(0, 0, 0, 0)); }
 
 // ---- forward 3x3 convolution: out[p][co] = b[co] + sum W[co][dy][dx][ci]
-// in[p + (dy-1, dx-1)][ci], periodic in x, zero outside 0 <= y < H.  One
+// in[p + d (dy-1, dx-1)][ci] (d = g.dil), periodic in x, zero outside
+// 0 <= y < H.  One
 // wave computes 16 pixels x COUT channels; 4 waves per workgroup.
 template <int CIN, int COUT, bool RELU, bool OUTF32>
 __global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
@@ -97,12 +118,11 @@ __global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restric
   for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
 #pragma unroll
   for (int dy = 0; dy < 3; ++dy) {
-    const int yy = y + dy - 1;
+    const int yy = y + (dy - 1) * g.dil;
     const bool ok = yy >= 0 && yy < g.H;
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
-      int xx = x + dx - 1;
-      xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+      const int xx = g.wrap(x + (dx - 1) * g.dil);
       const bf16_t* src = in + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * CIN + kq * 8;
 #pragma unroll
       for (int c = 0; c < CIN / 32; ++c) {
@@ -130,7 +150,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restric
   }
 }
 
-// ---- backward data: din[p][ci] = sum W[co][dy][dx][ci] dpre[p - (dy-1, dx-1)][co]
+// ---- backward data: din[p][ci] = sum W[co][dy][dx][ci] dpre[p - d (dy-1, dx-1)][co]
 // with wt = W transposed to [ci][dy][dx][co].
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __restrict__ dpre,
@@ -149,12 +169,11 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __re
   for (int nb = 0; nb < CIN / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
 #pragma unroll
   for (int dy = 0; dy < 3; ++dy) {
-    const int yy = y - dy + 1;
+    const int yy = y - (dy - 1) * g.dil;
     const bool ok = yy >= 0 && yy < g.H;
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
-      int xx = x - dx + 1;
-      xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+      const int xx = g.wrap(x - (dx - 1) * g.dil);
       const bf16_t* src = dpre + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * COUT + kq * 8;
 #pragma unroll
       for (int c = 0; c < COUT / 32; ++c) {
@@ -174,7 +193,7 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __re
 }
 
 // ---- backward weights, one (dy, dx) per blockIdx.y and one pixel chunk per
-// blockIdx.x: part[chunk][dydx][co][ci] = sum_p dpre[p][co] in[p+(dy-1,dx-1)][ci].
+// blockIdx.x: part[chunk][dydx][co][ci] = sum_p dpre[p][co] in[p+d(dy-1,dx-1)][ci].
 // GEMM with M = co, N = ci, K = pixels; 32-pixel slices of both operands
 // are staged in LDS and read back transposed.  Wave w owns co rows
 // [16 w', 16 w' + 16) for w' = w, w + 4, ... < COUT/16.
@@ -211,9 +230,8 @@ __global__ void __launch_bounds__(256) k_conv_bwd_w(Geo g, const bf16_t* __restr
         const int x = (int)(p % g.W);
         const long t = p / g.W;
         const int y = (int)(t % g.H), n = (int)(t / g.H);
-        const int yy = y + dy - 1;
-        int xx = x + dx - 1;
-        xx = xx < 0 ? xx + g.W : (xx >= g.W ? xx - g.W : xx);
+        const int yy = y + (dy - 1) * g.dil;
+        const int xx = g.wrap(x + (dx - 1) * g.dil);
         if (yy >= 0 && yy < g.H) v = in[(((long)n * g.H + yy) * g.W + xx) * CIN + c];
       }
       sI[pp][c] = v;
@@ -355,61 +373,285 @@ __global__ void k_w_bf16(int cout, int cin, const float* w32, bf16_t* w, bf16_t*
   wt[((long)ci * 9 + dydx) * cout + co] = h;
 }
 
-// ---- data movement between the model state and the network tensors.
-// Static per-pixel channels: sea masks and the normalised Coriolis
-// parameter; pixel q = y * W + x maps to Fortran (i, j) = (x + 2, y + 2).
+// ---- the first substep of the plain cycle (mod_int.f90:184-273 at nt = 1,
+// where vs - vs1 = 0), from the state the cycle starts from (chp, bup, bvp),
+// on every point of the halo'd arrays; the operators of pcom_dev.h apply
+// the reference's halo rule.  Predictor free surface first:
+__global__ void k_sub1_pred(Grid<double> g, const double* __restrict__ chp, const double* __restrict__ bup,
+                            const double* __restrict__ bvp, const double* __restrict__ chc, double bt,
+                            double* chpred) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  const int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  auto CHP = [&](int m, int jj) -> double { return chp[g.ix(m, jj)]; };
+  auto SCH = [&](int m, int jj) -> double {
+    return g.lev_u[g.ix(m, jj)] > 0 ? sqrt(ter_di<1>(g, CHP, m, jj, 0.0)) : 1.0;
+  };
+  auto FU = [&](int m, int jj) -> double { return g.prh_u[g.ix(m, jj)] * SCH(m, jj) * bup[g.ix(m, jj)]; };
+  auto FV = [&](int m, int jj) -> double { return g.prh_u[g.ix(m, jj)] * SCH(m, jj) * bvp[g.ix(m, jj)]; };
+  const double wk = chp[p] - divg<3, 3, 1>(g, FU, FV, i, j) * bt * GAMMA_B / g.prh_t[p];
+  chpred[p] = g.lev_t[p] > 0 ? wk : chc[p];
+}
+
+struct Sub1In {
+  const double *bup, *bvp, *but, *bvt, *ghx, *ghy, *cor;
+  const double *bphi[4], *bgra[4];
+};
+
+// the momentum stage: pressure gradient of the predictor free surface,
+// Coriolis predictor-corrector, new barotropic velocity (and its sqrt(ch)
+// factor for the corrector's fluxes)
+__global__ void k_sub1_mom(Grid<double> g, Sub1In a, const double* __restrict__ chpred, double bt, double* u1,
+                           double* v1, double* sch1) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n2) return;
+  const int i = (int)(p % g.ni) + 1, j = (int)(p / g.ni) + 1;
+  const int ic = g.canon(i);
+  auto CP = [&](int m, int jj) -> double { return chpred[g.ix(m, jj)]; };
+  const double su = g.lev_u[p] >= 1 ? 1.0 : 0.0;
+  const double sch = g.lev_u[p] > 0 ? sqrt(ter_di<1>(g, CP, i, j, 0.0)) : 1.0;
+  auto GX = [&](int m, int jj) -> double { return grax<1, 2>(g, CP, m, jj); };
+  auto GY = [&](int m, int jj) -> double { return gray<1, 4>(g, CP, m, jj); };
+  auto TE = [&](int m, int jj) -> double { return ter_ew<1>(g, CP, m, jj); };
+  auto TN = [&](int m, int jj) -> double { return ter_ns<1>(g, CP, m, jj, 0.0); };
+  double gi1 = 0.0, ci1 = 0.0;
+  if (j <= g.njm) {
+    gi1 = 0.5 * (GX(ic, j) * a.bphi[0][g.ix(ic, j)] + GX(ic, j + 1) * a.bphi[1][g.ix(ic, j + 1)]);
+    ci1 = 0.5 * (TE(ic, j) * a.bgra[0][g.ix(ic, j)] + TE(ic, j + 1) * a.bgra[1][g.ix(ic, j + 1)]);
+  }
+  const double gi2 = 0.5 * (GY(ic, j) * a.bphi[2][g.ix(ic, j)] + GY(ic + 1, j) * a.bphi[3][g.ix(ic + 1, j)]);
+  const double ci2 = 0.5 * (TN(ic, j) * a.bgra[2][g.ix(ic, j)] + TN(ic + 1, j) * a.bgra[3][g.ix(ic + 1, j)]);
+  const double pg1 = (gi1 + ci1 + a.ghx[p]) * su * sch;
+  const double pg2 = (gi2 + ci2 + a.ghy[p]) * su * sch;
+  const double up = a.bup[p], vp = a.bvp[p], f = a.cor[p];
+  double pu1 = su * (a.but[p] - pg1 + 0.5 * f * vp);
+  double pu2 = su * (a.bvt[p] - pg2 - 0.5 * f * up);
+  const double w1 = pu1 * bt + up, w2 = pu2 * bt + vp;
+  const double x = 0.5 * f * bt;
+  double uc = (w1 + 0.5 * f * bt * w2) / (1 + x * x);
+  double vc = (w2 - 0.5 * f * bt * w1) / (1 + x * x);
+  uc = 0.5 * (uc + up);
+  vc = 0.5 * (vc + vp);
+  pu1 = su * (0.0 - pg1 + a.but[p] + f * vc);
+  pu2 = su * (0.0 - pg2 + a.bvt[p] - f * uc);
+  u1[p] = pu1 * bt + up;
+  v1[p] = pu2 * bt + vp;
+  sch1[p] = sch;
+}
+
+// the network's raw inputs of one snapshot, kDyn channels [kDyn][HW]
+struct RawIn {
+  const double *chp, *bup, *bvp, *u1, *v1, *sch1, *am1;
+  const double *bphi[4], *bgra[4];
+};
+
+// corrector free surface, then r = x_1 - x_0 and the other raw channels of
+// the interior pixel q = y * W + x (Fortran (i, j) = (x + 2, y + 2))
+__global__ void k_sub1_raw(Grid<double> g, int H, int W, RawIn a, double bt, float* raw) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const int i = (int)(q % W) + 2, j = (int)(q / W) + 2;
+  const long p = g.ix(i, j);
+  auto FU = [&](int m, int jj) -> double {
+    const long k = g.ix(m, jj);
+    return g.prh_u[k] * a.sch1[k] * a.u1[k];
+  };
+  auto FV = [&](int m, int jj) -> double {
+    const long k = g.ix(m, jj);
+    return g.prh_u[k] * a.sch1[k] * a.v1[k];
+  };
+  const double ch1 = a.chp[p] - (double)(g.lev_t[p] >= 1 ? 1 : 0) * divg<3, 3, 1>(g, FU, FV, i, j) * bt / g.prh_t[p];
+  raw[q] = (float)(ch1 - a.chp[p]);
+  raw[hw + q] = (float)(a.u1[p] - a.bup[p]);
+  raw[2 * hw + q] = (float)(a.v1[p] - a.bvp[p]);
+  raw[3 * hw + q] = (float)a.am1[p];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    raw[(4 + f) * hw + q] = (float)a.bphi[f][p];
+    raw[(8 + f) * hw + q] = (float)a.bgra[f][p];
+  }
+}
+
+// the snapshot's scale s = sea RMS of r_u, r_v: per-block partials of
+// (sum r^2, count), then one block folds them in order into s
+__global__ void k_scale_part(int H, int W, const float* raw, const float* stat, double* part) {
+  __shared__ double s1[256], s2[256];
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  double a = 0.0, c = 0.0;
+  if (q < hw) {
+    const double mu = (double)stat[2 * hw + q];
+    const double u = raw[hw + q], v = raw[2 * hw + q];
+    a = mu * (u * u + v * v);
+    c = 2.0 * mu;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+      s2[threadIdx.x] += s2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+__global__ void k_scale_final(const double* part, int nb, double* scale, float* keep) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      a += part[2 * b];
+      c += part[2 * b + 1];
+    }
+    const double sc = c > 0.0 && a > 0.0 ? sqrt(a / c) : 1e-30;
+    *scale = sc;
+    if (keep) *keep = (float)sc;
+  }
+}
+
+// ---- perturbed snapshots.  The network must also map states it makes
+// itself, which carry unbalanced errors the plain model never shows; the
+// plain cycle damps them (a gravity wave averages out over the cycle), and
+// snapshots of perturbed states teach the network that response.  Noise:
+// a hashed uniform value per (field, pixel, seed), smoothed by `npass`
+// passes of a 5-point average (zonally periodic, rows clamped at the walls),
+// scaled to a chosen sea RMS.
+__device__ __forceinline__ float hash_uniform(unsigned long long seed, unsigned f, long q) {
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)q * 4 + f + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (float)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+}
+
+__global__ void k_noise(int H, int W, unsigned long long seed, float* out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= 3 * hw) return;
+  out[q] = hash_uniform(seed, (unsigned)(q / hw), q % hw);
+}
+
+__global__ void k_smooth(int H, int W, const float* in, float* out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (e >= 3 * hw) return;
+  const long base = (e / hw) * hw, q = e % hw;
+  const int x = (int)(q % W), y = (int)(q / W);
+  const int xw = x == 0 ? W - 1 : x - 1, xe = x == W - 1 ? 0 : x + 1;
+  const int ys = y == 0 ? 0 : y - 1, yn = y == H - 1 ? H - 1 : y + 1;
+  out[e] = 0.2f * (in[base + q] + in[base + (long)y * W + xw] + in[base + (long)y * W + xe] +
+                   in[base + (long)ys * W + x] + in[base + (long)yn * W + x]);
+}
+
+// per-block sums of noise^2 over sea (field 0 on T points, 1 and 2 on U)
+__global__ void k_noise_part(int H, int W, const float* nz, const float* stat, double* part) {
+  __shared__ double s[4][256];
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (q < hw) {
+    const double mt = stat[q], mu = stat[hw + q];
+    v[0] = mt * (double)nz[q] * nz[q];
+    v[1] = mu * ((double)nz[hw + q] * nz[hw + q] + (double)nz[2 * hw + q] * nz[2 * hw + q]);
+    v[2] = mt;
+    v[3] = 2.0 * mu;
+  }
+  for (int c = 0; c < 4; ++c) s[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 4; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[4 * blockIdx.x + c] = s[c][0];
+}
+
+// the factors that scale the noise of ch and of (u, v) to unit sea RMS
+__global__ void k_noise_final(const double* part, int nb, double* inv_rms) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < 4; ++c) a[c] += part[4 * b + c];
+    inv_rms[0] = a[0] > 0.0 ? sqrt(a[2] / a[0]) : 0.0;
+    inv_rms[1] = a[1] > 0.0 ? sqrt(a[3] / a[1]) : 0.0;
+  }
+}
+
+// add amp_ch (ch) and amp_u (u, v) times the unit-RMS noise to the state
+// the cycle starts from, on sea points of the interior and their periodic
+// halo copies
+__global__ void k_perturb(int ni, int H, int W, const float* nz, const double* inv_rms, double amp_ch,
+                          double amp_u, const float* stat, double* chc, double* chp, double* buc, double* bvc,
+                          double* bup, double* bvp) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long hw = (long)H * W;
+  if (q >= hw) return;
+  const double rc = inv_rms[0], ru = inv_rms[1];
+  const int x = (int)(q % W), y = (int)(q / W);
+  const double dc = stat[q] * amp_ch * rc * nz[q];
+  const double du = stat[hw + q] * amp_u * ru * nz[hw + q];
+  const double dv = stat[hw + q] * amp_u * ru * nz[2 * hw + q];
+  const int i = x + 2;
+  const int cols[2] = {i, i == W + 1 ? 1 : (i == 2 ? W + 2 : 0)};
+  for (int k = 0; k < 2; ++k) {
+    if (cols[k] == 0) continue;
+    const long o = (long)(cols[k] - 1) + (long)ni * (y + 1);
+    chc[o] += dc;
+    chp[o] = chc[o];
+    buc[o] += du;
+    bup[o] = buc[o];
+    bvc[o] += dv;
+    bvp[o] = bvc[o];
+  }
+}
+
+// ---- data movement between the snapshots and the network tensors.
+// Static per-pixel channels (stat, [kStat][HW]): T-point sea, U-point sea,
+// Coriolis / its largest magnitude, normalised bottom pressure.
 struct NnNorm {
   float in_mean[kDyn], in_std[kDyn], out_std[kOutUsed];
-  float cor_scale;
+  float cor_scale, prh_scale;
 };
-constexpr int kNormValues = 2 * kDyn + kOutUsed + 1;
+constexpr int kNormValues = 2 * kDyn + kOutUsed + 2;
 
 __device__ __forceinline__ long nn_ix(int ni, int x, int y) { return (long)(x + 1) + (long)ni * (y + 1); }
 
-// the kDyn recorded inputs of the live state: chc-1, buc, bvc, but, bvt,
-// bphi xn xs ye yw, bgraphi xn xs ye yw
-struct LiveIn {
-  const double* f[kDyn];
-};
-__device__ __forceinline__ float live_value(const LiveIn& in, int d, long k) {
-  return d == 0 ? (float)(in.f[0][k] - 1.0) : (float)in.f[d][k];
-}
-
-// record the inputs of one plain subcycle (raw values)
-__global__ void k_rec_in(int ni, int H, int W, LiveIn in, float* store) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long hw = (long)H * W;
-  if (q >= hw) return;
-  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
-#pragma unroll
-  for (int d = 0; d < kDyn; ++d) store[d * hw + q] = live_value(in, d, k);
-}
-
-// and its outputs, as increments over the recorded inputs
+// the outputs of one plain subcycle as increments over its initial state
+// (base: chc, chc, buc, bvc of the interior pixels)
 __global__ void k_rec_out(int ni, int H, int W, const double* mch2, const double* mch, const double* buc,
-                          const double* bvc, const float* in, float* tgt) {
+                          const double* bvc, const double* base, float* tgt) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
   if (q >= hw) return;
   const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
-  const double c0 = (double)in[q] + 1.0;
-  tgt[q] = (float)(mch2[k] - c0);
-  tgt[hw + q] = (float)(mch[k] - c0);
-  tgt[2 * hw + q] = (float)(buc[k] - (double)in[hw + q]);
-  tgt[3 * hw + q] = (float)(bvc[k] - (double)in[2 * hw + q]);
+  tgt[q] = (float)(mch2[k] - base[q]);
+  tgt[hw + q] = (float)(mch[k] - base[hw + q]);
+  tgt[2 * hw + q] = (float)(buc[k] - base[2 * hw + q]);
+  tgt[3 * hw + q] = (float)(bvc[k] - base[3 * hw + q]);
 }
 
-// channel moments of the store (one block per channel, FP64, fixed order)
-__global__ void k_moments(const float* store, long per_snap, long hw, int nsnap, int chan, double* out) {
+// channel moments of the store (one block per channel, FP64, fixed order);
+// the first `nscaled` channels are divided by their snapshot's scale
+__global__ void k_moments(const float* store, long per_snap, long hw, int nsnap, const float* scale, int nscaled,
+                          double* out) {
   __shared__ double s1[256], s2[256];
   const int c = blockIdx.x;
   double a = 0.0, b = 0.0;
-  for (int sn = 0; sn < nsnap; ++sn)
+  for (int sn = 0; sn < nsnap; ++sn) {
+    const double inv = c < nscaled ? 1.0 / (double)scale[sn] : 1.0;
     for (long q = threadIdx.x; q < hw; q += blockDim.x) {
-      const double v = store[(long)sn * per_snap + (long)c * hw + q];
+      const double v = store[(long)sn * per_snap + (long)c * hw + q] * inv;
       a += v;
       b += v * v;
     }
+  }
   s1[threadIdx.x] = a;
   s2[threadIdx.x] = b;
   __syncthreads();
@@ -424,59 +666,65 @@ __global__ void k_moments(const float* store, long per_snap, long hw, int nsnap,
     out[2 * c] = s1[0];
     out[2 * c + 1] = s2[0];
   }
-  (void)chan;
 }
 
-// pack a minibatch of recorded snapshots into the network input (BF16 NHWC,
-// 32 channels) and targets (FP32, 4 channels, normalised)
-__global__ void k_pack(int H, int W, const float* sin, const float* stg, const int* idx, int batch, NnNorm nm,
-                       const float* stat, bf16_t* x, float* t) {
+// one pixel's network input (BF16, 32 channels) from its raw channels and
+// the snapshot's scale
+__device__ __forceinline__ void pack_pixel(const float* raw, long hw, long q, float sc, const NnNorm& nm,
+                                           const float* stat, bf16_t* xo) {
+#pragma unroll
+  for (int d = 0; d < kDyn; ++d) {
+    const float v = raw[d * hw + q];
+    xo[dyn_channel(d)] = f2bf(d < kScaled ? v / (sc * nm.in_std[d]) : (v - nm.in_mean[d]) / nm.in_std[d]);
+  }
+  xo[4] = f2bf(stat[3 * hw + q]);
+  xo[5] = f2bf(stat[q]);
+  xo[6] = f2bf(stat[hw + q]);
+  xo[7] = f2bf(stat[2 * hw + q]);
+  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
+}
+
+// a minibatch of recorded snapshots into the network input and targets
+// (FP32, 4 channels, divided by the scale and the output normalisation)
+__global__ void k_pack(int H, int W, const float* sin, const float* stg, const float* sscale, const int* idx,
+                       int batch, NnNorm nm, const float* stat, bf16_t* x, float* t) {
   const long hw = (long)H * W;
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= hw * batch) return;
   const int b = (int)(e / hw);
   const long q = e % hw;
   const long s = idx[b];
-  const float* in = sin + s * kDyn * hw;
+  const float sc = sscale[s];
+  pack_pixel(sin + s * kDyn * hw, hw, q, sc, nm, stat, x + e * kInC);
   const float* tg = stg + s * kOutUsed * hw;
-  bf16_t* xo = x + e * kInC;
 #pragma unroll
-  for (int d = 0; d < kDyn; ++d) xo[dyn_channel(d)] = f2bf((in[d * hw + q] - nm.in_mean[d]) / nm.in_std[d]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) xo[5 + c] = f2bf(stat[c * hw + q]);
-  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
-#pragma unroll
-  for (int c = 0; c < kOutUsed; ++c) t[e * kOutUsed + c] = tg[c * hw + q] / nm.out_std[c];
+  for (int c = 0; c < kOutUsed; ++c) t[e * kOutUsed + c] = tg[c * hw + q] / (sc * nm.out_std[c]);
 }
 
-// the live state into the network input (one image)
-__global__ void k_pack_live(int ni, int H, int W, LiveIn in, NnNorm nm, const float* stat, bf16_t* x) {
+// the live snapshot into the network input (one image)
+__global__ void k_pack_live(int H, int W, const float* raw, const double* scale, NnNorm nm, const float* stat,
+                            bf16_t* x) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
   if (q >= hw) return;
-  const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
-  bf16_t* xo = x + q * kInC;
-#pragma unroll
-  for (int d = 0; d < kDyn; ++d) xo[dyn_channel(d)] = f2bf((live_value(in, d, k) - nm.in_mean[d]) / nm.in_std[d]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) xo[5 + c] = f2bf(stat[c * hw + q]);
-  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
+  pack_pixel(raw, hw, q, (float)*scale, nm, stat, x + q * kInC);
 }
 
 // prediction -> increments (FP64) with per-block sums over sea of
 // w*d(mch2), w*d(mch) and w, w = area * prh of the T points: the barotropic
 // continuity equation (mod_int.f90:189-193, 264-267) conserves sum(w chc)
-__global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const float* stat, const double* area,
-                       double* inc, double* part) {
+__global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const double* scale, const float* stat,
+                       const double* area, double* inc, double* part) {
   __shared__ double s[3][256];
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
   double v[3] = {0.0, 0.0, 0.0};
   if (q < hw) {
+    const double sc = *scale;
     const float mt = stat[q], mu = stat[hw + q];
 #pragma unroll
     for (int c = 0; c < kOutUsed; ++c)
-      inc[c * hw + q] = (double)(out[q * kOutC + c] * nm.out_std[c]) * (double)(c < 2 ? mt : mu);
+      inc[c * hw + q] = (double)(out[q * kOutC + c] * nm.out_std[c]) * sc * (double)(c < 2 ? mt : mu);
     const double ar = area[nn_ix(ni, (int)(q % W), (int)(q / W))] * (double)mt;
     v[0] = ar * inc[q];
     v[1] = ar * inc[hw + q];
@@ -587,17 +835,19 @@ __global__ void k_check(int ni, int H, int W, const double* inc, const double* c
 // squared prediction error and squared target (persistence error) per used
 // output channel, in physical units, summed over sea points of a batch
 __global__ void k_eval(long np, long hw, const float* __restrict__ out, const float* __restrict__ tgt, NnNorm nm,
-                       const float* __restrict__ stat, double* part) {
+                       const float* __restrict__ stat, const float* __restrict__ sscale, const int* __restrict__ idx,
+                       double* part) {
   __shared__ double s[9][256];
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (p < np) {
     const long q = p % hw;
+    const double sc = sscale[idx[p / hw]];
     const float mt = stat[q], mu = stat[hw + q];
     for (int c = 0; c < kOutUsed; ++c) {
       const double m = c < 2 ? mt : mu;
-      const double d = m * ((double)out[p * kOutC + c] - (double)tgt[p * kOutUsed + c]) * nm.out_std[c];
-      const double t = m * (double)tgt[p * kOutUsed + c] * nm.out_std[c];
+      const double d = m * ((double)out[p * kOutC + c] - (double)tgt[p * kOutUsed + c]) * nm.out_std[c] * sc;
+      const double t = m * (double)tgt[p * kOutUsed + c] * nm.out_std[c] * sc;
       v[c] = d * d;
       v[4 + c] = t * t;
     }
@@ -661,9 +911,18 @@ struct Nn {
   double* misc = nullptr;      // [16]: sea means, check errors
   NnNorm norm{};
   NnLayer L[kLayers];
-  float* store_in = nullptr;   // [cap][5][HW]
-  float* store_tg = nullptr;   // [cap][4][HW]
-  float* stat = nullptr;       // [3][HW] sea T, sea U, Coriolis / scale
+  float* store_in = nullptr;   // [cap][kDyn][HW] raw inputs
+  float* store_tg = nullptr;   // [cap][4][HW] increments
+  float* store_sc = nullptr;   // [cap] scale of each snapshot
+  float* stat = nullptr;       // [kStat][HW] sea T, sea U, Coriolis / scale, prh / scale
+  float* live = nullptr;       // [kDyn][HW] raw inputs of the live state
+  double *chpred = nullptr, *u1 = nullptr, *v1 = nullptr, *sch1 = nullptr;  // [n2] first substep
+  // perturbed snapshots: fraction of the recorded ones, noise amplitudes
+  // (ch, m/s), seed; state backup [9][n2] and noise [2][3][HW]
+  double aug_fraction = 0.0, aug_amp_ch = 0.0, aug_amp_u = 0.0;
+  unsigned long long aug_seed = 0, aug_count = 0;
+  double* bak = nullptr;
+  float* noise = nullptr;
   int batch_max = 0;
   bf16_t* act[kLayers + 1] = {};  // layer inputs / outputs (BF16), act[0] = x
   float* out = nullptr;        // last layer output (FP32) [batch][HW][32]
@@ -724,13 +983,16 @@ std::string nn_batch_buffers(Nn& n, int batch) {
 
 // forward pass over `batch` images already packed into act[0]
 std::string nn_forward(Nn& n, hipStream_t st, int batch) {
-  Geo g{batch, n.H, n.W};
+  Geo g{batch, n.H, n.W, layer_dil(0)};
   const unsigned blocks = (unsigned)((g.npix() / 16 + 3) / 4);
   NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
             (void*)n.act[1]);
-  for (int l = 1; l < kLayers - 1; ++l)
+  for (int l = 1; l < kLayers - 1; ++l) {
+    g.dil = layer_dil(l);
     NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
               (void*)n.act[l + 1]);
+  }
+  g.dil = layer_dil(kLayers - 1);
   NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
             n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
   return "";
@@ -738,13 +1000,14 @@ std::string nn_forward(Nn& n, hipStream_t st, int batch) {
 
 // backward pass from dpre of the last layer (in act[kLayers], BF16)
 std::string nn_backward(Nn& n, hipStream_t st, int batch) {
-  Geo g{batch, n.H, n.W};
+  Geo g{batch, n.H, n.W, 1};
   const long np = g.npix();
   const unsigned blocks = (unsigned)((np / 16 + 3) / 4);
   const int nch = (int)((np + n.wchunk - 1) / n.wchunk);
   bf16_t* dpre = n.act[kLayers];  // d loss / d (last pre-activation), 32 channels
   for (int l = kLayers - 1; l >= 0; --l) {
     NnLayer& L = n.L[l];
+    g.dil = layer_dil(l);
     // weight and bias gradients
     if (L.cin == kInC && L.cout == kHid)
       NN_LAUNCH((k_conv_bwd_w<kInC, kHid>), dim3(nch, 9), dim3(256), g, n.act[l], dpre, n.wpart, n.wchunk);
@@ -834,16 +1097,22 @@ std::string nn_create(Ctx& c, int capacity, unsigned long long seed) {
     if (hipMemcpy(L.w32, hw32.data(), nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
       return "upload surrogate weights";
   }
-  // static channels: sea masks and Coriolis of the interior
-  std::vector<float> stat((size_t)3 * hw);
-  double cmax = 0.0;
-  for (long q = 0; q < hw; ++q) cmax = std::max(cmax, std::fabs(c.hs.cor[(q % W) + 1 + (size_t)c.ni * (q / W + 1)]));
+  // static channels: sea masks, Coriolis and bottom pressure of the interior
+  std::vector<float> stat((size_t)kStat * hw);
+  double cmax = 0.0, pmax = 0.0;
+  for (long q = 0; q < hw; ++q) {
+    const size_t k = (size_t)(q % W) + 1 + (size_t)c.ni * (q / W + 1);
+    cmax = std::max(cmax, std::fabs(c.hs.cor[k]));
+    if (c.hs.lev_t[k] > 0) pmax = std::max(pmax, c.hs.prh_t[k]);
+  }
   n->norm.cor_scale = (float)(cmax > 0 ? cmax : 1.0);
+  n->norm.prh_scale = (float)(pmax > 0 ? pmax : 1.0);
   for (long q = 0; q < hw; ++q) {
     const size_t k = (size_t)(q % W) + 1 + (size_t)c.ni * (q / W + 1);
     stat[q] = c.hs.lev_t[k] > 0 ? 1.f : 0.f;
     stat[hw + q] = c.hs.lev_u[k] > 0 ? 1.f : 0.f;
     stat[2 * hw + q] = (float)(c.hs.cor[k] / n->norm.cor_scale);
+    stat[3 * hw + q] = c.hs.lev_t[k] > 0 ? (float)(c.hs.prh_t[k] / n->norm.prh_scale) : 0.f;
   }
   {
     std::vector<double> w(c.n2);
@@ -852,12 +1121,19 @@ std::string nn_create(Ctx& c, int capacity, unsigned long long seed) {
     if (hipMemcpy(n->massw, w.data(), c.n2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return "upload surrogate mass weights";
   }
-  if (!(e = nn_make(*n, &n->stat, (size_t)3 * hw)).empty()) return e;
+  if (!(e = nn_make(*n, &n->stat, (size_t)kStat * hw)).empty()) return e;
   if (hipMemcpy(n->stat, stat.data(), stat.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return "upload surrogate statics";
+  double** sub1[] = {&n->chpred, &n->u1, &n->v1, &n->sch1};
+  for (double** q : sub1)
+    if (!(e = nn_make(*n, q, c.n2)).empty()) return e;
+  if (!(e = nn_make(*n, &n->live, (size_t)kDyn * hw)).empty()) return e;
+  if (!(e = nn_make(*n, &n->bak, (size_t)9 * c.n2)).empty()) return e;
+  if (!(e = nn_make(*n, &n->noise, (size_t)6 * hw)).empty()) return e;
   if (capacity > 0) {
     if (!(e = nn_make(*n, &n->store_in, (size_t)capacity * kDyn * hw)).empty()) return e;
     if (!(e = nn_make(*n, &n->store_tg, (size_t)capacity * kOutUsed * hw)).empty()) return e;
+    if (!(e = nn_make(*n, &n->store_sc, (size_t)capacity)).empty()) return e;
   }
   if (!(e = nn_make(*n, &n->inc, (size_t)4 * hw)).empty()) return e;
   if (!(e = nn_make(*n, &n->misc, (size_t)24)).empty()) return e;
@@ -882,6 +1158,17 @@ void nn_set_record(Ctx& c, bool on) {
   if (c.nn) c.nn->recording = on && c.nn->cap > 0;
 }
 
+std::string nn_set_augment(Ctx& c, double fraction, double amp_ch, double amp_u, unsigned long long seed) {
+  if (!c.nn) return "no surrogate attached";
+  if (!(fraction >= 0.0 && fraction <= 1.0) || !(amp_ch >= 0.0) || !(amp_u >= 0.0)) return "bad augmentation";
+  c.nn->aug_fraction = fraction;
+  c.nn->aug_amp_ch = amp_ch;
+  c.nn->aug_amp_u = amp_u;
+  c.nn->aug_seed = seed;
+  c.nn->aug_count = 0;
+  return "";
+}
+
 std::string nn_set_use(Ctx& c, int every, double tol) {
   if (!c.nn) return "no surrogate attached";
   if (every > 0 && !c.nn->normalised) return "the surrogate is untrained (no normalisation statistics)";
@@ -891,13 +1178,34 @@ std::string nn_set_use(Ctx& c, int every, double tol) {
 }
 
 namespace {
-LiveIn live_inputs(const Ctx& c) {
+// the raw inputs of the live state (the first substep's increment and the
+// fields the subcycle's operator depends on) into `raw`, and its scale into
+// misc[20] (and *keep when recording)
+std::string nn_raw_inputs(Ctx& c, float* raw, float* keep) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
   const Fields2& h = c.f2;
-  LiveIn in;
-  const double* f[kDyn] = {h.chc, h.buc, h.bvc, h.but, h.bvt, h.bphi[0], h.bphi[1], h.bphi[2], h.bphi[3],
-                           h.bgra[0], h.bgra[1], h.bgra[2], h.bgra[3]};
-  for (int d = 0; d < kDyn; ++d) in.f[d] = f[d];
-  return in;
+  const long hw = (long)n->H * n->W;
+  const double bt = (double)c.dtbt;
+  NN_LAUNCH(k_sub1_pred, dim3(nblk((long)c.n2)), dim3(256), c.gd, h.chp, h.bup, h.bvp, h.chc, bt, n->chpred);
+  Sub1In a;
+  a.bup = h.bup; a.bvp = h.bvp; a.but = h.but; a.bvt = h.bvt;
+  a.ghx = h.graphihx; a.ghy = h.graphihy; a.cor = c.gd.cor;
+  for (int q = 0; q < 4; ++q) {
+    a.bphi[q] = h.bphi[q];
+    a.bgra[q] = h.bgra[q];
+  }
+  NN_LAUNCH(k_sub1_mom, dim3(nblk((long)c.n2)), dim3(256), c.gd, a, n->chpred, bt, n->u1, n->v1, n->sch1);
+  RawIn r;
+  r.chp = h.chp; r.bup = h.bup; r.bvp = h.bvp; r.u1 = n->u1; r.v1 = n->v1; r.sch1 = n->sch1; r.am1 = h.am1;
+  for (int q = 0; q < 4; ++q) {
+    r.bphi[q] = h.bphi[q];
+    r.bgra[q] = h.bgra[q];
+  }
+  NN_LAUNCH(k_sub1_raw, dim3(nblk(hw)), dim3(256), c.gd, n->H, n->W, r, bt, raw);
+  NN_LAUNCH(k_scale_part, dim3(nblk(hw)), dim3(256), n->H, n->W, raw, n->stat, n->dpart);
+  NN_LAUNCH(k_scale_final, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc + 20, keep);
+  return "";
 }
 
 // the network's increments for the live state into n->inc (mass-corrected)
@@ -905,14 +1213,72 @@ std::string nn_predict(Ctx& c) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
   const long hw = (long)n->H * n->W;
-  NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c), n->norm, n->stat,
-            n->act[0]);
-  std::string e = nn_forward(*n, st, 1);
+  std::string e = nn_raw_inputs(c, n->live, nullptr);
   if (!e.empty()) return e;
-  NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->stat, n->massw, n->inc,
-            n->dpart);
+  NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->H, n->W, n->live, n->misc + 20, n->norm, n->stat, n->act[0]);
+  e = nn_forward(*n, st, 1);
+  if (!e.empty()) return e;
+  NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->misc + 20, n->stat, n->massw,
+            n->inc, n->dpart);
   NN_LAUNCH(k_pred_mean, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc);
   NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
+  return "";
+}
+}  // namespace
+
+namespace {
+// with probability aug_fraction, two extra recorded snapshots: the state
+// perturbed by seeded noise (random smoothness, amplitudes log-uniform in
+// [0.02, 1] times aug_amp_*, on ch, on (u, v) or on both) and the state its
+// plain cycle leaves, each with its plain cycle; then the state is
+// restored and the step itself follows unperturbed
+std::string nn_record_perturbed(Ctx& c, std::string (*plain)(Ctx&)) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  Fields2& h = c.f2;
+  const long hw = (long)n->H * n->W;
+  std::mt19937_64 rng(n->aug_seed + 7919ull * (n->aug_count++));
+  std::uniform_real_distribution<double> u01(0.0, 1.0);
+  if (!(u01(rng) < n->aug_fraction)) return "";
+  static const int passes[5] = {0, 1, 3, 8, 20};
+  const int npass = passes[(int)(u01(rng) * 5.0) % 5];
+  const double fc = std::exp(std::log(0.02) + u01(rng) * std::log(50.0));
+  const double fu = std::exp(std::log(0.02) + u01(rng) * std::log(50.0));
+  const int which = (int)(u01(rng) * 3.0) % 3;  // 0 both, 1 ch only, 2 (u, v) only
+  const double amp_ch = which == 2 ? 0.0 : fc * n->aug_amp_ch, amp_u = which == 1 ? 0.0 : fu * n->aug_amp_u;
+  double* fields[9] = {h.chc, h.chp, h.mch, h.mch2, h.acch, h.buc, h.bvc, h.bup, h.bvp};
+  const size_t n2 = c.n2;
+  for (int f = 0; f < 9; ++f)
+    if (hipMemcpyAsync(n->bak + f * n2, fields[f], n2 * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return "surrogate state backup";
+  float* nz = n->noise;
+  float* tmp = n->noise + 3 * hw;
+  NN_LAUNCH(k_noise, dim3(nblk(3 * hw)), dim3(256), n->H, n->W, rng(), nz);
+  for (int k = 0; k < npass; ++k) {
+    NN_LAUNCH(k_smooth, dim3(nblk(3 * hw)), dim3(256), n->H, n->W, nz, tmp);
+    std::swap(nz, tmp);
+  }
+  NN_LAUNCH(k_noise_part, dim3(nblk(hw)), dim3(256), n->H, n->W, nz, n->stat, n->dpart);
+  NN_LAUNCH(k_noise_final, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc + 22);
+  NN_LAUNCH(k_perturb, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, nz, n->misc + 22, amp_ch, amp_u, n->stat, h.chc,
+            h.chp, h.buc, h.bvc, h.bup, h.bvp);
+  // the perturbed state's cycle, then the cycle from what it left: its
+  // averaging keeps the slow, nearly balanced part of the perturbation,
+  // whose evolution the network must follow as closely as it damps the rest
+  for (int pass = 0; pass < 2; ++pass) {
+    const int slot = n->cursor;
+    NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
+    std::string e = nn_raw_inputs(c, n->store_in + (long)slot * kDyn * hw, n->store_sc + slot);
+    if (!e.empty()) return e;
+    if (!(e = plain(c)).empty()) return e;
+    NN_LAUNCH(k_rec_out, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.mch2, h.mch, h.buc, h.bvc, n->base,
+              n->store_tg + (long)slot * kOutUsed * hw);
+    n->cursor = (n->cursor + 1) % n->cap;
+    n->nrec = std::min(n->nrec + 1, n->cap);
+  }
+  for (int f = 0; f < 9; ++f)
+    if (hipMemcpyAsync(fields[f], n->bak + f * n2, n2 * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return "surrogate state restore";
   return "";
 }
 }  // namespace
@@ -936,13 +1302,19 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
     n->surrogate_steps += 1;
     return "";
   }
+  if (n->recording && n->aug_fraction > 0.0) {
+    std::string e = nn_record_perturbed(c, plain);
+    if (!e.empty()) return e;
+  }
   const int slot = n->cursor;
-  if (n->recording)
-    NN_LAUNCH(k_rec_in, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, live_inputs(c),
-              n->store_in + (long)slot * kDyn * hw);
+  if (n->recording || check)
+    NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
+  if (n->recording) {
+    std::string e = nn_raw_inputs(c, n->store_in + (long)slot * kDyn * hw, n->store_sc + slot);
+    if (!e.empty()) return e;
+  }
   if (check) {
     // the prediction the network would have made, checked by the plain cycle
-    NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
     std::string e = nn_predict(c);
     if (!e.empty()) return e;
   }
@@ -950,8 +1322,8 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   if (!e.empty()) return e;
   n->plain_steps += 1;
   if (n->recording) {
-    NN_LAUNCH(k_rec_out, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.mch2, h.mch, h.buc, h.bvc,
-              n->store_in + (long)slot * kDyn * hw, n->store_tg + (long)slot * kOutUsed * hw);
+    NN_LAUNCH(k_rec_out, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.mch2, h.mch, h.buc, h.bvc, n->base,
+              n->store_tg + (long)slot * kOutUsed * hw);
     n->cursor = (n->cursor + 1) % n->cap;
     n->nrec = std::min(n->nrec + 1, n->cap);
   }
@@ -989,9 +1361,10 @@ std::string nn_normalise(Ctx& c) {
   if (n->nrec < 1) return "no recorded snapshots";
   const long hw = (long)n->H * n->W;
   double mom[2 * (kDyn + kOutUsed)];
-  NN_LAUNCH(k_moments, dim3(kDyn), dim3(256), n->store_in, (long)kDyn * hw, hw, n->nrec, kDyn, n->dpart);
-  NN_LAUNCH(k_moments, dim3(kOutUsed), dim3(256), n->store_tg, (long)kOutUsed * hw, hw, n->nrec, kOutUsed,
-            n->dpart + 2 * kDyn);
+  NN_LAUNCH(k_moments, dim3(kDyn), dim3(256), n->store_in, (long)kDyn * hw, hw, n->nrec, n->store_sc, kScaled,
+            n->dpart);
+  NN_LAUNCH(k_moments, dim3(kOutUsed), dim3(256), n->store_tg, (long)kOutUsed * hw, hw, n->nrec, n->store_sc,
+            kOutUsed, n->dpart + 2 * kDyn);
   // the null-stream copy does not wait for the context's (non-blocking) stream
   if (hipStreamSynchronize(st) != hipSuccess ||
       hipMemcpy(mom, n->dpart, sizeof mom, hipMemcpyDeviceToHost) != hipSuccess)
@@ -1002,7 +1375,11 @@ std::string nn_normalise(Ctx& c) {
     double var = mom[2 * q + 1] / cnt - mean * mean;
     double sd = var > 0 ? std::sqrt(var) : 0.0;
     if (!(sd > 1e-12)) sd = 1.0;
-    if (q < kDyn) {
+    if (q < kScaled) {  // r / s: a root-mean-square scale, no offset (the map is linear in r)
+      const double ms = mom[2 * q + 1] / cnt;
+      n->norm.in_mean[q] = 0.f;
+      n->norm.in_std[q] = (float)(ms > 0 ? std::sqrt(ms) : 1.0);
+    } else if (q < kDyn) {
       n->norm.in_mean[q] = (float)mean;
       n->norm.in_std[q] = (float)sd;
     } else {
@@ -1031,8 +1408,8 @@ std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long
     if (hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(n->idx, idx.data(), batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return "minibatch upload";
-    NN_LAUNCH(k_pack, dim3(nblk(hw * batch)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->idx, batch, n->norm,
-              n->stat, n->act[0], n->tgt);
+    NN_LAUNCH(k_pack, dim3(nblk(hw * batch)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->store_sc, n->idx,
+              batch, n->norm, n->stat, n->act[0], n->tgt);
     if (!(e = nn_forward(*n, st, batch)).empty()) return e;
     if (!(e = nn_loss(*n, st, batch, ldev)).empty()) return e;
     if (!(e = nn_backward(*n, st, batch)).empty()) return e;
@@ -1071,12 +1448,12 @@ std::string nn_eval(Ctx& c, int first, int count, double* res) {
     if (hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(n->idx, idx.data(), b * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return "index upload";
-    NN_LAUNCH(k_pack, dim3(nblk(hw * b)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->idx, b, n->norm,
-              n->stat, n->act[0], n->tgt);
+    NN_LAUNCH(k_pack, dim3(nblk(hw * b)), dim3(256), n->H, n->W, n->store_in, n->store_tg, n->store_sc, n->idx, b,
+              n->norm, n->stat, n->act[0], n->tgt);
     if (!(e = nn_forward(*n, st, b)).empty()) return e;
     const long np = hw * b;
     const int nb = (int)nblk(np);
-    NN_LAUNCH(k_eval, dim3(nb), dim3(256), np, hw, n->out, n->tgt, n->norm, n->stat, n->dpart);
+    NN_LAUNCH(k_eval, dim3(nb), dim3(256), np, hw, n->out, n->tgt, n->norm, n->stat, n->store_sc, n->idx, n->dpart);
     part.resize((size_t)9 * nb);
     if (hipMemcpyAsync(part.data(), n->dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
@@ -1122,8 +1499,13 @@ std::string nn_params(Ctx& c, float* host, bool set) {
       if (set) nv[a][q] = host[off];
       else host[off] = nv[a][q];
     }
-  if (set) n->norm.cor_scale = host[off];
-  else host[off] = n->norm.cor_scale;
+  if (set) {
+    n->norm.cor_scale = host[off];
+    n->norm.prh_scale = host[off + 1];
+  } else {
+    host[off] = n->norm.cor_scale;
+    host[off + 1] = n->norm.prh_scale;
+  }
   if (set) {
     n->normalised = true;
     hipStream_t st = c.stream;

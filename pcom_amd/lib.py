@@ -31,7 +31,7 @@ EXPORTS = [
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
     "pcom_set_clock", "pcom_output_means", "pcom_forcing_due", "pcom_set_forcing",
     "pcom_output_means_noalloc",
-    "pcom_nn_create", "pcom_nn_record", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
+    "pcom_nn_create", "pcom_nn_record", "pcom_nn_augment", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
     "pcom_tune_subcycle", "pcom_subcycle_tiling",
@@ -82,6 +82,7 @@ def lib():
         fp = C.POINTER(C.c_float)
         L.pcom_nn_create.argtypes = [vp, C.c_int, C.c_ulonglong]
         L.pcom_nn_record.argtypes = [vp, C.c_int]
+        L.pcom_nn_augment.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_ulonglong]
         L.pcom_nn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, dp]
         L.pcom_nn_use.argtypes = [vp, C.c_int, C.c_double]
         L.pcom_nn_eval.argtypes = [vp, C.c_int, C.c_int, dp]
@@ -292,6 +293,10 @@ class Model:
 
     def nn_record(self, on=True):
         _check(lib().pcom_nn_record(self.h, int(bool(on))))
+
+    def nn_augment(self, fraction, amp_ch, amp_u, seed=1):
+        """Record perturbed snapshots beside the plain ones (pcom_nn_augment)."""
+        _check(lib().pcom_nn_augment(self.h, float(fraction), float(amp_ch), float(amp_u), int(seed)))
 
     def nn_train(self, iters, batch=4, lr=1e-3, seed=1):
         loss = np.zeros(max(int(iters), 1))

@@ -8,7 +8,10 @@
    RMSE against the plain subcycle beside the RMSE of persistence (the
    step without the network: no change of the barotropic mode);
 5. time plain steps and surrogate steps (plain judging subcycle every
-   `check_every`-th step).
+   `check_every`-th step);
+6. with --closed-loop-days D: run D days with the network in use beside a
+   plain twin started from the same state, and report the RMSE of SSH,
+   SST, the free surface and the barotropic velocity between them daily.
 
     python -m pcom_amd.surrogate --config global --spin-days 10 \\
         --snapshots 480 --iters 9000 --out report.json
@@ -26,16 +29,35 @@ from .lib import Model
 OUTPUTS = ["mch2", "mch", "ubt", "vbt"]
 
 
+def step_scales(m, cfg):
+    """Sea RMS of one step's change of ch and of the barotropic velocity
+    (the model takes that step): the size of the perturbations the
+    augmented snapshots use."""
+    from . import rmse
+    area, lev_t, inner, wet_u = rmse.weights(m, cfg)
+    sea = (lev_t > 0) & inner
+    c0, u0, v0 = m.get("chc"), m.get("buc"), m.get("bvc")
+    m.step(1)
+    dc = m.get("chc") - c0
+    du = np.concatenate([(m.get("buc") - u0)[wet_u], (m.get("bvc") - v0)[wet_u]])
+    return float(np.sqrt(np.mean(dc[sea] ** 2))), float(np.sqrt(np.mean(du ** 2)))
+
+
 def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=48, iters=9000, batch=8,
-            lr=1e-3, seed=1, log=print):
-    """Spin up, record, train and evaluate; returns (model, report)."""
+            lr=1e-3, seed=1, augment=0.5, amp=1.0, log=print):
+    """Spin up, record (with a fraction `augment` of perturbed snapshots
+    beside the plain ones), train and evaluate; returns (model, report).
+    report["steps"] is the number of steps the model took."""
     spd = 86400 // cfg.dtbc
     m = Model(cfg, precision=precision)
     m.step(spin_days * spd)
-    m.nn_create(capacity=snapshots + holdout, seed=seed)
+    amp_ch, amp_u = step_scales(m, cfg)
+    m.nn_create(capacity=int(snapshots * (1 + 2 * augment)) + holdout + 8, seed=seed)
+    m.nn_augment(augment, amp * amp_ch, amp * amp_u, seed=seed + 100)
     m.nn_record(True)
     m.step(snapshots)
     m.nn_record(False)
+    m.nn_augment(0.0, 0.0, 0.0)
     t0 = time.time()
     parts = []
     for chunk in range(0, iters, 1000):
@@ -45,12 +67,15 @@ def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=
     loss = np.concatenate(parts)
     train_s = time.time() - t0
     log(f"trained {iters} iterations in {train_s:.1f} s, loss {loss[:20].mean():.3e} -> {loss[-100:].mean():.3e}")
+    first = m.nn_stats().recorded
     m.nn_record(True)
     m.step(holdout)
     m.nn_record(False)
-    net, per = m.nn_eval(snapshots, holdout)
+    net, per = m.nn_eval(first, holdout)
     report = {
         "config": cfg.name, "spin_days": spin_days, "snapshots": snapshots, "holdout": holdout,
+        "augment": augment, "perturbation_scales": [amp_ch, amp_u], "recorded": first + holdout,
+        "steps": spin_days * spd + 1 + snapshots + holdout,
         "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
         "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-100:].mean()),
         "holdout_rmse_network": dict(zip(OUTPUTS, map(float, net))),
@@ -59,6 +84,30 @@ def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=
     }
     log("held-out gain over persistence: " + ", ".join(f"{k} {v:.3f}" for k, v in report["holdout_gain"].items()))
     return m, report
+
+
+def closed_loop(m, cfg, steps_done, days, check_every, tol=0.0, log=print):
+    """The network in use for `days` days beside a plain twin from the same
+    state (the twin repeats the `steps_done` steps the model has taken;
+    recording does not change the state); daily RMSE between the two."""
+    from . import rmse
+    twin = Model(cfg, precision=m.precision)
+    twin.step(steps_done)
+    spd = 86400 // cfg.dtbc
+    m.nn_use(check_every, tol)
+    recs = []
+    for d in range(days):
+        m.step(spd)
+        twin.step(spd)
+        r = rmse.rmse_report(m, twin, cfg)
+        st = m.nn_stats()
+        rec = {"day": d + 1, "sst": r["sst"], "ssh": r["ssh"], "chc": r["chc"], "buc": r["buc"], "bvc": r["bvc"],
+               "tc": r["tc"], "uc": r["uc"], "in_use": bool(st.in_use), "surrogate_steps": st.surrogate_steps,
+               "check_rmse": list(st.last_check_rmse), "persist_rmse": list(st.last_persist_rmse)}
+        recs.append(rec)
+        log(json.dumps(rec))
+    twin.close()
+    return recs
 
 
 def time_steps(m, nsteps, check_every):
@@ -90,11 +139,18 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--time-steps", type=int, default=240)
+    ap.add_argument("--augment", type=float, default=0.5, help="fraction of recorded steps with a perturbed snapshot")
+    ap.add_argument("--amp", type=float, default=1.0,
+                    help="largest perturbation, in units of one step's change of ch and velocity")
+    ap.add_argument("--closed-loop-days", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=0.0, help="fallback bound of the closed-loop run (pcom_nn_use)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     cfg = configs.CONFIGS[args.config]()
     m, rep = prepare(cfg, abi.FORMS[args.form], args.spin_days, args.snapshots, args.holdout, args.iters,
-                     args.batch, args.lr)
+                     args.batch, args.lr, augment=args.augment, amp=args.amp)
+    if args.closed_loop_days > 0:
+        rep["closed_loop"] = closed_loop(m, cfg, rep["steps"], args.closed_loop_days, args.check_every, args.tol)
     plain, sur = time_steps(m, args.time_steps, args.check_every)
     st = m.nn_stats()
     rep.update({"form": args.form, "check_every": args.check_every, "ms_per_step_plain": plain,

@@ -41,6 +41,36 @@ def test_conv_matches_brute_force():
     np.testing.assert_allclose(out, bf, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("dil", [2, 4])
+def test_dilated_conv_matches_brute_force(dil):
+    """Taps dil pixels apart, periodic in x (also past a full period) and
+    zero outside the rows."""
+    rng = np.random.default_rng(dil)
+    x = rng.standard_normal((1, 9, 5, 2))
+    w = rng.standard_normal((2, 3, 3, 2))
+    b = np.zeros(2)
+    out = ref.conv(x, w, b, dil)
+    bf = np.zeros_like(out)
+    for y in range(9):
+        for xx in range(5):
+            for co in range(2):
+                s = 0.0
+                for dy in range(3):
+                    for dx in range(3):
+                        yy = y + (dy - 1) * dil
+                        if 0 <= yy < 9:
+                            s += np.dot(w[co, dy, dx], x[0, yy, (xx + (dx - 1) * dil) % 5])
+                bf[0, y, xx, co] = s
+    np.testing.assert_allclose(out, bf, rtol=1e-12, atol=1e-12)
+    # the data gradient is the adjoint of the convolution
+    d = rng.standard_normal(out.shape)
+    lhs = np.sum(ref.conv(x, w, b, dil) * d)
+    rhs = np.sum(x * ref.conv_data_grad(d, w, dil))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    g = ref.conv_weight_grad(x, d, dil)
+    assert abs(np.sum(g * w) - lhs) <= 1e-10 * abs(lhs)
+
+
 def test_bf16_rounding_cases():
     v = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, -2.5, 3.0e-3, 65504.0], np.float32)
     got = ref.bf16(v)

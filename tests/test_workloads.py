@@ -139,24 +139,28 @@ def test_config4_network_at_1deg_matches_oracle(gpu_lib):
 
 
 def test_config4_record_train_use_at_1deg(gpu_lib):
-    """At 1 degree: record plain subcycles, train on the device, then step
-    with the network and a plain checking subcycle every 10th step; the
-    network learns (loss falls), the run stays finite and conserves the
-    mass of the plain run."""
+    """At 1 degree: record plain subcycles (half of the steps with perturbed
+    snapshots beside them), train on the device, then step with the network
+    and a plain checking subcycle every 10th step; the network learns (loss
+    falls), the run stays finite and conserves the mass of the plain run."""
     from pcom_amd.lib import Model
+    from pcom_amd.surrogate import step_scales
     cfg = configs.global_1deg()
     m = Model(cfg, precision=abi.PCOM_MIXED)
     plain = Model(cfg, precision=abi.PCOM_MIXED)
-    _say("config 4: record 96 steps, train 600 iterations, use 20 steps")
-    m.step(24)
-    plain.step(24)
-    m.nn_create(capacity=96, seed=3)
+    _say("config 4: record 240 steps, train 3000 iterations, use 20 steps")
+    m.step(120)
+    amp_ch, amp_u = step_scales(m, cfg)
+    plain.step(121)
+    m.nn_create(capacity=500, seed=3)
+    m.nn_augment(0.5, amp_ch, amp_u, seed=9)
     m.nn_record(True)
-    m.step(96)
-    plain.step(96)
+    m.step(240)
+    plain.step(240)
     m.nn_record(False)
-    loss = m.nn_train(600, batch=4, lr=1e-3, seed=4)
-    assert np.all(np.isfinite(loss)) and loss[-50:].mean() < 0.7 * loss[:10].mean()
+    assert m.nn_stats().recorded > 240
+    loss = m.nn_train(3000, batch=8, lr=1e-3, seed=4)
+    assert np.all(np.isfinite(loss)) and loss[-50:].mean() < 0.5 * loss[:10].mean()
     m.nn_use(10, tol=0.0)
     m.step(20)
     plain.step(20)
