@@ -99,175 +99,238 @@ __device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uin
 
 // ---- forward 3x3 convolution: out[p][co] = b[co] + sum W[co][dy][dx][ci]
 // in[p + d (dy-1, dx-1)][ci] (d = g.dil), periodic in x, zero outside
-// 0 <= y < H.  One
-// wave computes 16 pixels x COUT channels; 4 waves per workgroup.
-template <int CIN, int COUT, bool RELU, bool OUTF32>
+// 0 <= y < H.  One wave computes PB blocks of 16 pixels x COUT channels, so
+// each weight fragment it loads serves PB matrix-core products; 4 waves per
+// workgroup.
+template <int CIN, int COUT, bool RELU, bool OUTF32, int PB>
 __global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
                                                   const float* __restrict__ bias, void* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long np = g.npix();
-  const long p0 = ((long)xcd_block() * 4 + wave) * 16;
-  if (p0 >= np) return;
+  const long pb0 = ((long)xcd_block() * 4 + wave) * PB;  // first 16-pixel block of this wave
+  if (pb0 * 16 >= np) return;
   const int kq = lane >> 4, r = lane & 15;
-  const long pa = p0 + r;
-  const int x = (int)(pa % g.W);
-  const long t = pa / g.W;
-  const int y = (int)(t % g.H), n = (int)(t / g.H);
-  f4 acc[COUT / 16];
+  int x[PB], y[PB], n[PB];
+  bool live[PB];
 #pragma unroll
-  for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
+  for (int b = 0; b < PB; ++b) {
+    const long pa = (pb0 + b) * 16 + r;
+    live[b] = (pb0 + b) * 16 < np;
+    const long pc = live[b] ? pa : 0;
+    x[b] = (int)(pc % g.W);
+    const long t = pc / g.W;
+    y[b] = (int)(t % g.H);
+    n[b] = (int)(t / g.H);
+  }
+  f4 acc[PB][COUT / 16];
+#pragma unroll
+  for (int b = 0; b < PB; ++b)
+#pragma unroll
+    for (int nb = 0; nb < COUT / 16; ++nb) acc[b][nb] = f4{0, 0, 0, 0};
 #pragma unroll
   for (int dy = 0; dy < 3; ++dy) {
-    const int yy = y + (dy - 1) * g.dil;
-    const bool ok = yy >= 0 && yy < g.H;
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
-      const int xx = g.wrap(x + (dx - 1) * g.dil);
-      const bf16_t* src = in + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * CIN + kq * 8;
+      const bf16_t* src[PB];
+      bool ok[PB];
+#pragma unroll
+      for (int b = 0; b < PB; ++b) {
+        const int yy = y[b] + (dy - 1) * g.dil;
+        ok[b] = live[b] && yy >= 0 && yy < g.H;
+        const int xx = g.wrap(x[b] + (dx - 1) * g.dil);
+        src[b] = in + (((long)n[b] * g.H + (ok[b] ? yy : 0)) * g.W + xx) * CIN + kq * 8;
+      }
 #pragma unroll
       for (int c = 0; c < CIN / 32; ++c) {
-        const bf8 a = ok ? load8(src + c * 32) : zero8();
+        bf8 a[PB];
+#pragma unroll
+        for (int b = 0; b < PB; ++b) a[b] = ok[b] ? load8(src[b] + c * 32) : zero8();
 #pragma unroll
         for (int nb = 0; nb < COUT / 16; ++nb) {
-          const bf8 b = load8(w + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * CIN + c * 32 + kq * 8);
-          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+          const bf8 wb = load8(w + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * CIN + c * 32 + kq * 8);
+#pragma unroll
+          for (int b = 0; b < PB; ++b) acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], wb, acc[b][nb], 0, 0, 0);
         }
       }
     }
   }
 #pragma unroll
-  for (int nb = 0; nb < COUT / 16; ++nb) {
-    const int co = nb * 16 + r;
-    const float bb = bias[co];
+  for (int b = 0; b < PB; ++b) {
+    if (!live[b]) continue;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long pd = p0 + kq * 4 + i;
-      float v = acc[nb][i] + bb;
-      if (RELU) v = v > 0.f ? v : 0.f;
-      if (OUTF32) static_cast<float*>(out)[pd * COUT + co] = v;
-      else static_cast<bf16_t*>(out)[pd * COUT + co] = f2bf(v);
+    for (int nb = 0; nb < COUT / 16; ++nb) {
+      const int co = nb * 16 + r;
+      const float bb = bias[co];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long pd = (pb0 + b) * 16 + kq * 4 + i;
+        float v = acc[b][nb][i] + bb;
+        if (RELU) v = v > 0.f ? v : 0.f;
+        if (OUTF32) static_cast<float*>(out)[pd * COUT + co] = v;
+        else static_cast<bf16_t*>(out)[pd * COUT + co] = f2bf(v);
+      }
     }
   }
 }
 
 // ---- backward data: din[p][ci] = sum W[co][dy][dx][ci] dpre[p - d (dy-1, dx-1)][co]
-// with wt = W transposed to [ci][dy][dx][co].
-template <int CIN, int COUT>
+// with wt = W transposed to [ci][dy][dx][co], masked by the ReLU that made
+// this layer's input (post > 0); PB pixel blocks per wave as in the forward
+// pass.
+template <int CIN, int COUT, int PB>
 __global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __restrict__ dpre,
-                                                       const bf16_t* __restrict__ wt, bf16_t* din) {
+                                                       const bf16_t* __restrict__ wt, const bf16_t* __restrict__ post,
+                                                       bf16_t* din) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long np = g.npix();
-  const long p0 = ((long)xcd_block() * 4 + wave) * 16;
-  if (p0 >= np) return;
+  const long pb0 = ((long)xcd_block() * 4 + wave) * PB;
+  if (pb0 * 16 >= np) return;
   const int kq = lane >> 4, r = lane & 15;
-  const long pa = p0 + r;
-  const int x = (int)(pa % g.W);
-  const long t = pa / g.W;
-  const int y = (int)(t % g.H), n = (int)(t / g.H);
-  f4 acc[CIN / 16];
+  int x[PB], y[PB], n[PB];
+  bool live[PB];
 #pragma unroll
-  for (int nb = 0; nb < CIN / 16; ++nb) acc[nb] = f4{0, 0, 0, 0};
+  for (int b = 0; b < PB; ++b) {
+    const long pa = (pb0 + b) * 16 + r;
+    live[b] = (pb0 + b) * 16 < np;
+    const long pc = live[b] ? pa : 0;
+    x[b] = (int)(pc % g.W);
+    const long t = pc / g.W;
+    y[b] = (int)(t % g.H);
+    n[b] = (int)(t / g.H);
+  }
+  f4 acc[PB][CIN / 16];
+#pragma unroll
+  for (int b = 0; b < PB; ++b)
+#pragma unroll
+    for (int nb = 0; nb < CIN / 16; ++nb) acc[b][nb] = f4{0, 0, 0, 0};
 #pragma unroll
   for (int dy = 0; dy < 3; ++dy) {
-    const int yy = y - (dy - 1) * g.dil;
-    const bool ok = yy >= 0 && yy < g.H;
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
-      const int xx = g.wrap(x - (dx - 1) * g.dil);
-      const bf16_t* src = dpre + (((long)n * g.H + (ok ? yy : 0)) * g.W + xx) * COUT + kq * 8;
+      const bf16_t* src[PB];
+      bool ok[PB];
+#pragma unroll
+      for (int b = 0; b < PB; ++b) {
+        const int yy = y[b] - (dy - 1) * g.dil;
+        ok[b] = live[b] && yy >= 0 && yy < g.H;
+        const int xx = g.wrap(x[b] - (dx - 1) * g.dil);
+        src[b] = dpre + (((long)n[b] * g.H + (ok[b] ? yy : 0)) * g.W + xx) * COUT + kq * 8;
+      }
 #pragma unroll
       for (int c = 0; c < COUT / 32; ++c) {
-        const bf8 a = ok ? load8(src + c * 32) : zero8();
+        bf8 a[PB];
+#pragma unroll
+        for (int b = 0; b < PB; ++b) a[b] = ok[b] ? load8(src[b] + c * 32) : zero8();
 #pragma unroll
         for (int nb = 0; nb < CIN / 16; ++nb) {
-          const bf8 b = load8(wt + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * COUT + c * 32 + kq * 8);
-          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+          const bf8 wb = load8(wt + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * COUT + c * 32 + kq * 8);
+#pragma unroll
+          for (int b = 0; b < PB; ++b) acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], wb, acc[b][nb], 0, 0, 0);
         }
       }
     }
   }
 #pragma unroll
-  for (int nb = 0; nb < CIN / 16; ++nb)
+  for (int b = 0; b < PB; ++b) {
+    if (!live[b]) continue;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) din[(p0 + kq * 4 + i) * CIN + nb * 16 + r] = f2bf(acc[nb][i]);
+    for (int nb = 0; nb < CIN / 16; ++nb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // through the ReLU whose output `post` is this layer's input
+        const long e = ((pb0 + b) * 16 + kq * 4 + i) * CIN + nb * 16 + r;
+        din[e] = bf2f(post[e]) > 0.f ? f2bf(acc[b][nb][i]) : (bf16_t)0;
+      }
+  }
 }
 
-// ---- backward weights, one (dy, dx) per blockIdx.y and one pixel chunk per
-// blockIdx.x: part[chunk][dydx][co][ci] = sum_p dpre[p][co] in[p+d(dy-1,dx-1)][ci].
-// GEMM with M = co, N = ci, K = pixels; 32-pixel slices of both operands
-// are staged in LDS and read back transposed.  Wave w owns co rows
-// [16 w', 16 w' + 16) for w' = w, w + 4, ... < COUT/16.
+// ---- backward weights, one tap (dy, dx) per blockIdx.y and one pixel chunk
+// per blockIdx.x: part[chunk][tap][co][ci] = sum_p dpre[p][co] in[p+d(dy-1,dx-1)][ci].
+// GEMM with M = co, N = ci, K = pixels.  Tiles of 64 pixels of both
+// operands are staged in LDS transposed ([channel][pixel], rows padded to
+// 72 entries), so each matrix-core operand (8 consecutive pixels of one
+// channel) is one 16-byte LDS read; a thread stages 4 pixels x 8 channels
+// of one operand per tile (four 16-byte global loads, eight 8-byte LDS
+// writes).  The MB x NB output tiles of 16 x 16 are dealt to the 4 waves.
+constexpr int kWTile = 64, kWPad = 72;
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_conv_bwd_w(Geo g, const bf16_t* __restrict__ in,
                                                     const bf16_t* __restrict__ dpre, float* part, long chunk) {
-  __shared__ bf16_t sD[32][COUT + 2];
-  __shared__ bf16_t sI[32][CIN + 2];
+  __shared__ __align__(16) bf16_t sD[COUT][kWPad];
+  __shared__ __align__(16) bf16_t sI[CIN][kWPad];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kq = lane >> 4, r = lane & 15;
-  const int dydx = blockIdx.y, dy = dydx / 3, dx = dydx % 3;
+  const int tap = blockIdx.y, dy = tap / 3, dx = tap % 3;
   const long np = g.npix();
   const long pbeg = (long)blockIdx.x * chunk;
   const long pend = pbeg + chunk < np ? pbeg + chunk : np;
-  constexpr int MB = COUT / 16, NB = CIN / 16;
-  constexpr int MW = (MB + 3) / 4;  // co blocks per wave
-  f4 acc[MW][NB];
+  constexpr int MB = COUT / 16, NB = CIN / 16, T = MB * NB, TW = (T + 3) / 4;
+  f4 acc[TW];
 #pragma unroll
-  for (int a = 0; a < MW; ++a)
+  for (int j = 0; j < TW; ++j) acc[j] = f4{0, 0, 0, 0};
+  // staging tasks: (4-pixel quad, 8-channel group) of dpre, then of the input
+  constexpr int QD = (kWTile / 4) * (COUT / 8), QI = (kWTile / 4) * (CIN / 8);
+  for (long ps = pbeg; ps < pend; ps += kWTile) {
+    for (int task = threadIdx.x; task < QD + QI; task += 256) {
+      const bool isd = task < QD;
+      const int tt = isd ? task : task - QD;
+      const int ng = isd ? COUT / 8 : CIN / 8;
+      const int quad = tt / ng, cg = tt % ng;
+      uint4 v[4];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = f4{0, 0, 0, 0};
-  for (long ps = pbeg; ps < pend; ps += 32) {
-    // stage 32 pixels of dpre (COUT) and of the shifted input (CIN)
-    for (int e = threadIdx.x; e < 32 * COUT; e += 256) {
-      const int pp = e / COUT, c = e % COUT;
-      const long p = ps + pp;
-      sD[pp][c] = p < pend ? dpre[p * COUT + c] : (bf16_t)0;
-    }
-    for (int e = threadIdx.x; e < 32 * CIN; e += 256) {
-      const int pp = e / CIN, c = e % CIN;
-      const long p = ps + pp;
-      bf16_t v = 0;
-      if (p < pend) {
-        const int x = (int)(p % g.W);
-        const long t = p / g.W;
-        const int y = (int)(t % g.H), n = (int)(t / g.H);
-        const int yy = y + (dy - 1) * g.dil;
-        const int xx = g.wrap(x + (dx - 1) * g.dil);
-        if (yy >= 0 && yy < g.H) v = in[(((long)n * g.H + yy) * g.W + xx) * CIN + c];
+      for (int e = 0; e < 4; ++e) {
+        const long p = ps + quad * 4 + e;
+        v[e] = make_uint4(0, 0, 0, 0);
+        if (p < pend) {
+          if (isd) {
+            v[e] = *reinterpret_cast<const uint4*>(dpre + p * COUT + cg * 8);
+          } else {
+            const int x = (int)(p % g.W);
+            const long t = p / g.W;
+            const int y = (int)(t % g.H), n = (int)(t / g.H);
+            const int yy = y + (dy - 1) * g.dil;
+            const int xx = g.wrap(x + (dx - 1) * g.dil);
+            if (yy >= 0 && yy < g.H) v[e] = *reinterpret_cast<const uint4*>(in + (((long)n * g.H + yy) * g.W + xx) * CIN + cg * 8);
+          }
+        }
       }
-      sI[pp][c] = v;
+      const unsigned short* h0 = reinterpret_cast<const unsigned short*>(&v[0]);
+      const unsigned short* h1 = reinterpret_cast<const unsigned short*>(&v[1]);
+      const unsigned short* h2 = reinterpret_cast<const unsigned short*>(&v[2]);
+      const unsigned short* h3 = reinterpret_cast<const unsigned short*>(&v[3]);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const unsigned long long q = (unsigned long long)h0[c] | ((unsigned long long)h1[c] << 16) |
+                                     ((unsigned long long)h2[c] << 32) | ((unsigned long long)h3[c] << 48);
+        bf16_t* dst = isd ? &sD[cg * 8 + c][quad * 4] : &sI[cg * 8 + c][quad * 4];
+        *reinterpret_cast<unsigned long long*>(dst) = q;
+      }
     }
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < MW; ++a) {
-      const int mb = wave + 4 * a;
-      if (mb < MB) {
-        bf16_t av[8];
+    for (int j = 0; j < TW; ++j) {
+      const int t = wave * TW + j;
+      if (t < T) {
+        const int mb = t / NB, nb = t % NB;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) av[e] = sD[kq * 8 + e][mb * 16 + r];
-        bf8 A;
-        __builtin_memcpy(&A, av, 16);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          bf16_t bv[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) bv[e] = sI[kq * 8 + e][b * 16 + r];
-          bf8 B;
-          __builtin_memcpy(&B, bv, 16);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[a][b], 0, 0, 0);
+        for (int kb = 0; kb < kWTile / 32; ++kb) {
+          const bf8 A = __builtin_bit_cast(bf8, *reinterpret_cast<const uint4*>(&sD[mb * 16 + r][kb * 32 + kq * 8]));
+          const bf8 B = __builtin_bit_cast(bf8, *reinterpret_cast<const uint4*>(&sI[nb * 16 + r][kb * 32 + kq * 8]));
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
-  float* dst = part + ((long)blockIdx.x * 9 + dydx) * COUT * CIN;
+  float* dst = part + ((long)blockIdx.x * 9 + tap) * COUT * CIN;
 #pragma unroll
-  for (int a = 0; a < MW; ++a) {
-    const int mb = wave + 4 * a;
-    if (mb < MB)
+  for (int j = 0; j < TW; ++j) {
+    const int t = wave * TW + j;
+    if (t < T) {
+      const int mb = t / NB, nb = t % NB;
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) dst[(mb * 16 + kq * 4 + i) * CIN + b * 16 + r] = acc[a][b][i];
+      for (int i = 0; i < 4; ++i) dst[(mb * 16 + kq * 4 + i) * CIN + nb * 16 + r] = acc[j][i];
+    }
   }
 }
 
@@ -280,35 +343,79 @@ __global__ void k_reduce_w(const float* part, int nchunk, int cout, int cin, flo
   const int ci = (int)(e % cin);
   const long t = e / cin;
   const int co = (int)(t % cout), dydx = (int)(t / cout);
-  float s = 0.f;
-  for (int c = 0; c < nchunk; ++c) s += part[((long)c * 9 + dydx) * cout * cin + (long)co * cin + ci];
-  gw[((long)co * 9 + dydx) * cin + ci] = s;
+  // four independent running sums (chunks c = 4 m + j), added at the end
+  const long stride = 9L * cout * cin, off = (long)dydx * cout * cin + (long)co * cin + ci;
+  float s4[4] = {0.f, 0.f, 0.f, 0.f};
+  int c = 0;
+  for (; c + 4 <= nchunk; c += 4)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s4[j] += part[(long)(c + j) * stride + off];
+  for (; c < nchunk; ++c) s4[c & 3] += part[(long)c * stride + off];
+  gw[((long)co * 9 + dydx) * cin + ci] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
 }
 
-__global__ void k_bias_part(Geo g, const bf16_t* __restrict__ dpre, int cout, long chunk, float* part) {
-  const int co = threadIdx.x;
-  if (co >= cout) return;
+// bias gradient partials: part[block][co] = sum over the block's kBiasChunk
+// pixels of dpre[p][co]; 256 / cout groups of cout threads take every
+// (256 / cout)-th pixel (each group's loads are one contiguous row), folded
+// in LDS in group order
+constexpr int kBiasChunk = 256;
+__global__ void __launch_bounds__(256) k_bias_part(Geo g, const bf16_t* __restrict__ dpre, int cout, float* part) {
+  __shared__ float sh[256];
+  const int groups = 256 / cout;
+  const int co = threadIdx.x % cout, grp = threadIdx.x / cout;
   const long np = g.npix();
-  const long pbeg = (long)blockIdx.x * chunk;
-  const long pend = pbeg + chunk < np ? pbeg + chunk : np;
+  const long pbeg = (long)blockIdx.x * kBiasChunk;
   float s = 0.f;
-  for (long p = pbeg; p < pend; ++p) s += bf2f(dpre[p * cout + co]);
-  part[(long)blockIdx.x * cout + co] = s;
+  if (grp < groups)
+    for (long p = pbeg + grp; p < pbeg + kBiasChunk && p < np; p += groups) s += bf2f(dpre[p * cout + co]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < cout) {
+    float t = 0.f;
+    for (int q = 0; q < groups; ++q) t += sh[q * cout + threadIdx.x];
+    part[(long)blockIdx.x * cout + threadIdx.x] = t;
+  }
 }
 
-__global__ void k_bias_reduce(const float* part, int nchunk, int cout, float* gb) {
-  const int co = threadIdx.x;
-  if (co >= cout) return;
+// one block per output channel folds the partials: 256 strided lanes, then
+// a tree in LDS
+__global__ void __launch_bounds__(256) k_bias_reduce(const float* part, int nchunk, int cout, float* gb) {
+  __shared__ float sh[256];
+  const int co = blockIdx.x;
   float s = 0.f;
-  for (int c = 0; c < nchunk; ++c) s += part[(long)c * cout + co];
-  gb[co] = s;
+  for (int c = threadIdx.x; c < nchunk; c += 256) s += part[(long)c * cout + co];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gb[co] = sh[0];
 }
 
-// dpre = dpost where post > 0 (ReLU)
-__global__ void k_relu_bwd(long n, const bf16_t* __restrict__ post, const bf16_t* __restrict__ dpost, bf16_t* dpre) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  dpre[e] = bf2f(post[e]) > 0.f ? dpost[e] : (bf16_t)0;
+// the sums over nb per-block partials of NF fields (part[NF * b + f]) by
+// one block of 256 threads: a strided pass per thread, then a tree in LDS
+// (a fixed order, so the result does not vary from run to run)
+template <int NF>
+__device__ __forceinline__ void fold_parts(const double* part, int nb, double out[NF]) {
+  __shared__ double sh[NF][256];
+  double a[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) a[f] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) a[f] += part[NF * b + f];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) sh[f][threadIdx.x] = a[f];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sh[f][threadIdx.x] += sh[f][threadIdx.x + st];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) out[f] = sh[f][0];
 }
 
 // masked mean-squared error over the 4 used outputs; channel c is masked by
@@ -340,12 +447,10 @@ __global__ void k_loss(long np, const float* __restrict__ out, const float* __re
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_parts(const double* part, int n, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += part[i];
-    *out = s;
-  }
+__global__ void __launch_bounds__(256) k_sum_parts(const double* part, int n, double* out) {
+  double s[1];
+  fold_parts<1>(part, n, s);
+  if (threadIdx.x == 0) *out = s[0];
 }
 
 // Adam on the FP32 master weights, then the BF16 copies: w[co][dy][dx][ci]
@@ -502,14 +607,11 @@ __global__ void k_scale_part(int H, int W, const float* raw, const float* stat, 
   }
 }
 
-__global__ void k_scale_final(const double* part, int nb, double* scale, float* keep) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0.0, c = 0.0;
-    for (int b = 0; b < nb; ++b) {
-      a += part[2 * b];
-      c += part[2 * b + 1];
-    }
-    const double sc = c > 0.0 && a > 0.0 ? sqrt(a / c) : 1e-30;
+__global__ void __launch_bounds__(256) k_scale_final(const double* part, int nb, double* scale, float* keep) {
+  double a[2];
+  fold_parts<2>(part, nb, a);
+  if (threadIdx.x == 0) {
+    const double sc = a[1] > 0.0 && a[0] > 0.0 ? sqrt(a[0] / a[1]) : 1e-30;
     *scale = sc;
     if (keep) *keep = (float)sc;
   }
@@ -574,11 +676,10 @@ __global__ void k_noise_part(int H, int W, const float* nz, const float* stat, d
 }
 
 // the factors that scale the noise of ch and of (u, v) to unit sea RMS
-__global__ void k_noise_final(const double* part, int nb, double* inv_rms) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < nb; ++b)
-      for (int c = 0; c < 4; ++c) a[c] += part[4 * b + c];
+__global__ void __launch_bounds__(256) k_noise_final(const double* part, int nb, double* inv_rms) {
+  double a[4];
+  fold_parts<4>(part, nb, a);
+  if (threadIdx.x == 0) {
     inv_rms[0] = a[0] > 0.0 ? sqrt(a[2] / a[0]) : 0.0;
     inv_rms[1] = a[1] > 0.0 ? sqrt(a[3] / a[1]) : 0.0;
   }
@@ -672,16 +773,21 @@ __global__ void k_moments(const float* store, long per_snap, long hw, int nsnap,
 // the snapshot's scale
 __device__ __forceinline__ void pack_pixel(const float* raw, long hw, long q, float sc, const NnNorm& nm,
                                            const float* stat, bf16_t* xo) {
+  __attribute__((aligned(16))) bf16_t v[kInC];
 #pragma unroll
   for (int d = 0; d < kDyn; ++d) {
-    const float v = raw[d * hw + q];
-    xo[dyn_channel(d)] = f2bf(d < kScaled ? v / (sc * nm.in_std[d]) : (v - nm.in_mean[d]) / nm.in_std[d]);
+    const float r = raw[d * hw + q];
+    v[dyn_channel(d)] = f2bf(d < kScaled ? r / (sc * nm.in_std[d]) : (r - nm.in_mean[d]) / nm.in_std[d]);
   }
-  xo[4] = f2bf(stat[3 * hw + q]);
-  xo[5] = f2bf(stat[q]);
-  xo[6] = f2bf(stat[hw + q]);
-  xo[7] = f2bf(stat[2 * hw + q]);
-  for (int c = kInUsed; c < kInC; ++c) xo[c] = 0;
+  v[4] = f2bf(stat[3 * hw + q]);
+  v[5] = f2bf(stat[q]);
+  v[6] = f2bf(stat[hw + q]);
+  v[7] = f2bf(stat[2 * hw + q]);
+#pragma unroll
+  for (int c = kInUsed; c < kInC; ++c) v[c] = 0;
+  // one pixel's 64 bytes as four 16-byte stores
+#pragma unroll
+  for (int k = 0; k < kInC / 8; ++k) reinterpret_cast<uint4*>(xo)[k] = reinterpret_cast<const uint4*>(v)[k];
 }
 
 // a minibatch of recorded snapshots into the network input and targets
@@ -742,16 +848,12 @@ __global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const 
 }
 
 // sea means of the two free-surface increments, then removed from them
-__global__ void k_pred_mean(const double* part, int nb, double* mean) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0.0, b = 0.0, w = 0.0;
-    for (int i = 0; i < nb; ++i) {
-      a += part[3 * i];
-      b += part[3 * i + 1];
-      w += part[3 * i + 2];
-    }
-    mean[0] = w > 0.0 ? a / w : 0.0;
-    mean[1] = w > 0.0 ? b / w : 0.0;
+__global__ void __launch_bounds__(256) k_pred_mean(const double* part, int nb, double* mean) {
+  double a[3];
+  fold_parts<3>(part, nb, a);
+  if (threadIdx.x == 0) {
+    mean[0] = a[2] > 0.0 ? a[0] / a[2] : 0.0;
+    mean[1] = a[2] > 0.0 ? a[1] / a[2] : 0.0;
   }
 }
 
@@ -971,7 +1073,7 @@ std::string nn_batch_buffers(Nn& n, int batch) {
   if (!(e = nn_make(n, &n.tgt, (size_t)np * kOutUsed)).empty()) return e;
   if (!(e = nn_make(n, &n.g0, (size_t)np * kHid)).empty()) return e;
   if (!(e = nn_make(n, &n.g1, (size_t)np * kHid)).empty()) return e;
-  n.wchunk = 2048;
+  n.wchunk = 4096;
   n.wnchunk = (int)((np + n.wchunk - 1) / n.wchunk);
   if (!(e = nn_make(n, &n.wpart, (size_t)n.wnchunk * 9 * kHid * kHid)).empty()) return e;
   // block partials: up to 9 per block of 256 pixels (k_eval), with room
@@ -984,16 +1086,33 @@ std::string nn_batch_buffers(Nn& n, int batch) {
 // forward pass over `batch` images already packed into act[0]
 std::string nn_forward(Nn& n, hipStream_t st, int batch) {
   Geo g{batch, n.H, n.W, layer_dil(0)};
-  const unsigned blocks = (unsigned)((g.npix() / 16 + 3) / 4);
-  NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
+  // two pixel blocks per wave for one image (more waves on the device), four
+  // for minibatches (fewer weight loads)
+  const long pblocks = g.npix() / 16;
+  if (batch == 1) {
+    const unsigned blocks = (unsigned)((pblocks + 7) / 8);
+    NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false, 2>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
+              (void*)n.act[1]);
+    for (int l = 1; l < kLayers - 1; ++l) {
+      g.dil = layer_dil(l);
+      NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false, 2>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
+                (void*)n.act[l + 1]);
+    }
+    g.dil = layer_dil(kLayers - 1);
+    NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true, 2>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
+              n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
+    return "";
+  }
+  const unsigned blocks = (unsigned)((pblocks + 15) / 16);
+  NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false, 4>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
             (void*)n.act[1]);
   for (int l = 1; l < kLayers - 1; ++l) {
     g.dil = layer_dil(l);
-    NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
+    NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false, 4>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
               (void*)n.act[l + 1]);
   }
   g.dil = layer_dil(kLayers - 1);
-  NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
+  NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true, 4>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
             n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
   return "";
 }
@@ -1002,8 +1121,9 @@ std::string nn_forward(Nn& n, hipStream_t st, int batch) {
 std::string nn_backward(Nn& n, hipStream_t st, int batch) {
   Geo g{batch, n.H, n.W, 1};
   const long np = g.npix();
-  const unsigned blocks = (unsigned)((np / 16 + 3) / 4);
+  const unsigned blocks = (unsigned)((np / 16 + 15) / 16);
   const int nch = (int)((np + n.wchunk - 1) / n.wchunk);
+  const int nbias = (int)((np + kBiasChunk - 1) / kBiasChunk);
   bf16_t* dpre = n.act[kLayers];  // d loss / d (last pre-activation), 32 channels
   for (int l = kLayers - 1; l >= 0; --l) {
     NnLayer& L = n.L[l];
@@ -1016,17 +1136,16 @@ std::string nn_backward(Nn& n, hipStream_t st, int batch) {
     else
       NN_LAUNCH((k_conv_bwd_w<kHid, kOutC>), dim3(nch, 9), dim3(256), g, n.act[l], dpre, n.wpart, n.wchunk);
     NN_LAUNCH(k_reduce_w, dim3(nblk(L.nw())), dim3(256), n.wpart, nch, L.cout, L.cin, L.gw);
-    NN_LAUNCH(k_bias_part, dim3(nch), dim3(64), g, dpre, L.cout, n.wchunk, (float*)n.wpart);
-    NN_LAUNCH(k_bias_reduce, dim3(1), dim3(64), (const float*)n.wpart, nch, L.cout, L.gb);
+    NN_LAUNCH(k_bias_part, dim3(nbias), dim3(256), g, dpre, L.cout, (float*)n.wpart);
+    NN_LAUNCH(k_bias_reduce, dim3(L.cout), dim3(256), (const float*)n.wpart, nbias, L.cout, L.gb);
     if (l == 0) break;
     // gradient of this layer's input = previous layer's post-activation
     bf16_t* dpost = (dpre == n.g0) ? n.g1 : n.g0;
+    // through the ReLU of layer l-1, whose output is act[l]
     if (L.cout == kOutC)
-      NN_LAUNCH((k_conv_bwd_data<kHid, kOutC>), dim3(blocks), dim3(256), g, dpre, L.wt, dpost);
+      NN_LAUNCH((k_conv_bwd_data<kHid, kOutC, 4>), dim3(blocks), dim3(256), g, dpre, L.wt, n.act[l], dpost);
     else
-      NN_LAUNCH((k_conv_bwd_data<kHid, kHid>), dim3(blocks), dim3(256), g, dpre, L.wt, dpost);
-    // through the ReLU of layer l-1 (its output is act[l])
-    NN_LAUNCH(k_relu_bwd, dim3(nblk(np * kHid)), dim3(256), np * kHid, n.act[l], dpost, dpost);
+      NN_LAUNCH((k_conv_bwd_data<kHid, kHid, 4>), dim3(blocks), dim3(256), g, dpre, L.wt, n.act[l], dpost);
     dpre = dpost;
   }
   return "";
@@ -1037,7 +1156,7 @@ std::string nn_loss(Nn& n, hipStream_t st, int batch, double* loss_dev) {
   // masked count: sum of masks over used channels (computed from the input)
   const float inv = 1.0f / (float)(np * kOutUsed);
   NN_LAUNCH(k_loss, dim3(nblk(np)), dim3(256), np, n.out, n.tgt, n.act[0], inv, n.act[kLayers], n.dpart);
-  NN_LAUNCH(k_sum_parts, dim3(1), dim3(64), n.dpart, (int)nblk(np), loss_dev);
+  NN_LAUNCH(k_sum_parts, dim3(1), dim3(256), n.dpart, (int)nblk(np), loss_dev);
   return "";
 }
 
@@ -1204,7 +1323,7 @@ std::string nn_raw_inputs(Ctx& c, float* raw, float* keep) {
   }
   NN_LAUNCH(k_sub1_raw, dim3(nblk(hw)), dim3(256), c.gd, n->H, n->W, r, bt, raw);
   NN_LAUNCH(k_scale_part, dim3(nblk(hw)), dim3(256), n->H, n->W, raw, n->stat, n->dpart);
-  NN_LAUNCH(k_scale_final, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc + 20, keep);
+  NN_LAUNCH(k_scale_final, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc + 20, keep);
   return "";
 }
 
@@ -1220,7 +1339,7 @@ std::string nn_predict(Ctx& c) {
   if (!e.empty()) return e;
   NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->misc + 20, n->stat, n->massw,
             n->inc, n->dpart);
-  NN_LAUNCH(k_pred_mean, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc);
+  NN_LAUNCH(k_pred_mean, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc);
   NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
   return "";
 }
@@ -1259,7 +1378,7 @@ std::string nn_record_perturbed(Ctx& c, std::string (*plain)(Ctx&)) {
     std::swap(nz, tmp);
   }
   NN_LAUNCH(k_noise_part, dim3(nblk(hw)), dim3(256), n->H, n->W, nz, n->stat, n->dpart);
-  NN_LAUNCH(k_noise_final, dim3(1), dim3(64), n->dpart, (int)nblk(hw), n->misc + 22);
+  NN_LAUNCH(k_noise_final, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc + 22);
   NN_LAUNCH(k_perturb, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, nz, n->misc + 22, amp_ch, amp_u, n->stat, h.chc,
             h.chp, h.buc, h.bvc, h.bup, h.bvp);
   // the perturbed state's cycle, then the cycle from what it left: its
