@@ -1,17 +1,24 @@
 """Benchmark: simulated years per day (SYPD) of the PCOM time stepper on one
 MI355X at 1 degree (360 x 180 x 30, dtbt = 60 s, dtbc = 3600 s).
 
-Prints ONE JSON line.  `value` is SYPD of the reported form over exactly
-`--steps` baroclinic steps (each = daily forcing when due, EOS, pressure
-gradient integrals, advection/friction, the 120-substep barotropic cycle,
-baroclinic momentum, SSH, tracers, convection), state resident in HBM.
-Alongside: the other precision form, a per-phase breakdown from HIP events
-on the library's stream, the roofline of the dominant kernel (the
-barotropic subcycle, k_trop_tiles: FP64 flops per launch over its
-event-timed duration), the HBM roofline of the whole step, and the CPU
-FP64 oracle timed on a bounded sample on the host.
+Prints ONE JSON line.  `value` is SYPD of the reported form (by default
+mixed+surrogate: FP32 tendency stencils with FP64 state, and the learned
+barotropic surrogate in place of most subcycles) over exactly `--steps`
+baroclinic steps (each = daily forcing when due, EOS, pressure gradient
+integrals, advection/friction, the 120-substep barotropic cycle or the
+network, baroclinic momentum, SSH, tracers, convection), state resident in
+HBM.  The surrogate form is first fitted untimed (pcom_amd.surrogate.fit:
+10 days of spin-up, 1000 recorded steps with perturbed snapshots beside
+half of them, 6000 training iterations) and then used under the policy
+pcom_amd.surrogate.USE.  Alongside: the plain FP64 form and the other
+mixed forms, a per-phase breakdown from HIP events on the library's
+stream, the roofline of the dominant kernel (the barotropic subcycle,
+k_trop_tiles: FP64 flops per launch over its event-timed duration), the
+HBM roofline of the whole step, each form's RMSE against FP64 after
+--rmse-days days, and the CPU FP64 oracle timed on a bounded sample on
+the host.
 
-    python bench.py [--steps K] [--warmup W] [--form fp64|mixed]
+    python bench.py [--steps K] [--warmup W] [--form mixed+surrogate|fp64|mixed|mixed_eos32]
     python bench.py --calibrate-copy    # a plain 1 GiB copy, for PMC scaling
 """
 import argparse
@@ -54,38 +61,41 @@ def step_bytes(cfg, s3):
     return n3 * 31 * s3 + n2 * 50 * 8
 
 
-SUR_ITERS = 1000
-SUR_RECORD = 48
-SUR_TOL = -1.0  # a check fails when the network's error exceeds persistence's
+def _nn_counts(m):
+    st = m.nn_stats()
+    return st, {"surrogate_steps": st.surrogate_steps, "plain_steps": st.plain_steps, "checks": st.checks,
+                "fallbacks": st.fallbacks}
 
 
-def run_form(cfg, precision, steps, warmup, surrogate=False, check_every=10):
+FITTED = {}  # the surrogate form's trained parameters and fit steps, for accuracy()
+
+
+def run_form(cfg, precision, steps, warmup, surrogate=False):
     from pcom_amd import abi
+    from pcom_amd import surrogate as sm
     from pcom_amd.lib import Model
     m = Model(cfg, precision=precision)
-    m.step(warmup)
     sur = None
     if surrogate:
-        # a network trained briefly on 48 recorded subcycles, used with the
-        # plain subcycle judging every check_every-th step: a prediction
-        # whose error exceeds that of persistence (tol < 0) hands the run
-        # back to the plain subcycle for good, so the form is timed at the
-        # accuracy it keeps (the full training workflow: python -m
-        # pcom_amd.surrogate)
-        m.nn_create(capacity=48, seed=1)
-        m.nn_record(True)
-        m.step(48)
-        m.nn_record(False)
+        # fitted untimed, then used under the policy USE
         t0 = time.perf_counter()
-        m.nn_train(SUR_ITERS, batch=8, lr=1e-3, seed=2)
-        sur = {"train_iters": SUR_ITERS, "train_s": round(time.perf_counter() - t0, 2),
-               "check_every": check_every, "tol": SUR_TOL}
-        m.nn_use(check_every, tol=SUR_TOL)
+        rep = sm.fit(m, cfg, log=lambda s: None)
+        sur = {"fit_s": round(time.perf_counter() - t0, 2), "fit_steps": rep["steps"], "recorded": rep["recorded"],
+               "train_iters": rep["iters"], "train_s": round(rep["train_s"], 2),
+               "holdout_gain": {k: round(v, 3) for k, v in rep["holdout_gain"].items()}, "policy": sm.USE}
+        FITTED[precision] = (m.nn_params(), rep["steps"])
+        m.nn_use(**sm.USE)
+    m.step(warmup)
+    if sur is not None:
+        _, before = _nn_counts(m)
     m.sync()
     t0 = time.perf_counter()
     m.step(steps)
     m.sync()
     wall = time.perf_counter() - t0
+    if sur is not None:
+        _, after = _nn_counts(m)
+        sur["timed"] = {k: after[k] - before[k] for k in after}
     # per-phase device time from HIP events on the library's stream
     # (phase 2 brackets the subcycle kernel alone)
     m.phase_times(True)
@@ -101,11 +111,10 @@ def run_form(cfg, precision, steps, warmup, surrogate=False, check_every=10):
            "subcycle_mode": mode, "subcycle_tiling": list(tile),
            "subcycle_launches": 1 if mode == abi.PCOM_TROP_PERSISTENT else -(-nstep // tile[2])}
     if sur is not None:
-        st = m.nn_stats()
-        sur.update({"surrogate_steps": st.surrogate_steps, "plain_steps": st.plain_steps, "checks": st.checks,
-                    "fallbacks": st.fallbacks, "in_use_at_end": bool(st.in_use),
-                    "last_check_rmse": [float(x) for x in st.last_check_rmse],
-                    "last_persistence_rmse": [float(x) for x in st.last_persist_rmse]})
+        st, _ = _nn_counts(m)
+        sur.update({"in_use_at_end": bool(st.in_use),
+                    "last_check_rmse": [float(f"{x:.4g}") for x in st.last_check_rmse],
+                    "last_persistence_rmse": [float(f"{x:.4g}") for x in st.last_persist_rmse]})
         out["surrogate"] = sur
     m.close()
     return out
@@ -125,41 +134,56 @@ def calibrate_copy():
 
 
 def accuracy(cfg, forms, days):
-    """SST / SSH RMSE (and every prognostic field's) of each form against
-    the FP64 form after `days` simulated days from the initial state, run
-    side by side (the FP64 form is bit-identical to the FP64 oracle).  A
-    surrogate form records its first days, trains, then runs checked."""
+    """RMSE of SST, SSH and every prognostic field of each form against the
+    FP64 form run side by side from the same state (the FP64 form is
+    bit-identical to the FP64 oracle): the plain forms after `days`
+    simulated days from the initial state; a surrogate form after `days`
+    days of use, counted from the whole day after its fit (the FP64 run
+    takes the same steps)."""
     from pcom_amd import abi, rmse
+    from pcom_amd import surrogate as sm
     from pcom_amd.lib import Model
     spd = 86400 // cfg.dtbc
-    ref = Model(cfg, precision=abi.PCOM_FP64)
-    ref.step(days * spd)
     res = {"days": days, "gates": {"sst_K": 1e-3, "ssh_m": 1e-3}, "forms": {}}
+    ref = None
     for name in forms:
         if name == "fp64":
             continue
         base, sur = name.split("+")[0], name.endswith("+surrogate")
         m = Model(cfg, precision=abi.FORMS[base])
+        r = {}
         if sur:
-            rec_steps = min(SUR_RECORD, days * spd // 2)
-            m.nn_create(capacity=rec_steps, seed=1)
-            m.nn_record(True)
-            m.step(rec_steps)
-            m.nn_record(False)
-            m.nn_train(SUR_ITERS, batch=8, lr=1e-3, seed=2)
-            m.nn_use(10, tol=SUR_TOL)
-            m.step(days * spd - rec_steps)
-        else:
+            # the fitted network of the timed run (recording leaves the state
+            # as plain steps leave it, so plain steps reach the same state)
+            if abi.FORMS[base] in FITTED:
+                params, fit_steps = FITTED[abi.FORMS[base]]
+                d0 = -(-fit_steps // spd)
+                m.step(d0 * spd)
+                m.nn_create(capacity=0, seed=1)
+                m.nn_params(params)
+            else:
+                fit_steps = sm.fit(m, cfg, log=lambda s: None)["steps"]
+                d0 = -(-fit_steps // spd)
+                m.step(d0 * spd - fit_steps)
+            m.nn_use(**sm.USE)
             m.step(days * spd)
-        r = {k: float(f"{v:.4g}") for k, v in rmse.rmse_report(m, ref, cfg).items()}
+            ref_s = Model(cfg, precision=abi.PCOM_FP64)
+            ref_s.step((d0 + days) * spd)
+            r = {k: float(f"{v:.4g}") for k, v in rmse.rmse_report(m, ref_s, cfg).items()}
+            ref_s.close()
+            _, cnt = _nn_counts(m)
+            r.update({"in_use_from_day": d0, "to_day": d0 + days, **cnt})
+        else:
+            if ref is None:
+                ref = Model(cfg, precision=abi.PCOM_FP64)
+                ref.step(days * spd)
+            m.step(days * spd)
+            r = {k: float(f"{v:.4g}") for k, v in rmse.rmse_report(m, ref, cfg).items()}
         r["finite"] = bool(m.check_finite())
-        if sur:
-            st = m.nn_stats()
-            r["surrogate_steps"] = st.surrogate_steps
-            r["plain_steps"] = st.plain_steps
         res["forms"][name] = r
         m.close()
-    ref.close()
+    if ref is not None:
+        ref.close()
     return res
 
 
@@ -186,7 +210,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=24)
-    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32", "mixed+surrogate"], default="fp64",
+    ap.add_argument("--form", choices=["fp64", "mixed", "mixed_eos32", "mixed+surrogate"], default="mixed+surrogate",
                     help="form reported as `value`; the others are measured alongside")
     ap.add_argument("--cpu-steps", type=int, default=30, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--config", default="global", choices=["global", "box", "eddy"])
@@ -222,7 +246,10 @@ def main():
     step_gbs = b / (rep["ms_per_step"] / 1000.0) / 1e9
     nstep = 2 * cfg.dtbc // cfg.dtbt
     flops = TROP_FLOPS_PER_POINT * cfg.glo_ni * cfg.glo_nj * nstep
-    trop_ms = rep["phase_ms"]["subcycle_kernel"]
+    # the subcycle kernel timed where it runs every step (the surrogate form
+    # runs it on its checks and holds only)
+    kform = next((f for f in ("mixed", "fp64", "mixed_eos32") if f in forms), args.form)
+    trop_ms = forms[kform]["phase_ms"]["subcycle_kernel"]
     trop_tflops = flops / (trop_ms / 1000.0) / 1e12
     out = {
         "metric": METRIC,
@@ -242,6 +269,8 @@ def main():
                                f"dtbc={cfg.dtbc} s, {nstep} barotropic substeps/step",
                    "form": args.form, "parallelism": "single GPU, whole domain"},
         "forms": forms,
+        "plain_fp64_sypd": forms["fp64"]["sypd"] if "fp64" in forms else None,
+        "surrogate_timed": rep.get("surrogate", {}).get("timed"),
         "accuracy_1yr": "1-year SST/SSH RMSE of each form against FP64: profiles/rmse_1yr_global1deg.md",
         "roofline": {"bound": "fp64-valu",
                      "limiter": "latency: five barrier-separated stages per substep on a cache-resident "
@@ -255,7 +284,7 @@ def main():
                      "frac": round(trop_tflops / PEAK_FP64_TFLOPS, 5),
                      "traffic": TROP_TRAFFIC_BYTES if cfg.name == "global" else None,
                      "traffic_source": TROP_TRAFFIC_SOURCE if cfg.name == "global" else None,
-                     "flops_per_launch": flops, "launch_ms": round(trop_ms, 4),
+                     "flops_per_launch": flops, "launch_ms": round(trop_ms, 4), "timed_in_form": kform,
                      "definition": f"{TROP_FLOPS_PER_POINT} FP64 flops per point per substep x "
                                    f"{cfg.glo_ni}x{cfg.glo_nj} points x {nstep} substeps / event-timed "
                                    "kernel duration; peak = MI355X FP64 (vector and matrix) 78.6 TFLOP/s"},

@@ -360,9 +360,18 @@ int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long 
 /* Use the network in pcom_step with a plain, checking subcycle every
  * check_every steps (0 turns the surrogate off).  At each check, tol > 0
  * bounds the sea RMSE of every output, tol < 0 bounds it by -tol times the
- * RMSE of persistence (no change over the step), tol = 0 never falls back;
- * a failed check turns the surrogate off.                                */
+ * RMSE of persistence (no change over the step), both at this check and at
+ * the first check after this call, whichever is smaller (a run drifting
+ * into noise raises the first with the error); tol = 0 never falls back;
+ * a failed check turns the surrogate off, or holds it for a number of
+ * plain steps set by pcom_nn_resume.                                     */
 int pcom_nn_use(pcom_ctx *ctx, int check_every, double tol);
+/* After a failed check, run plain_steps plain steps and then resume the
+ * network (counted in pcom_nn_stats.fallbacks); 0 (the default) keeps the
+ * plain subcycle for good.  Extends the check of the surrogate config
+ * (plain subcycle every 10th step as check and fallback): the reference
+ * has no surrogate.                                                      */
+int pcom_nn_resume(pcom_ctx *ctx, int plain_steps);
 /* Root-mean-square error over sea points of the network on recorded
  * snapshots [first, first+count) (in recording order), in physical units:
  * rmse[0..3] for the mch2, mch, ubt, vbt increments and rmse[4..7] for

@@ -809,6 +809,16 @@ int pcom_nn_use(pcom_ctx* ctx, int check_every, double tol) {
   return PCOM_OK;
 }
 
+int pcom_nn_resume(pcom_ctx* ctx, int plain_steps) {
+  if (!ctx || plain_steps < 0) return fail(PCOM_ERR_ARG, "pcom_nn_resume: bad argument");
+  Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
+  if (!c->nn) return fail(PCOM_ERR_STATE, "pcom_nn_resume: no surrogate attached");
+  std::string e = nn_set_resume(*c, plain_steps);
+  if (!e.empty()) return fail(PCOM_ERR_STATE, "pcom_nn_resume: " + e);
+  return PCOM_OK;
+}
+
 int pcom_nn_eval(pcom_ctx* ctx, int first, int count, double rmse[8]) {
   if (!ctx || !rmse) return fail(PCOM_ERR_ARG, "pcom_nn_eval: NULL argument");
   Ctx* c = C(ctx);

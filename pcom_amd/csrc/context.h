@@ -132,6 +132,7 @@ void nn_destroy(Ctx& c);
 void nn_set_record(Ctx& c, bool on);
 std::string nn_set_augment(Ctx& c, double fraction, double amp_ch, double amp_u, unsigned long long seed);
 std::string nn_set_use(Ctx& c, int every, double tol);
+std::string nn_set_resume(Ctx& c, int plain_steps);
 std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&));
 std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long seed, double* loss);
 size_t nn_param_count(const Ctx& c);

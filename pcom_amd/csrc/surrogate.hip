@@ -819,8 +819,12 @@ __global__ void k_pack_live(int H, int W, const float* raw, const double* scale,
 // prediction -> increments (FP64) with per-block sums over sea of
 // w*d(mch2), w*d(mch) and w, w = area * prh of the T points: the barotropic
 // continuity equation (mod_int.f90:189-193, 264-267) conserves sum(w chc)
+struct NnLimit {
+  double v[kOutUsed];  // largest increment magnitude per output (0: no bound)
+};
+
 __global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const double* scale, const float* stat,
-                       const double* area, double* inc, double* part) {
+                       const double* area, NnLimit lim, double* inc, double* part) {
   __shared__ double s[3][256];
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
@@ -829,8 +833,13 @@ __global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const 
     const double sc = *scale;
     const float mt = stat[q], mu = stat[hw + q];
 #pragma unroll
-    for (int c = 0; c < kOutUsed; ++c)
-      inc[c * hw + q] = (double)(out[q * kOutC + c] * nm.out_std[c]) * sc * (double)(c < 2 ? mt : mu);
+    for (int c = 0; c < kOutUsed; ++c) {
+      double d = (double)(out[q * kOutC + c] * nm.out_std[c]) * sc * (double)(c < 2 ? mt : mu);
+      // a non-finite prediction is no change; one beyond the limit is cut
+      if (!(fabs(d) < HUGE_VAL)) d = 0.0;
+      if (lim.v[c] > 0.0) d = fmin(fmax(d, -lim.v[c]), lim.v[c]);
+      inc[c * hw + q] = d;
+    }
     const double ar = area[nn_ix(ni, (int)(q % W), (int)(q / W))] * (double)mt;
     v[0] = ar * inc[q];
     v[1] = ar * inc[hw + q];
@@ -1000,14 +1009,26 @@ struct NnLayer {
   long nw() const { return 9L * cin * cout; }
 };
 
+// pointwise bound of a predicted increment, in units of the sea RMS of one
+// step's change when the network took over
+constexpr double kLimitAnchors = 50.0;
+
 struct Nn {
   int H = 0, W = 0, ni = 0;
   int cap = 0, nrec = 0, cursor = 0;
   bool recording = false;
   int use_every = 0;  // surrogate in use: plain subcycle every use_every-th step
   double tol = 0.0;
+  // after a failed check: plain steps before the network resumes (0: it
+  // does not), and the last step of the current hold
+  int resume_after = 0;
+  long long hold_until = 0;
   long long step_count = 0, surrogate_steps = 0, plain_steps = 0, checks = 0, fallbacks = 0;
   double last_err[4] = {0, 0, 0, 0}, max_err[4] = {0, 0, 0, 0}, last_persist[4] = {0, 0, 0, 0};
+  // persistence RMSE at the first check after pcom_nn_use: the size of one
+  // step's change when the network took over, which a relative bound uses
+  double anchor[4] = {0, 0, 0, 0};
+  bool anchored = false;
   bool normalised = false;
   int adam_t = 0;              // Adam steps taken
   double* misc = nullptr;      // [16]: sea means, check errors
@@ -1293,6 +1314,14 @@ std::string nn_set_use(Ctx& c, int every, double tol) {
   if (every > 0 && !c.nn->normalised) return "the surrogate is untrained (no normalisation statistics)";
   c.nn->use_every = every;
   c.nn->tol = tol;
+  c.nn->hold_until = 0;
+  c.nn->anchored = false;
+  return "";
+}
+
+std::string nn_set_resume(Ctx& c, int plain_steps) {
+  if (!c.nn) return "no surrogate attached";
+  c.nn->resume_after = plain_steps;
   return "";
 }
 
@@ -1337,8 +1366,13 @@ std::string nn_predict(Ctx& c) {
   NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->H, n->W, n->live, n->misc + 20, n->norm, n->stat, n->act[0]);
   e = nn_forward(*n, st, 1);
   if (!e.empty()) return e;
+  // once anchored, increments are bounded pointwise by kLimitAnchors times
+  // the sea RMS of one step's change at the first check
+  NnLimit lim{};
+  if (n->anchored)
+    for (int q = 0; q < kOutUsed; ++q) lim.v[q] = kLimitAnchors * n->anchor[q];
   NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->misc + 20, n->stat, n->massw,
-            n->inc, n->dpart);
+            lim, n->inc, n->dpart);
   NN_LAUNCH(k_pred_mean, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc);
   NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
   return "";
@@ -1410,7 +1444,8 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   Fields2& h = c.f2;
   const long hw = (long)n->H * n->W;
   n->step_count += 1;
-  const bool use = n->use_every > 0;
+  // held: plain steps after a failed check, before the network resumes
+  const bool use = n->use_every > 0 && n->step_count > n->hold_until;
   const bool check = use && (n->step_count % n->use_every == 0);
   if (c.timing) (void)hipEventRecord(c.ev[1], st);
   if (use && !check) {
@@ -1455,19 +1490,27 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
         hipStreamSynchronize(st) != hipSuccess)
       return "surrogate check read-back";
     n->checks += 1;
+    if (!n->anchored) {
+      for (int q = 0; q < 4; ++q) n->anchor[q] = he[4 + q];
+      n->anchored = true;
+    }
     bool bad = false;
     for (int q = 0; q < 4; ++q) {
       n->last_persist[q] = he[4 + q];
       n->last_err[q] = he[q];
       n->max_err[q] = std::max(n->max_err[q], he[q]);
-      // tol > 0 bounds the error, tol < 0 bounds it relative to persistence;
-      // NaN counts as a failed check
-      const double bound = n->tol > 0.0 ? n->tol : -n->tol * he[4 + q];
+      // tol > 0 bounds the error, tol < 0 bounds it relative to persistence,
+      // now and at the first check (a run drifting into noise, where both
+      // grow together, fails the second); NaN counts as a failed check
+      const double bound = n->tol > 0.0 ? n->tol : -n->tol * std::min(he[4 + q], n->anchor[q]);
       if (!(he[q] <= bound)) bad = true;
     }
     if (bad && n->tol != 0.0) {
       n->fallbacks += 1;
-      n->use_every = 0;  // the plain subcycle takes over
+      if (n->resume_after > 0)
+        n->hold_until = n->step_count + n->resume_after;  // the plain subcycle takes over for a while
+      else
+        n->use_every = 0;  // ... or for good
     }
   }
   return "";
@@ -1686,7 +1729,7 @@ void nn_stats(const Ctx& c, pcom_nn_stats* s) {
   s->plain_steps = n->plain_steps;
   s->checks = n->checks;
   s->fallbacks = n->fallbacks;
-  s->in_use = n->use_every > 0 ? 1 : 0;
+  s->in_use = n->use_every > 0 && n->step_count >= n->hold_until ? 1 : 0;
   for (int q = 0; q < 4; ++q) {
     s->last_check_rmse[q] = n->last_err[q];
     s->last_persist_rmse[q] = n->last_persist[q];

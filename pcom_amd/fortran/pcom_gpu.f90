@@ -55,6 +55,12 @@ module pcom_gpu
 
   real(c_float), parameter :: PCOM_MISSING_FLOAT = -9.99999e30
 
+  ! forcing records of one refresh (c_loc of a (glo_ni, glo_nj) array, or
+  ! c_null_ptr to keep int_bnd's value)
+  type, bind(C) :: pcom_forcing
+    type(c_ptr) :: taux, tauy, pa, bct
+  end type pcom_forcing
+
   type, bind(C) :: pcom_counters
     integer(c_long_long) :: step
     integer(c_int) :: n_ts, n_uv, n_w, n_ch, n_ssh
@@ -115,6 +121,19 @@ module pcom_gpu
       integer(c_long_long), value :: step
       integer(c_int), intent(in) :: cdate(6), pdate(6)
     end function pcom_set_clock
+
+    integer(c_int) function pcom_forcing_due(ctx, due) bind(C, name='pcom_forcing_due')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: due(2)
+    end function pcom_forcing_due
+
+    integer(c_int) function pcom_set_forcing(ctx, f, interpolate) bind(C, name='pcom_set_forcing')
+      import :: c_int, c_ptr, pcom_forcing
+      type(c_ptr), value :: ctx
+      type(pcom_forcing), intent(in) :: f
+      integer(c_int), value :: interpolate
+    end function pcom_set_forcing
 
     integer(c_int) function pcom_output_means(ctx, out) bind(C, name='pcom_output_means')
       import :: c_int, c_ptr, pcom_means
@@ -281,6 +300,12 @@ module pcom_gpu
       integer(c_int), value :: check_every
       real(c_double), value :: tol
     end function pcom_nn_use
+
+    integer(c_int) function pcom_nn_resume(ctx, plain_steps) bind(C, name='pcom_nn_resume')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: plain_steps
+    end function pcom_nn_resume
 
     integer(c_int) function pcom_nn_eval(ctx, first, count, rmse) bind(C, name='pcom_nn_eval')
       import :: c_int, c_ptr, c_double

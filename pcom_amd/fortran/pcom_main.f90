@@ -14,6 +14,10 @@
 !   means    means_<date>.bin: int32 glo_ni, glo_nj, nk, int32 date(6),
 !            float64 pt, sa, rho, u, v, w (glo_ni, glo_nj, nk), ch, ssh
 !   restart  restart_<date>.bin: the layout of pcom_amd/restart.py
+!   forcing  ff_ws / ff_pa / ff_bct (optional, main.f90:167-202): float64
+!            records of (glo_ni, glo_nj); ff_ws holds taux then tauy per
+!            record.  Record nday on each daily refresh, and with
+!            out_per = '6hour' record (nday-1)*4 + hour/6 + 1 every 6 hours
 !
 ! Usage: pcom_main [namelist-file]      (default: ./namelist)
 !        pcom_main --abi                 (print the sizes of the C structs)
@@ -26,12 +30,13 @@ program pcom_main
   character(len=19) :: bdate = '0001-01-01 00:00:00', edate = '0001-01-02 00:00:00'
   integer :: dtbt = 60, dtbc = 3600
   character(len=256) :: fname_ini = 'input/inputs.bin', fname_rst = '', out_dir = 'output/'
+  character(len=256) :: ff_ws = '', ff_pa = '', ff_bct = ''
   character(len=16) :: out_per = 'day'
   integer :: restart_per = 1, restart_run = 0
   character(len=12) :: precision = 'fp64'
   integer :: device = 0, tile_i = 0, tile_j = 0, tile_s = 0, trop_mode = 0
   namelist /time_ctrl/ bdate, edate, dtbt, dtbc
-  namelist /io_ctrl/ fname_ini, fname_rst, out_dir, out_per, restart_per, restart_run
+  namelist /io_ctrl/ fname_ini, fname_rst, out_dir, out_per, restart_per, restart_run, ff_ws, ff_pa, ff_bct
   namelist /gpu_ctrl/ precision, device, tile_i, tile_j, tile_s, trop_mode
 
   character(len=256) :: nml_file
@@ -47,15 +52,19 @@ program pcom_main
   type(pcom_counters) :: cnt
   integer :: bd(6), ed(6), cd(6), pd(6)
   integer(c_long_long) :: i, nt, ist
-  integer :: nrec
+  integer :: nrec, nday
+  integer(c_int) :: due(2)
+  real(c_double), allocatable, target :: ftx(:,:), fty(:,:), fpa(:,:), fbt(:,:)
   real(c_double) :: sums(3)
   integer(c_int64_t) :: t0, t1, rate
   type(pcom_means) :: means_probe
+  type(pcom_forcing) :: forcing_probe
 
   nml_file = 'namelist'
   if (command_argument_count() >= 1) call get_command_argument(1, nml_file)
   if (trim(nml_file) == '--abi') then   ! sizes of the interoperable types
-    write(*, '(a, 4(1x, i0))') 'abi', c_sizeof(inp), c_sizeof(opts), c_sizeof(cnt), c_sizeof(means_probe)
+    write(*, '(a, 5(1x, i0))') 'abi', c_sizeof(inp), c_sizeof(opts), c_sizeof(cnt), c_sizeof(means_probe), &
+      c_sizeof(forcing_probe)
     stop
   end if
   open(newunit=u, file=trim(nml_file), status='old', action='read', iostat=ios)
@@ -128,8 +137,23 @@ program pcom_main
   ! ---- baroclinic loop (main.f90:156-238); the library advances the
   ! calendar and refreshes the daily forcing itself
   nrec = 0
+  nday = 0
+  allocate(ftx(glo_ni, glo_nj), fty(glo_ni, glo_nj), fpa(glo_ni, glo_nj), fbt(glo_ni, glo_nj))
   call system_clock(t0, rate)
   do i = ist, nt
+    ! forcing files (main.f90:164-202): the library says which refreshes
+    ! this step performs; the records replace int_bnd's fields before
+    ! den_prho
+    call pcom_check(pcom_forcing_due(ctx, due), 'pcom_forcing_due')
+    if (due(1) /= 0) then
+      nday = nday + 1
+      if (len_trim(ff_ws) + len_trim(ff_pa) + len_trim(ff_bct) > 0) call apply_forcing(nday, 1_c_int)
+    end if
+    if (due(2) /= 0 .and. trim(out_per) == '6hour' .and. &
+        len_trim(ff_ws) + len_trim(ff_pa) + len_trim(ff_bct) > 0) then
+      call pcom_check(pcom_get_counters(ctx, cnt), 'pcom_get_counters')
+      call apply_forcing((nday - 1) * 4 + cnt%hour / 6 + 1, 0_c_int)
+    end if
     call pcom_check(pcom_step(ctx, 1_c_int), 'pcom_step')
     call pcom_check(pcom_get_counters(ctx, cnt), 'pcom_get_counters')
     cd = [cnt%year, cnt%month, cnt%day, cnt%hour, cnt%minute, cnt%second]
@@ -151,6 +175,50 @@ program pcom_main
   call pcom_check(pcom_destroy(ctx), 'pcom_destroy')
 
 contains
+
+  ! record `rec` of each forcing file named in the namelist, handed to the
+  ! library as one refresh (interpolate = 1: the daily one after int_bnd)
+  subroutine apply_forcing(rec, interpolate)
+    integer, intent(in) :: rec
+    integer(c_int), intent(in) :: interpolate
+    type(pcom_forcing) :: f
+    integer(c_int64_t) :: rb
+    rb = int(glo_ni, c_int64_t) * glo_nj * 8
+    f%taux = c_null_ptr; f%tauy = c_null_ptr; f%pa = c_null_ptr; f%bct = c_null_ptr
+    if (len_trim(ff_ws) > 0) then
+      call read_record(ff_ws, 2 * rb * (rec - 1) + 1, ftx)
+      call read_record(ff_ws, 2 * rb * (rec - 1) + rb + 1, fty)
+      f%taux = c_loc(ftx); f%tauy = c_loc(fty)
+    end if
+    if (len_trim(ff_pa) > 0) then
+      call read_record(ff_pa, rb * (rec - 1) + 1, fpa)
+      f%pa = c_loc(fpa)
+    end if
+    if (len_trim(ff_bct) > 0) then
+      call read_record(ff_bct, rb * (rec - 1) + 1, fbt)
+      f%bct = c_loc(fbt)
+    end if
+    call pcom_check(pcom_set_forcing(ctx, f, interpolate), 'pcom_set_forcing')
+  end subroutine apply_forcing
+
+  subroutine read_record(fname, pos, a)
+    character(len=*), intent(in) :: fname
+    integer(c_int64_t), intent(in) :: pos
+    real(c_double), intent(out) :: a(:,:)
+    integer :: v, ios
+    open(newunit=v, file=trim(fname), access='stream', form='unformatted', status='old', action='read', &
+         iostat=ios)
+    if (ios /= 0) then
+      write(*, '(a, a)') 'pcom_main: cannot open forcing file ', trim(fname)
+      stop 1
+    end if
+    read(v, pos=pos, iostat=ios) a
+    if (ios /= 0) then
+      write(*, '(a, a, a, i0)') 'pcom_main: forcing file ', trim(fname), ' has no record at byte ', pos
+      stop 1
+    end if
+    close(v)
+  end subroutine read_record
 
   subroutine parse_date(s, d)
     character(len=*), intent(in) :: s

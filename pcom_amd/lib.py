@@ -31,7 +31,7 @@ EXPORTS = [
     "pcom_op_px2", "pcom_op_div", "pcom_op_lap", "pcom_den_rho", "pcom_get_metric",
     "pcom_set_clock", "pcom_output_means", "pcom_forcing_due", "pcom_set_forcing",
     "pcom_output_means_noalloc",
-    "pcom_nn_create", "pcom_nn_record", "pcom_nn_augment", "pcom_nn_train", "pcom_nn_use", "pcom_nn_eval", "pcom_nn_get_stats",
+    "pcom_nn_create", "pcom_nn_record", "pcom_nn_augment", "pcom_nn_train", "pcom_nn_use", "pcom_nn_resume", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
     "pcom_tune_subcycle", "pcom_subcycle_tiling",
@@ -85,6 +85,7 @@ def lib():
         L.pcom_nn_augment.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_ulonglong]
         L.pcom_nn_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, dp]
         L.pcom_nn_use.argtypes = [vp, C.c_int, C.c_double]
+        L.pcom_nn_resume.argtypes = [vp, C.c_int]
         L.pcom_nn_eval.argtypes = [vp, C.c_int, C.c_int, dp]
         L.pcom_nn_get_stats.argtypes = [vp, C.POINTER(abi.PcomNnStats)]
         L.pcom_nn_param_count.argtypes = [vp]
@@ -303,8 +304,10 @@ class Model:
         _check(lib().pcom_nn_train(self.h, int(iters), int(batch), float(lr), int(seed), _dp(loss)))
         return loss[:iters]
 
-    def nn_use(self, check_every, tol=0.0):
+    def nn_use(self, check_every, tol=0.0, resume=None):
         _check(lib().pcom_nn_use(self.h, int(check_every), float(tol)))
+        if resume is not None:
+            _check(lib().pcom_nn_resume(self.h, int(resume)))
 
     def nn_eval(self, first, count):
         """(network RMSE[4], persistence RMSE[4]) on recorded snapshots."""

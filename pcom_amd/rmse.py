@@ -9,10 +9,11 @@ for the oracle over runs the CPU cannot finish.
 
     python -m pcom_amd.rmse --days 365 --every 30 [--forms mixed,mixed+surrogate]
 
-A form "<form>+surrogate" records the first --train-days of plain
-subcycles, trains the learned barotropic surrogate on them (pcom_amd.
-surrogate) and then runs with it, the plain subcycle judging every
---check-every-th step.  The form "fp64+f32init" is the FP64 form started
+A form "<form>+surrogate" spins up --spin-days, records --snapshots plain
+subcycles and trains the learned barotropic surrogate on them
+(pcom_amd.surrogate.fit), then runs with it under the policy
+pcom_amd.surrogate.USE (the plain subcycle judging every 10th step); the
+other forms take the same steps alongside.  The form "fp64+f32init" is the FP64 form started
 from the initial T and S rounded to FP32: its distance from FP64 is the
 model's own sensitivity to perturbations at FP32 rounding level, the floor
 any non-bit-identical form meets.
@@ -24,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import abi, configs
+from . import abi, configs, surrogate
 from .lib import Model
 
 # the prognostic state a restart file holds (mod_io.f90:80-113), less the
@@ -75,11 +76,14 @@ def main(argv=None):
     ap.add_argument("--days", type=int, default=365)
     ap.add_argument("--every", type=int, default=30, help="report every N days")
     ap.add_argument("--forms", default="mixed", help="comma-separated forms to compare with FP64")
-    ap.add_argument("--train-days", type=int, default=20, help="surrogate forms: days recorded for training")
-    ap.add_argument("--iters", type=int, default=9000, help="surrogate forms: training iterations")
-    ap.add_argument("--check-every", type=int, default=10)
-    ap.add_argument("--tol", type=float, default=0.0,
-                    help="surrogate fallback bound (see pcom_nn_use; <0: relative to persistence)")
+    ap.add_argument("--spin-days", type=int, default=10, help="surrogate forms: plain days before recording")
+    ap.add_argument("--snapshots", type=int, default=1000, help="surrogate forms: steps recorded for training")
+    ap.add_argument("--iters", type=int, default=6000, help="surrogate forms: training iterations")
+    ap.add_argument("--check-every", type=int, default=surrogate.USE["check_every"])
+    ap.add_argument("--tol", type=float, default=surrogate.USE["tol"],
+                    help="surrogate check bound (pcom_nn_use; <0: relative to persistence)")
+    ap.add_argument("--resume", type=int, default=surrogate.USE["resume"],
+                    help="plain steps after a failed check (pcom_nn_resume)")
     ap.add_argument("--out", default=None, help="write the JSON report here")
     args = ap.parse_args(argv)
     cfg = configs.CONFIGS[args.config]()
@@ -93,32 +97,33 @@ def main(argv=None):
         if f.endswith("+f32init"):
             for name in ("tc", "sc"):
                 models[f].set(name, models[f].get(name).astype(np.float32).astype(np.float64))
-        if f.endswith("+surrogate"):
-            models[f].nn_create(capacity=args.train_days * steps_per_day, seed=1)
-            models[f].nn_record(True)
     report = {"config": cfg.name, "dims": [cfg.glo_ni, cfg.glo_nj, cfg.nk], "records": {f: [] for f in forms},
               "surrogate": {}}
     t0 = time.time()
+    # surrogate forms: fit first, then every model catches up to the next
+    # whole day, from which the network is in use
+    taken = {}
+    for f, m in models.items():
+        if f.endswith("+surrogate"):
+            rep = surrogate.fit(m, cfg, spin_days=args.spin_days, snapshots=args.snapshots, iters=args.iters,
+                                log=lambda s: print(s, flush=True))
+            taken[f] = rep["steps"]
+            report["surrogate"][f] = {k: rep[k] for k in ("steps", "recorded", "holdout_gain", "train_s")}
     day = 0
+    if taken:
+        day = -(-max(taken.values()) // steps_per_day)
+        ref.step(day * steps_per_day)
+        for f, m in models.items():
+            m.step(day * steps_per_day - taken.get(f, 0))
+            if f in taken:
+                m.nn_use(args.check_every, args.tol, args.resume)
+                report["surrogate"][f]["in_use_from_day"] = day
     while day < args.days:
         n = min(args.every, args.days - day)
-        if day < args.train_days:
-            n = min(n, args.train_days - day)
         ref.step(n * steps_per_day)
         for m in models.values():
             m.step(n * steps_per_day)
         day += n
-        if day == args.train_days:
-            for f, m in models.items():
-                if f.endswith("+surrogate"):
-                    m.nn_record(False)
-                    for chunk in range(0, args.iters, 1000):
-                        it = min(1000, args.iters - chunk)
-                        lr = 1e-3 if chunk < args.iters // 3 else 5e-4
-                        loss = m.nn_train(it, batch=8, lr=lr, seed=2 + chunk)
-                        print(json.dumps({"form": f, "training_iters": chunk + it, "loss": float(loss[-50:].mean())}),
-                              flush=True)
-                    m.nn_use(args.check_every, args.tol)
         for f, m in models.items():
             rec = {"form": f, "day": day, "finite": bool(ref.check_finite() and m.check_finite())}
             rec.update(rmse_report(m, ref, cfg))

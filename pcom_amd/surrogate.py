@@ -8,13 +8,13 @@
    RMSE against the plain subcycle beside the RMSE of persistence (the
    step without the network: no change of the barotropic mode);
 5. time plain steps and surrogate steps (plain judging subcycle every
-   `check_every`-th step);
+   `check_every`-th step, the policy USE);
 6. with --closed-loop-days D: run D days with the network in use beside a
    plain twin started from the same state, and report the RMSE of SSH,
    SST, the free surface and the barotropic velocity between them daily.
 
     python -m pcom_amd.surrogate --config global --spin-days 10 \\
-        --snapshots 480 --iters 9000 --out report.json
+        --snapshots 1000 --iters 6000 --closed-loop-days 30 --out report.json
 """
 import argparse
 import json
@@ -43,13 +43,36 @@ def step_scales(m, cfg):
     return float(np.sqrt(np.mean(dc[sea] ** 2))), float(np.sqrt(np.mean(du ** 2)))
 
 
-def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=48, iters=9000, batch=8,
-            lr=1e-3, seed=1, augment=0.5, amp=1.0, log=print):
-    """Spin up, record (with a fraction `augment` of perturbed snapshots
-    beside the plain ones), train and evaluate; returns (model, report).
-    report["steps"] is the number of steps the model took."""
-    spd = 86400 // cfg.dtbc
+def train(m, iters, batch=8, lr=1e-3, seed=2, log=None):
+    """`iters` Adam iterations on the recorded snapshots, the learning rate
+    on a cosine from `lr` to 0 over the run (pcom_nn_train); returns the
+    loss of every iteration."""
+    loss = m.nn_train(iters, batch=batch, lr=lr, seed=seed)
+    if log:
+        for k in range(1000, iters + 1, 1000):
+            log(f"  {k} iterations, loss {loss[k - 50:k].mean():.3e}")
+    return loss
+
+
+# how the network is used once trained: a plain checking subcycle every
+# 10th step; a check fails when an output's error exceeds twice that of
+# persistence (now, or at the first check); a failure hands 10 steps to
+# the plain subcycle, after which the network resumes
+USE = {"check_every": 10, "tol": -2.0, "resume": 10}
+
+
+def prepare(cfg, precision=abi.PCOM_MIXED, **kw):
+    """A new model of `cfg` put through fit(); returns (model, report)."""
     m = Model(cfg, precision=precision)
+    return m, fit(m, cfg, **kw)
+
+
+def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, lr=1e-3, seed=1, augment=0.5,
+        amp=1.0, log=print):
+    """On model m: spin up, record (with a fraction `augment` of perturbed
+    snapshots beside the plain ones), train and evaluate; returns the
+    report.  report["steps"] is the number of steps the model took."""
+    spd = 86400 // cfg.dtbc
     m.step(spin_days * spd)
     amp_ch, amp_u = step_scales(m, cfg)
     m.nn_create(capacity=int(snapshots * (1 + 2 * augment)) + holdout + 8, seed=seed)
@@ -59,12 +82,7 @@ def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=
     m.nn_record(False)
     m.nn_augment(0.0, 0.0, 0.0)
     t0 = time.time()
-    parts = []
-    for chunk in range(0, iters, 1000):
-        it = min(1000, iters - chunk)
-        parts.append(m.nn_train(it, batch=batch, lr=lr if chunk < iters // 3 else lr / 2, seed=seed + 1 + chunk))
-        log(f"  {chunk + it} iterations, loss {parts[-1][-50:].mean():.3e}")
-    loss = np.concatenate(parts)
+    loss = train(m, iters, batch, lr, seed + 1, log)
     train_s = time.time() - t0
     log(f"trained {iters} iterations in {train_s:.1f} s, loss {loss[:20].mean():.3e} -> {loss[-100:].mean():.3e}")
     first = m.nn_stats().recorded
@@ -83,10 +101,11 @@ def prepare(cfg, precision=abi.PCOM_MIXED, spin_days=10, snapshots=480, holdout=
         "holdout_gain": dict(zip(OUTPUTS, map(float, per / np.maximum(net, 1e-300)))),
     }
     log("held-out gain over persistence: " + ", ".join(f"{k} {v:.3f}" for k, v in report["holdout_gain"].items()))
-    return m, report
+    return report
 
 
-def closed_loop(m, cfg, steps_done, days, check_every, tol=0.0, log=print):
+def closed_loop(m, cfg, steps_done, days, check_every=USE["check_every"], tol=USE["tol"], resume=USE["resume"],
+                log=print):
     """The network in use for `days` days beside a plain twin from the same
     state (the twin repeats the `steps_done` steps the model has taken;
     recording does not change the state); daily RMSE between the two."""
@@ -94,7 +113,7 @@ def closed_loop(m, cfg, steps_done, days, check_every, tol=0.0, log=print):
     twin = Model(cfg, precision=m.precision)
     twin.step(steps_done)
     spd = 86400 // cfg.dtbc
-    m.nn_use(check_every, tol)
+    m.nn_use(check_every, tol, resume)
     recs = []
     for d in range(days):
         m.step(spd)
@@ -103,6 +122,7 @@ def closed_loop(m, cfg, steps_done, days, check_every, tol=0.0, log=print):
         st = m.nn_stats()
         rec = {"day": d + 1, "sst": r["sst"], "ssh": r["ssh"], "chc": r["chc"], "buc": r["buc"], "bvc": r["bvc"],
                "tc": r["tc"], "uc": r["uc"], "in_use": bool(st.in_use), "surrogate_steps": st.surrogate_steps,
+               "fallbacks": st.fallbacks,
                "check_rmse": list(st.last_check_rmse), "persist_rmse": list(st.last_persist_rmse)}
         recs.append(rec)
         log(json.dumps(rec))
@@ -110,7 +130,7 @@ def closed_loop(m, cfg, steps_done, days, check_every, tol=0.0, log=print):
     return recs
 
 
-def time_steps(m, nsteps, check_every):
+def time_steps(m, nsteps, check_every=USE["check_every"], tol=USE["tol"], resume=USE["resume"]):
     """ms per step of plain steps, then of steps with the surrogate."""
     m.nn_use(0)
     m.sync()
@@ -118,7 +138,7 @@ def time_steps(m, nsteps, check_every):
     m.step(nsteps)
     m.sync()
     plain = 1000.0 * (time.perf_counter() - t0) / nsteps
-    m.nn_use(check_every)
+    m.nn_use(check_every, tol, resume)
     m.sync()
     t0 = time.perf_counter()
     m.step(nsteps)
@@ -132,28 +152,32 @@ def main(argv=None):
     ap.add_argument("--config", default="global", choices=list(configs.CONFIGS))
     ap.add_argument("--form", default="mixed", choices=list(abi.FORMS))
     ap.add_argument("--spin-days", type=int, default=10)
-    ap.add_argument("--snapshots", type=int, default=480)
+    ap.add_argument("--snapshots", type=int, default=1000)
     ap.add_argument("--holdout", type=int, default=48)
-    ap.add_argument("--iters", type=int, default=9000)
+    ap.add_argument("--iters", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--check-every", type=int, default=USE["check_every"])
     ap.add_argument("--time-steps", type=int, default=240)
     ap.add_argument("--augment", type=float, default=0.5, help="fraction of recorded steps with a perturbed snapshot")
     ap.add_argument("--amp", type=float, default=1.0,
                     help="largest perturbation, in units of one step's change of ch and velocity")
     ap.add_argument("--closed-loop-days", type=int, default=0)
-    ap.add_argument("--tol", type=float, default=0.0, help="fallback bound of the closed-loop run (pcom_nn_use)")
+    ap.add_argument("--tol", type=float, default=USE["tol"], help="check bound (pcom_nn_use)")
+    ap.add_argument("--resume", type=int, default=USE["resume"], help="plain steps after a failed check (pcom_nn_resume)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     cfg = configs.CONFIGS[args.config]()
-    m, rep = prepare(cfg, abi.FORMS[args.form], args.spin_days, args.snapshots, args.holdout, args.iters,
-                     args.batch, args.lr, augment=args.augment, amp=args.amp)
+    m, rep = prepare(cfg, abi.FORMS[args.form], spin_days=args.spin_days, snapshots=args.snapshots,
+                     holdout=args.holdout, iters=args.iters, batch=args.batch, lr=args.lr, augment=args.augment,
+                     amp=args.amp)
     if args.closed_loop_days > 0:
-        rep["closed_loop"] = closed_loop(m, cfg, rep["steps"], args.closed_loop_days, args.check_every, args.tol)
-    plain, sur = time_steps(m, args.time_steps, args.check_every)
+        rep["closed_loop"] = closed_loop(m, cfg, rep["steps"], args.closed_loop_days, args.check_every, args.tol,
+                                         args.resume)
+    plain, sur = time_steps(m, args.time_steps, args.check_every, args.tol, args.resume)
     st = m.nn_stats()
-    rep.update({"form": args.form, "check_every": args.check_every, "ms_per_step_plain": plain,
+    rep.update({"form": args.form, "check_every": args.check_every, "tol": args.tol, "resume": args.resume,
+                "ms_per_step_plain": plain,
                 "ms_per_step_surrogate": sur, "speedup": plain / sur,
                 "last_check_rmse": list(st.last_check_rmse), "last_persist_rmse": list(st.last_persist_rmse),
                 "finite": bool(m.check_finite())})

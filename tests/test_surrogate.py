@@ -172,3 +172,28 @@ def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
     # mass: the surrogate removes its mean free-surface increment
     s_nn, s_pl = m.global_sums()[2], plain.global_sums()[2]
     assert abs(s_nn - s_pl) <= 1e-9 * abs(s_pl)
+
+
+@pytest.mark.gpu
+def test_gpu_failed_check_holds_then_resumes(gpu_lib):
+    """pcom_nn_resume: with a bound no prediction meets (tol = 1e-300), every
+    check fails; each failure runs 3 plain steps and the network resumes.
+    Checks every 2nd step: network at step 1, checks at 2, 6 and 10, plain
+    steps 2-10; without resume the first failure ends the surrogate."""
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    m = Model(cfg)
+    m.nn_create(capacity=8, seed=2)
+    m.nn_record(True)
+    m.step(8)
+    m.nn_record(False)
+    m.nn_train(20, batch=2, lr=1e-3, seed=3)
+    m.nn_use(2, tol=1e-300, resume=3)
+    m.step(10)
+    st = m.nn_stats()
+    assert (st.surrogate_steps, st.checks, st.fallbacks, st.plain_steps) == (1, 3, 3, 9 + 8)
+    assert m.check_finite()
+    m.nn_use(2, tol=1e-300, resume=0)
+    m.step(6)
+    st = m.nn_stats()
+    assert (st.surrogate_steps, st.checks, st.fallbacks, st.in_use) == (2, 4, 4, 0)

@@ -142,7 +142,9 @@ def test_config4_record_train_use_at_1deg(gpu_lib):
     """At 1 degree: record plain subcycles (half of the steps with perturbed
     snapshots beside them), train on the device, then step with the network
     and a plain checking subcycle every 10th step; the network learns (loss
-    falls), the run stays finite and conserves the mass of the plain run."""
+    falls), beats persistence on 48 held-out steps in all four outputs, and
+    the run stays finite and conserves the mass of the plain run.  Recording
+    leaves the state bit for bit as plain steps do."""
     from pcom_amd.lib import Model
     from pcom_amd.surrogate import step_scales
     cfg = configs.global_1deg()
@@ -152,15 +154,29 @@ def test_config4_record_train_use_at_1deg(gpu_lib):
     m.step(120)
     amp_ch, amp_u = step_scales(m, cfg)
     plain.step(121)
-    m.nn_create(capacity=500, seed=3)
+    m.nn_create(capacity=600, seed=3)
     m.nn_augment(0.5, amp_ch, amp_u, seed=9)
     m.nn_record(True)
     m.step(240)
     plain.step(240)
     m.nn_record(False)
     assert m.nn_stats().recorded > 240
+    # recording, perturbed snapshots included, leaves the state as plain
+    # steps leave it
+    for n in ("tc", "sc", "uc", "vc", "chc", "chp", "mch", "mch2", "buc", "bvc", "bup", "bvp", "acch", "am"):
+        assert np.array_equal(m.get(n), plain.get(n)), n
     loss = m.nn_train(3000, batch=8, lr=1e-3, seed=4)
     assert np.all(np.isfinite(loss)) and loss[-50:].mean() < 0.5 * loss[:10].mean()
+    first = m.nn_stats().recorded
+    m.nn_augment(0.0, 0.0, 0.0)
+    m.nn_record(True)
+    m.step(48)
+    plain.step(48)
+    m.nn_record(False)
+    net, per = m.nn_eval(first, 48)
+    gain = per / net
+    _say("config 4: held-out gain over persistence (mch2, mch, ubt, vbt) " + " ".join(f"{g:.2f}" for g in gain))
+    assert np.all(gain > 1.2), gain
     m.nn_use(10, tol=0.0)
     m.step(20)
     plain.step(20)
