@@ -97,152 +97,108 @@ __device__ __forceinline__ bf8 load8(const bf16_t* p) {
 }
 __device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uint4(0, 0, 0, 0)); }
 
-// ---- forward 3x3 convolution: out[p][co] = b[co] + sum W[co][dy][dx][ci]
-// in[p + d (dy-1, dx-1)][ci] (d = g.dil), periodic in x, zero outside
-// 0 <= y < H.  One wave computes PB blocks of 16 pixels x COUT channels, so
-// each weight fragment it loads serves PB matrix-core products; 4 waves per
-// workgroup.
-template <int CIN, int COUT, bool RELU, bool OUTF32, int PB>
-__global__ void __launch_bounds__(256) k_conv_fwd(Geo g, const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
-                                                  const float* __restrict__ bias, void* out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long np = g.npix();
-  const long pb0 = ((long)xcd_block() * 4 + wave) * PB;  // first 16-pixel block of this wave
-  if (pb0 * 16 >= np) return;
-  const int kq = lane >> 4, r = lane & 15;
-  int x[PB], y[PB], n[PB];
-  bool live[PB];
-#pragma unroll
-  for (int b = 0; b < PB; ++b) {
-    const long pa = (pb0 + b) * 16 + r;
-    live[b] = (pb0 + b) * 16 < np;
-    const long pc = live[b] ? pa : 0;
-    x[b] = (int)(pc % g.W);
-    const long t = pc / g.W;
-    y[b] = (int)(t % g.H);
-    n[b] = (int)(t / g.H);
+
+// ---- the forward convolution, and the backward data pass, with the layer's
+// weights staged in LDS once per workgroup of 8 waves (a wave re-reading
+// them through the vector cache streams 9 KIN KOUT BF16 from L2 per pixel
+// block otherwise).  out[p][o] = sum_t,i Wl[o][t][i] in[p + s d t][i] with
+// KIN input and KOUT output channels: the forward pass (s = +1, Wl = W,
+// bias, then ReLU to BF16 or linear to FP32) or the data gradient (s = -1,
+// Wl = W transposed to [ci][dy][dx][co], masked by the ReLU whose output
+// `post` is the layer's input).  LDS holds 16-byte B fragments in the order
+// [tap][c][kq][o], so the 16 lanes of one kq read 256 contiguous bytes.
+// Each wave takes PB pixel blocks at a time, grid-strided.
+constexpr int kConvLdsWaves = 8;
+enum ConvMode { kFwdRelu, kFwdLinear, kBwdData };
+template <int KIN, int KOUT>
+constexpr size_t conv_lds_bytes() { return (size_t)9 * KIN * KOUT * 2; }
+
+template <int KIN, int KOUT, int MODE, int PB>
+__global__ void __launch_bounds__(64 * kConvLdsWaves) k_conv_lds(Geo g, const bf16_t* __restrict__ in,
+                                                                 const bf16_t* __restrict__ w,
+                                                                 const float* __restrict__ bias,
+                                                                 const bf16_t* __restrict__ post, void* out) {
+  extern __shared__ uint4 wl[];
+  constexpr int NC = KIN / 32, NF = 9 * NC * 4 * KOUT, S = MODE == kBwdData ? -1 : 1;
+  // global order (o, tap, 8-channel group), LDS order (tap, c, kq, o)
+  for (int f = threadIdx.x; f < NF; f += blockDim.x) {
+    const int o = f / (9 * NC * 4), rem = f % (9 * NC * 4);
+    const int t = rem / (NC * 4), c8 = rem % (NC * 4);
+    wl[((t * NC + c8 / 4) * 4 + c8 % 4) * KOUT + o] = reinterpret_cast<const uint4*>(w)[f];
   }
-  f4 acc[PB][COUT / 16];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kq = lane >> 4, r = lane & 15;
+  const long np = g.npix(), npb = np / 16;
+  float bb[KOUT / 16];
 #pragma unroll
-  for (int b = 0; b < PB; ++b)
+  for (int nb = 0; nb < KOUT / 16; ++nb) bb[nb] = MODE == kBwdData ? 0.f : bias[nb * 16 + r];
+  const long stride = (long)gridDim.x * kConvLdsWaves * PB;
+  for (long pb0 = ((long)xcd_block() * kConvLdsWaves + wave) * PB; pb0 < npb; pb0 += stride) {
+    int x[PB], y[PB], n[PB];
+    bool live[PB];
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; ++nb) acc[b][nb] = f4{0, 0, 0, 0};
+    for (int b = 0; b < PB; ++b) {
+      live[b] = pb0 + b < npb;
+      const long pc = live[b] ? (pb0 + b) * 16 + r : 0;
+      x[b] = (int)(pc % g.W);
+      const long t = pc / g.W;
+      y[b] = (int)(t % g.H);
+      n[b] = (int)(t / g.H);
+    }
+    f4 acc[PB][KOUT / 16];
 #pragma unroll
-  for (int dy = 0; dy < 3; ++dy) {
+    for (int b = 0; b < PB; ++b)
 #pragma unroll
-    for (int dx = 0; dx < 3; ++dx) {
-      const bf16_t* src[PB];
-      bool ok[PB];
+      for (int nb = 0; nb < KOUT / 16; ++nb) acc[b][nb] = f4{0, 0, 0, 0};
+    // one row of taps at a time (the registers of all nine would spill)
+#pragma unroll 1
+    for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
-      for (int b = 0; b < PB; ++b) {
-        const int yy = y[b] + (dy - 1) * g.dil;
-        ok[b] = live[b] && yy >= 0 && yy < g.H;
-        const int xx = g.wrap(x[b] + (dx - 1) * g.dil);
-        src[b] = in + (((long)n[b] * g.H + (ok[b] ? yy : 0)) * g.W + xx) * CIN + kq * 8;
-      }
+      for (int dx = 0; dx < 3; ++dx) {
+        const bf16_t* src[PB];
+        bool ok[PB];
 #pragma unroll
-      for (int c = 0; c < CIN / 32; ++c) {
-        bf8 a[PB];
+        for (int b = 0; b < PB; ++b) {
+          const int yy = y[b] + S * (dy - 1) * g.dil;
+          ok[b] = live[b] && yy >= 0 && yy < g.H;
+          const int xx = g.wrap(x[b] + S * (dx - 1) * g.dil);
+          src[b] = in + (((long)n[b] * g.H + (ok[b] ? yy : 0)) * g.W + xx) * KIN + kq * 8;
+        }
 #pragma unroll
-        for (int b = 0; b < PB; ++b) a[b] = ok[b] ? load8(src[b] + c * 32) : zero8();
+        for (int c = 0; c < NC; ++c) {
+          bf8 a[PB];
 #pragma unroll
-        for (int nb = 0; nb < COUT / 16; ++nb) {
-          const bf8 wb = load8(w + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * CIN + c * 32 + kq * 8);
+          for (int b = 0; b < PB; ++b) a[b] = ok[b] ? load8(src[b] + c * 32) : zero8();
 #pragma unroll
-          for (int b = 0; b < PB; ++b) acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], wb, acc[b][nb], 0, 0, 0);
+          for (int nb = 0; nb < KOUT / 16; ++nb) {
+            const bf8 wb = __builtin_bit_cast(bf8, wl[(((dy * 3 + dx) * NC + c) * 4 + kq) * KOUT + nb * 16 + r]);
+#pragma unroll
+            for (int b = 0; b < PB; ++b)
+              acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], wb, acc[b][nb], 0, 0, 0);
+          }
         }
       }
     }
-  }
 #pragma unroll
-  for (int b = 0; b < PB; ++b) {
-    if (!live[b]) continue;
+    for (int b = 0; b < PB; ++b) {
+      if (!live[b]) continue;
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; ++nb) {
-      const int co = nb * 16 + r;
-      const float bb = bias[co];
+      for (int nb = 0; nb < KOUT / 16; ++nb) {
+        const int o = nb * 16 + r;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long pd = (pb0 + b) * 16 + kq * 4 + i;
-        float v = acc[b][nb][i] + bb;
-        if (RELU) v = v > 0.f ? v : 0.f;
-        if (OUTF32) static_cast<float*>(out)[pd * COUT + co] = v;
-        else static_cast<bf16_t*>(out)[pd * COUT + co] = f2bf(v);
+        for (int i = 0; i < 4; ++i) {
+          const long e = ((pb0 + b) * 16 + kq * 4 + i) * KOUT + o;
+          const float v = acc[b][nb][i] + bb[nb];
+          if (MODE == kFwdRelu) static_cast<bf16_t*>(out)[e] = f2bf(v > 0.f ? v : 0.f);
+          if (MODE == kFwdLinear) static_cast<float*>(out)[e] = v;
+          if (MODE == kBwdData) static_cast<bf16_t*>(out)[e] = bf2f(post[e]) > 0.f ? f2bf(v) : (bf16_t)0;
+        }
       }
     }
   }
 }
 
-// ---- backward data: din[p][ci] = sum W[co][dy][dx][ci] dpre[p - d (dy-1, dx-1)][co]
-// with wt = W transposed to [ci][dy][dx][co], masked by the ReLU that made
-// this layer's input (post > 0); PB pixel blocks per wave as in the forward
-// pass.
-template <int CIN, int COUT, int PB>
-__global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16_t* __restrict__ dpre,
-                                                       const bf16_t* __restrict__ wt, const bf16_t* __restrict__ post,
-                                                       bf16_t* din) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long np = g.npix();
-  const long pb0 = ((long)xcd_block() * 4 + wave) * PB;
-  if (pb0 * 16 >= np) return;
-  const int kq = lane >> 4, r = lane & 15;
-  int x[PB], y[PB], n[PB];
-  bool live[PB];
-#pragma unroll
-  for (int b = 0; b < PB; ++b) {
-    const long pa = (pb0 + b) * 16 + r;
-    live[b] = (pb0 + b) * 16 < np;
-    const long pc = live[b] ? pa : 0;
-    x[b] = (int)(pc % g.W);
-    const long t = pc / g.W;
-    y[b] = (int)(t % g.H);
-    n[b] = (int)(t / g.H);
-  }
-  f4 acc[PB][CIN / 16];
-#pragma unroll
-  for (int b = 0; b < PB; ++b)
-#pragma unroll
-    for (int nb = 0; nb < CIN / 16; ++nb) acc[b][nb] = f4{0, 0, 0, 0};
-#pragma unroll
-  for (int dy = 0; dy < 3; ++dy) {
-#pragma unroll
-    for (int dx = 0; dx < 3; ++dx) {
-      const bf16_t* src[PB];
-      bool ok[PB];
-#pragma unroll
-      for (int b = 0; b < PB; ++b) {
-        const int yy = y[b] - (dy - 1) * g.dil;
-        ok[b] = live[b] && yy >= 0 && yy < g.H;
-        const int xx = g.wrap(x[b] - (dx - 1) * g.dil);
-        src[b] = dpre + (((long)n[b] * g.H + (ok[b] ? yy : 0)) * g.W + xx) * COUT + kq * 8;
-      }
-#pragma unroll
-      for (int c = 0; c < COUT / 32; ++c) {
-        bf8 a[PB];
-#pragma unroll
-        for (int b = 0; b < PB; ++b) a[b] = ok[b] ? load8(src[b] + c * 32) : zero8();
-#pragma unroll
-        for (int nb = 0; nb < CIN / 16; ++nb) {
-          const bf8 wb = load8(wt + ((long)((nb * 16 + r) * 3 + dy) * 3 + dx) * COUT + c * 32 + kq * 8);
-#pragma unroll
-          for (int b = 0; b < PB; ++b) acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], wb, acc[b][nb], 0, 0, 0);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < PB; ++b) {
-    if (!live[b]) continue;
-#pragma unroll
-    for (int nb = 0; nb < CIN / 16; ++nb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // through the ReLU whose output `post` is this layer's input
-        const long e = ((pb0 + b) * 16 + kq * 4 + i) * CIN + nb * 16 + r;
-        din[e] = bf2f(post[e]) > 0.f ? f2bf(acc[b][nb][i]) : (bf16_t)0;
-      }
-  }
-}
 
 // ---- backward weights, one tap (dy, dx) per blockIdx.y and one pixel chunk
 // per blockIdx.x: part[chunk][tap][co][ci] = sum_p dpre[p][co] in[p+d(dy-1,dx-1)][ci].
@@ -607,11 +563,16 @@ __global__ void k_scale_part(int H, int W, const float* raw, const float* stat, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_scale_final(const double* part, int nb, double* scale, float* keep) {
+// the snapshot scale s = sea RMS of (r_u, r_v) from k_scale_part's partials
+__device__ __forceinline__ double fold_scale(const double* part, int nb) {
   double a[2];
   fold_parts<2>(part, nb, a);
+  return a[1] > 0.0 && a[0] > 0.0 ? sqrt(a[0] / a[1]) : 1e-30;
+}
+
+__global__ void __launch_bounds__(256) k_scale_final(const double* part, int nb, double* scale, float* keep) {
+  const double sc = fold_scale(part, nb);
   if (threadIdx.x == 0) {
-    const double sc = a[1] > 0.0 && a[0] > 0.0 ? sqrt(a[0] / a[1]) : 1e-30;
     *scale = sc;
     if (keep) *keep = (float)sc;
   }
@@ -808,12 +769,16 @@ __global__ void k_pack(int H, int W, const float* sin, const float* stg, const f
 }
 
 // the live snapshot into the network input (one image)
-__global__ void k_pack_live(int H, int W, const float* raw, const double* scale, NnNorm nm, const float* stat,
-                            bf16_t* x) {
+// the live state's network input; every block folds the scale partials
+// itself (the same fixed order, so the same s), block 0 keeps s in *scale
+__global__ void __launch_bounds__(256) k_pack_live(int H, int W, const float* raw, const double* part, int nb,
+                                                   double* scale, NnNorm nm, const float* stat, bf16_t* x) {
+  const double sc = fold_scale(part, nb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *scale = sc;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
   if (q >= hw) return;
-  pack_pixel(raw, hw, q, (float)*scale, nm, stat, x + q * kInC);
+  pack_pixel(raw, hw, q, (float)sc, nm, stat, x + q * kInC);
 }
 
 // prediction -> increments (FP64) with per-block sums over sea of
@@ -856,22 +821,18 @@ __global__ void k_pred(int ni, int H, int W, const float* out, NnNorm nm, const 
     for (int c = 0; c < 3; ++c) part[3 * blockIdx.x + c] = s[c][0];
 }
 
-// sea means of the two free-surface increments, then removed from them
-__global__ void __launch_bounds__(256) k_pred_mean(const double* part, int nb, double* mean) {
+// the weighted sea means of the two free-surface increments (every block
+// folds k_pred's partials itself, in the same order), removed from them
+__global__ void __launch_bounds__(256) k_pred_fix(long hw, const double* part, int nb, const float* stat,
+                                                  double* inc) {
   double a[3];
   fold_parts<3>(part, nb, a);
-  if (threadIdx.x == 0) {
-    mean[0] = a[2] > 0.0 ? a[0] / a[2] : 0.0;
-    mean[1] = a[2] > 0.0 ? a[1] / a[2] : 0.0;
-  }
-}
-
-__global__ void k_pred_fix(long hw, const double* mean, const float* stat, double* inc) {
+  const double m0 = a[2] > 0.0 ? a[0] / a[2] : 0.0, m1 = a[2] > 0.0 ? a[1] / a[2] : 0.0;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= hw) return;
   const double sea = (double)stat[q];
-  inc[q] = inc[q] - mean[0] * sea;
-  inc[hw + q] = inc[hw + q] - mean[1] * sea;
+  inc[q] = inc[q] - m0 * sea;
+  inc[hw + q] = inc[hw + q] - m1 * sea;
 }
 
 // write the end-of-cycle fields (mod_int.f90:276-287) from the increments,
@@ -1104,45 +1065,53 @@ std::string nn_batch_buffers(Nn& n, int batch) {
   return "";
 }
 
-// forward pass over `batch` images already packed into act[0]
-std::string nn_forward(Nn& n, hipStream_t st, int batch) {
-  Geo g{batch, n.H, n.W, layer_dil(0)};
-  // two pixel blocks per wave for one image (more waves on the device), four
-  // for minibatches (fewer weight loads)
+// forward pass over `batch` images already packed into act[0]: the layers'
+// weights staged in LDS, workgroups of 8 waves taking two pixel blocks per
+// wave at a time, at most kConvGroups workgroups
+constexpr long kConvGroups = 512;
+template <int KIN, int KOUT, int MODE, int PB = 2>
+std::string conv_lds(hipStream_t st, const Geo& g, const bf16_t* in, const bf16_t* w, const float* b,
+                     const bf16_t* post, void* out) {
+  const auto kern = k_conv_lds<KIN, KOUT, MODE, PB>;
+  static bool attr = false;  // once per process: the attribute holds for every device
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)conv_lds_bytes<KIN, KOUT>());
+    if (e != hipSuccess) return std::string("hipFuncSetAttribute k_conv_lds: ") + hipGetErrorString(e);
+    attr = true;
+  }
   const long pblocks = g.npix() / 16;
-  if (batch == 1) {
-    const unsigned blocks = (unsigned)((pblocks + 7) / 8);
-    NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false, 2>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
-              (void*)n.act[1]);
-    for (int l = 1; l < kLayers - 1; ++l) {
-      g.dil = layer_dil(l);
-      NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false, 2>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
-                (void*)n.act[l + 1]);
-    }
-    g.dil = layer_dil(kLayers - 1);
-    NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true, 2>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
-              n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
-    return "";
-  }
-  const unsigned blocks = (unsigned)((pblocks + 15) / 16);
-  NN_LAUNCH((k_conv_fwd<kInC, kHid, true, false, 4>), dim3(blocks), dim3(256), g, n.act[0], n.L[0].w, n.L[0].b,
-            (void*)n.act[1]);
-  for (int l = 1; l < kLayers - 1; ++l) {
-    g.dil = layer_dil(l);
-    NN_LAUNCH((k_conv_fwd<kHid, kHid, true, false, 4>), dim3(blocks), dim3(256), g, n.act[l], n.L[l].w, n.L[l].b,
-              (void*)n.act[l + 1]);
-  }
-  g.dil = layer_dil(kLayers - 1);
-  NN_LAUNCH((k_conv_fwd<kHid, kOutC, false, true, 4>), dim3(blocks), dim3(256), g, n.act[kLayers - 1],
-            n.L[kLayers - 1].w, n.L[kLayers - 1].b, (void*)n.out);
+  const unsigned blocks = (unsigned)std::min(kConvGroups, (pblocks + kConvLdsWaves * PB - 1) / (kConvLdsWaves * PB));
+  const size_t lds = conv_lds_bytes<KIN, KOUT>();
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kConvLdsWaves), lds, st, g, in, w, b, post, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("k_conv_lds: ") + hipGetErrorString(e);
   return "";
+}
+
+template <int PB>
+std::string nn_forward_pb(Nn& n, hipStream_t st, int batch) {
+  Geo g{batch, n.H, n.W, layer_dil(0)};
+  std::string e = conv_lds<kInC, kHid, kFwdRelu, PB>(st, g, n.act[0], n.L[0].w, n.L[0].b, nullptr, (void*)n.act[1]);
+  for (int l = 1; l < kLayers - 1 && e.empty(); ++l) {
+    g.dil = layer_dil(l);
+    e = conv_lds<kHid, kHid, kFwdRelu, PB>(st, g, n.act[l], n.L[l].w, n.L[l].b, nullptr, (void*)n.act[l + 1]);
+  }
+  if (!e.empty()) return e;
+  g.dil = layer_dil(kLayers - 1);
+  return conv_lds<kHid, kOutC, kFwdLinear, PB>(st, g, n.act[kLayers - 1], n.L[kLayers - 1].w, n.L[kLayers - 1].b,
+                                               nullptr, (void*)n.out);
+}
+
+// one image: a pixel block per wave (more waves in flight); minibatches: two
+std::string nn_forward(Nn& n, hipStream_t st, int batch) {
+  return batch == 1 ? nn_forward_pb<1>(n, st, batch) : nn_forward_pb<2>(n, st, batch);
 }
 
 // backward pass from dpre of the last layer (in act[kLayers], BF16)
 std::string nn_backward(Nn& n, hipStream_t st, int batch) {
   Geo g{batch, n.H, n.W, 1};
   const long np = g.npix();
-  const unsigned blocks = (unsigned)((np / 16 + 15) / 16);
   const int nch = (int)((np + n.wchunk - 1) / n.wchunk);
   const int nbias = (int)((np + kBiasChunk - 1) / kBiasChunk);
   bf16_t* dpre = n.act[kLayers];  // d loss / d (last pre-activation), 32 channels
@@ -1163,10 +1132,10 @@ std::string nn_backward(Nn& n, hipStream_t st, int batch) {
     // gradient of this layer's input = previous layer's post-activation
     bf16_t* dpost = (dpre == n.g0) ? n.g1 : n.g0;
     // through the ReLU of layer l-1, whose output is act[l]
-    if (L.cout == kOutC)
-      NN_LAUNCH((k_conv_bwd_data<kHid, kOutC, 4>), dim3(blocks), dim3(256), g, dpre, L.wt, n.act[l], dpost);
-    else
-      NN_LAUNCH((k_conv_bwd_data<kHid, kHid, 4>), dim3(blocks), dim3(256), g, dpre, L.wt, n.act[l], dpost);
+    std::string e = L.cout == kOutC
+                        ? conv_lds<kOutC, kHid, kBwdData>(st, g, dpre, L.wt, nullptr, n.act[l], (void*)dpost)
+                        : conv_lds<kHid, kHid, kBwdData>(st, g, dpre, L.wt, nullptr, n.act[l], (void*)dpost);
+    if (!e.empty()) return e;
     dpre = dpost;
   }
   return "";
@@ -1329,7 +1298,7 @@ namespace {
 // the raw inputs of the live state (the first substep's increment and the
 // fields the subcycle's operator depends on) into `raw`, and its scale into
 // misc[20] (and *keep when recording)
-std::string nn_raw_inputs(Ctx& c, float* raw, float* keep) {
+std::string nn_raw_inputs(Ctx& c, float* raw, float* keep, bool final = true) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
   const Fields2& h = c.f2;
@@ -1352,7 +1321,8 @@ std::string nn_raw_inputs(Ctx& c, float* raw, float* keep) {
   }
   NN_LAUNCH(k_sub1_raw, dim3(nblk(hw)), dim3(256), c.gd, n->H, n->W, r, bt, raw);
   NN_LAUNCH(k_scale_part, dim3(nblk(hw)), dim3(256), n->H, n->W, raw, n->stat, n->dpart);
-  NN_LAUNCH(k_scale_final, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc + 20, keep);
+  // (the live path folds the partials in k_pack_live instead)
+  if (final) NN_LAUNCH(k_scale_final, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc + 20, keep);
   return "";
 }
 
@@ -1361,9 +1331,10 @@ std::string nn_predict(Ctx& c) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
   const long hw = (long)n->H * n->W;
-  std::string e = nn_raw_inputs(c, n->live, nullptr);
+  std::string e = nn_raw_inputs(c, n->live, nullptr, false);
   if (!e.empty()) return e;
-  NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->H, n->W, n->live, n->misc + 20, n->norm, n->stat, n->act[0]);
+  NN_LAUNCH(k_pack_live, dim3(nblk(hw)), dim3(256), n->H, n->W, n->live, n->dpart, (int)nblk(hw), n->misc + 20, n->norm,
+            n->stat, n->act[0]);
   e = nn_forward(*n, st, 1);
   if (!e.empty()) return e;
   // once anchored, increments are bounded pointwise by kLimitAnchors times
@@ -1373,8 +1344,7 @@ std::string nn_predict(Ctx& c) {
     for (int q = 0; q < kOutUsed; ++q) lim.v[q] = kLimitAnchors * n->anchor[q];
   NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->misc + 20, n->stat, n->massw,
             lim, n->inc, n->dpart);
-  NN_LAUNCH(k_pred_mean, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc);
-  NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->misc, n->stat, n->inc);
+  NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->dpart, (int)nblk(hw), n->stat, n->inc);
   return "";
 }
 }  // namespace
