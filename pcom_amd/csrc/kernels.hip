@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kColCW * kColKL) k_upw(Grid<R> g, const double
     const int lev = g.lev_t[p];
     R s = 0;
     for (int k = 1; k <= nk; ++k) s += w3[k - 1][tx] * g.dpr2[k - 1] * (R)(lev >= k ? 1 : 0);
-    R wk2 = -(s / g.prh_t_r[p]);
+    R wk2 = -dv(s, g.prh_t_r[p]);
     R w = wm[p];  // wm(1) is never written by upwelling
     R d = w3[0][tx];
     w3[0][tx] = w;
@@ -355,9 +355,9 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
   __shared__ R fuu[kAdvTY][kAdvTX + 1], fm1[2][kAdvTY][kAdvTX + 1];
   __shared__ R fvv[kAdvTY + 1][kAdvTX], fm2[2][kAdvTY + 1][kAdvTX];
   auto SCH = [&](int m, int jj) -> R { return (R)schr[g.ix(m, jj)]; };
-  auto X = [&](int m, int jj) -> R { return (R)u[g.ix(m, jj)] / SCH(m, jj); };
+  auto X = [&](int m, int jj) -> R { return dv((R)u[g.ix(m, jj)], SCH(m, jj)); };
   auto Y = [&](int m, int jj) -> R {
-    R y = (R)v[g.ix(m, jj)] / SCH(m, jj);
+    R y = dv((R)v[g.ix(m, jj)], SCH(m, jj));
     return y * g.RH(3, m, jj);
   };
   auto VK = [&](int c, int m, int jj) -> R { return (R)(c ? v : u)[g.ix(m, jj)]; };
@@ -418,23 +418,24 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
     // w = ter(wm/chc, g1 -> g3) on level kk of nkp (ter_di: rows 1..njm)
     auto W = [&](int o) -> R {
       if (!rows) return (R)0;
-      return (in.wk[o][0] / in.ch[0] + in.wk[o][1] / in.ch[1] + in.wk[o][2] / in.ch[2] + in.wk[o][3] / in.ch[3]) *
+      return (dv(in.wk[o][0], in.ch[0]) + dv(in.wk[o][1], in.ch[1]) + dv(in.wk[o][2], in.ch[2]) +
+              dv(in.wk[o][3], in.ch[3])) *
              (R)0.25;
     };
     const R w_k = W(0), w_k1 = W(1);
     const R rh3 = in.rh3;
     const R dpr = g.dpr2[k - 1];
-    const R d1 = (fuu[ty][tx + 1] - fuu[ty][tx]) / in.dx13;
-    const R d2 = (rows ? fvv[ty + 1][tx] - fvv[ty][tx] : (R)0) / in.dx23;
-    const R d3 = (w_k - w_k1) / dpr;
+    const R d1 = dv(fuu[ty][tx + 1] - fuu[ty][tx], in.dx13);
+    const R d2 = dv(rows ? fvv[ty + 1][tx] - fvv[ty][tx] : (R)0, in.dx23);
+    const R d3 = dv(w_k - w_k1, dpr);
     const int lev = in.lev;
     const R mu = (R)(lev >= k ? 1 : 0);
     R out[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const R p1m = (fm1[c][ty][tx + 1] - fm1[c][ty][tx]) / in.dx13;
-      const R p2m = (rows ? fm2[c][ty + 1][tx] - fm2[c][ty][tx] : (R)0) / in.dx23;
-      R ans = (p1m + p2m) / ((R)ARAD * rh3);
+      const R p1m = dv(fm1[c][ty][tx + 1] - fm1[c][ty][tx], in.dx13);
+      const R p2m = dv(rows ? fm2[c][ty + 1][tx] - fm2[c][ty][tx] : (R)0, in.dx23);
+      R ans = dv(p1m + p2m, (R)ARAD * rh3);
       // vertical flux through the U-cell faces, zero unless both layers wet
       auto WB = [&](int kk) -> R {
         if (kk < 2 || kk > g.nk) return (R)0;
@@ -443,8 +444,8 @@ __global__ void __launch_bounds__(kAdvTX * kAdvTY)
         a = a * (kk == k ? w_k : w_k1);
         return a * (R)(lev >= kk - 1 ? 1 : 0) * (R)(lev >= kk ? 1 : 0);
       };
-      ans = ans + (WB(k) - WB(k + 1)) / dpr;
-      ans = ans - (R)0.5 * (R)in.var[c][1] * ((d1 + d2) / ((R)ARAD * rh3) + d3);
+      ans = ans + dv(WB(k) - WB(k + 1), dpr);
+      ans = ans - (R)0.5 * (R)in.var[c][1] * (dv(d1 + d2, (R)ARAD * rh3) + d3);
       out[c] = ans * mu;
     }
     auc[p] = out[0];
@@ -503,13 +504,13 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
   __shared__ R fa[2][kFriTY][kFriTX + 1];
   __shared__ R fb[2][kFriTY + 1][kFriTX];
   // the horizontal views at level k: v = uc/sch (u) or vc/sch (v)
-  auto VV = [&](int c, int m, int jj) -> R { return (R)(c ? vc : uc)[ko + g.ix(m, jj)] / (R)schr[g.ix(m, jj)]; };
+  auto VV = [&](int c, int m, int jj) -> R { return dv((R)(c ? vc : uc)[ko + g.ix(m, jj)], (R)schr[g.ix(m, jj)]); };
   auto CH = [&](int m, int jj) -> R { return (R)chc[g.ix(m, jj)]; };
   // VA(m, jj) = ter_ns<1>(ch) * grax<3,4>(v), m canonical
   auto face_ew = [&](int m, int jj, int row, int f) {
     const R chy = (jj < 1 || jj > g.njm) ? (R)0 : (R)0.5 * (CH(m, jj) + CH(m, jj + 1));
     for (int c = 0; c < 2; ++c) {
-      const R gx = (VV(c, m, jj) - VV(c, m - 1, jj)) / g.DX1(4, m, jj) / ((R)ARAD * g.RH(4, m, jj));
+      const R gx = dv(dv(VV(c, m, jj) - VV(c, m - 1, jj), g.DX1(4, m, jj)), (R)ARAD * g.RH(4, m, jj));
       fa[c][row][f] = chy * gx;
     }
   };
@@ -518,7 +519,7 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
     const R chx = (R)0.5 * (CH(m, jj) + CH(m + 1, jj));
     for (int c = 0; c < 2; ++c) {
       const R d = (jj < 2 || jj > g.nj) ? (R)0 : (VV(c, m, jj) - VV(c, m, jj - 1));
-      const R gy = d / g.DX2(2, m, jj) / (R)ARAD;
+      const R gy = dv(dv(d, g.DX2(2, m, jj)), (R)ARAD);
       fb[c][row][f] = chx * gy * g.RH(2, m, jj);
     }
   };
@@ -583,36 +584,36 @@ __global__ void __launch_bounds__(kFriTX * kFriTY)
       // fx from (u, uc, -v, taux), fy from (v, vc, u, tauy) (mod_op.f90:169-184)
       const R tau = (R)in.tau[c];
       // u or v / sch at the bottom level of this column
-      auto v3b = [&]() -> R { return (R)(c ? in.vb : in.ub) / sch; };
+      auto v3b = [&]() -> R { return dv((R)(c ? in.vb : in.ub), sch); };
       // -v for fx, u for fy, at the bottom level and at the east and west
       // neighbours of level k
-      auto vccb = [&]() -> R { return c ? (R)in.ub / sch : -((R)in.vb / sch); };
-      auto vcce = [&]() -> R { return c ? (R)in.ue / in.sche : -((R)in.ve / in.sche); };
-      auto vccw = [&]() -> R { return c ? (R)in.uw / in.schw : -((R)in.vw / in.schw); };
+      auto vccb = [&]() -> R { return c ? dv((R)in.ub, sch) : -dv((R)in.vb, sch); };
+      auto vcce = [&]() -> R { return c ? dv((R)in.ue, in.sche) : -dv((R)in.ve, in.sche); };
+      auto vccw = [&]() -> R { return c ? dv((R)in.uw, in.schw) : -dv((R)in.vw, in.schw); };
       // divg<4,2,3>(VA, VB) (mod_op.f90:813-848)
-      const R a1 = (fa[c][ty][tx + 1] - fa[c][ty][tx]) / in.dx13;
+      const R a1 = dv(fa[c][ty][tx + 1] - fa[c][ty][tx], in.dx13);
       const R d = (j < 1 || j > g.njm) ? (R)0 : (fb[c][ty + 1][tx] - fb[c][ty][tx]);
-      const R b1 = d / in.dx23;
-      const R wka = (a1 + b1) / ((R)ARAD * in.rh3);
-      R ans = a / sch * wka;
+      const R b1 = dv(d, in.dx23);
+      const R wka = dv(a1 + b1, (R)ARAD * in.rh3);
+      R ans = dv(a, sch) * wka;
       ans = ans + a * in.cv1 * (R)in.vp[c][1];
       const R dx = (R)ARAD * in.rh4 * (in.d14 + in.d14e);
-      const R pxc = (vcce() - vccw()) / dx;
+      const R pxc = dv(vcce() - vccw(), dx);
       ans = ans + a * in.cv2 * sch * pxc;
       auto WKP = [&](int kk) -> R {
-        if (kk == 1) return (R)GRAV * tau / sch;
+        if (kk == 1) return dv((R)GRAV * tau, sch);
         if (lev > 0 && kk == lev + 1) {
           R vv = v3b(), cc = vccb();
           R mag = sqrt(vv * vv + cc * cc);
-          return vv * (R)GRAV * (R)CDBOT * mag / sch;
+          return dv(vv * (R)GRAV * (R)CDBOT * mag, sch);
         }
         if (kk > g.nk) return (R)0;
         const int o = kk - k;
-        R pz = ((R)in.vp[c][o] - (R)in.vp[c][o + 1]) * (R)(lev >= kk - 1 ? 1 : 0) * (R)(lev >= kk ? 1 : 0) /
-               g.dz1[kk - 1];
-        return (R)GRAV * (R)RHO0 * (kk == k ? in.km0 : in.km1) * pz / (sch * sch);
+        R pz = dv(((R)in.vp[c][o] - (R)in.vp[c][o + 1]) * (R)(lev >= kk - 1 ? 1 : 0) * (R)(lev >= kk ? 1 : 0),
+                  g.dz1[kk - 1]);
+        return dv((R)GRAV * (R)RHO0 * (kk == k ? in.km0 : in.km1) * pz, sch * sch);
       };
-      const R vert = (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1];
+      const R vert = dv(WKP(k) - WKP(k + 1), g.dpr2[k - 1]);
       out[c] = ans * (R)(lev >= k ? 1 : 0) + vert;
     }
     fx[p] = out[0];
@@ -827,14 +828,14 @@ __global__ void __launch_bounds__(kSmagTX * kSmagTY)
     R dx = (R)ARAD * (g.RH(4, ic, j) * g.DX1(4, ic, j) + g.RH(4, ic + 1, j) * g.DX1(4, ic + 1, j));
     R dy = (R)ARAD * (g.DX2(2, ic, j) + g.DX2(2, ic, j + 1));
     R tn = g.tn[3][g.ix(ic, j)];
-    R dt = (R)4 * (su1[rr][cc + 1] - su1[rr][cc - 1]) / dx - (R)4 * (su2[rr + 1][cc] - su2[rr - 1][cc]) / dy +
-           su2[rr][cc] * tn / (R)ARAD;
-    R ds = (R)4 * (su2[rr][cc + 1] - su2[rr][cc - 1]) / dx + (R)4 * (su1[rr + 1][cc] - su1[rr - 1][cc]) / dy -
-           su1[rr][cc] * tn / (R)ARAD;
-    R cs = (R)0.5 * (R)SMAG_C / (R)PI_F * ((R)0.5 * dx + (R)0.5 * dy);
+    R dt = dv((R)4 * (su1[rr][cc + 1] - su1[rr][cc - 1]), dx) - dv((R)4 * (su2[rr + 1][cc] - su2[rr - 1][cc]), dy) +
+           dv(su2[rr][cc] * tn, (R)ARAD);
+    R ds = dv((R)4 * (su2[rr][cc + 1] - su2[rr][cc - 1]), dx) + dv((R)4 * (su1[rr + 1][cc] - su1[rr - 1][cc]), dy) -
+           dv(su1[rr][cc] * tn, (R)ARAD);
+    R cs = dv((R)0.5 * (R)SMAG_C, (R)PI_F) * ((R)0.5 * dx + (R)0.5 * dy);
     R c = cs * cs;
     R wk = c * sqrt(dt * dt + ds * ds);
-    R amax = (dx * dx + dy * dy) / (R)(4 * 8 * dtbc);
+    R amax = dv(dx * dx + dy * dy, (R)(4 * 8 * dtbc));
     amv = (amax < wk) ? amax : wk;
   }
   auto cell = [&](int i, bool own) {
@@ -954,11 +955,11 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) __attribute__((amdgpu_waves_per
     const R um = (jj < 2 || jj > g.nj) ? (R)0 : (R)0.5 * (UM1(m, jj - 1) + UM1(m, jj));  // ter_ns<3>
     const R chx = (R)0.5 * (CH(m, jj) + CH(m + 1, jj));                                 // ter_ew<1>
     for (int v = 0; v < 2; ++v) {
-      const R dv = VAR(v, m + 1, jj) - VAR(v, m, jj);
-      fx[v][row][col] = um * (dv / g.DX1(2, m, jj));  // * p1<1,2>
-      R d = dv * (R)g.mskt(m + 1, jj, k) * (R)g.mskt(m, jj, k);
-      d = d / g.DX1(2, m, jj);
-      fa[v][row][col] = chx * (d / ((R)ARAD * g.RH(2, m, jj)));
+      const R dvar = VAR(v, m + 1, jj) - VAR(v, m, jj);
+      fx[v][row][col] = um * dv(dvar, g.DX1(2, m, jj));  // * p1<1,2>
+      R d = dvar * (R)g.mskt(m + 1, jj, k) * (R)g.mskt(m, jj, k);
+      d = dv(d, g.DX1(2, m, jj));
+      fa[v][row][col] = chx * dv(d, (R)ARAD * g.RH(2, m, jj));
     }
   };
   // north-south face at g4 point (m, jj): Y and VB * rh_g4
@@ -966,11 +967,11 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) __attribute__((amdgpu_waves_per
     const R vm = (R)0.5 * (UM2(m - 1, jj) + UM2(m, jj));                                     // ter_ew<3>
     const R chy = (jj < 1 || jj > g.njm) ? (R)0 : (R)0.5 * (CH(m, jj) + CH(m, jj + 1));      // ter_ns<1>
     for (int v = 0; v < 2; ++v) {
-      const R dv = (jj < 1 || jj > g.njm) ? (R)0 : VAR(v, m, jj + 1) - VAR(v, m, jj);
-      fy[v][row][col] = vm * (dv / g.DX2(4, m, jj));  // * p2<1,4>
+      const R dvar = (jj < 1 || jj > g.njm) ? (R)0 : VAR(v, m, jj + 1) - VAR(v, m, jj);
+      fy[v][row][col] = vm * dv(dvar, g.DX2(4, m, jj));  // * p2<1,4>
       R d = (jj > g.njm) ? (R)0 : (VAR(v, m, jj + 1) - VAR(v, m, jj)) * (R)g.mskt(m, jj + 1, k) * (R)g.mskt(m, jj, k);
-      d = d / g.DX2(4, m, jj);
-      fb[v][row][col] = chy * (d / (R)ARAD) * g.RH(4, m, jj);
+      d = dv(d, g.DX2(4, m, jj));
+      fb[v][row][col] = chy * dv(d, (R)ARAD) * g.RH(4, m, jj);
     }
   };
   // the column inputs of cell (i, j): T/S at k-1, k, k+1 (0 outside 1..nk),
@@ -1020,9 +1021,9 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) __attribute__((amdgpu_waves_per
     for (int v = 0; v < 2; ++v) {
       // op_adv_ts (mod_op.f90:280-325)
       const R rh1 = in.rh1;
-      R ans = (R)0.5 * (fx[v][ty][tx] + fx[v][ty][tx + 1]) / ((R)ARAD * rh1);                      // ter_ew<2>
-      ans = ans + ((j < 2 || j > g.nj) ? (R)0 : (R)0.5 * (fy[v][ty][tx] + fy[v][ty + 1][tx])) /    // ter_ns<4>
-                      ((R)ARAD * rh1);
+      R ans = dv((R)0.5 * (fx[v][ty][tx] + fx[v][ty][tx + 1]), (R)ARAD * rh1);                      // ter_ew<2>
+      ans = ans + dv((j < 2 || j > g.nj) ? (R)0 : (R)0.5 * (fy[v][ty][tx] + fy[v][ty + 1][tx]),    // ter_ns<4>
+                     (R)ARAD * rh1);
       // var at kk - 1 minus var at kk (dx3_r3d_b), kk = k or k + 1
       auto DX3 = [&](int kk) -> R {
         if (kk < 2 || kk > g.nk) return (R)0;
@@ -1030,31 +1031,31 @@ __global__ void __launch_bounds__(kTsTX * kTsTY) __attribute__((amdgpu_waves_per
         return (R)in.var[v][o] - (R)in.var[v][o + 1];
       };
       R wka = (in.wm0 * DX3(k) + in.wm1 * DX3(k + 1)) * (R)0.5;
-      wka = wka / g.dpr2[k - 1];
+      wka = dv(wka, g.dpr2[k - 1]);
       pt[v] = -(ans + wka);
     }
     for (int v = 0; v < 2; ++v) {
       // op_dif (mod_op.f90:186-222): divg<2,4,1> of (VA, VB*rh)
-      const R a1 = (fa[v][ty][tx + 1] - fa[v][ty][tx]) / in.dx1c;
+      const R a1 = dv(fa[v][ty][tx + 1] - fa[v][ty][tx], in.dx1c);
       const R d = (j < 2 || j > g.nj) ? (R)0 : (fb[v][ty + 1][tx] - fb[v][ty][tx]);
-      const R b1 = d / in.dx2c;
-      R ans = (R)AH_C * ((a1 + b1) / ((R)ARAD * in.rh1c));
+      const R b1 = dv(d, in.dx2c);
+      R ans = (R)AH_C * dv(a1 + b1, (R)ARAD * in.rh1c);
       auto WKP = [&](int kk) -> R {
         R w = (R)0;
         if (kk >= 2 && kk <= g.nk) {
           const int o = kk - k;
-          w = ((R)in.var[v][o] - (R)in.var[v][o + 1]) * (R)(in.lev >= kk - 1 ? 1 : 0) *
-              (R)(in.lev >= kk ? 1 : 0) / g.dz1[kk - 1];
+          w = dv(((R)in.var[v][o] - (R)in.var[v][o + 1]) * (R)(in.lev >= kk - 1 ? 1 : 0) * (R)(in.lev >= kk ? 1 : 0),
+                 g.dz1[kk - 1]);
         }
         return (R)GRAV * (R)RHO0 * (kk == k ? in.kh0 : in.kh1) * w;
       };
-      pt[v] = pt[v] + (ans + (WKP(k) - WKP(k + 1)) / g.dpr2[k - 1]);
+      pt[v] = pt[v] + (ans + dv(WKP(k) - WKP(k + 1), g.dpr2[k - 1]));
     }
     const R mk = (R)(in.lev >= k ? 1 : 0);
     const R m = (R)in.mch;
     const double t0 = in.var[0][1], s0 = in.var[1][1];
     if (k == 1) {
-      pt[0] = pt[0] + (R)GAMMA_T * ((R)in.bt - (R)t0) * mk / g.dpr2[0];
+      pt[0] = pt[0] + dv((R)GAMMA_T * ((R)in.bt - (R)t0) * mk, g.dpr2[0]);
       pt[1] = pt[1] + (R)GAMMA_S * ((R)in.bs - (R)s0) * mk * m;
     }
     // the forward step of the FP64 state (mod_int.f90:484-487)
@@ -1201,10 +1202,10 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
       sc[q] = s;
       act[q] = act[q] + (R)t;
       acs[q] = acs[q] + (R)s;
-      acw[q] = acw[q] + wm[q] * (R)alpha[q] / ((R)GRAV * (R)mch2[p]);
+      acw[q] = acw[q] + dv(wm[q] * (R)alpha[q], (R)GRAV * (R)mch2[p]);
       if (nlev >= k) {
-        E wk = (E)1.0 / den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]);
-        acr[q] = acr[q] + (R)((E)1.0 / wk);
+        E wk = dv((E)1.0, den_rho<E>((E)t, (E)s, ch * g.pr2[k - 1]));
+        acr[q] = acr[q] + (R)dv((E)1.0, wk);
         if (fuse) {
           alphn[q] = wk;
           l1[o] = wk * g.dpr2[k - 1] * (E)1;

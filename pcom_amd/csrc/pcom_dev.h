@@ -47,6 +47,18 @@ constexpr double AH_C = 1.0e3;
 // the contiguous range [start(x), start(x+1)) of logical tiles.  The map is
 // a bijection of [0, n) for any n.
 constexpr unsigned kXcds = 8;
+// a / b in the stencils' type: IEEE division in FP64 (the FP64 form is the
+// reference's arithmetic), and in FP32 a hardware reciprocal refined by one
+// residual step (Markstein, IBM J. Res. Dev. 34(1), 1990: q = a r,
+// q + (a - q b) r), within an ulp of the IEEE quotient for the normal,
+// nonzero divisors of the stencils, at a third of the instructions
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, b, a), r, q);
+}
+
 __device__ __forceinline__ unsigned xcd_block() {
   const unsigned b = blockIdx.x, n = gridDim.x;
   const unsigned q = n / kXcds, rem = n % kXcds;

@@ -182,8 +182,11 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const int x0 = tx * TW;
   const int y0 = 1 + ty * TH;
   const int HL = 2 * a.S, HR = 3 * a.S;
-  // tiles whose owned points fall in this tile's region
-  const int nx_lo = -((HL + TW - 1) / TW), nx_hi = (HR + TW - 1) / TW;
+  // tiles whose owned points fall in this tile's region; a last column of
+  // tiles narrower than a halo lets that halo reach one tile further
+  const int wlast = G - (a.tiles_x - 1) * TW;
+  const int xext = wlast < (HL > HR ? HL : HR) ? 1 : 0;
+  const int nx_lo = -((HL + TW - 1) / TW) - xext, nx_hi = (HR + TW - 1) / TW + xext;
   const int ny_lo = -((HL + TH - 1) / TH), ny_hi = (HR + TH - 1) / TH;
   const int nx_span = nx_hi - nx_lo + 1 > a.tiles_x ? 0 : 1;  // region wraps the whole circle
   const int wx_lo = nx_span ? nx_lo : 0, wx_hi = nx_span ? nx_hi : a.tiles_x - 1;

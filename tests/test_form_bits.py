@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 FIELDS = ["tc", "sc", "uc", "vc", "chc", "mch2", "buc", "am", "ssh"]
 EXPECTED = {
     "mixed": "598d2b2d27eaeaae",
-    "mixed_eos32": "547f57c915ec48a7",
+    "mixed_eos32": "c37817a392e4d876",
     "fp64": "7aff16207ed32fb4",
 }
 
