@@ -3,13 +3,14 @@
 // co-resident LDS tiles with temporal blocking.
 //
 // Each workgroup owns a TW x TH tile of the (i, j) index space and keeps
-// the tile plus a halo of 2S (west/south) and 3S (east/north) points of
-// the prognostic free surface ch and barotropic velocity in LDS.  One
-// substep needs ch within [-2, +3] and ub within [-2, +2] points of a point
-// (predictor div -> sch -> viscosity / pressure gradient -> corrector div),
-// so S substeps run on LDS alone, recomputing the halo redundantly; then
-// tiles exchange their owned points through double-buffered global arrays
-// and wait for their neighbour tiles' flags.  That needs every tile
+// the tile plus a halo of 2S points on every side of the prognostic free
+// surface ch and barotropic velocity in LDS.  One substep needs ch within
+// [-2, +2] and ub within [-2, +1] points of a point (predictor fluxes ->
+// predictor div -> sch, viscosity and pressure gradient -> momentum ->
+// corrector div), so S substeps run on LDS alone, recomputing the halo
+// redundantly, each stage only where a later stage of the round reads it;
+// then tiles exchange their owned points through double-buffered global
+// arrays and wait for their neighbour tiles' flags.  That needs every tile
 // co-resident, which holds up to about 1 degree; larger grids run the same
 // kernel in launch mode instead: one launch per S substeps over all tiles,
 // each tile loading its region from the previous launch's buffer and
@@ -73,6 +74,37 @@ struct TropTileArgs {
   int nt0, nt1, src, dst;
   double bt;
 };
+
+// The stages of a substep and the area each must cover.  With k substeps
+// left in the round after this one, and T the tile side, the stages read
+// (per dimension, tile points 0..T-1, east/north positive):
+//   A1 predictor fluxes at x   <- ch at x, x+1; ub at x
+//   A2 predictor ch at x       <- A1 at x-1, x
+//   B1 Laplacian faces, ch gradients and sch at x <- ub at x-1, x; A2 at x, x+1
+//   B2 momentum and corrector fluxes at x <- B1 at x, x+1; A2 at x, x+1
+//   C  corrector ch at x       <- B2 at x-1, x
+// so C covers [-2k, T-1+2k], B2 [-2k-1, T-1+2k], A2 and B1 [-2k-1, T+2k]
+// and A1 [-2k-2, T+2k]: nested rectangles, the area of "level" L = 4k + j
+// (j = 0 for C, 1 for B2, 2 for A2 and B1, 3 for A1) being
+// [-ceil(L/2), T-1+floor(L/2)], T + L points wide; the region is level 4S.
+// A point's level is the least L whose area holds it.
+__device__ __forceinline__ int ring_level(int d, int T) {
+  if (d < 0) return -2 * d - 1;
+  if (d > T - 1) return 2 * (d - T + 1);
+  return 0;
+}
+// region points (row-major, RW wide) of the level-M area before (bx, by)
+__device__ __forceinline__ int ring_before(int M, int bx, int by, int H, int TW, int TH) {
+  if (M < 0) return 0;
+  const int xl = H - (M + 1) / 2, yl = H - (M + 1) / 2;
+  const int w = TW + M, h = TH + M;
+  const int rows = min(max(by - yl, 0), h);
+  const int cols = (by >= yl && by < yl + h) ? min(max(bx - xl, 0), w) : 0;
+  return rows * w + cols;
+}
+constexpr int kStageLevel[5] = {3, 2, 2, 1, 0};  // A1, A2, B1, B2, C
+// modelled share of a full-region substep's instructions per stage
+constexpr double kStageWeight[5] = {0.10, 0.10, 0.22, 0.48, 0.10};
 
 constexpr int kFlagStride = 32;                 // 256 B between flags
 constexpr unsigned long long kSpinLimit = 1ull << 22;  // ~seconds; a normal wait is microseconds
@@ -175,25 +207,40 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
                                                 const unsigned long long epoch) {
   extern __shared__ double2 lds2[];
   __shared__ int abort_flag;
+  __shared__ unsigned short order[kB];  // region points by level, then row-major
   if (threadIdx.x == 0) abort_flag = 0;
   const int G = a.G, nj = a.nj, ni = a.ni, njm = a.nj - 1;
   const int TW = a.TW, TH = a.TH;
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   const int x0 = tx * TW;
   const int y0 = 1 + ty * TH;
-  const int HL = 2 * a.S, HR = 3 * a.S;
+  const int H = 2 * a.S;
   // tiles whose owned points fall in this tile's region; a last column of
   // tiles narrower than a halo lets that halo reach one tile further
   const int wlast = G - (a.tiles_x - 1) * TW;
-  const int xext = wlast < (HL > HR ? HL : HR) ? 1 : 0;
-  const int nx_lo = -((HL + TW - 1) / TW) - xext, nx_hi = (HR + TW - 1) / TW + xext;
-  const int ny_lo = -((HL + TH - 1) / TH), ny_hi = (HR + TH - 1) / TH;
+  const int xext = wlast < H ? 1 : 0;
+  const int nx_lo = -((H + TW - 1) / TW) - xext, nx_hi = (H + TW - 1) / TW + xext;
+  const int ny_lo = -((H + TH - 1) / TH), ny_hi = (H + TH - 1) / TH;
   const int nx_span = nx_hi - nx_lo + 1 > a.tiles_x ? 0 : 1;  // region wraps the whole circle
   const int wx_lo = nx_span ? nx_lo : 0, wx_hi = nx_span ? nx_hi : a.tiles_x - 1;
   const unsigned long long base = epoch << 16;
-  const int RW = TW + HL + HR, RHt = TH + HL + HR;
-  const int bx0 = x0 - HL, by0 = y0 - HL;
+  const int RW = TW + 2 * H, RHt = TH + 2 * H;
+  const int bx0 = x0 - H, by0 = y0 - H;
   const int NR = RW * RHt;
+  // Persistent mode: threads take the region's points in order of level (the
+  // stage areas described at ring_level), so a stage that covers levels up
+  // to L runs on the first threads and whole wavefronts beyond them skip it.
+  // Launch mode loads every region point's statics and state at each launch,
+  // so its threads keep the row-major order, whose loads coalesce.
+  if (threadIdx.x < NR) {
+    const int rx = threadIdx.x % RW, ry = threadIdx.x / RW;
+    const int L = max(ring_level(rx - H, TW), ring_level(ry - H, TH));
+    const int pos = kLaunches ? (int)threadIdx.x
+                              : (L > 0 ? (TW + L - 1) * (TH + L - 1) : 0) + ring_before(L, rx, ry, H, TW, TH) -
+                                    ring_before(L - 1, rx, ry, H, TW, TH);
+    order[pos] = (unsigned short)threadIdx.x;
+  }
+  __syncthreads();
   double* CHP = reinterpret_cast<double*>(lds2 + P_CH * kB);
   double* PR = CHP + kB;
   double2* UV = lds2 + P_UV * kB;
@@ -245,16 +292,22 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     for (int d : div) c[(d + 1) * kTabStride] = 1.0 / c[d * kTabStride];
   }
 
-  // ---- this thread's point, its neighbours and its static fields
-  const int q = threadIdx.x;
-  const bool inreg = q < NR;
-  const int bx = inreg ? q % RW : 0, by = inreg ? q / RW : 0;
+  // ---- this thread's point, its level, its neighbours and its static fields
+  const bool inreg = threadIdx.x < NR;
+  const int pt = inreg ? (int)order[threadIdx.x] : 0;
+  const int bx = inreg ? pt % RW : 0, by = inreg ? pt / RW : 0;
+  const int lvl = max(ring_level(bx - H, TW), ring_level(by - H, TH));
+  // LDS rows RWp apart: an odd pitch spreads the columns of a level's ring,
+  // which neighbouring lanes take, over every bank (where the region still
+  // fits the workgroup's arrays)
+  const int RWp = (RW | 1) * RHt <= kB ? (RW | 1) : RW;
+  const int q = bx + RWp * by;
   const int x = bx0 + bx, y = by0 + by;
   const bool live = inreg && y >= 1 && y <= nj;
   const int bxe = bx + 1 < RW ? bx + 1 : RW - 1, bxw = bx > 0 ? bx - 1 : 0;
   const int byn = by + 1 < RHt ? by + 1 : RHt - 1, bys = by > 0 ? by - 1 : 0;
-  const int qe = bxe + RW * by, qw = bxw + RW * by, qn = bx + RW * byn, qs = bx + RW * bys;
-  const int qne = bxe + RW * byn, qsw = bxw + RW * bys;
+  const int qe = bxe + RWp * by, qw = bxw + RWp * by, qn = bx + RWp * byn, qs = bx + RWp * bys;
+  const int qne = bxe + RWp * byn, qsw = bxw + RWp * bys;
   int w = x % G;
   if (w < 0) w += G;
   const int we = w + 1 < G ? w + 1 : 0;
@@ -263,7 +316,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   const int p = (w + 1) + ni * (yc - 1);
   const int pe = (we + 1) + ni * (yc - 1);
   const int pn = y < nj ? p + ni : p;
-  const bool owned = live && bx >= HL && bx < HL + TW && by >= HL && by < HL + TH && x < G;
+  const bool owned = live && bx >= H && bx < H + TW && by >= H && by < H + TH && x < G;
   int levt = 0, levu = 0;
   double prh_u = 0, prh_t = 1, rprh_t = 1, chc0 = 0, am1 = 0, z1 = 0, z2 = 0, ghx = 0, ghy = 0, ut = 0, vt = 0;
   double bph0 = 0, bph1 = 0, bph2 = 0, bph3 = 0, bgr0 = 0, bgr1 = 0, bgr2 = 0, bgr3 = 0;
@@ -351,15 +404,19 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   int round = 0;
   const int nt1 = kLaunches ? a.nt1 : nstep;
   for (int nt = kLaunches ? a.nt0 : 1; nt <= nt1; ++nt) {
+    // the level of the stages of a substep with k substeps left before the
+    // next exchange (or the end of the launch): 4k + kStageLevel[stage]
+    const int rend = kLaunches ? nt1 : min((nt + a.S - 1) / a.S * a.S, nt1);
+    const int k4 = 4 * (rend - nt);
     // A) predictor (mod_int.f90:184-193): sch = sqrt(ter(chp)) (1 on land),
     // fluxes prh*sch*ub_p, chc = chp - gamma_b*dtbt*div/prh on sea
-    if (live) {
+    if (live && lvl <= k4 + kStageLevel[0]) {
       const double sc = sch_of(CHP[q], CHP[qn], CHP[qe], CHP[qne]);
       const double2 uv = UV[q];
       SF[q] = make_double2(prh_u * sc * uv.x, prh_u * sc * uv.y);
     }
     __syncthreads();
-    if (live) {
+    if (live && lvl <= k4 + kStageLevel[1]) {
       double wk = div331();
       const double chp = CHP[q];
       wk = chp - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
@@ -372,7 +429,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     // (mod_op.f90:913-920) and gradients of the predictor ch per point.
     double sch = 1.0;
     double2 uv = make_double2(0.0, 0.0);
-    if (live) {
+    if (live && lvl <= k4 + kStageLevel[2]) {
       const double pr = PR[q], pre = PR[qe], prn = PR[qn];
       sch = sch_of(pr, prn, pre, PR[qne]);
       const bool r2 = y >= 2 && y <= nj;
@@ -387,7 +444,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     }
     __syncthreads();
     double un = 0, vn = 0;
-    if (live) {
+    if (live && lvl <= k4 + kStageLevel[3]) {
       const double2 wa = WA[q], wae = WA[qe], wb = WB[q], wbn = WB[qn];
       const double l1 = qdiv(qdiv(wae.x - wa.x, COL(C_DX13), COL(C_DX13I)) +
                                  qdiv(rin ? wbn.x - wb.x : 0.0, ROW(R_DX23), ROW(R_DX23I)),
@@ -439,7 +496,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     __syncthreads();
     // C) corrector ch and the time-filter accumulators (mod_int.f90:263-273)
     double cn = 0;
-    if (live) {
+    if (live && lvl <= k4 + kStageLevel[4]) {
       const double wk = div331();
       cn = CHP[q] - qdiv(mt * wk * bt, prh_t, rprh_t);
       CHP[q] = cn;
@@ -604,17 +661,33 @@ std::string trop_resident(const Ctx& c, int block, long* resident) {
 
 namespace {
 // A candidate tiling and its modelled cost per cycle, in units of one
-// wavefront of region points through one substep.  Persistent mode: every
-// substep costs the region's wavefronts and an exchange about
-// kExchangeWaves units.  Launch mode: each launch runs its tiles in waves
-// of co-resident workgroups, each paying its substeps plus about
-// kLoadSubsteps substeps for loading its region and statics, and a launch
-// adds about kLaunchWaves units.
+// wavefront of points through one substep on each of a CU's four SIMDs.
+// Persistent mode: a substep's stages cover nested areas (ring_level), each
+// costing the wavefronts per SIMD of its area weighted by its share of the
+// substep's instructions; the rounds of S substeps plus an exchange of
+// about kExchangeWaves units between rounds.  Launch mode (row-major
+// threads): each launch runs its tiles in waves of co-resident workgroups,
+// each paying its substeps on the whole region plus about kLoadSubsteps
+// substeps for loading its region and statics, and a launch adds about
+// kLaunchWaves units.
 struct TropCand {
-  long cost;
+  double cost;
   int tw, th, s;
 };
-constexpr long kExchangeWaves = 14, kLoadSubsteps = 2, kLaunchWaves = 14;
+constexpr double kExchangeWaves = 11, kLoadSubsteps = 2, kLaunchWaves = 14;
+
+inline double simd_waves(long points) { return 4.0 * (((points + 63) / 64 + 3) / 4); }
+
+// substeps 1..s of a round (or launch) on a tw x th tile
+double round_cost(int tw, int th, int s) {
+  double cost = 0;
+  for (int k = 0; k < s; ++k)
+    for (int st = 0; st < 5; ++st) {
+      const int L = 4 * k + kStageLevel[st];
+      cost += kStageWeight[st] * simd_waves((long)(tw + L) * (th + L));
+    }
+  return cost;
+}
 
 std::string trop_candidates(const Ctx& c, bool launches, std::vector<TropCand>* out) {
   const int G = c.ni - 2;
@@ -628,19 +701,20 @@ std::string trop_candidates(const Ctx& c, bool launches, std::vector<TropCand>* 
     if (launches && s >= nstep) continue;  // a cycle takes at least two launches
     for (int tw = 4; tw <= 64; ++tw)
       for (int th = 4; th <= 64; ++th) {
-        const long nr = (long)(tw + 5 * s) * (th + 5 * s);
+        const long nr = (long)(tw + 4 * s) * (th + 4 * s);
         if (nr > kTropBlock) continue;
         const long resident = trop_block_for(nr) == kTropSmall ? res_small : res_big;
         if (resident == 0) continue;
-        const long waves = (nr + 63) / 64;
         const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
-        long cost;
+        const long nl = (nstep + s - 1) / s;  // rounds (launches) per cycle
+        const double rounds_cost = (nstep / s) * round_cost(tw, th, s) + round_cost(tw, th, (int)(nstep % s));
+        double cost;
         if (!launches) {
           if (tiles > resident || (c.trop_flag_cap > 0 && tiles > c.trop_flag_cap)) continue;
-          cost = nstep * waves + (nstep / s) * kExchangeWaves;
-        } else {
-          const long nl = (nstep + s - 1) / s, rounds = (tiles + resident - 1) / resident;
-          cost = nl * (rounds * (s + kLoadSubsteps) * waves + kLaunchWaves);
+          cost = rounds_cost + (nl - 1) * kExchangeWaves;
+        } else {  // row-major threads: the whole region's wavefronts every substep
+          const long rounds = (tiles + resident - 1) / resident;
+          cost = rounds * (nstep + nl * kLoadSubsteps) * simd_waves(nr) + nl * kLaunchWaves;
         }
         out->push_back({cost, tw, th, s});
       }
@@ -686,13 +760,13 @@ std::string trop_configure(Ctx& c) {
     c.trop_s = cands[0].s;
   }
   if (c.trop_s > 16) return "at most 16 barotropic substeps per exchange";
-  if ((long)(c.trop_tw + 5 * c.trop_s) * (c.trop_th + 5 * c.trop_s) > kTropBlock)
+  if ((long)(c.trop_tw + 4 * c.trop_s) * (c.trop_th + 4 * c.trop_s) > kTropBlock)
     return "barotropic tile " + std::to_string(c.trop_tw) + "x" + std::to_string(c.trop_th) + "x" +
-           std::to_string(c.trop_s) + ": the tile plus its halo of 5 points per substep must hold at most " +
+           std::to_string(c.trop_s) + ": the tile plus its halo of 2 points per substep on each side must hold at most " +
            std::to_string(kTropBlock) + " points";
-  if (c.trop_tw + 5 * c.trop_s > kTabStride || c.trop_th + 5 * c.trop_s > kTabStride)
+  if (c.trop_tw + 4 * c.trop_s > kTabStride || c.trop_th + 4 * c.trop_s > kTabStride)
     return "barotropic tile sides plus halo must be at most " + std::to_string(kTabStride);
-  const int block = trop_block_for((long)(c.trop_tw + 5 * c.trop_s) * (c.trop_th + 5 * c.trop_s));
+  const int block = trop_block_for((long)(c.trop_tw + 4 * c.trop_s) * (c.trop_th + 4 * c.trop_s));
   const size_t lds = trop_lds_bytes(block);
   long resident;
   if (!(err = trop_resident(c, block, &resident)).empty()) return err;

@@ -127,11 +127,12 @@ def test_step_bitwise(gpu_lib, cfgname, nsteps):
         assert getattr(cm, f) == getattr(co, f), f
 
 
-@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (8, 8, 4), (24, 12, 2), (7, 7, 5), (8, 6, 5), (20, 14, 3), (7, 8, 2)])
+@pytest.mark.parametrize("tile", [(4, 4, 1), (5, 3, 3), (8, 8, 4), (24, 12, 2), (7, 7, 5), (8, 6, 5), (20, 14, 3),
+                                  (7, 8, 2), (7, 8, 3), (16, 17, 3), (20, 13, 3)])
 def test_subcycle_tiling_is_exact(gpu_lib, tile):
     """The LDS-tiled barotropic subcycle gives the reference's bits for any
     tile shape and any number of substeps between exchanges, also where the
-    last column of tiles is narrower than a halo (7 x 8 x 2 on the box: 4
+    last column of tiles is narrower than a halo (7 x 8 x 3 on the box: 4
     columns against a halo of 6, which then reaches two tiles east)."""
     for cfgname in ("small", "box"):
         cfg = configs.CONFIGS[cfgname]()
