@@ -342,7 +342,11 @@ typedef struct pcom_nn_stats {
 /* Attach a network with seeded random weights and a device store of up to
  * `capacity` snapshots (0: no store).  Needs glo_ni*glo_nj % 16 == 0.   */
 int pcom_nn_create(pcom_ctx *ctx, int capacity, unsigned long long seed);
-/* Record (inputs, outputs) of every plain subcycle from now on (1) or not. */
+/* Record (inputs, outputs) of every plain subcycle from now on (1) or not.
+ * While the network is in use (pcom_nn_use), a step it takes is recorded
+ * too: its inputs with the plain subcycle's outputs from the same state,
+ * run beside it and then undone, so that training sees the states the
+ * network itself leads to.                                              */
 int pcom_nn_record(pcom_ctx *ctx, int on);
 /* While recording, also record, before a `fraction` of the steps, two
  * extra snapshots: the state perturbed by seeded noise of random smoothness

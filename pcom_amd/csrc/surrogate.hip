@@ -1406,8 +1406,41 @@ std::string nn_record_perturbed(Ctx& c, std::string (*plain)(Ctx&)) {
 }
 }  // namespace
 
+namespace {
+// one recorded snapshot of the state the network is about to step: its
+// inputs and what the plain cycle makes of them, after which the state is
+// restored.  Snapshots of the states the network itself leads to teach it
+// to correct the errors it made (its own inputs drift from those of plain
+// runs as soon as it drives the model).
+std::string nn_record_beside(Ctx& c, std::string (*plain)(Ctx&)) {
+  Nn* n = c.nn;
+  hipStream_t st = c.stream;
+  Fields2& h = c.f2;
+  const long hw = (long)n->H * n->W;
+  double* fields[9] = {h.chc, h.chp, h.mch, h.mch2, h.acch, h.buc, h.bvc, h.bup, h.bvp};
+  const size_t n2 = c.n2;
+  for (int f = 0; f < 9; ++f)
+    if (hipMemcpyAsync(n->bak + f * n2, fields[f], n2 * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return "surrogate state backup";
+  const int slot = n->cursor;
+  NN_LAUNCH(k_save_base, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.chc, h.buc, h.bvc, n->base);
+  std::string e = nn_raw_inputs(c, n->store_in + (long)slot * kDyn * hw, n->store_sc + slot);
+  if (!e.empty()) return e;
+  if (!(e = plain(c)).empty()) return e;
+  NN_LAUNCH(k_rec_out, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, h.mch2, h.mch, h.buc, h.bvc, n->base,
+            n->store_tg + (long)slot * kOutUsed * hw);
+  n->cursor = (n->cursor + 1) % n->cap;
+  n->nrec = std::min(n->nrec + 1, n->cap);
+  for (int f = 0; f < 9; ++f)
+    if (hipMemcpyAsync(fields[f], n->bak + f * n2, n2 * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return "surrogate state restore";
+  return "";
+}
+}  // namespace
+
 // the barotropic part of one step: plain subcycle, or the network's
-// prediction; records snapshots of plain subcycles when recording.
+// prediction; records snapshots of plain subcycles when recording, and of
+// the plain cycle beside the network's steps when recording in use.
 std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
@@ -1419,6 +1452,10 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   const bool check = use && (n->step_count % n->use_every == 0);
   if (c.timing) (void)hipEventRecord(c.ev[1], st);
   if (use && !check) {
+    if (n->recording) {
+      std::string e = nn_record_beside(c, plain);
+      if (!e.empty()) return e;
+    }
     std::string e = nn_predict(c);
     if (!e.empty()) return e;
     NN_LAUNCH(k_apply, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->inc, h.chc, h.chp, h.mch, h.mch2, h.acch,

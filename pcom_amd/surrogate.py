@@ -4,13 +4,17 @@
 2. record `snapshots` consecutive steps (inputs and outputs of the plain
    subcycle) into the device store;
 3. train the BF16 network on them (pcom_nn_train);
+3b. a few on-policy rounds: the network in use while recording, each of
+   its steps beside the plain subcycle's answer from the same state (the
+   states the network itself leads to), then more training on all;
 4. record `holdout` further steps and measure, on those, the network's
    RMSE against the plain subcycle beside the RMSE of persistence (the
    step without the network: no change of the barotropic mode);
 5. time plain steps and surrogate steps (plain judging subcycle every
    `check_every`-th step, the policy USE);
-6. with --closed-loop-days D: run D days with the network in use beside a
-   plain twin started from the same state, and report the RMSE of SSH,
+6. with --closed-loop-days D: run D days with the network in use (on a
+   model that took the fit's steps plainly) beside a plain twin started
+   from the same state, and report the RMSE of SSH,
    SST, the free surface and the barotropic velocity between them daily.
 
     python -m pcom_amd.surrogate --config global --spin-days 10 \\
@@ -68,14 +72,20 @@ def prepare(cfg, precision=abi.PCOM_MIXED, **kw):
 
 
 def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, lr=1e-3, seed=1, augment=0.5,
-        amp=1.0, log=print):
+        amp=1.0, onpolicy_rounds=3, onpolicy_steps=240, onpolicy_iters=2000, onpolicy_lr=3e-4, log=print):
     """On model m: spin up, record (with a fraction `augment` of perturbed
-    snapshots beside the plain ones), train and evaluate; returns the
-    report.  report["steps"] is the number of steps the model took."""
+    snapshots beside the plain ones), train; then `onpolicy_rounds` rounds
+    of `onpolicy_steps` steps with the network in use (policy USE) while
+    recording, each step's state with the plain subcycle's answer, each
+    round followed by `onpolicy_iters` more training iterations on all
+    snapshots; then evaluate on `holdout` new plain steps.  Returns the
+    report; report["steps"] is the number of steps the model took (the
+    on-policy steps leave it off the plain trajectory)."""
     spd = 86400 // cfg.dtbc
     m.step(spin_days * spd)
     amp_ch, amp_u = step_scales(m, cfg)
-    m.nn_create(capacity=int(snapshots * (1 + 2 * augment)) + holdout + 8, seed=seed)
+    cap = int(snapshots * (1 + 2 * augment)) + onpolicy_rounds * onpolicy_steps + holdout + 8
+    m.nn_create(capacity=cap, seed=seed)
     m.nn_augment(augment, amp * amp_ch, amp * amp_u, seed=seed + 100)
     m.nn_record(True)
     m.step(snapshots)
@@ -85,6 +95,22 @@ def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, l
     loss = train(m, iters, batch, lr, seed + 1, log)
     train_s = time.time() - t0
     log(f"trained {iters} iterations in {train_s:.1f} s, loss {loss[:20].mean():.3e} -> {loss[-100:].mean():.3e}")
+    onpolicy = []
+    for r in range(onpolicy_rounds):
+        m.nn_use(**USE)
+        m.nn_record(True)
+        m.step(onpolicy_steps)
+        m.nn_record(False)
+        st = m.nn_stats()
+        m.nn_use(0)
+        t0 = time.time()
+        ol = train(m, onpolicy_iters, batch, onpolicy_lr, seed + 2 + r, log)
+        train_s += time.time() - t0
+        rec = {"round": r + 1, "surrogate_steps": st.surrogate_steps, "fallbacks": st.fallbacks,
+               "check_rmse": list(st.last_check_rmse), "persist_rmse": list(st.last_persist_rmse),
+               "loss_last": float(ol[-100:].mean())}
+        onpolicy.append(rec)
+        log(json.dumps(rec))
     first = m.nn_stats().recorded
     m.nn_record(True)
     m.step(holdout)
@@ -93,8 +119,9 @@ def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, l
     report = {
         "config": cfg.name, "spin_days": spin_days, "snapshots": snapshots, "holdout": holdout,
         "augment": augment, "perturbation_scales": [amp_ch, amp_u], "recorded": first + holdout,
-        "steps": spin_days * spd + 1 + snapshots + holdout,
+        "steps": spin_days * spd + 1 + snapshots + onpolicy_rounds * onpolicy_steps + holdout,
         "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
+        "onpolicy": onpolicy, "onpolicy_iters": onpolicy_iters, "onpolicy_lr": onpolicy_lr,
         "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-100:].mean()),
         "holdout_rmse_network": dict(zip(OUTPUTS, map(float, net))),
         "holdout_rmse_persistence": dict(zip(OUTPUTS, map(float, per))),
@@ -106,10 +133,16 @@ def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, l
 
 def closed_loop(m, cfg, steps_done, days, check_every=USE["check_every"], tol=USE["tol"], resume=USE["resume"],
                 log=print):
-    """The network in use for `days` days beside a plain twin from the same
-    state (the twin repeats the `steps_done` steps the model has taken;
-    recording does not change the state); daily RMSE between the two."""
+    """The network of model m (fitted in `steps_done` steps) in use for
+    `days` days on a new model of the same form that took `steps_done`
+    plain steps, beside a plain twin from that state; daily RMSE between
+    the two."""
     from . import rmse
+    params = m.nn_params()
+    m = Model(cfg, precision=m.precision)
+    m.step(steps_done)
+    m.nn_create(capacity=0, seed=1)
+    m.nn_params(params)
     twin = Model(cfg, precision=m.precision)
     twin.step(steps_done)
     spd = 86400 // cfg.dtbc
@@ -127,6 +160,7 @@ def closed_loop(m, cfg, steps_done, days, check_every=USE["check_every"], tol=US
         recs.append(rec)
         log(json.dumps(rec))
     twin.close()
+    m.close()
     return recs
 
 
@@ -162,6 +196,10 @@ def main(argv=None):
     ap.add_argument("--augment", type=float, default=0.5, help="fraction of recorded steps with a perturbed snapshot")
     ap.add_argument("--amp", type=float, default=1.0,
                     help="largest perturbation, in units of one step's change of ch and velocity")
+    ap.add_argument("--onpolicy-rounds", type=int, default=3)
+    ap.add_argument("--onpolicy-steps", type=int, default=240)
+    ap.add_argument("--onpolicy-iters", type=int, default=2000)
+    ap.add_argument("--onpolicy-lr", type=float, default=3e-4)
     ap.add_argument("--closed-loop-days", type=int, default=0)
     ap.add_argument("--tol", type=float, default=USE["tol"], help="check bound (pcom_nn_use)")
     ap.add_argument("--resume", type=int, default=USE["resume"], help="plain steps after a failed check (pcom_nn_resume)")
@@ -170,7 +208,8 @@ def main(argv=None):
     cfg = configs.CONFIGS[args.config]()
     m, rep = prepare(cfg, abi.FORMS[args.form], spin_days=args.spin_days, snapshots=args.snapshots,
                      holdout=args.holdout, iters=args.iters, batch=args.batch, lr=args.lr, augment=args.augment,
-                     amp=args.amp)
+                     amp=args.amp, onpolicy_rounds=args.onpolicy_rounds, onpolicy_steps=args.onpolicy_steps,
+                     onpolicy_iters=args.onpolicy_iters, onpolicy_lr=args.onpolicy_lr)
     if args.closed_loop_days > 0:
         rep["closed_loop"] = closed_loop(m, cfg, rep["steps"], args.closed_loop_days, args.check_every, args.tol,
                                          args.resume)

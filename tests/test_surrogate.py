@@ -175,6 +175,37 @@ def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
 
 
 @pytest.mark.gpu
+def test_gpu_recording_in_use_leaves_the_network_run_unchanged(gpu_lib):
+    """pcom_nn_record while the network is in use: every step records a
+    snapshot (the network's steps with the plain subcycle run beside them
+    and undone, the checks as plain steps), and the run keeps the bits of
+    the same network stepping without recording."""
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    a = Model(cfg)
+    a.nn_create(capacity=48, seed=2)
+    a.nn_record(True)
+    a.step(24)
+    a.nn_record(False)
+    a.nn_train(200, batch=4, lr=2e-3, seed=3)
+    b = Model(cfg)
+    b.step(24)
+    b.nn_create(capacity=0, seed=5)
+    b.nn_params(a.nn_params())
+    a.nn_use(4, tol=0.0)
+    b.nn_use(4, tol=0.0)
+    a.nn_record(True)
+    a.step(16)
+    a.nn_record(False)
+    b.step(16)
+    sa, sb = a.nn_stats(), b.nn_stats()
+    assert sa.recorded == 24 + 16
+    assert (sa.surrogate_steps, sa.checks) == (sb.surrogate_steps, sb.checks) == (12, 4)
+    for n in ("tc", "sc", "uc", "vc", "chc", "chp", "mch", "mch2", "buc", "bvc", "bup", "bvp", "acch", "am"):
+        assert np.array_equal(a.get(n), b.get(n)), n
+
+
+@pytest.mark.gpu
 def test_gpu_failed_check_holds_then_resumes(gpu_lib):
     """pcom_nn_resume: with a bound no prediction meets (tol = 1e-300), every
     check fails; each failure runs 3 plain steps and the network resumes.
