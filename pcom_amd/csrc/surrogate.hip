@@ -542,7 +542,7 @@ __global__ void k_scale_part(int H, int W, const float* raw, const float* stat, 
   const long hw = (long)H * W;
   double a = 0.0, c = 0.0;
   if (q < hw) {
-    const double mu = (double)stat[2 * hw + q];
+    const double mu = (double)stat[hw + q];  // sea U mask
     const double u = raw[hw + q], v = raw[2 * hw + q];
     a = mu * (u * u + v * v);
     c = 2.0 * mu;

@@ -32,8 +32,9 @@ from .lib import Ensemble, Model
 
 YEAR_S = 31536000.0
 
-# larger-tile options, to put more small members into one launch
-PACKING_TILES = [(26, 16, 2), (32, 12, 2), (22, 22, 2)]
+# larger-tile options, to put more small members into one launch (a tile
+# plus its halo of 2S points a side must fit one 1024-thread workgroup)
+PACKING_TILES = [(26, 16, 2), (32, 12, 2), (22, 22, 2), (24, 24, 2), (20, 20, 3)]
 
 
 def member_configs(n2=24, n1=8, seed0=100, sd=0.01):

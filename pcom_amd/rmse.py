@@ -9,11 +9,13 @@ for the oracle over runs the CPU cannot finish.
 
     python -m pcom_amd.rmse --days 365 --every 30 [--forms mixed,mixed+surrogate]
 
-A form "<form>+surrogate" spins up --spin-days, records --snapshots plain
-subcycles and trains the learned barotropic surrogate on them
-(pcom_amd.surrogate.fit), then runs with it under the policy
-pcom_amd.surrogate.USE (the plain subcycle judging every 10th step); the
-other forms take the same steps alongside.  The form "fp64+f32init" is the FP64 form started
+A form "<form>+surrogate" fits the learned barotropic surrogate on a
+scratch model of the form (pcom_amd.surrogate.fit: --spin-days, then
+--snapshots recorded plain subcycles, training, on-policy rounds); its own
+model steps plainly, alongside the other forms, to the first whole day
+after the fit's steps, and from then on runs with the fitted network under
+the policy pcom_amd.surrogate.USE (the plain subcycle judging every 10th
+step).  The form "fp64+f32init" is the FP64 form started
 from the initial T and S rounded to FP32: its distance from FP64 is the
 model's own sensitivity to perturbations at FP32 rounding level, the floor
 any non-bit-identical form meets.
@@ -79,6 +81,9 @@ def main(argv=None):
     ap.add_argument("--spin-days", type=int, default=10, help="surrogate forms: plain days before recording")
     ap.add_argument("--snapshots", type=int, default=1000, help="surrogate forms: steps recorded for training")
     ap.add_argument("--iters", type=int, default=6000, help="surrogate forms: training iterations")
+    ap.add_argument("--onpolicy-rounds", type=int, default=3, help="surrogate forms: on-policy rounds")
+    ap.add_argument("--onpolicy-steps", type=int, default=240, help="surrogate forms: steps per on-policy round")
+    ap.add_argument("--onpolicy-iters", type=int, default=2000, help="surrogate forms: iterations after each round")
     ap.add_argument("--check-every", type=int, default=surrogate.USE["check_every"])
     ap.add_argument("--tol", type=float, default=surrogate.USE["tol"],
                     help="surrogate check bound (pcom_nn_use; <0: relative to persistence)")
@@ -100,22 +105,30 @@ def main(argv=None):
     report = {"config": cfg.name, "dims": [cfg.glo_ni, cfg.glo_nj, cfg.nk], "records": {f: [] for f in forms},
               "surrogate": {}}
     t0 = time.time()
-    # surrogate forms: fit first, then every model catches up to the next
-    # whole day, from which the network is in use
-    taken = {}
+    # surrogate forms: fit on a scratch model of the form (its on-policy
+    # rounds leave that model off the plain trajectory), then every model
+    # steps plainly to the next whole day after the fit, from which the
+    # fitted network is in use
+    taken, params = {}, {}
     for f, m in models.items():
         if f.endswith("+surrogate"):
-            rep = surrogate.fit(m, cfg, spin_days=args.spin_days, snapshots=args.snapshots, iters=args.iters,
-                                log=lambda s: print(s, flush=True))
+            scratch = Model(cfg, precision=m.precision)
+            rep = surrogate.fit(scratch, cfg, spin_days=args.spin_days, snapshots=args.snapshots, iters=args.iters,
+                                onpolicy_rounds=args.onpolicy_rounds, onpolicy_steps=args.onpolicy_steps,
+                                onpolicy_iters=args.onpolicy_iters, log=lambda s: print(s, flush=True))
+            params[f] = scratch.nn_params()
+            scratch.close()
             taken[f] = rep["steps"]
-            report["surrogate"][f] = {k: rep[k] for k in ("steps", "recorded", "holdout_gain", "train_s")}
+            report["surrogate"][f] = {k: rep[k] for k in ("steps", "recorded", "holdout_gain", "train_s", "onpolicy")}
     day = 0
     if taken:
         day = -(-max(taken.values()) // steps_per_day)
         ref.step(day * steps_per_day)
         for f, m in models.items():
-            m.step(day * steps_per_day - taken.get(f, 0))
+            m.step(day * steps_per_day)
             if f in taken:
+                m.nn_create(capacity=0, seed=1)
+                m.nn_params(params[f])
                 m.nn_use(args.check_every, args.tol, args.resume)
                 report["surrogate"][f]["in_use_from_day"] = day
     while day < args.days:

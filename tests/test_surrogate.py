@@ -175,6 +175,32 @@ def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
 
 
 @pytest.mark.gpu
+def test_gpu_snapshot_scale_is_sea_rms_of_velocity_increment(gpu_lib):
+    """Each snapshot's first-substep increment r is divided by s, the RMS
+    of (r_u, r_v) over sea U points, so over sea mean(r_u^2 + r_v^2) / s^2
+    is 2 in every snapshot; the normalisation's root-mean-square scales of
+    r_u / s and r_v / s over all interior pixels and snapshots therefore
+    satisfy in_std(u)^2 + in_std(v)^2 = 2 x (sea fraction of U points).
+    Parameters end with the normalisation: input means (12), input scales
+    (12), output scales (4), Coriolis and bottom-pressure scales."""
+    from pcom_amd.lib import Model
+    cfg = configs.box()
+    m = Model(cfg)
+    m.step(4)
+    m.nn_create(capacity=8, seed=2)
+    m.nn_record(True)
+    m.step(6)
+    m.nn_record(False)
+    m.nn_train(1, batch=2, lr=1e-4, seed=3)
+    norm = m.nn_params()[-30:]
+    in_std = norm[12:24]
+    sea_u = m.get("prh_u")[1:-1, 1:-1] != np.float64(np.float32(0.1))
+    frac = sea_u.mean()
+    assert 0.1 < frac < 1.0
+    np.testing.assert_allclose(in_std[1] ** 2 + in_std[2] ** 2, 2.0 * frac, rtol=1e-4)
+
+
+@pytest.mark.gpu
 def test_gpu_recording_in_use_leaves_the_network_run_unchanged(gpu_lib):
     """pcom_nn_record while the network is in use: every step records a
     snapshot (the network's steps with the plain subcycle run beside them
