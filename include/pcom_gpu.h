@@ -359,7 +359,9 @@ int pcom_nn_augment(pcom_ctx *ctx, double fraction, double amp_ch, double amp_u,
 /* Adam training on the recorded snapshots: `iters` minibatches of `batch`
  * snapshots drawn with `seed`, learning rate lr decayed to 0 by a cosine
  * over the call; loss[iters] (mean squared error in normalised units) may
- * be NULL.                                                               */
+ * be NULL.  A non-finite gradient element leaves its weight unchanged;
+ * weights that still end non-finite fail the call (PCOM_ERR_HIP,
+ * "training diverged").                                                 */
 int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long long seed, double *loss);
 /* Use the network in pcom_step with a plain, checking subcycle every
  * check_every steps (0 turns the surrogate off).  At each check, tol > 0
@@ -385,9 +387,10 @@ int pcom_nn_eval(pcom_ctx *ctx, int first, int count, double rmse[8]);
 int pcom_nn_get_stats(pcom_ctx *ctx, pcom_nn_stats *s);
 /* Parameters: per layer the FP32 master weights [cout][3][3][cin] (channels
  * padded: input 32 (16 used), hidden 64, output 32 (4 used)) and biases
- * [cout], then 30 normalisation values (offsets and scales of the 12
- * recorded inputs, output scales 4, Coriolis and bottom-pressure scales).
- * Setting them marks the network as trained.                             */
+ * [cout], then 31 normalisation values (offsets and scales of the 12
+ * recorded inputs, output scales 4, Coriolis and bottom-pressure scales,
+ * the floor of the per-snapshot scale).  Setting them marks the network as
+ * trained.                                                               */
 size_t pcom_nn_param_count(const pcom_ctx *ctx);
 int pcom_nn_get_params(pcom_ctx *ctx, float *host, size_t n);
 int pcom_nn_set_params(pcom_ctx *ctx, const float *host, size_t n);

@@ -11,7 +11,7 @@ The loss is the masked mean-squared error over the 4 used outputs: channel
 c < 2 is masked by input channel 5 (T-point sea) and the others by input
 channel 6 (U-point sea), divided by pixels x 4.  Parameters are laid out as
 pcom_nn_get_params gives them: per layer W [cout][3][3][cin] then b
-[cout], then 30 normalisation values.
+[cout], then 31 normalisation values.
 
 With rnd=True the same BF16 roundings as the device are applied at the
 same points:
@@ -28,7 +28,7 @@ import numpy as np
 IN_C, HID, OUT_C, IN_USED, OUT_USED = 32, 64, 32, 16, 4
 DIMS = [(IN_C, HID), (HID, HID), (HID, HID), (HID, HID), (HID, HID), (HID, OUT_C)]
 DIL = [1, 2, 4, 8, 4, 1]
-N_NORM = 30
+N_NORM = 31
 
 
 def param_count():

@@ -38,6 +38,7 @@
 // lane % 16, same k; D row = 4 (lane / 16) + i, column = lane % 16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -416,6 +417,7 @@ __global__ void k_adam(long n, float* w32, float* m, float* v, const float* g, f
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const float gg = g[e];
+  if (!isfinite(gg)) return;  // a non-finite gradient leaves the weight and its moments
   const float mm = b1 * m[e] + (1.f - b1) * gg;
   const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
   m[e] = mm;
@@ -680,8 +682,18 @@ __global__ void k_perturb(int ni, int H, int W, const float* nz, const double* i
 struct NnNorm {
   float in_mean[kDyn], in_std[kDyn], out_std[kOutUsed];
   float cor_scale, prh_scale;
+  // least snapshot scale s: a fraction of the recorded snapshots' median,
+  // so that a snapshot of unusually small first-substep velocity increment
+  // cannot blow its divided values up
+  float s_floor;
 };
-constexpr int kNormValues = 2 * kDyn + kOutUsed + 2;
+constexpr int kNormValues = 2 * kDyn + kOutUsed + 3;
+constexpr float kFloorOfMedian = 0.2f;
+// normalised values beyond this are clamped: far above any real snapshot's,
+// only a guard against infinities
+constexpr float kNormClip = 1024.f;
+__device__ __forceinline__ float floored(float s, const NnNorm& nm) { return s > nm.s_floor ? s : nm.s_floor; }
+__device__ __forceinline__ float nclip(float v) { return fminf(fmaxf(v, -kNormClip), kNormClip); }
 
 __device__ __forceinline__ long nn_ix(int ni, int x, int y) { return (long)(x + 1) + (long)ni * (y + 1); }
 
@@ -702,12 +714,12 @@ __global__ void k_rec_out(int ni, int H, int W, const double* mch2, const double
 // channel moments of the store (one block per channel, FP64, fixed order);
 // the first `nscaled` channels are divided by their snapshot's scale
 __global__ void k_moments(const float* store, long per_snap, long hw, int nsnap, const float* scale, int nscaled,
-                          double* out) {
+                          float s_floor, double* out) {
   __shared__ double s1[256], s2[256];
   const int c = blockIdx.x;
   double a = 0.0, b = 0.0;
   for (int sn = 0; sn < nsnap; ++sn) {
-    const double inv = c < nscaled ? 1.0 / (double)scale[sn] : 1.0;
+    const double inv = c < nscaled ? 1.0 / (double)(scale[sn] > s_floor ? scale[sn] : s_floor) : 1.0;
     for (long q = threadIdx.x; q < hw; q += blockDim.x) {
       const double v = store[(long)sn * per_snap + (long)c * hw + q] * inv;
       a += v;
@@ -738,7 +750,7 @@ __device__ __forceinline__ void pack_pixel(const float* raw, long hw, long q, fl
 #pragma unroll
   for (int d = 0; d < kDyn; ++d) {
     const float r = raw[d * hw + q];
-    v[dyn_channel(d)] = f2bf(d < kScaled ? r / (sc * nm.in_std[d]) : (r - nm.in_mean[d]) / nm.in_std[d]);
+    v[dyn_channel(d)] = f2bf(nclip(d < kScaled ? r / (sc * nm.in_std[d]) : (r - nm.in_mean[d]) / nm.in_std[d]));
   }
   v[4] = f2bf(stat[3 * hw + q]);
   v[5] = f2bf(stat[q]);
@@ -761,11 +773,11 @@ __global__ void k_pack(int H, int W, const float* sin, const float* stg, const f
   const int b = (int)(e / hw);
   const long q = e % hw;
   const long s = idx[b];
-  const float sc = sscale[s];
+  const float sc = floored(sscale[s], nm);
   pack_pixel(sin + s * kDyn * hw, hw, q, sc, nm, stat, x + e * kInC);
   const float* tg = stg + s * kOutUsed * hw;
 #pragma unroll
-  for (int c = 0; c < kOutUsed; ++c) t[e * kOutUsed + c] = tg[c * hw + q] / (sc * nm.out_std[c]);
+  for (int c = 0; c < kOutUsed; ++c) t[e * kOutUsed + c] = nclip(tg[c * hw + q] / (sc * nm.out_std[c]));
 }
 
 // the live snapshot into the network input (one image)
@@ -773,7 +785,7 @@ __global__ void k_pack(int H, int W, const float* sin, const float* stg, const f
 // itself (the same fixed order, so the same s), block 0 keeps s in *scale
 __global__ void __launch_bounds__(256) k_pack_live(int H, int W, const float* raw, const double* part, int nb,
                                                    double* scale, NnNorm nm, const float* stat, bf16_t* x) {
-  const double sc = fold_scale(part, nb);
+  const double sc = floored((float)fold_scale(part, nb), nm);
   if (blockIdx.x == 0 && threadIdx.x == 0) *scale = sc;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
@@ -914,7 +926,7 @@ __global__ void k_eval(long np, long hw, const float* __restrict__ out, const fl
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (p < np) {
     const long q = p % hw;
-    const double sc = sscale[idx[p / hw]];
+    const double sc = floored(sscale[idx[p / hw]], nm);
     const float mt = stat[q], mu = stat[hw + q];
     for (int c = 0; c < kOutUsed; ++c) {
       const double m = c < 2 ? mt : mu;
@@ -1530,10 +1542,20 @@ std::string nn_normalise(Ctx& c) {
   if (n->nrec < 1) return "no recorded snapshots";
   const long hw = (long)n->H * n->W;
   double mom[2 * (kDyn + kOutUsed)];
+  // the scale floor: a fraction of the recorded snapshots' median scale
+  {
+    std::vector<float> sc(n->nrec);
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(sc.data(), n->store_sc, sc.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return "snapshot scales read-back";
+    std::nth_element(sc.begin(), sc.begin() + sc.size() / 2, sc.end());
+    const float med = sc[sc.size() / 2];
+    n->norm.s_floor = med > 0.f && std::isfinite(med) ? kFloorOfMedian * med : 0.f;
+  }
   NN_LAUNCH(k_moments, dim3(kDyn), dim3(256), n->store_in, (long)kDyn * hw, hw, n->nrec, n->store_sc, kScaled,
-            n->dpart);
+            n->norm.s_floor, n->dpart);
   NN_LAUNCH(k_moments, dim3(kOutUsed), dim3(256), n->store_tg, (long)kOutUsed * hw, hw, n->nrec, n->store_sc,
-            kOutUsed, n->dpart + 2 * kDyn);
+            kOutUsed, n->norm.s_floor, n->dpart + 2 * kDyn);
   // the null-stream copy does not wait for the context's (non-blocking) stream
   if (hipStreamSynchronize(st) != hipSuccess ||
       hipMemcpy(mom, n->dpart, sizeof mom, hipMemcpyDeviceToHost) != hipSuccess)
@@ -1594,6 +1616,14 @@ std::string nn_train(Ctx& c, int iters, int batch, double lr, unsigned long long
     }
   }
   if (hipStreamSynchronize(st) != hipSuccess) return "training sync";
+  // a diverged training is an error, not a network that predicts no change
+  for (int l = 0; l < kLayers; ++l) {
+    std::vector<float> w(n->L[l].nw());
+    if (hipMemcpy(w.data(), n->L[l].w32, w.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return "weights read-back";
+    for (float v : w)
+      if (!std::isfinite(v)) return "training diverged: non-finite weights";
+  }
   return "";
 }
 
@@ -1671,9 +1701,11 @@ std::string nn_params(Ctx& c, float* host, bool set) {
   if (set) {
     n->norm.cor_scale = host[off];
     n->norm.prh_scale = host[off + 1];
+    n->norm.s_floor = host[off + 2];
   } else {
     host[off] = n->norm.cor_scale;
     host[off + 1] = n->norm.prh_scale;
+    host[off + 2] = n->norm.s_floor;
   }
   if (set) {
     n->normalised = true;

@@ -84,7 +84,10 @@ def fit(m, cfg, spin_days=10, snapshots=1000, holdout=48, iters=6000, batch=8, l
     spd = 86400 // cfg.dtbc
     m.step(spin_days * spd)
     amp_ch, amp_u = step_scales(m, cfg)
-    cap = int(snapshots * (1 + 2 * augment)) + onpolicy_rounds * onpolicy_steps + holdout + 8
+    # perturbed snapshots come two at a time before a random fraction
+    # `augment` of the steps: room for six standard deviations more
+    extra = 2 * (snapshots * augment + 6.0 * np.sqrt(snapshots * augment * (1.0 - augment)))
+    cap = snapshots + int(np.ceil(extra)) + onpolicy_rounds * onpolicy_steps + holdout + 8
     m.nn_create(capacity=cap, seed=seed)
     m.nn_augment(augment, amp * amp_ch, amp * amp_u, seed=seed + 100)
     m.nn_record(True)

@@ -182,7 +182,8 @@ def test_gpu_snapshot_scale_is_sea_rms_of_velocity_increment(gpu_lib):
     r_u / s and r_v / s over all interior pixels and snapshots therefore
     satisfy in_std(u)^2 + in_std(v)^2 = 2 x (sea fraction of U points).
     Parameters end with the normalisation: input means (12), input scales
-    (12), output scales (4), Coriolis and bottom-pressure scales."""
+    (12), output scales (4), Coriolis and bottom-pressure scales, the scale
+    floor (a fifth of the median scale, which these snapshots stay above)."""
     from pcom_amd.lib import Model
     cfg = configs.box()
     m = Model(cfg)
@@ -192,8 +193,9 @@ def test_gpu_snapshot_scale_is_sea_rms_of_velocity_increment(gpu_lib):
     m.step(6)
     m.nn_record(False)
     m.nn_train(1, batch=2, lr=1e-4, seed=3)
-    norm = m.nn_params()[-30:]
+    norm = m.nn_params()[-31:]
     in_std = norm[12:24]
+    assert norm[30] > 0.0  # a scale floor was set
     sea_u = m.get("prh_u")[1:-1, 1:-1] != np.float64(np.float32(0.1))
     frac = sea_u.mean()
     assert 0.1 < frac < 1.0
