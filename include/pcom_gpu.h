@@ -314,6 +314,13 @@ int pcom_tune_subcycle(pcom_ctx *ctx, int trials, int tile[3], double *ms);
  * the tiling in use, its mode resolved (set = 0).  The results do not
  * depend on the tiling or the mode.                                      */
 int pcom_subcycle_tiling(pcom_ctx *ctx, int set, int tile[4]);
+/* Bound on each neighbour wait of the persistent subcycle: a wait that has
+ * polled `polls` times without success ends in an error (0: the default,
+ * 2^22 polls, seconds, where a normal wait takes microseconds).  The tile
+ * stops, the launch still ends, the state is marked invalid and every
+ * later synchronising call reports PCOM_ERR_STATE.  No counterpart in the
+ * reference; a bound of 1 forces that path, for tests of failure handling. */
+int pcom_subcycle_wait_limit(pcom_ctx *ctx, unsigned long long polls);
 
 /* ---- Learned barotropic surrogate.  New: the reference has no such part.
  * A BF16 convolutional network (six 3x3 layers, 64 hidden channels, FP32

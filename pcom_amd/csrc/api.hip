@@ -756,6 +756,14 @@ int pcom_subcycle_tiling(pcom_ctx* ctx, int set, int tile[4]) {
   return PCOM_OK;
 }
 
+int pcom_subcycle_wait_limit(pcom_ctx* ctx, unsigned long long polls) {
+  if (!ctx) return fail(PCOM_ERR_ARG, "pcom_subcycle_wait_limit: NULL context");
+  Ctx* c = C(ctx);
+  if (int rc = quiesce(*c, false)) return rc;
+  c->trop_spin_limit = polls;
+  return PCOM_OK;
+}
+
 int pcom_nn_create(pcom_ctx* ctx, int capacity, unsigned long long seed) {
   if (!ctx || capacity < 0) return fail(PCOM_ERR_ARG, "pcom_nn_create: bad argument");
   Ctx* c = C(ctx);

@@ -92,6 +92,7 @@ struct Ctx {
   unsigned long long* trop_flags = nullptr;  // per-tile exchange flags (zeroed at create)
   long trop_flag_cap = 0;                    // tiles the flag array holds
   int* trop_error = nullptr;                 // set by a timed-out neighbour wait
+  unsigned long long trop_spin_limit = 0;    // polls per neighbour wait (0: kSpinLimit)
   unsigned long long trop_epoch = 0;
   // phase timing (HIP events on `stream`)
   bool timing = false;

@@ -66,6 +66,7 @@ struct TropTileArgs {
   double* acc[4];             // accumulators between launches (launch mode)
   unsigned long long* flags;  // per tile, kFlagStride apart: epoch * 2^16 + round
   int* error;                 // set when a neighbour wait exceeds its bound
+  unsigned long long spin_limit;  // unsuccessful polls that end a neighbour wait
   unsigned long long epoch;
   int nstep, S, TW, TH, tiles_x, tiles_y;
   // launch mode (k_trop_block): substeps nt0..nt1 of this launch; the state
@@ -154,7 +155,8 @@ size_t trop_lds_bytes(int block) { return 16 * (size_t)P_N * block + 8 * (R_N + 
 // store and an acquire fence would add an L2 writeback and an L2
 // invalidate per exchange, which only non-atomic accesses need.
 //
-// Waits are bounded: past kSpinLimit polls the tile records an error and
+// Waits are bounded: after spin_limit unsuccessful polls (kSpinLimit unless
+// the host set another) the tile records an error and
 // stops waiting, so the launch always completes (the host then reports the
 // failure).
 __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int nx_hi, int ny_lo, int ny_hi,
@@ -174,7 +176,7 @@ __device__ void tile_sync(const TropTileArgs& a, int tx, int ty, int nx_lo, int 
     const unsigned long long* f = &a.flags[(size_t)(x + a.tiles_x * y) * kFlagStride];
     unsigned long long it = 0;
     while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < value) {
-      if (++it > kSpinLimit) {
+      if (++it >= a.spin_limit) {
         atomicOr(a.error, 1);
         *abort_flag = 1;
         break;
@@ -864,6 +866,7 @@ TropTileArgs trop_args(const Ctx& c) {
   a.tiles_y = c.trop_blocks / c.trop_tiles_x;
   a.flags = c.trop_flags;
   a.error = c.trop_error;
+  a.spin_limit = c.trop_spin_limit ? c.trop_spin_limit : kSpinLimit;
   a.epoch = 0;
   a.bt = (double)c.dtbt;
   return a;

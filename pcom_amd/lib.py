@@ -34,7 +34,7 @@ EXPORTS = [
     "pcom_nn_create", "pcom_nn_record", "pcom_nn_augment", "pcom_nn_train", "pcom_nn_use", "pcom_nn_resume", "pcom_nn_eval", "pcom_nn_get_stats",
     "pcom_nn_param_count", "pcom_nn_get_params", "pcom_nn_set_params", "pcom_nn_forward_backward",
     "pcom_ensemble_create", "pcom_ensemble_step", "pcom_ensemble_packs", "pcom_ensemble_destroy",
-    "pcom_tune_subcycle", "pcom_subcycle_tiling",
+    "pcom_tune_subcycle", "pcom_subcycle_tiling", "pcom_subcycle_wait_limit",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
         L.pcom_ensemble_destroy.argtypes = [vp]
         L.pcom_tune_subcycle.argtypes = [vp, C.c_int, ip, dp]
         L.pcom_subcycle_tiling.argtypes = [vp, C.c_int, ip]
+        L.pcom_subcycle_wait_limit.argtypes = [vp, C.c_ulonglong]
         for name in EXPORTS:
             if name not in ("pcom_field_size", "pcom_last_error", "pcom_nn_param_count"):
                 getattr(L, name).restype = C.c_int
@@ -268,6 +269,11 @@ class Model:
         ms = C.c_double()
         _check(lib().pcom_tune_subcycle(self.h, int(trials), t, C.byref(ms)))
         return tuple(t), ms.value
+
+    def wait_limit(self, polls=0):
+        """Bound each neighbour wait of the persistent subcycle to `polls`
+        unsuccessful polls (pcom_subcycle_wait_limit; 0: the default)."""
+        _check(lib().pcom_subcycle_wait_limit(self.h, int(polls)))
 
     def tiling(self, tile=None, mode=None):
         """The subcycle tiling (tile_i, tile_j, substeps); sets it if given

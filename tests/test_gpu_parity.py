@@ -188,6 +188,32 @@ def test_large_grid_runs_in_launch_mode(gpu_lib):
     assert a.check_finite()
 
 
+def test_subcycle_wait_bound_reports_and_ends(gpu_lib):
+    """A neighbour wait of the persistent subcycle past its bound (forced from
+    the host: one unsuccessful poll ends a wait) stops its tile; the launch
+    ends, the next synchronising call reports PCOM_ERR_STATE, and so does
+    every later one.  A new context on the device then steps as the oracle
+    does, bit for bit."""
+    cfg = configs.box()
+    m = Model(cfg)
+    assert m.subcycle_mode() == abi.PCOM_TROP_PERSISTENT
+    m.wait_limit(1)
+    m.step(3)
+    with pytest.raises(PcomError) as ei:
+        m.sync()
+    assert ei.value.code == abi.PCOM_ERR_STATE and "wait" in str(ei.value)
+    with pytest.raises(PcomError) as ei:
+        m.get("chc")
+    assert ei.value.code == abi.PCOM_ERR_STATE
+    m.close()
+    fresh = Model(cfg)
+    o = Oracle(cfg)
+    fresh.step(2)
+    o.step(2)
+    bad = _compare(fresh, o, ["chc", "mch2", "buc", "bvc", "tc", "uc"])
+    assert not bad, "\n".join(bad)
+
+
 def test_box_many_steps_bitwise(gpu_lib):
     """Config 1 (box, 60x40x10): 30 steps crossing a day boundary."""
     cfg = configs.box()
