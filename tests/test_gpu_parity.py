@@ -215,7 +215,7 @@ def test_subcycle_wait_bound_reports_and_ends(gpu_lib):
 
 
 def test_box_many_steps_bitwise(gpu_lib):
-    """Config 1 (box, 60x40x10): 30 steps crossing a day boundary."""
+    """The box basin (60x40x10): 30 steps crossing a day boundary."""
     cfg = configs.box()
     m = Model(cfg)
     o = Oracle(cfg)

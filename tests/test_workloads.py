@@ -1,15 +1,15 @@
 """The five workloads at their stated sizes on the GPU:
 
-* config 1, the box basin: 100 steps bit for bit against the oracle;
-* config 2, the 1-degree grid: the mixed forms' error against FP64 over
+* the box basin, 60 x 40 x 10: 100 steps bit for bit against the oracle;
+* the 1-degree 360 x 180 x 30 grid: the mixed forms' error against FP64 over
   ten days, field by field (FP64 itself is bit-identical to the oracle,
   tests/test_gpu_parity.py);
-* config 3, 24 members at 2 degrees plus 8 at 1 degree stepped as one
+* 24 members at 2 degrees plus 8 at 1 degree stepped as one
   ensemble: a 2-degree and a 1-degree member (FP64) bit for bit against the
   oracle, mixed members bit for bit against themselves stepped alone;
-* config 4, the surrogate network at 360 x 180: forward and backward
+* the surrogate network at 360 x 180: forward and backward
   against the CPU restatement, and a record -> train -> checked-use cycle;
-* config 5, the 0.25-degree 1440 x 720 x 50 grid: one FP64 step bit for
+* the 0.25-degree 1440 x 720 x 50 grid: one FP64 step bit for
   bit against the oracle, two FP64 and two mixed steps conserving mass and
   staying finite, and the same bits under another launch tiling.
 """
@@ -28,10 +28,10 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def test_config1_box_100_steps_bitwise(gpu_lib):
+def test_box_60x40x10_100_steps_bitwise(gpu_lib):
     from pcom_amd.lib import Model
     cfg = configs.box()
-    _say("config 1: box 60x40x10, 100 steps")
+    _say("box 60x40x10, 100 steps")
     m, o = Model(cfg), Oracle(cfg)
     m.step(100)
     o.step(100)
@@ -57,12 +57,12 @@ MIXED_BOUNDS = {
 }
 
 
-def test_config2_mixed_forms_error_per_field(gpu_lib):
+def test_global_1deg_mixed_forms_error_per_field(gpu_lib):
     """Every prognostic field of the mixed forms against FP64 after ten
     days (240 steps) at 1 degree, area-weighted over sea."""
     from pcom_amd.lib import Model
     cfg = configs.global_1deg()
-    _say("config 2: mixed forms vs fp64, 240 steps")
+    _say("1 degree: mixed forms vs fp64, 240 steps")
     ref64 = Model(cfg)
     ref64.step(240)
     for form, bounds in MIXED_BOUNDS.items():
@@ -78,7 +78,7 @@ def test_config2_mixed_forms_error_per_field(gpu_lib):
         m.close()
 
 
-def test_config3_ensemble_members_bitwise(gpu_lib):
+def test_ensemble_24x2deg_8x1deg_members_bitwise(gpu_lib):
     """The 24 + 8 member ensemble in packed launches: member 0 (2 degrees)
     and member 24 (1 degree) run FP64 and match the oracle; members 5 and
     30 (mixed) match themselves stepped alone."""
@@ -86,14 +86,14 @@ def test_config3_ensemble_members_bitwise(gpu_lib):
     cfgs = ensemble.member_configs()
     assert len(cfgs) == 32 and sum(d == 2 for d, _ in cfgs) == 24
     fp64 = {0, 24}
-    _say("config 3: creating 32 members")
+    _say("ensemble 24 x 2 deg + 8 x 1 deg: creating 32 members")
     # 2-degree members on a tiling that packs six to a launch (the tiling
     # does not change the bits)
     tile = {2: ensemble.PACKING_TILES[0], 1: (0, 0, 0)}
     members = [Model(c, precision=abi.PCOM_FP64 if i in fp64 else abi.PCOM_MIXED, trop_tile=tile[d])
                for i, (d, c) in enumerate(cfgs)]
     ens = Ensemble(members)
-    _say("config 3: 3 ensemble steps")
+    _say("ensemble: 3 steps")
     ens.step(3)
     npk, of = ens.packs()
     assert npk < 32 and all(p >= 0 for p in of), (npk, of)
@@ -112,9 +112,9 @@ def test_config3_ensemble_members_bitwise(gpu_lib):
     ens.close()
 
 
-def test_config4_network_at_1deg_matches_oracle(gpu_lib):
+def test_network_at_360x180_matches_oracle(gpu_lib):
     """Forward and backward of the network on one 360 x 180 snapshot (the
-    size config 4 trains on) against the CPU restatement with the device's
+    size the 1-degree surrogate trains on) against the CPU restatement with the device's
     BF16 roundings."""
     from pcom_amd.lib import Model
     from test_surrogate import _inputs, _random_params
@@ -125,7 +125,7 @@ def test_config4_network_at_1deg_matches_oracle(gpu_lib):
     p = _random_params(rng).astype(np.float32)
     m.nn_params(p)
     x, t = _inputs(cfg, rng, 1)
-    _say("config 4: forward/backward at 360x180, batch 1")
+    _say("network: forward/backward at 360x180, batch 1")
     loss, out, g = m.nn_forward_backward(x, t)
     rl, rout, rg = ref.loss_and_grads(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rnd=True)
     assert abs(loss - rl) <= 2e-3 * rl
@@ -138,7 +138,7 @@ def test_config4_network_at_1deg_matches_oracle(gpu_lib):
         off += n
 
 
-def test_config4_record_train_use_at_1deg(gpu_lib):
+def test_surrogate_record_train_use_at_1deg(gpu_lib):
     """At 1 degree: record plain subcycles (half of the steps with perturbed
     snapshots beside them), train on the device, then step with the network
     and a plain checking subcycle every 10th step; the network learns (loss
@@ -150,7 +150,7 @@ def test_config4_record_train_use_at_1deg(gpu_lib):
     cfg = configs.global_1deg()
     m = Model(cfg, precision=abi.PCOM_MIXED)
     plain = Model(cfg, precision=abi.PCOM_MIXED)
-    _say("config 4: record 240 steps, train 3000 iterations, use 20 steps")
+    _say("surrogate at 1 degree: record 240 steps, train 3000 iterations, use 20 steps")
     m.step(120)
     amp_ch, amp_u = step_scales(m, cfg)
     plain.step(121)
@@ -175,7 +175,7 @@ def test_config4_record_train_use_at_1deg(gpu_lib):
     m.nn_record(False)
     net, per = m.nn_eval(first, 48)
     gain = per / net
-    _say("config 4: held-out gain over persistence (mch2, mch, ubt, vbt) " + " ".join(f"{g:.2f}" for g in gain))
+    _say("surrogate at 1 degree: held-out gain over persistence (mch2, mch, ubt, vbt) " + " ".join(f"{g:.2f}" for g in gain))
     assert np.all(gain > 1.2), gain
     m.nn_use(10, tol=0.0)
     m.step(20)
@@ -195,7 +195,7 @@ def _mass(m, cfg):
     return float(np.sum((m.get("prh_t") * m.get("chc") * area)[sea]))
 
 
-def test_config5_quarter_degree_50_levels(gpu_lib):
+def test_quarter_degree_1440x720x50(gpu_lib):
     """1440 x 720 x 50, dtbt 15 s, dtbc 900 s (launch-mode subcycle): the
     first FP64 step bit for bit against the oracle; two more FP64 steps and
     two mixed steps conserve the bottom-pressure mass to 1e-12 and stay
@@ -203,12 +203,12 @@ def test_config5_quarter_degree_50_levels(gpu_lib):
     from pcom_amd.lib import Model
     cfg = configs.eddy_025deg()
     assert (cfg.glo_ni, cfg.glo_nj, cfg.nk) == (1440, 720, 50)
-    _say("config 5: fp64 model, 1 step")
+    _say("0.25 degree: fp64 model, 1 step")
     m = Model(cfg)
     assert m.subcycle_mode() == abi.PCOM_TROP_LAUNCHES
     m0 = _mass(m, cfg)
     m.step(1)
-    _say("config 5: oracle, 1 step")
+    _say("0.25 degree: oracle, 1 step")
     o = Oracle(cfg)
     o.step(1)
     bad = _compare(m, o, ["tc", "sc", "uc", "vc", "chc", "mch", "mch2", "buc", "bvc", "am", "ssh", "wm", "adp",
@@ -218,7 +218,7 @@ def test_config5_quarter_degree_50_levels(gpu_lib):
     m.step(2)
     assert m.check_finite()
     assert abs(_mass(m, cfg) - m0) <= 1e-12 * abs(m0)
-    _say("config 5: other tiling, mixed form")
+    _say("0.25 degree: other tiling, mixed form")
     b = Model(cfg, trop_tile=(12, 10, 1))
     b.step(3)
     for n in ("chc", "mch", "buc", "bvc", "tc", "uc"):
