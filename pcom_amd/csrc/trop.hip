@@ -104,8 +104,6 @@ __device__ __forceinline__ int ring_before(int M, int bx, int by, int H, int TW,
   return rows * w + cols;
 }
 constexpr int kStageLevel[5] = {3, 2, 2, 1, 0};  // A1, A2, B1, B2, C
-// modelled share of a full-region substep's instructions per stage
-constexpr double kStageWeight[5] = {0.10, 0.10, 0.22, 0.48, 0.10};
 
 constexpr int kFlagStride = 32;                 // 256 B between flags
 constexpr unsigned long long kSpinLimit = 1ull << 22;  // ~seconds; a normal wait is microseconds
@@ -512,16 +510,30 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     if (exch) {
       // uniform selects keep the buffer addresses in scalar registers
       const bool odd = round & 1;
-      double* const xc = (odd ? a.xc[1] : a.xc[0]) + p;
-      double* const xu = (odd ? a.xu[1] : a.xu[0]) + p;
-      double* const xv = (odd ? a.xv[1] : a.xv[0]) + p;
-      if (owned) xstore(xc, cn);
+      double* const xc = odd ? a.xc[1] : a.xc[0];
+      double* const xu = odd ? a.xu[1] : a.xu[0];
+      double* const xv = odd ? a.xv[1] : a.xv[0];
+      if (owned) xstore(xc + p, cn);
       tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
       if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
-      // the halo from the neighbours (owned points already hold these values)
-      if (live && !owned) {
-        CHP[q] = xload(xc);
-        UV[q] = make_double2(xload(xu), xload(xv));
+      // the halo from the neighbours (owned points already hold these
+      // values), taken by the threads in row-major order of the region: the
+      // level order puts a column of the ring on neighbouring lanes, one
+      // memory line each, where row-major lanes share lines.  The three
+      // loads are issued before either value is used, so one round trip
+      // serves them.
+      if (threadIdx.x < NR) {
+        const int hx = threadIdx.x % RW, hy = threadIdx.x / RW;
+        const int gx = bx0 + hx, gy = by0 + hy;
+        if (gy >= 1 && gy <= nj && !(hx >= H && hx < H + TW && hy >= H && hy < H + TH && gx < G)) {
+          int hw = gx % G;
+          if (hw < 0) hw += G;
+          const int hp = (hw + 1) + ni * (gy - 1);
+          const int hq = hx + RWp * hy;
+          const double hc = xload(xc + hp), hu = xload(xu + hp), hv = xload(xv + hp);
+          CHP[hq] = hc;
+          UV[hq] = make_double2(hu, hv);
+        }
       }
       ++round;
     }
@@ -662,31 +674,38 @@ std::string trop_resident(const Ctx& c, int block, long* resident) {
 }  // namespace
 
 namespace {
-// A candidate tiling and its modelled cost per cycle, in units of one
-// wavefront of points through one substep on each of a CU's four SIMDs.
-// Persistent mode: a substep's stages cover nested areas (ring_level), each
-// costing the wavefronts per SIMD of its area weighted by its share of the
-// substep's instructions; the rounds of S substeps plus an exchange of
-// about kExchangeWaves units between rounds.  Launch mode (row-major
-// threads): each launch runs its tiles in waves of co-resident workgroups,
-// each paying its substeps on the whole region plus about kLoadSubsteps
-// substeps for loading its region and statics, and a launch adds about
-// kLaunchWaves units.
+// A candidate tiling and its modelled cost per cycle, in hundredths of one
+// wavefront of points through one substep on each of a CU's four SIMDs, in
+// integers so that equal costs compare equal.  Persistent mode: a substep's
+// stages cover nested areas (ring_level), each costing the wavefronts per
+// SIMD of its area weighted by its share (percent) of the substep's
+// instructions, the substeps of the 1024-thread kernel (which spills
+// registers) at kBigBlockPercent percent; the rounds of S substeps plus
+// kExchangeCost between rounds.  Both constants are a least-squares fit to
+// 17 tilings timed at 1 degree (an exchange is about 2.1 us).  Launch mode
+// (row-major threads): each launch runs its tiles in waves of co-resident
+// workgroups, each paying its substeps on the whole region plus about
+// kLoadSubsteps substeps for loading its region and statics, and a launch
+// adds kLaunchCost.  Equal costs go to the lower tile, then the wider: the
+// halo exchange moves region rows, which are contiguous in memory, so fewer
+// and longer rows cost less (24x11 < 16x17 < 20x13 < 19x14 < 10x26 at 1
+// degree, all of one modelled cost).
 struct TropCand {
-  double cost;
+  long cost;
   int tw, th, s;
 };
-constexpr double kExchangeWaves = 11, kLoadSubsteps = 2, kLaunchWaves = 14;
+constexpr long kExchangeCost = 670, kBigBlockPercent = 107, kLoadSubsteps = 2, kLaunchCost = 1400;
+constexpr long kStagePercent[5] = {10, 10, 22, 48, 10};  // A1, A2, B1, B2, C
 
-inline double simd_waves(long points) { return 4.0 * (((points + 63) / 64 + 3) / 4); }
+inline long simd_waves(long points) { return 4 * (((points + 63) / 64 + 3) / 4); }
 
 // substeps 1..s of a round (or launch) on a tw x th tile
-double round_cost(int tw, int th, int s) {
-  double cost = 0;
+long round_cost(int tw, int th, int s) {
+  long cost = 0;
   for (int k = 0; k < s; ++k)
     for (int st = 0; st < 5; ++st) {
       const int L = 4 * k + kStageLevel[st];
-      cost += kStageWeight[st] * simd_waves((long)(tw + L) * (th + L));
+      cost += kStagePercent[st] * simd_waves((long)(tw + L) * (th + L));
     }
   return cost;
 }
@@ -709,19 +728,25 @@ std::string trop_candidates(const Ctx& c, bool launches, std::vector<TropCand>* 
         if (resident == 0) continue;
         const long tiles = (long)((G + tw - 1) / tw) * ((c.nj + th - 1) / th);
         const long nl = (nstep + s - 1) / s;  // rounds (launches) per cycle
-        const double rounds_cost = (nstep / s) * round_cost(tw, th, s) + round_cost(tw, th, (int)(nstep % s));
-        double cost;
+        const long rounds_cost = (nstep / s) * round_cost(tw, th, s) + round_cost(tw, th, (int)(nstep % s));
+        long cost;
         if (!launches) {
           if (tiles > resident || (c.trop_flag_cap > 0 && tiles > c.trop_flag_cap)) continue;
-          cost = rounds_cost + (nl - 1) * kExchangeWaves;
+          cost = rounds_cost * (trop_block_for(nr) == kTropBlock ? kBigBlockPercent : 100) / 100 +
+                 (nl - 1) * kExchangeCost;
         } else {  // row-major threads: the whole region's wavefronts every substep
           const long rounds = (tiles + resident - 1) / resident;
-          cost = rounds * (nstep + nl * kLoadSubsteps) * simd_waves(nr) + nl * kLaunchWaves;
+          cost = rounds * (nstep + nl * kLoadSubsteps) * simd_waves(nr) * 100 + nl * kLaunchCost;
         }
         out->push_back({cost, tw, th, s});
       }
   }
-  std::stable_sort(out->begin(), out->end(), [](const TropCand& a, const TropCand& b) { return a.cost < b.cost; });
+  std::sort(out->begin(), out->end(), [](const TropCand& a, const TropCand& b) {
+    if (a.cost != b.cost) return a.cost < b.cost;
+    if (a.th != b.th) return a.th < b.th;
+    if (a.tw != b.tw) return a.tw > b.tw;
+    return a.s < b.s;
+  });
   return "";
 }
 }  // namespace
