@@ -44,7 +44,7 @@ TROP_FLOPS_PER_POINT = 184
 # passes over this bench (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
 # factors being what the counters read for a plain 1 GiB copy); the passes
 # and their commands are in the profile named here
-TROP_TRAFFIC_BYTES = 402_000_000
+TROP_TRAFFIC_BYTES = 403_400_000
 TROP_TRAFFIC_SOURCE = "profiles/k_trop_tiles_global1deg_pmc.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
 
 

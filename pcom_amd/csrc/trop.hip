@@ -137,9 +137,6 @@ constexpr int kTabStride = 176;
 
 size_t trop_lds_bytes(int block) { return 16 * (size_t)P_N * block + 8 * (R_N + C_N) * kTabStride; }
 
-// Publish this tile's round and wait until every tile whose owned points
-// lie in this tile's region has published it.
-//
 // The exchange values and the flags are written and read only with
 // agent-scope atomic accesses.  Those carry the device-scope cache policy
 // (sc1) that keeps each of them coherent across the XCDs' L2 caches on its
@@ -149,9 +146,13 @@ size_t trop_lds_bytes(int block) { return 16 * (size_t)P_N * block + 8 * (R_N + 
 // workgroup barrier that precedes the flag store, so the tile's exchange
 // stores, and its loads of the entry fields and of the last round, are
 // complete before its flag can be seen; a consumer issues its data loads
-// only after the barrier that follows its flag polls returned.  A release
-// store and an acquire fence would add an L2 writeback and an L2
-// invalidate per exchange, which only non-atomic accesses need.
+// only after its flag poll returned.  A release store and an acquire fence
+// would add an L2 writeback and an L2 invalidate per exchange, which only
+// non-atomic accesses need.  The rounds' exchanges are in the substep loop
+// (each thread polls the owner of its halo point); tile_sync publishes the
+// end of the launch and waits until every tile whose owned points lie in
+// this tile's region has published it, before a tile overwrites its owned
+// points of the entry fields, which those tiles' regions read at the start.
 //
 // Waits are bounded: after spin_limit unsuccessful polls (kSpinLimit unless
 // the host set another) the tile records an error and
@@ -302,6 +303,23 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   // fits the workgroup's arrays)
   const int RWp = (RW | 1) * RHt <= kB ? (RW | 1) : RW;
   const int q = bx + RWp * by;
+  // The halo reload takes the region's points in row-major order: the level
+  // order puts a column of the ring on neighbouring lanes, one memory line
+  // each, where row-major lanes share lines.  This thread's row-major point:
+  // its 2-D offset, its LDS slot and the flag of the tile that owns it (-1
+  // when the point is owned here or lies outside the rows).
+  int hp = 0, hq = 0, hf = -1;
+  if (!kLaunches && threadIdx.x < NR) {
+    const int hx = threadIdx.x % RW, hy = threadIdx.x / RW;
+    const int gx = bx0 + hx, gy = by0 + hy;
+    if (gy >= 1 && gy <= nj && !(hx >= H && hx < H + TW && hy >= H && hy < H + TH && gx < G)) {
+      int hw = gx % G;
+      if (hw < 0) hw += G;
+      hp = (hw + 1) + ni * (gy - 1);
+      hq = hx + RWp * hy;
+      hf = hw / TW + a.tiles_x * ((gy - 1) / TH);
+    }
+  }
   const int x = bx0 + bx, y = by0 + by;
   const bool live = inreg && y >= 1 && y <= nj;
   const int bxe = bx + 1 < RW ? bx + 1 : RW - 1, bxw = bx > 0 ? bx - 1 : 0;
@@ -514,30 +532,41 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       double* const xu = odd ? a.xu[1] : a.xu[0];
       double* const xv = odd ? a.xv[1] : a.xv[0];
       if (owned) xstore(xc + p, cn);
-      tile_sync(a, tx, ty, wx_lo, wx_hi, ny_lo, ny_hi, base + (unsigned long long)(round + 1), &abort_flag);
-      if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
+      // publish this round: every wave's accesses complete, then the flag
+      const unsigned long long value = base + (unsigned long long)(round + 1);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&a.flags[(size_t)tile * kFlagStride], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // the halo from the neighbours (owned points already hold these
-      // values), taken by the threads in row-major order of the region: the
-      // level order puts a column of the ring on neighbouring lanes, one
-      // memory line each, where row-major lanes share lines.  The three
-      // loads are issued before either value is used, so one round trip
-      // serves them.
-      if (threadIdx.x < NR) {
-        const int hx = threadIdx.x % RW, hy = threadIdx.x / RW;
-        const int gx = bx0 + hx, gy = by0 + hy;
-        if (gy >= 1 && gy <= nj && !(hx >= H && hx < H + TW && hy >= H && hy < H + TH && gx < G)) {
-          int hw = gx % G;
-          if (hw < 0) hw += G;
-          const int hp = (hw + 1) + ni * (gy - 1);
-          const int hq = hx + RWp * hy;
-          const double hc = xload(xc + hp), hu = xload(xu + hp), hv = xload(xv + hp);
-          CHP[hq] = hc;
-          UV[hq] = make_double2(hu, hv);
+      // values): each thread waits for the flag of the tile that owns its
+      // row-major point, then issues the point's three loads before using
+      // either value, so one round trip serves them.  Every tile that reads
+      // this tile's points owns points of this tile's halo (the region
+      // relation is symmetric), so a tile stores round r + 2 only after every
+      // reader of its round r has published round r + 1, which it does once
+      // its loads of round r are complete.
+      if (hf >= 0) {
+        const unsigned long long* f = &a.flags[(size_t)hf * kFlagStride];
+        unsigned long long it = 0;
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < value) {
+          if (++it >= a.spin_limit) {
+            atomicOr(a.error, 1);
+            abort_flag = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
+        const double hc = xload(xc + hp), hu = xload(xu + hp), hv = xload(xv + hp);
+        CHP[hq] = hc;
+        UV[hq] = make_double2(hu, hv);
       }
-      ++round;
     }
     __syncthreads();
+    if (exch) {
+      if (__builtin_amdgcn_readfirstlane(abort_flag)) return;
+      ++round;
+    }
   }
 #undef ROW
 #undef COL
