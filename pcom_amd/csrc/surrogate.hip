@@ -881,16 +881,26 @@ __global__ void k_apply(int ni, int H, int W, const double* inc, double* chc, do
 // root-mean-square error over sea of the prediction against the plain
 // subcycle's result (err[c]), and of persistence, i.e. no change over the
 // step (err[4 + c]); one block per channel, fixed order
-__global__ void k_check(int ni, int H, int W, const double* inc, const double* c0, const double* mch2,
-                        const double* mch, const double* buc, const double* bvc, const float* stat, double* err) {
-  __shared__ double s[256], s0[256], cnt[256];
-  const int c = blockIdx.x;
+// The check's RMS of the prediction's and of persistence's error against
+// the plain cycle, per output over sea pixels: partial sums of 512 pixels
+// (two per thread) in part[(3 * c + v) * nb + block], then one pass over
+// the partials in block order, so the result does not depend on timing.
+constexpr int kCheckPix = 512;
+__global__ void k_check_part(int ni, int H, int W, const double* inc, const double* c0, const double* mch2,
+                             const double* mch, const double* buc, const double* bvc, const float* stat,
+                             double* part) {
+  __shared__ double s[3][256];
+  const int c = blockIdx.y;
   const long hw = (long)H * W;
+  const int nb = gridDim.x;
+  const double* truth = c == 0 ? mch2 : c == 1 ? mch : c == 2 ? buc : bvc;
   double a = 0.0, a0 = 0.0, n = 0.0;
-  for (long q = threadIdx.x; q < hw; q += blockDim.x) {
-    const long k = nn_ix(ni, (int)(q % W), (int)(q / W));
+  for (int h = 0; h < 2; ++h) {
+    const long q = (long)blockIdx.x * kCheckPix + h * 256 + threadIdx.x;
+    if (q >= hw) break;
+    const int y = (int)(q / W), x = (int)(q - (long)y * W);
+    const long k = nn_ix(ni, x, y);
     const double m = (double)stat[(c < 2 ? 0 : hw) + q];
-    const double* truth = c == 0 ? mch2 : c == 1 ? mch : c == 2 ? buc : bvc;
     const double base = c0[c * hw + q];
     const double d = m * (base + inc[c * hw + q] - truth[k]);
     const double d0 = m * (base - truth[k]);
@@ -898,21 +908,34 @@ __global__ void k_check(int ni, int H, int W, const double* inc, const double* c
     a0 += d0 * d0;
     n += m;
   }
-  s[threadIdx.x] = a;
-  s0[threadIdx.x] = a0;
-  cnt[threadIdx.x] = n;
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = a0;
+  s[2][threadIdx.x] = n;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      s[threadIdx.x] += s[threadIdx.x + st];
-      s0[threadIdx.x] += s0[threadIdx.x + st];
-      cnt[threadIdx.x] += cnt[threadIdx.x + st];
-    }
+    if ((int)threadIdx.x < st)
+      for (int v = 0; v < 3; ++v) s[v][threadIdx.x] += s[v][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[(3 * c + threadIdx.x) * nb + blockIdx.x] = s[threadIdx.x][0];
+}
+__global__ void k_check_final(int nb, const double* part, double* err) {
+  __shared__ double s[3][256];
+  const int c = blockIdx.x;
+  for (int v = 0; v < 3; ++v) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += part[(3 * c + v) * nb + b];
+    s[v][threadIdx.x] = t;
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int v = 0; v < 3; ++v) s[v][threadIdx.x] += s[v][threadIdx.x + st];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    err[c] = cnt[0] > 0 ? sqrt(s[0] / cnt[0]) : 0.0;
-    err[4 + c] = cnt[0] > 0 ? sqrt(s0[0] / cnt[0]) : 0.0;
+    err[c] = s[2][0] > 0 ? sqrt(s[0][0] / s[2][0]) : 0.0;
+    err[4 + c] = s[2][0] > 0 ? sqrt(s[1][0] / s[2][0]) : 0.0;
   }
 }
 
@@ -1502,8 +1525,12 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   }
   if (check) {
     double* err = n->misc + 12;  // 8 values, read back below
-    NN_LAUNCH(k_check, dim3(4), dim3(256), n->ni, n->H, n->W, n->inc, n->base, h.mch2, h.mch, h.buc, h.bvc, n->stat,
-              err);
+    // the partials fit the prediction's (12 per 512 pixels against 9 per
+    // 256), which k_pred_fix has consumed
+    const int nb = (int)nblk(hw, kCheckPix);
+    NN_LAUNCH(k_check_part, dim3(nb, 4), dim3(256), n->ni, n->H, n->W, n->inc, n->base, h.mch2, h.mch, h.buc, h.bvc,
+              n->stat, n->dpart);
+    NN_LAUNCH(k_check_final, dim3(4), dim3(256), nb, n->dpart, err);
     double he[8];
     if (hipMemcpyAsync(he, err, sizeof he, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
