@@ -374,14 +374,18 @@ int pcom_nn_train(pcom_ctx *ctx, int iters, int batch, double lr, unsigned long 
  * check_every steps (0 turns the surrogate off).  At each check, tol > 0
  * bounds the sea RMSE of every output, tol < 0 bounds it by -tol times the
  * RMSE of persistence (no change over the step), both at this check and at
- * the first check after this call, whichever is smaller (a run drifting
- * into noise raises the first with the error); tol = 0 never falls back;
- * a failed check turns the surrogate off, or holds it for a number of
- * plain steps set by pcom_nn_resume.                                     */
+ * the network's first check, whichever is smaller (a run drifting into
+ * noise raises the first with the error); tol = 0 never falls back; a
+ * failed check turns the surrogate off, or holds it for a number of plain
+ * steps set by pcom_nn_resume.  The first check also sets the pointwise
+ * limit of the network's increments (50 times its persistence RMSE); the
+ * network's first step after this call is a check until one has done so,
+ * and the limit then holds across later calls.                          */
 int pcom_nn_use(pcom_ctx *ctx, int check_every, double tol);
 /* After a failed check, run plain_steps plain steps and then resume the
- * network (counted in pcom_nn_stats.fallbacks); 0 (the default) keeps the
- * plain subcycle for good.  Extends the check of the surrogate config
+ * network (counted in pcom_nn_stats.fallbacks), whose first step after them
+ * is a check, so that a network that failed never runs unchecked; 0 (the
+ * default) keeps the plain subcycle for good.  Extends the check of the surrogate config
  * (plain subcycle every 10th step as check and fallback): the reference
  * has no surrogate.                                                      */
 int pcom_nn_resume(pcom_ctx *ctx, int plain_steps);

@@ -1019,10 +1019,16 @@ struct Nn {
   // does not), and the last step of the current hold
   int resume_after = 0;
   long long hold_until = 0;
+  // the network's first step after a hold is a check: a network that failed
+  // is never trusted again unchecked
+  bool check_next = false;
   long long step_count = 0, surrogate_steps = 0, plain_steps = 0, checks = 0, fallbacks = 0;
   double last_err[4] = {0, 0, 0, 0}, max_err[4] = {0, 0, 0, 0}, last_persist[4] = {0, 0, 0, 0};
-  // persistence RMSE at the first check after pcom_nn_use: the size of one
-  // step's change when the network took over, which a relative bound uses
+  // persistence RMSE at the network's first check: the size of one step's
+  // change when the network first took over, which the relative bound and
+  // the pointwise limit of increments use.  It is kept across pcom_nn_use
+  // calls: anchored afresh where the network has just gone wrong, it would
+  // measure the damage instead of the flow.
   double anchor[4] = {0, 0, 0, 0};
   bool anchored = false;
   bool normalised = false;
@@ -1319,7 +1325,9 @@ std::string nn_set_use(Ctx& c, int every, double tol) {
   c.nn->use_every = every;
   c.nn->tol = tol;
   c.nn->hold_until = 0;
-  c.nn->anchored = false;
+  // a network not yet anchored starts with a check, so that no step of it
+  // runs without the pointwise limit
+  c.nn->check_next = every > 0 && !c.nn->anchored;
   return "";
 }
 
@@ -1484,7 +1492,7 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
   n->step_count += 1;
   // held: plain steps after a failed check, before the network resumes
   const bool use = n->use_every > 0 && n->step_count > n->hold_until;
-  const bool check = use && (n->step_count % n->use_every == 0);
+  const bool check = use && (n->check_next || n->step_count % n->use_every == 0);
   if (c.timing) (void)hipEventRecord(c.ev[1], st);
   if (use && !check) {
     if (n->recording) {
@@ -1536,6 +1544,7 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
         hipStreamSynchronize(st) != hipSuccess)
       return "surrogate check read-back";
     n->checks += 1;
+    n->check_next = false;
     if (!n->anchored) {
       for (int q = 0; q < 4; ++q) n->anchor[q] = he[4 + q];
       n->anchored = true;
@@ -1553,9 +1562,10 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
     }
     if (bad && n->tol != 0.0) {
       n->fallbacks += 1;
-      if (n->resume_after > 0)
+      if (n->resume_after > 0) {
         n->hold_until = n->step_count + n->resume_after;  // the plain subcycle takes over for a while
-      else
+        n->check_next = true;
+      } else
         n->use_every = 0;  // ... or for good
     }
   }

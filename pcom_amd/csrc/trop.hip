@@ -48,8 +48,13 @@ inline std::string hip_error_string(hipError_t e, const char* what) {
 // 34(1), 1990), so the result is the IEEE quotient for every finite,
 // non-subnormal quotient.  All divisors of a substep are static, so their
 // reciprocals are computed once per launch and each division costs a
-// multiply and two FMAs instead of the iterative IEEE sequence.
+// multiply and two FMAs instead of the iterative IEEE sequence.  The mixed
+// forms, which need accuracy but not the reference's bits, take a * rb
+// alone (within an ulp of the quotient): a third of the substep's FP64
+// operations are these corrections.
+template <bool kFast>
 __device__ __forceinline__ double qdiv(double a, double b, double rb) {
+  if constexpr (kFast) return a * rb;
   const double q = a * rb;
   const double r = __builtin_fma(-q, b, a);
   return __builtin_fma(r, rb, q);
@@ -73,6 +78,7 @@ struct TropTileArgs {
   // enters from buffer `src` of xc/xu/xv (nt0 > 1) and leaves into buffer
   // `dst` (nt1 < nstep)
   int nt0, nt1, src, dst;
+  int fast;  // divisions as multiplications by the reciprocal (mixed forms)
   double bt;
 };
 
@@ -203,7 +209,7 @@ __device__ __forceinline__ double xload(const double* p) {
 // result is independent of the tiling; points near the region edge read
 // clamped neighbours and are garbage, but no owned point depends on them
 // within S substeps.
-template <bool kLaunches, int kB>
+template <bool kLaunches, int kB, bool kFast>
 __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int tile,
                                                 const unsigned long long epoch) {
   extern __shared__ double2 lds2[];
@@ -395,7 +401,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     const double2 fs = SF[qs], fq = SF[q], fsw = SF[qsw], fw = SF[qw];
     const double t1 = r2 ? 0.5 * (fs.x + fq.x) : 0.0;
     const double t0 = r2 ? 0.5 * (fsw.x + fw.x) : 0.0;
-    const double a1 = qdiv(t1 - t0, COL(C_DX11), COL(C_DX11I));
+    const double a1 = qdiv<kFast>(t1 - t0, COL(C_DX11), COL(C_DX11I));
     double d = 0.0;
     if (r2) {
       const double rh3 = ROW(R_RH3), rh3s = rws[R_RH3 * kTabStride];
@@ -403,8 +409,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       const double u0 = 0.5 * (fsw.y * rh3s + fs.y * rh3s);
       d = u1 - u0;
     }
-    const double b1 = qdiv(d, ROW(R_DX21), ROW(R_DX21I));
-    return qdiv(a1 + b1, ROW(R_ARH1), ROW(R_ARH1I));
+    const double b1 = qdiv<kFast>(d, ROW(R_DX21), ROW(R_DX21I));
+    return qdiv<kFast>(a1 + b1, ROW(R_ARH1), ROW(R_ARH1I));
   };
   // sqrt(ter(v, g1 -> g3)) on sea U points, 1 on land, from v at the point
   // and its north, east and north-east neighbours
@@ -437,7 +443,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     if (live && lvl <= k4 + kStageLevel[1]) {
       double wk = div331();
       const double chp = CHP[q];
-      wk = chp - qdiv(wk * bt * GAMMA_B, prh_t, rprh_t);
+      wk = chp - qdiv<kFast>(wk * bt * GAMMA_B, prh_t, rprh_t);
       // land keeps its ch: the entry value in the first substep, afterwards
       // the corrector's chp - 0 (mod_int.f90:193, 267)
       PR[q] = levt > 0 ? wk : (nt == 1 ? a.chc[p] : chp);
@@ -453,22 +459,22 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       const bool r2 = y >= 2 && y <= nj;
       uv = UV[q];
       const double2 uvw = UV[qw], uvs = UV[qs];
-      WA[q] = make_double2(qdiv(qdiv(uv.x - uvw.x, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)),
-                           qdiv(qdiv(uv.y - uvw.y, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)));
-      WB[q] = make_double2(qdiv(r2 ? uv.x - uvs.x : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2),
-                           qdiv(r2 ? uv.y - uvs.y : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2));
-      GX[q] = qdiv(qdiv(pre - pr, COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I));
-      GY[q] = qdiv(qdiv(rin ? prn - pr : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD);
+      WA[q] = make_double2(qdiv<kFast>(qdiv<kFast>(uv.x - uvw.x, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)),
+                           qdiv<kFast>(qdiv<kFast>(uv.y - uvw.y, COL(C_DX14), COL(C_DX14I)), ROW(R_RH4), ROW(R_RH4I)));
+      WB[q] = make_double2(qdiv<kFast>(r2 ? uv.x - uvs.x : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2),
+                           qdiv<kFast>(r2 ? uv.y - uvs.y : 0.0, ROW(R_DX22), ROW(R_DX22I)) * ROW(R_RH2));
+      GX[q] = qdiv<kFast>(qdiv<kFast>(pre - pr, COL(C_DX12), COL(C_DX12I)), ROW(R_ARH2), ROW(R_ARH2I));
+      GY[q] = qdiv<kFast>(qdiv<kFast>(rin ? prn - pr : 0.0, ROW(R_DX24), ROW(R_DX24I)), ARAD, 1.0 / ARAD);
     }
     __syncthreads();
     double un = 0, vn = 0;
     if (live && lvl <= k4 + kStageLevel[3]) {
       const double2 wa = WA[q], wae = WA[qe], wb = WB[q], wbn = WB[qn];
-      const double l1 = qdiv(qdiv(wae.x - wa.x, COL(C_DX13), COL(C_DX13I)) +
-                                 qdiv(rin ? wbn.x - wb.x : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+      const double l1 = qdiv<kFast>(qdiv<kFast>(wae.x - wa.x, COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv<kFast>(rin ? wbn.x - wb.x : 0.0, ROW(R_DX23), ROW(R_DX23I)),
                              ROW(R_DEN), ROW(R_DENI));
-      const double l2 = qdiv(qdiv(wae.y - wa.y, COL(C_DX13), COL(C_DX13I)) +
-                                 qdiv(rin ? wbn.y - wb.y : 0.0, ROW(R_DX23), ROW(R_DX23I)),
+      const double l2 = qdiv<kFast>(qdiv<kFast>(wae.y - wa.y, COL(C_DX13), COL(C_DX13I)) +
+                                 qdiv<kFast>(rin ? wbn.y - wb.y : 0.0, ROW(R_DX23), ROW(R_DX23I)),
                              ROW(R_DEN), ROW(R_DENI));
       const double vs1 = sam1 * l1;
       const double vs2 = sam1 * l2;
@@ -489,8 +495,8 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
       double pu2 = su * (vs2 - z2 + vt - pg2 - 0.5 * cor * up);
       const double w1 = pu1 * bt + up;
       const double w2 = pu2 * bt + vp;
-      double uc = qdiv(w1 + 0.5 * cor * bt * w2, ROW(R_DXX), ROW(R_DXXI));
-      double vc = qdiv(w2 - 0.5 * cor * bt * w1, ROW(R_DXX), ROW(R_DXXI));
+      double uc = qdiv<kFast>(w1 + 0.5 * cor * bt * w2, ROW(R_DXX), ROW(R_DXXI));
+      double vc = qdiv<kFast>(w2 - 0.5 * cor * bt * w1, ROW(R_DXX), ROW(R_DXXI));
       uc = 0.5 * (uc + up);
       vc = 0.5 * (vc + vp);
       pu1 = su * (vs1 - z1 - pg1 + ut + cor * vc);
@@ -516,7 +522,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     double cn = 0;
     if (live && lvl <= k4 + kStageLevel[4]) {
       const double wk = div331();
-      cn = CHP[q] - qdiv(mt * wk * bt, prh_t, rprh_t);
+      cn = CHP[q] - qdiv<kFast>(mt * wk * bt, prh_t, rprh_t);
       CHP[q] = cn;
       if (owned) {
         if (nt <= nstep / 2) acm = acm + cn;
@@ -614,20 +620,22 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
   }
 }
 
-template <int kB>
-__global__ void __launch_bounds__(kB) k_trop_tiles(TropTileArgs a) { trop_tiles_body<false, kB>(a, blockIdx.x, a.epoch); }
+template <int kB, bool kFast>
+__global__ void __launch_bounds__(kB) k_trop_tiles(TropTileArgs a) {
+  trop_tiles_body<false, kB, kFast>(a, blockIdx.x, a.epoch);
+}
 
 // launch mode: tiles of a launch do not wait for each other, so neighbouring
 // tiles go to one XCD, whose L2 then serves their shared halo
-template <int kB>
+template <int kB, bool kFast>
 __global__ void __launch_bounds__(kB) k_trop_block(TropTileArgs a) {
-  trop_tiles_body<true, kB>(a, (int)xcd_block(), 0);
+  trop_tiles_body<true, kB, kFast>(a, (int)xcd_block(), 0);
 }
 
 // The subcycles of several contexts (ensemble members) in one launch:
 // workgroups [start[m], start[m+1]) are member m's tiles, each member with
 // its own fields, flags and epoch.  Tiles wait only for tiles of their own
-// member, so members of any grid sizes share the launch.
+// member, so members of any grid sizes and forms share the launch.
 constexpr int kMaxPack = 16;
 struct MultiLaunch {
   int n;
@@ -640,21 +648,25 @@ __global__ void __launch_bounds__(kB) k_trop_multi(const TropTileArgs* __restric
   const int b = blockIdx.x;
   int m = 0;
   while (m + 1 < ml.n && b >= ml.start[m + 1]) ++m;
-  trop_tiles_body<false, kB>(args[m], b - ml.start[m], ml.epoch[m]);
+  if (args[m].fast)
+    trop_tiles_body<false, kB, true>(args[m], b - ml.start[m], ml.epoch[m]);
+  else
+    trop_tiles_body<false, kB, false>(args[m], b - ml.start[m], ml.epoch[m]);
 }
 
-// the kernels of one workgroup size
+// the kernels of one workgroup size and division kind
 struct TropKernels {
   const void* tiles;
   const void* block;
   const void* multi;
 };
-TropKernels trop_kernels(int block) {
-  if (block == kTropSmall)
-    return {(const void*)k_trop_tiles<kTropSmall>, (const void*)k_trop_block<kTropSmall>,
-            (const void*)k_trop_multi<kTropSmall>};
-  return {(const void*)k_trop_tiles<kTropBlock>, (const void*)k_trop_block<kTropBlock>,
-          (const void*)k_trop_multi<kTropBlock>};
+template <int kB, bool kFast>
+TropKernels trop_kernels_of() {
+  return {(const void*)k_trop_tiles<kB, kFast>, (const void*)k_trop_block<kB, kFast>, (const void*)k_trop_multi<kB>};
+}
+TropKernels trop_kernels(int block, bool fast) {
+  if (block == kTropSmall) return fast ? trop_kernels_of<kTropSmall, true>() : trop_kernels_of<kTropSmall, false>();
+  return fast ? trop_kernels_of<kTropBlock, true>() : trop_kernels_of<kTropBlock, false>();
 }
 
 // vs at the first substep (mod_int.f90:198-207): seau*am(:,:,1)*lap(ub),
@@ -688,8 +700,8 @@ std::string trop_resident(const Ctx& c, int block, long* resident) {
   *resident = 0;
   if (lds > (size_t)prop.sharedMemPerBlock) return "";
   int per_cu = 1 << 30;
-  const TropKernels tk = trop_kernels(block);
-  for (const void* k : {tk.tiles, tk.block}) {
+  const TropKernels te = trop_kernels(block, false), tf = trop_kernels(block, true);
+  for (const void* k : {te.tiles, te.block, tf.tiles, tf.block}) {
     e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     int n = 0;
@@ -921,6 +933,7 @@ TropTileArgs trop_args(const Ctx& c) {
   a.flags = c.trop_flags;
   a.error = c.trop_error;
   a.spin_limit = c.trop_spin_limit ? c.trop_spin_limit : kSpinLimit;
+  a.fast = c.precision != PCOM_FP64;
   a.epoch = 0;
   a.bt = (double)c.dtbt;
   return a;
@@ -976,11 +989,10 @@ std::string launch_trop_kernel(Ctx& c) {
       a.nt1 = std::min(a.nstep, nt0 + c.trop_s - 1);
       a.src = (b + 1) & 1;
       a.dst = b & 1;
-      if (c.trop_block == kTropSmall)
-        hipLaunchKernelGGL(k_trop_block<kTropSmall>, dim3(c.trop_blocks), dim3(kTropSmall), c.trop_lds, c.stream, a);
-      else
-        hipLaunchKernelGGL(k_trop_block<kTropBlock>, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
-      hipError_t e = hipGetLastError();
+      const void* k = trop_kernels(c.trop_block, a.fast).block;
+      void* kargs[] = {&a};
+      hipError_t e = hipLaunchKernel(k, dim3(c.trop_blocks), dim3(c.trop_block), kargs, c.trop_lds, c.stream);
+      if (e == hipSuccess) e = hipGetLastError();
       if (e != hipSuccess) return std::string("k_trop_block: ") + hipGetErrorString(e);
     }
     return "";
@@ -988,11 +1000,10 @@ std::string launch_trop_kernel(Ctx& c) {
   a.epoch = ++c.trop_epoch;
   return chained_launch(c.device, c.stream, [&]() -> std::string {
     if (c.timing) (void)hipEventRecord(c.ev[1], c.stream);
-    if (c.trop_block == kTropSmall)
-      hipLaunchKernelGGL(k_trop_tiles<kTropSmall>, dim3(c.trop_blocks), dim3(kTropSmall), c.trop_lds, c.stream, a);
-    else
-      hipLaunchKernelGGL(k_trop_tiles<kTropBlock>, dim3(c.trop_blocks), dim3(kTropBlock), c.trop_lds, c.stream, a);
-    hipError_t e = hipGetLastError();
+    const void* k = trop_kernels(c.trop_block, a.fast).tiles;
+    void* kargs[] = {&a};
+    hipError_t e = hipLaunchKernel(k, dim3(c.trop_blocks), dim3(c.trop_block), kargs, c.trop_lds, c.stream);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return std::string("k_trop_tiles: ") + hipGetErrorString(e);
     return "";
   });
@@ -1034,7 +1045,7 @@ std::string multi_capacity(int device, int block, long* cap) {
   hipError_t e = hipGetDeviceProperties(&prop, device);
   if (e != hipSuccess) return std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
   const size_t lds = trop_lds_bytes(block);
-  const void* k = trop_kernels(block).multi;
+  const void* k = trop_kernels(block, false).multi;
   e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
   int per_cu = 0;
@@ -1191,7 +1202,7 @@ std::string ensemble_step(Ensemble& en, int nsteps) {
       const int block = en.pack_block[p];
       const size_t lds = trop_lds_bytes(block);
       err = chained_launch(en.device, en.stream, [&]() -> std::string {
-        hipError_t ea = hipFuncSetAttribute(trop_kernels(block).multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipError_t ea = hipFuncSetAttribute(trop_kernels(block, false).multi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)lds);
         if (ea != hipSuccess) return hip_error_string(ea, "hipFuncSetAttribute");
         if (block == kTropSmall)

@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["tc", "sc", "uc", "vc", "chc", "mch2", "buc", "am", "ssh"]
 EXPECTED = {
-    "mixed": "598d2b2d27eaeaae",
-    "mixed_eos32": "c37817a392e4d876",
+    "mixed": "7702e86b23698d0a",
+    "mixed_eos32": "9964d307ba8e2fce",
     "fp64": "7aff16207ed32fb4",
 }
 

@@ -144,7 +144,8 @@ def test_gpu_forward_backward_matches_oracle(gpu_lib):
 @pytest.mark.gpu
 def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
     """Snapshots from plain box steps, a short training run, then steps
-    with the network checked by the plain subcycle every 4th step."""
+    with the network checked by the plain subcycle every 4th step; its
+    first step is a check too (which sets the limit of its increments)."""
     from pcom_amd.lib import Model
     cfg = configs.box()
     m = Model(cfg)
@@ -162,7 +163,7 @@ def test_gpu_surrogate_trains_and_beats_persistence(gpu_lib):
     m.step(16)
     plain.step(16)
     st = m.nn_stats()
-    assert (st.surrogate_steps, st.checks) == (12, 4)
+    assert (st.surrogate_steps, st.checks) == (11, 5)
     assert m.check_finite()
     err = np.array(st.last_check_rmse[:])
     per = np.array(st.last_persist_rmse[:])
@@ -228,7 +229,7 @@ def test_gpu_recording_in_use_leaves_the_network_run_unchanged(gpu_lib):
     b.step(16)
     sa, sb = a.nn_stats(), b.nn_stats()
     assert sa.recorded == 24 + 16
-    assert (sa.surrogate_steps, sa.checks) == (sb.surrogate_steps, sb.checks) == (12, 4)
+    assert (sa.surrogate_steps, sa.checks) == (sb.surrogate_steps, sb.checks) == (11, 5)
     for n in ("tc", "sc", "uc", "vc", "chc", "chp", "mch", "mch2", "buc", "bvc", "bup", "bvp", "acch", "am"):
         assert np.array_equal(a.get(n), b.get(n)), n
 
@@ -236,9 +237,12 @@ def test_gpu_recording_in_use_leaves_the_network_run_unchanged(gpu_lib):
 @pytest.mark.gpu
 def test_gpu_failed_check_holds_then_resumes(gpu_lib):
     """pcom_nn_resume: with a bound no prediction meets (tol = 1e-300), every
-    check fails; each failure runs 3 plain steps and the network resumes.
-    Checks every 2nd step: network at step 1, checks at 2, 6 and 10, plain
-    steps 2-10; without resume the first failure ends the surrogate."""
+    check fails; each failure runs 3 plain steps and the network resumes
+    with a check.  Checks every 2nd step, and the network's first step is a
+    check (it sets the limit of the increments): checks at steps 1, 5 and 9,
+    all 10 steps plain.  Later, with the limit set, the cadence alone rules
+    (network at step 11, check at 12); without resume the first failure
+    ends the surrogate."""
     from pcom_amd.lib import Model
     cfg = configs.box()
     m = Model(cfg)
@@ -250,9 +254,9 @@ def test_gpu_failed_check_holds_then_resumes(gpu_lib):
     m.nn_use(2, tol=1e-300, resume=3)
     m.step(10)
     st = m.nn_stats()
-    assert (st.surrogate_steps, st.checks, st.fallbacks, st.plain_steps) == (1, 3, 3, 9 + 8)
+    assert (st.surrogate_steps, st.checks, st.fallbacks, st.plain_steps) == (0, 3, 3, 10 + 8)
     assert m.check_finite()
     m.nn_use(2, tol=1e-300, resume=0)
     m.step(6)
     st = m.nn_stats()
-    assert (st.surrogate_steps, st.checks, st.fallbacks, st.in_use) == (2, 4, 4, 0)
+    assert (st.surrogate_steps, st.checks, st.fallbacks, st.in_use) == (1, 4, 4, 0)
