@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["tc", "sc", "uc", "vc", "chc", "mch2", "buc", "am", "ssh"]
 EXPECTED = {
-    "mixed": "7702e86b23698d0a",
-    "mixed_eos32": "9964d307ba8e2fce",
+    "mixed": "8e29dab40aa87e82",
+    "mixed_eos32": "e9f38985eb489849",
     "fp64": "7aff16207ed32fb4",
 }
 
