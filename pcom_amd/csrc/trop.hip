@@ -62,6 +62,25 @@ __device__ __forceinline__ double qdiv(double a, double b, double rb) {
   return __builtin_fma(r, rb, q);
 }
 
+// sqrt(t) for a positive normal t above 2^-767: the hardware reciprocal
+// square root refined by two Newton steps in FMA form, which is the
+// compiler's own correctly rounded sequence for sqrt without its range
+// scaling (inputs below 2^-767) and its special cases (0, infinities, NaN),
+// so it gives the same bits for such t; 0 gives 0.  The subcycle's roots
+// are of averages of ch, near 1.
+__device__ __forceinline__ double sqrt_normal(double t) {
+  const double r = __builtin_amdgcn_rsq(t);
+  double s = t * r, h = 0.5 * r;
+  const double e = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, e, s);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-s, s, t);
+  s = __builtin_fma(d, h, s);
+  d = __builtin_fma(-s, s, t);
+  s = __builtin_fma(d, h, s);
+  return t > 0.0 ? s : 0.0;
+}
+
 struct TropTileArgs {
   int ni, nj, G;
   const double *rh[5], *dx1[5], *dx2[5];
@@ -424,7 +443,7 @@ __device__ __forceinline__ void trop_tiles_body(const TropTileArgs& a, const int
     // the correctly rounded one is a refinement sequence, with a relative
     // error of up to about 2e-8 (FP32's accuracy, like their stencils)
     if constexpr (kFast) return __builtin_amdgcn_sqrt(t);
-    return sqrt(t);
+    return sqrt_normal(t);
   };
   const double su = levu >= 1 ? 1.0 : 0.0;
   const double sam1 = su * am1;  // su * am1 * lap, left to right
