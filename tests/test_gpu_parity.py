@@ -159,11 +159,14 @@ def test_launch_mode_is_exact(gpu_lib, tile):
         assert not bad, f"{cfgname} tile {tile}:\n" + "\n".join(bad)
 
 
-def test_launch_mode_matches_persistent_at_1deg(gpu_lib):
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_launch_mode_matches_persistent_at_1deg(gpu_lib, form):
+    """Both modes run the same arithmetic: the exact one in FP64, the mixed
+    forms' reciprocal divisions and hardware roots in mixed."""
     cfg = configs.global_1deg()
-    a = Model(cfg)
+    a = Model(cfg, precision=abi.FORMS[form])
     assert a.subcycle_mode() == abi.PCOM_TROP_PERSISTENT  # the 1-degree grid fits
-    b = Model(cfg, trop_mode=abi.PCOM_TROP_LAUNCHES)
+    b = Model(cfg, precision=abi.FORMS[form], trop_mode=abi.PCOM_TROP_LAUNCHES)
     a.step(3)
     b.step(3)
     for n in STEP_FIELDS:
