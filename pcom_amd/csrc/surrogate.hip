@@ -114,8 +114,8 @@ enum ConvMode { kFwdRelu, kFwdLinear, kBwdData };
 template <int KIN, int KOUT>
 constexpr size_t conv_lds_bytes() { return (size_t)9 * KIN * KOUT * 2; }
 
-template <int KIN, int KOUT, int MODE, int PB>
-__global__ void __launch_bounds__(64 * kConvLdsWaves) k_conv_lds(Geo g, const bf16_t* __restrict__ in,
+template <int KIN, int KOUT, int MODE, int PB, int NW = kConvLdsWaves>
+__global__ void __launch_bounds__(64 * NW) k_conv_lds(Geo g, const bf16_t* __restrict__ in,
                                                                  const bf16_t* __restrict__ w,
                                                                  const float* __restrict__ bias,
                                                                  const bf16_t* __restrict__ post, void* out) {
@@ -134,8 +134,8 @@ __global__ void __launch_bounds__(64 * kConvLdsWaves) k_conv_lds(Geo g, const bf
   float bb[KOUT / 16];
 #pragma unroll
   for (int nb = 0; nb < KOUT / 16; ++nb) bb[nb] = MODE == kBwdData ? 0.f : bias[nb * 16 + r];
-  const long stride = (long)gridDim.x * kConvLdsWaves * PB;
-  for (long pb0 = ((long)xcd_block() * kConvLdsWaves + wave) * PB; pb0 < npb; pb0 += stride) {
+  const long stride = (long)gridDim.x * NW * PB;
+  for (long pb0 = ((long)xcd_block() * NW + wave) * PB; pb0 < npb; pb0 += stride) {
     int x[PB], y[PB], n[PB];
     bool live[PB];
 #pragma unroll
@@ -1107,13 +1107,13 @@ std::string nn_batch_buffers(Nn& n, int batch) {
 }
 
 // forward pass over `batch` images already packed into act[0]: the layers'
-// weights staged in LDS, workgroups of 8 waves taking two pixel blocks per
+// weights staged in LDS, workgroups of NW waves taking PB pixel blocks per
 // wave at a time, at most kConvGroups workgroups
 constexpr long kConvGroups = 512;
-template <int KIN, int KOUT, int MODE, int PB = 2>
+template <int KIN, int KOUT, int MODE, int PB = 2, int NW = kConvLdsWaves>
 std::string conv_lds(hipStream_t st, const Geo& g, const bf16_t* in, const bf16_t* w, const float* b,
                      const bf16_t* post, void* out) {
-  const auto kern = k_conv_lds<KIN, KOUT, MODE, PB>;
+  const auto kern = k_conv_lds<KIN, KOUT, MODE, PB, NW>;
   static bool attr = false;  // once per process: the attribute holds for every device
   if (!attr) {
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1122,31 +1122,35 @@ std::string conv_lds(hipStream_t st, const Geo& g, const bf16_t* in, const bf16_
     attr = true;
   }
   const long pblocks = g.npix() / 16;
-  const unsigned blocks = (unsigned)std::min(kConvGroups, (pblocks + kConvLdsWaves * PB - 1) / (kConvLdsWaves * PB));
+  const unsigned blocks = (unsigned)std::min(kConvGroups, (pblocks + NW * PB - 1) / (NW * PB));
   const size_t lds = conv_lds_bytes<KIN, KOUT>();
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kConvLdsWaves), lds, st, g, in, w, b, post, out);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * NW), lds, st, g, in, w, b, post, out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("k_conv_lds: ") + hipGetErrorString(e);
   return "";
 }
 
-template <int PB>
+template <int PB, int NW = kConvLdsWaves>
 std::string nn_forward_pb(Nn& n, hipStream_t st, int batch) {
   Geo g{batch, n.H, n.W, layer_dil(0)};
-  std::string e = conv_lds<kInC, kHid, kFwdRelu, PB>(st, g, n.act[0], n.L[0].w, n.L[0].b, nullptr, (void*)n.act[1]);
+  std::string e =
+      conv_lds<kInC, kHid, kFwdRelu, PB, NW>(st, g, n.act[0], n.L[0].w, n.L[0].b, nullptr, (void*)n.act[1]);
   for (int l = 1; l < kLayers - 1 && e.empty(); ++l) {
     g.dil = layer_dil(l);
-    e = conv_lds<kHid, kHid, kFwdRelu, PB>(st, g, n.act[l], n.L[l].w, n.L[l].b, nullptr, (void*)n.act[l + 1]);
+    e = conv_lds<kHid, kHid, kFwdRelu, PB, NW>(st, g, n.act[l], n.L[l].w, n.L[l].b, nullptr, (void*)n.act[l + 1]);
   }
   if (!e.empty()) return e;
   g.dil = layer_dil(kLayers - 1);
-  return conv_lds<kHid, kOutC, kFwdLinear, PB>(st, g, n.act[kLayers - 1], n.L[kLayers - 1].w, n.L[kLayers - 1].b,
-                                               nullptr, (void*)n.out);
+  return conv_lds<kHid, kOutC, kFwdLinear, PB, NW>(st, g, n.act[kLayers - 1], n.L[kLayers - 1].w,
+                                                   n.L[kLayers - 1].b, nullptr, (void*)n.out);
 }
 
-// one image: a pixel block per wave (more waves in flight); minibatches: two
+// one image: a pixel block per wave in workgroups of 16 waves (one per CU
+// at 1 degree: half as many stagings of the layer's weights, each thread
+// loading half as many of them); minibatches: two blocks per wave, 8 waves
+constexpr int kConvOneWaves = 16;
 std::string nn_forward(Nn& n, hipStream_t st, int batch) {
-  return batch == 1 ? nn_forward_pb<1>(n, st, batch) : nn_forward_pb<2>(n, st, batch);
+  return batch == 1 ? nn_forward_pb<1, kConvOneWaves>(n, st, batch) : nn_forward_pb<2>(n, st, batch);
 }
 
 // backward pass from dpre of the last layer (in act[kLayers], BF16)
