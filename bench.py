@@ -84,6 +84,8 @@ def run_form(cfg, precision, steps, warmup, surrogate=False):
                "train_iters": rep["iters"], "train_s": round(rep["train_s"], 2),
                "holdout_gain": {k: round(v, 3) for k, v in rep["holdout_gain"].items()}, "policy": sm.USE}
         FITTED[precision] = (m.nn_params(), rep["steps"])
+        # the model time of the timed steps: the fit's steps and the warm-up
+        sur["timed_from_step"] = rep["steps"] + warmup
         m.nn_use(**sm.USE)
     m.step(warmup)
     if sur is not None:
@@ -118,6 +120,22 @@ def run_form(cfg, precision, steps, warmup, surrogate=False):
         out["surrogate"] = sur
     m.close()
     return out
+
+
+def time_plain_window(cfg, precision, start, steps):
+    """ms per step of a plain form over steps start+1 .. start+steps of a
+    run (the surrogate form is timed after its fit, later in model time,
+    where the 3-D kernels cost more than in the first days)."""
+    from pcom_amd.lib import Model
+    m = Model(cfg, precision=precision)
+    m.step(start)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(steps)
+    m.sync()
+    wall = time.perf_counter() - t0
+    m.close()
+    return 1000.0 * wall / steps
 
 
 def calibrate_copy():
@@ -292,6 +310,22 @@ def main():
                           "frac": round(step_gbs / PEAK_HBM_GBS, 5), "bytes_per_step": b,
                           "definition": "algorithmic bytes per baroclinic step (N3*31*s + N2*50*8) / wall time"},
     }
+    # the surrogate form's timed steps come after its fit: the plain forms
+    # over the same steps of a run, for a comparison in one window
+    sur_name = next((f for f in forms if f.endswith("+surrogate")), None)
+    if sur_name is not None:
+        start = forms[sur_name]["surrogate"]["timed_from_step"]
+        spd = 86400 // cfg.dtbc
+        win = {"from_step": start, "days": [round(start / spd, 2), round((start + args.steps) / spd, 2)],
+               "ms_per_step": {}, "sypd": {}}
+        for f in ("fp64", sur_name.split("+")[0]):
+            if f in forms:
+                win["ms_per_step"][f] = round(time_plain_window(cfg, abi.FORMS[f], start, args.steps), 4)
+        win["ms_per_step"][sur_name] = round(forms[sur_name]["ms_per_step"], 4)
+        win["sypd"] = {f: round(sypd(ms, cfg.dtbc), 2) for f, ms in win["ms_per_step"].items()}
+        if "fp64" in win["ms_per_step"]:
+            win["surrogate_vs_fp64"] = round(win["ms_per_step"]["fp64"] / win["ms_per_step"][sur_name], 3)
+        out["same_window"] = win
     if args.rmse_days > 0:
         out["accuracy"] = accuracy(cfg, wanted, args.rmse_days)
     if rank == 0 and args.cpu_steps > 0:
