@@ -24,6 +24,7 @@
 
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 #include "context.h"
 
@@ -1109,6 +1110,60 @@ struct Column {
   }
 };
 
+// The same search for the mixed forms, which need not reproduce the
+// reference's summation order: a range mixed to (tv, sv) has sum t d =
+// tv sum d, so extending it by a level costs one update instead of the
+// reference's new sums over the whole range, and the mixed values are
+// written once per range instead of once per extension.  Deep convection
+// then costs O(levels) per column instead of O(levels^2).  The tests
+// (comparisons and mixing order) are the reference's, on the values the
+// range holds.
+template <typename E>
+__device__ void convect_linear(const Column<E>& c, int nlev) {
+  int kcnt = 1;
+  while (kcnt < nlev) {
+    if (!(c.rhob(kcnt) > c.rhoa(kcnt + 1))) {
+      kcnt = kcnt + 1;
+      continue;
+    }
+    int na = kcnt, nb = kcnt + 1;
+    const double da = c.dpr[na - 1], db = c.dpr[nb - 1];
+    double sumd = da + db;
+    double tv = (c.t[c.st * (na - 1)] * da + c.t[c.st * (nb - 1)] * db) / sumd;
+    double sv = (c.s[c.st * (na - 1)] * da + c.s[c.st * (nb - 1)] * db) / sumd;
+    // rho of the range's own levels, with its mixed values
+    auto ra = [&](int k) { return (double)c.p1[c.st * (k - 1)] * tv + (double)c.p2[c.st * (k - 1)] * sv; };
+    auto rb = [&](int k) { return (double)c.p1[c.st * k] * tv + (double)c.p2[c.st * k] * sv; };
+    auto take = [&](int k) {
+      const double d = c.dpr[k - 1];
+      const double nd = sumd + d;
+      tv = (tv * sumd + c.t[c.st * (k - 1)] * d) / nd;
+      sv = (sv * sumd + c.s[c.st * (k - 1)] * d) / nd;
+      sumd = nd;
+    };
+    while (nb < nlev) {
+      if (rb(nb) > c.rhoa(nb + 1)) {
+        nb = nb + 1;
+        take(nb);
+      } else {
+        if (na <= 1 || c.rhob(na - 1) <= ra(na)) break;
+        na = na - 1;
+        take(na);
+      }
+    }
+    while (na > 1) {
+      if (c.rhob(na - 1) <= ra(na)) break;
+      na = na - 1;
+      take(na);
+    }
+    for (int k = na; k <= nb; ++k) {
+      c.t[c.st * (k - 1)] = tv;
+      c.s[c.st * (k - 1)] = sv;
+    }
+    kcnt = nb;
+  }
+}
+
 constexpr int kConvCW = 32, kConvKL = 8;
 
 // kConvCW columns per workgroup (fewer than the other column kernels: the
@@ -1161,7 +1216,12 @@ __global__ void __launch_bounds__(kConvCW * kConvKL) k_conv(Grid<E> g, const dou
   const int col = tx + kPer * kl;
   const long pcol = (long)xcd_block() * kConvCW + col;
   const int ncol = tx < kPer && pcol < g.n2 ? g.lev_t[pcol] : 0;
-  if (ncol > 1) {
+  // the FP64 form (R = double) follows the reference's sums; the mixed
+  // forms (FP32 tendencies) take the linear search
+  if (ncol > 1 && !std::is_same<R, double>::value) {
+    const Column<E> c{lt + col, ls + col, l1 + col, l2 + col, ldpr, kConvCW};
+    convect_linear(c, ncol);
+  } else if (ncol > 1) {
     Column<E> c{lt + col, ls + col, l1 + col, l2 + col, ldpr, kConvCW};
     const int nlev = ncol;
     int kcnt = 1;

@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["tc", "sc", "uc", "vc", "chc", "mch2", "buc", "am", "ssh"]
 EXPECTED = {
-    "mixed": "8e29dab40aa87e82",
-    "mixed_eos32": "e9f38985eb489849",
+    "mixed": "cd35c12b6c63d32f",
+    "mixed_eos32": "c3e441d83e6f1bb1",
     "fp64": "7aff16207ed32fb4",
 }
 
