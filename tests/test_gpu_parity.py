@@ -229,13 +229,10 @@ def test_box_many_steps_bitwise(gpu_lib):
     assert m.check_finite()
 
 
-def test_deep_convection_bitwise(gpu_lib):
-    """Box columns made statically unstable to seeded random depths (water
-    warmed by up to 8 K towards the bottom in 60% of the columns, salinity
-    noise of +-0.5 psu): convect then mixes layers that deepen downwards and
-    upwards many times over, and the result stays bit-identical to the
-    oracle after one step and after two more in one call (whose later steps
-    start from what the earlier ones formed)."""
+def _unstable_box():
+    """The box with its columns made statically unstable to seeded random
+    depths: water warmed by up to 8 K towards the bottom in 60% of the
+    columns, salinity noise of +-0.5 psu."""
     import dataclasses
     cfg = configs.box()
     rng = np.random.default_rng(7)
@@ -244,7 +241,82 @@ def test_deep_convection_bitwise(gpu_lib):
     warm = rng.uniform(0.0, 8.0, shape) * (rng.random(shape[:2] + (1,)) < 0.6) * ramp
     pt = np.asfortranarray(np.asarray(cfg.pt) + warm)
     sa = np.asfortranarray(np.asarray(cfg.sa) + rng.uniform(-0.5, 0.5, shape))
-    cfg = dataclasses.replace(cfg, pt=pt, sa=sa)
+    return dataclasses.replace(cfg, pt=pt, sa=sa)
+
+
+def _runs_of_four(t):
+    """Columns holding 4 or more equal consecutive levels (mixed layers)."""
+    t = t[1:-1, 1:-1, :]
+    run4 = (t[:, :, :-3] == t[:, :, 1:-2]) & (t[:, :, 1:-2] == t[:, :, 2:-1]) & (t[:, :, 2:-1] == t[:, :, 3:])
+    return int(run4.any(axis=2).sum())
+
+
+def _worst_instability(x, lev):
+    """Largest rhob(k) - rhoa(k+1) over the levels k < nlev of every column,
+    with convect's linearised density (mod_int.f90:509-542): rhob(k) =
+    prho_t(k+1) T(k) + prho_s(k+1) S(k), rhoa(k+1) = prho_t(k+1) T(k+1) +
+    prho_s(k+1) S(k+1).  The adjustment mixes wherever this is positive, so
+    after it every column has it <= 0 up to rounding."""
+    t, s, p1, p2 = (x.get(n) for n in ("tc", "sc", "prho_t", "prho_s"))
+    rb = p1[:, :, 1:] * t[:, :, :-1] + p2[:, :, 1:] * s[:, :, :-1]
+    ra = p1[:, :, 1:] * t[:, :, 1:] + p2[:, :, 1:] * s[:, :, 1:]
+    active = lev[:, :, None] > np.arange(1, t.shape[2])[None, None, :]
+    active[[0, -1], :, :] = False
+    active[:, [0, -1], :] = False
+    return float(np.max(np.where(active, rb - ra, -np.inf))), float(np.max(np.abs(ra[active])))
+
+
+def test_deep_convection_mixed_form(gpu_lib):
+    """The mixed forms' convective adjustment (a mixed range's sums updated
+    as it grows instead of summed again, values written once per range) on
+    the unstable box: after one step and after two more, every column is
+    statically stable under convect's own test (which the reference's
+    search guarantees); each column's heat and salt content sum(T dpr),
+    sum(S dpr), which mixing conserves, matches the oracle's to 1e-4 K and
+    1e-4 psu (the FP32 tendencies differ by far less); the first step mixes
+    the same deep layers, and almost every point matches the oracle to
+    1e-3."""
+    cfg = _unstable_box()
+    m = Model(cfg, precision=abi.PCOM_MIXED)
+    o = Oracle(cfg)
+    lev = o.igrid(2)
+    dpr = o.vgrid(2, 3)
+    sea = lev[:, :, None] >= np.arange(1, cfg.nk + 1)[None, None, :]
+    sea[[0, -1], :, :] = False
+    sea[:, [0, -1], :] = False
+    col = sea.any(axis=2)
+    for n in (1, 2):
+        m.step(n)
+        o.step(n)
+        assert m.check_finite()
+        worst, scale = _worst_instability(m, lev)
+        assert worst <= 1e-12 * scale, (n, worst, scale)
+        w = np.where(sea, dpr[None, None, :], 0.0)
+        for name, tol in (("tc", 1e-4), ("sc", 1e-4)):
+            a, b = m.get(name), o.get(name)
+            hm = np.sum(a * w, axis=2)[col] / np.sum(w, axis=2)[col]
+            ho = np.sum(b * w, axis=2)[col] / np.sum(w, axis=2)[col]
+            assert np.max(np.abs(hm - ho)) <= tol, (n, name, np.max(np.abs(hm - ho)))
+            far = np.abs(a - b)[sea] > 1e-3
+            assert far.mean() < 0.01, (n, name, far.mean())
+        if n == 1:
+            # (later steps re-mix layers that the tendencies left nearly
+            # neutral, where the sign of an FP32 rounding decides)
+            rm, ro = _runs_of_four(m.get("tc")), _runs_of_four(o.get("tc"))
+            assert ro > 1000 and abs(rm - ro) <= 0.02 * ro, (rm, ro)
+    # the oracle's own columns pass the same stability test
+    worst, scale = _worst_instability(o, lev)
+    assert worst <= 1e-12 * scale, (worst, scale)
+
+
+def test_deep_convection_bitwise(gpu_lib):
+    """Box columns made statically unstable to seeded random depths (water
+    warmed by up to 8 K towards the bottom in 60% of the columns, salinity
+    noise of +-0.5 psu): convect then mixes layers that deepen downwards and
+    upwards many times over, and the result stays bit-identical to the
+    oracle after one step and after two more in one call (whose later steps
+    start from what the earlier ones formed)."""
+    cfg = _unstable_box()
     m = Model(cfg)
     o = Oracle(cfg)
     m.step(1)
@@ -252,9 +324,7 @@ def test_deep_convection_bitwise(gpu_lib):
     bad = _compare(m, o, STEP_FIELDS)
     assert not bad, "\n".join(bad)
     # the adjustment did mix deep layers: runs of 4 or more equal levels
-    t = m.get("tc")[1:-1, 1:-1, :]
-    run4 = (t[:, :, :-3] == t[:, :, 1:-2]) & (t[:, :, 1:-2] == t[:, :, 2:-1]) & (t[:, :, 2:-1] == t[:, :, 3:])
-    assert run4.any(axis=2).sum() > 100
+    assert _runs_of_four(m.get("tc")) > 100
     m.step(2)
     o.step(2)
     bad = _compare(m, o, STEP_FIELDS)
