@@ -11,6 +11,7 @@ from pcom_amd import abi, configs
 from pcom_amd.lib import Model, PcomError, den_rho as gpu_den_rho
 
 from oracle.oracle import Oracle, den_rho as orc_den_rho
+from test_oracle_pins import unstable_box, worst_instability
 
 pytestmark = pytest.mark.gpu
 
@@ -229,41 +230,11 @@ def test_box_many_steps_bitwise(gpu_lib):
     assert m.check_finite()
 
 
-def _unstable_box():
-    """The box with its columns made statically unstable to seeded random
-    depths: water warmed by up to 8 K towards the bottom in 60% of the
-    columns, salinity noise of +-0.5 psu."""
-    import dataclasses
-    cfg = configs.box()
-    rng = np.random.default_rng(7)
-    shape = (cfg.glo_ni, cfg.glo_nj, cfg.nk)
-    ramp = np.linspace(0.0, 1.0, cfg.nk)[None, None, :]
-    warm = rng.uniform(0.0, 8.0, shape) * (rng.random(shape[:2] + (1,)) < 0.6) * ramp
-    pt = np.asfortranarray(np.asarray(cfg.pt) + warm)
-    sa = np.asfortranarray(np.asarray(cfg.sa) + rng.uniform(-0.5, 0.5, shape))
-    return dataclasses.replace(cfg, pt=pt, sa=sa)
-
-
 def _runs_of_four(t):
     """Columns holding 4 or more equal consecutive levels (mixed layers)."""
     t = t[1:-1, 1:-1, :]
     run4 = (t[:, :, :-3] == t[:, :, 1:-2]) & (t[:, :, 1:-2] == t[:, :, 2:-1]) & (t[:, :, 2:-1] == t[:, :, 3:])
     return int(run4.any(axis=2).sum())
-
-
-def _worst_instability(x, lev):
-    """Largest rhob(k) - rhoa(k+1) over the levels k < nlev of every column,
-    with convect's linearised density (mod_int.f90:509-542): rhob(k) =
-    prho_t(k+1) T(k) + prho_s(k+1) S(k), rhoa(k+1) = prho_t(k+1) T(k+1) +
-    prho_s(k+1) S(k+1).  The adjustment mixes wherever this is positive, so
-    after it every column has it <= 0 up to rounding."""
-    t, s, p1, p2 = (x.get(n) for n in ("tc", "sc", "prho_t", "prho_s"))
-    rb = p1[:, :, 1:] * t[:, :, :-1] + p2[:, :, 1:] * s[:, :, :-1]
-    ra = p1[:, :, 1:] * t[:, :, 1:] + p2[:, :, 1:] * s[:, :, 1:]
-    active = lev[:, :, None] > np.arange(1, t.shape[2])[None, None, :]
-    active[[0, -1], :, :] = False
-    active[:, [0, -1], :] = False
-    return float(np.max(np.where(active, rb - ra, -np.inf))), float(np.max(np.abs(ra[active])))
 
 
 def test_deep_convection_mixed_form(gpu_lib):
@@ -276,7 +247,7 @@ def test_deep_convection_mixed_form(gpu_lib):
     1e-4 psu (the FP32 tendencies differ by far less); the first step mixes
     the same deep layers, and almost every point matches the oracle to
     1e-3."""
-    cfg = _unstable_box()
+    cfg = unstable_box()
     m = Model(cfg, precision=abi.PCOM_MIXED)
     o = Oracle(cfg)
     lev = o.igrid(2)
@@ -289,7 +260,7 @@ def test_deep_convection_mixed_form(gpu_lib):
         m.step(n)
         o.step(n)
         assert m.check_finite()
-        worst, scale = _worst_instability(m, lev)
+        worst, scale = worst_instability(m, lev)
         assert worst <= 1e-12 * scale, (n, worst, scale)
         w = np.where(sea, dpr[None, None, :], 0.0)
         for name, tol in (("tc", 1e-4), ("sc", 1e-4)):
@@ -305,7 +276,7 @@ def test_deep_convection_mixed_form(gpu_lib):
             rm, ro = _runs_of_four(m.get("tc")), _runs_of_four(o.get("tc"))
             assert ro > 1000 and abs(rm - ro) <= 0.02 * ro, (rm, ro)
     # the oracle's own columns pass the same stability test
-    worst, scale = _worst_instability(o, lev)
+    worst, scale = worst_instability(o, lev)
     assert worst <= 1e-12 * scale, (worst, scale)
 
 
@@ -316,7 +287,7 @@ def test_deep_convection_bitwise(gpu_lib):
     upwards many times over, and the result stays bit-identical to the
     oracle after one step and after two more in one call (whose later steps
     start from what the earlier ones formed)."""
-    cfg = _unstable_box()
+    cfg = unstable_box()
     m = Model(cfg)
     o = Oracle(cfg)
     m.step(1)

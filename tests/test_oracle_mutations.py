@@ -163,6 +163,11 @@ MUTANTS = {
                            "restoring_budget"),
     "ts_no_mch": ("o->tc[q] = o->tp[q] + pt1[q] * bc * (double)o->msk_t[q] / o->mch[p];",
                   "o->tc[q] = o->tp[q] + pt1[q] * bc * (double)o->msk_t[q];", "restoring_budget"),
+    # convect
+    "convect_not_called": ("  convect(o);\n  copy3(o, o->tp, o->tc);", "  copy3(o, o->tp, o->tc);",
+                           "convect_leaves_unstable", PINS),
+    "convect_rhob_own_level": ("rhob[k] = p1c[k + 1] * t[k] + p2c[k + 1] * s[k];",
+                               "rhob[k] = p1c[k] * t[k] + p2c[k] * s[k];", "convect_leaves_unstable", PINS),
 }
 
 _RUN = """

@@ -21,7 +21,10 @@ evaluates them:
   with its bootstrap on steps 1 and 2 unless the run is a restart;
 * int_ts's accumulators (mod_int.f90:456-461, 500-504): acr adds the
   density of the new T, S at the layer-centre pressure, acw the pressure
-  velocity converted to height units with the step-mean mch2.
+  velocity converted to height units with the step-mean mch2;
+* convect at the end of int_ts (mod_int.f90:509-542): a step from columns
+  made statically unstable leaves every column stable under convect's own
+  test with the step's prho.
 
 The GPU kernels match the oracle bit for bit (tests/test_gpu_parity.py), so
 these pins carry over to them."""
@@ -450,3 +453,55 @@ def test_int_ts_vertical_velocity_accumulator_in_height_units():
     np.testing.assert_allclose(acw, want, rtol=1e-14, atol=1e-30)
     k0 = wm[:, :, :nk] / (C["rho0"] * C["g"])
     np.testing.assert_allclose(acw[:, :, 0] * mch2, k0[:, :, 0], rtol=1e-14, atol=1e-30)
+
+
+def unstable_box():
+    """The box with its columns made statically unstable to seeded random
+    depths: water warmed by up to 8 K towards the bottom in 60% of the
+    columns, salinity noise of +-0.5 psu."""
+    import dataclasses
+    cfg = configs.box()
+    rng = np.random.default_rng(7)
+    shape = (cfg.glo_ni, cfg.glo_nj, cfg.nk)
+    ramp = np.linspace(0.0, 1.0, cfg.nk)[None, None, :]
+    warm = rng.uniform(0.0, 8.0, shape) * (rng.random(shape[:2] + (1,)) < 0.6) * ramp
+    pt = np.asfortranarray(np.asarray(cfg.pt) + warm)
+    sa = np.asfortranarray(np.asarray(cfg.sa) + rng.uniform(-0.5, 0.5, shape))
+    return dataclasses.replace(cfg, pt=pt, sa=sa)
+
+
+def worst_instability(x, lev):
+    """Largest rhob(k) - rhoa(k+1) over the levels k < nlev of the inner
+    columns of x (anything with get()), with convect's linearised density
+    (mod_int.f90:509-542): rhob(k) = prho_t(k+1) T(k) + prho_s(k+1) S(k),
+    rhoa(k+1) = prho_t(k+1) T(k+1) + prho_s(k+1) S(k+1); and the largest
+    |rhoa| compared.  The adjustment mixes wherever the first is positive,
+    and its search leaves no such level, so after it the first is <= 0 up
+    to rounding."""
+    t, s, p1, p2 = (x.get(n) for n in ("tc", "sc", "prho_t", "prho_s"))
+    rb = p1[:, :, 1:] * t[:, :, :-1] + p2[:, :, 1:] * s[:, :, :-1]
+    ra = p1[:, :, 1:] * t[:, :, 1:] + p2[:, :, 1:] * s[:, :, 1:]
+    active = lev[:, :, None] > np.arange(1, t.shape[2])[None, None, :]
+    active[[0, -1], :, :] = False
+    active[:, [0, -1], :] = False
+    return float(np.max(np.where(active, rb - ra, -np.inf))), float(np.max(np.abs(ra[active])))
+
+
+def test_convect_leaves_unstable_columns_stable():
+    """Steps from the statically unstable box: before the first step most
+    columns are unstable somewhere; after it and after two more every
+    column passes convect's test with the step's prho, and deep layers
+    (4 or more levels) were mixed to equal values."""
+    cfg = unstable_box()
+    o = Oracle(cfg)
+    lev = o.igrid(2)
+    o.run_stage("den_prho")
+    worst0, scale = worst_instability(o, lev)
+    assert worst0 > 1e-3 * scale
+    for n in (1, 2):
+        o.step(n)
+        worst, scale = worst_instability(o, lev)
+        assert worst <= 1e-12 * scale, (n, worst, scale)
+    t = o.get("tc")[1:-1, 1:-1, :]
+    run4 = (t[:, :, :-3] == t[:, :, 1:-2]) & (t[:, :, 1:-2] == t[:, :, 2:-1]) & (t[:, :, 2:-1] == t[:, :, 3:])
+    assert run4.any(axis=2).sum() > 100
