@@ -509,48 +509,46 @@ struct RawIn {
 };
 
 // corrector free surface, then r = x_1 - x_0 and the other raw channels of
-// the interior pixel q = y * W + x (Fortran (i, j) = (x + 2, y + 2))
-__global__ void k_sub1_raw(Grid<double> g, int H, int W, RawIn a, double bt, float* raw) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long hw = (long)H * W;
-  if (q >= hw) return;
-  const int i = (int)(q % W) + 2, j = (int)(q / W) + 2;
-  const long p = g.ix(i, j);
-  auto FU = [&](int m, int jj) -> double {
-    const long k = g.ix(m, jj);
-    return g.prh_u[k] * a.sch1[k] * a.u1[k];
-  };
-  auto FV = [&](int m, int jj) -> double {
-    const long k = g.ix(m, jj);
-    return g.prh_u[k] * a.sch1[k] * a.v1[k];
-  };
-  const double ch1 = a.chp[p] - (double)(g.lev_t[p] >= 1 ? 1 : 0) * divg<3, 3, 1>(g, FU, FV, i, j) * bt / g.prh_t[p];
-  raw[q] = (float)(ch1 - a.chp[p]);
-  raw[hw + q] = (float)(a.u1[p] - a.bup[p]);
-  raw[2 * hw + q] = (float)(a.v1[p] - a.bvp[p]);
-  raw[3 * hw + q] = (float)a.am1[p];
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    raw[(4 + f) * hw + q] = (float)a.bphi[f][p];
-    raw[(8 + f) * hw + q] = (float)a.bgra[f][p];
-  }
-}
-
-// the snapshot's scale s = sea RMS of r_u, r_v: per-block partials of
-// (sum r^2, count), then one block folds them in order into s
-__global__ void k_scale_part(int H, int W, const float* raw, const float* stat, double* part) {
+// the interior pixel q = y * W + x (Fortran (i, j) = (x + 2, y + 2)); and
+// the snapshot's scale s = sea RMS of r_u, r_v (as the FP32 channels hold
+// them): per-block partials of (sum r^2, count) in part, which one block
+// folds in order into s (k_scale_final, or the live path's k_pack_live)
+__global__ void __launch_bounds__(256) k_sub1_raw(Grid<double> g, int H, int W, RawIn a, double bt, float* raw,
+                                                  const float* stat, double* part) {
   __shared__ double s1[256], s2[256];
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
-  double a = 0.0, c = 0.0;
+  double ssq = 0.0, cnt = 0.0;
   if (q < hw) {
+    const int i = (int)(q % W) + 2, j = (int)(q / W) + 2;
+    const long p = g.ix(i, j);
+    auto FU = [&](int m, int jj) -> double {
+      const long k = g.ix(m, jj);
+      return g.prh_u[k] * a.sch1[k] * a.u1[k];
+    };
+    auto FV = [&](int m, int jj) -> double {
+      const long k = g.ix(m, jj);
+      return g.prh_u[k] * a.sch1[k] * a.v1[k];
+    };
+    const double ch1 =
+        a.chp[p] - (double)(g.lev_t[p] >= 1 ? 1 : 0) * divg<3, 3, 1>(g, FU, FV, i, j) * bt / g.prh_t[p];
+    const float ru = (float)(a.u1[p] - a.bup[p]), rv = (float)(a.v1[p] - a.bvp[p]);
+    raw[q] = (float)(ch1 - a.chp[p]);
+    raw[hw + q] = ru;
+    raw[2 * hw + q] = rv;
+    raw[3 * hw + q] = (float)a.am1[p];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      raw[(4 + f) * hw + q] = (float)a.bphi[f][p];
+      raw[(8 + f) * hw + q] = (float)a.bgra[f][p];
+    }
     const double mu = (double)stat[hw + q];  // sea U mask
-    const double u = raw[hw + q], v = raw[2 * hw + q];
-    a = mu * (u * u + v * v);
-    c = 2.0 * mu;
+    const double u = ru, v = rv;
+    ssq = mu * (u * u + v * v);
+    cnt = 2.0 * mu;
   }
-  s1[threadIdx.x] = a;
-  s2[threadIdx.x] = c;
+  s1[threadIdx.x] = ssq;
+  s2[threadIdx.x] = cnt;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st) {
@@ -565,7 +563,7 @@ __global__ void k_scale_part(int H, int W, const float* raw, const float* stat, 
   }
 }
 
-// the snapshot scale s = sea RMS of (r_u, r_v) from k_scale_part's partials
+// the snapshot scale s = sea RMS of (r_u, r_v) from k_sub1_raw's partials
 __device__ __forceinline__ double fold_scale(const double* part, int nb) {
   double a[2];
   fold_parts<2>(part, nb, a);
@@ -848,17 +846,28 @@ __global__ void __launch_bounds__(256) k_pred_fix(long hw, const double* part, i
 }
 
 // write the end-of-cycle fields (mod_int.f90:276-287) from the increments,
-// interior points and their periodic halo copies
-__global__ void k_apply(int ni, int H, int W, const double* inc, double* chc, double* chp, double* mch,
-                        double* mch2, double* acch, double* buc, double* bvc, double* bup, double* bvp) {
+// interior points and their periodic halo copies, after removing the two
+// free-surface increments' weighted sea means as k_pred_fix does (the same
+// fold of k_pred's partials and the same operations; inc is left as
+// k_pred_fix leaves it)
+__global__ void __launch_bounds__(256) k_apply(int ni, int H, int W, const double* part, int nb, const float* stat,
+                                               double* inc, double* chc, double* chp, double* mch, double* mch2,
+                                               double* acch, double* buc, double* bvc, double* bup, double* bvp) {
+  double fm[3];
+  fold_parts<3>(part, nb, fm);
+  const double f0 = fm[2] > 0.0 ? fm[0] / fm[2] : 0.0, f1 = fm[2] > 0.0 ? fm[1] / fm[2] : 0.0;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W;
   if (q >= hw) return;
+  const double sea = (double)stat[q];
+  const double i0 = inc[q] - f0 * sea, i1 = inc[hw + q] - f1 * sea;
+  inc[q] = i0;
+  inc[hw + q] = i1;
   const int x = (int)(q % W), y = (int)(q / W);
   const long k = nn_ix(ni, x, y);
   const double c0 = chc[k];
-  const double m2 = c0 + inc[q];
-  const double m1 = c0 + inc[hw + q];
+  const double m2 = c0 + i0;
+  const double m1 = c0 + i1;
   const double u = buc[k] + inc[2 * hw + q];
   const double v = bvc[k] + inc[3 * hw + q];
   const int i = x + 2;
@@ -1366,15 +1375,15 @@ std::string nn_raw_inputs(Ctx& c, float* raw, float* keep, bool final = true) {
     r.bphi[q] = h.bphi[q];
     r.bgra[q] = h.bgra[q];
   }
-  NN_LAUNCH(k_sub1_raw, dim3(nblk(hw)), dim3(256), c.gd, n->H, n->W, r, bt, raw);
-  NN_LAUNCH(k_scale_part, dim3(nblk(hw)), dim3(256), n->H, n->W, raw, n->stat, n->dpart);
+  NN_LAUNCH(k_sub1_raw, dim3(nblk(hw)), dim3(256), c.gd, n->H, n->W, r, bt, raw, n->stat, n->dpart);
   // (the live path folds the partials in k_pack_live instead)
   if (final) NN_LAUNCH(k_scale_final, dim3(1), dim3(256), n->dpart, (int)nblk(hw), n->misc + 20, keep);
   return "";
 }
 
-// the network's increments for the live state into n->inc (mass-corrected)
-std::string nn_predict(Ctx& c) {
+// the network's increments for the live state into n->inc, mass-corrected
+// (with fix false the correction is left to k_apply, which makes it itself)
+std::string nn_predict(Ctx& c, bool fix = true) {
   Nn* n = c.nn;
   hipStream_t st = c.stream;
   const long hw = (long)n->H * n->W;
@@ -1391,7 +1400,7 @@ std::string nn_predict(Ctx& c) {
     for (int q = 0; q < kOutUsed; ++q) lim.v[q] = kLimitAnchors * n->anchor[q];
   NN_LAUNCH(k_pred, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->out, n->norm, n->misc + 20, n->stat, n->massw,
             lim, n->inc, n->dpart);
-  NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->dpart, (int)nblk(hw), n->stat, n->inc);
+  if (fix) NN_LAUNCH(k_pred_fix, dim3(nblk(hw)), dim3(256), hw, n->dpart, (int)nblk(hw), n->stat, n->inc);
   return "";
 }
 }  // namespace
@@ -1503,10 +1512,10 @@ std::string nn_trop_step(Ctx& c, std::string (*plain)(Ctx&)) {
       std::string e = nn_record_beside(c, plain);
       if (!e.empty()) return e;
     }
-    std::string e = nn_predict(c);
+    std::string e = nn_predict(c, false);
     if (!e.empty()) return e;
-    NN_LAUNCH(k_apply, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->inc, h.chc, h.chp, h.mch, h.mch2, h.acch,
-              h.buc, h.bvc, h.bup, h.bvp);
+    NN_LAUNCH(k_apply, dim3(nblk(hw)), dim3(256), n->ni, n->H, n->W, n->dpart, (int)nblk(hw), n->stat, n->inc, h.chc,
+              h.chp, h.mch, h.mch2, h.acch, h.buc, h.bvc, h.bup, h.bvp);
     n->surrogate_steps += 1;
     return "";
   }
