@@ -1641,27 +1641,24 @@ std::string step_post_t(Ctx& c) {
   }
   // int_ts's upwelling (its w accumulator is convect's)
   LAUNCH_UPW();
-  // int_ssh, int_ts and convect on the side stream once the upwelling is
-  // done.  Until the next step's int_readyc join nothing on c.stream writes
-  // what they read (T, S, uc, vc, wm, alpha, mch, mch2, chc, schs, bnd, prho)
-  // or touches what they write (alpha, ssh, acssh, T, S, the T/S/rho/w
-  // accumulators, alphn, adpn), so they overlap smag and the next step's
-  // int_readyc; a daily forcing update (which writes bnd and prho) settles
-  // the side stream first.
+  // int_ts and convect on the side stream once the upwelling is done, and
+  // int_ssh ahead of them there or beside them on this stream: int_ts does
+  // not read the alpha that int_ssh writes, convect does, and on this
+  // stream int_ssh is followed by fork 2, which convect waits for.  Until
+  // the next step's int_readyc join nothing on c.stream writes what the
+  // side stream reads (T, S, uc, vc, wm, alpha after int_ssh, mch, mch2,
+  // chc, schs, bnd, prho) or touches what it writes (T, S, the T/S/rho/w
+  // accumulators, alphn, adpn), so int_ts and convect overlap smag and the
+  // next step's int_readyc; a daily forcing update (which writes bnd and
+  // prho) settles the side stream first.  The side stream is the longer in
+  // the mixed forms (their int_ts keeps the FP64 state, their int_readyc
+  // kernels are FP32), the main stream in FP64, so int_ssh joins the
+  // shorter one.
   hipError_t e;
   if ((e = fork_side(c, 1)) != hipSuccess) return hip_err(e, "fork");
   c.side_pending = true;
-  LAUNCH_COLS_ON(c.side, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
-  c.n_ssh += 1;
-  {
-    const int G = c.ni - 2;
-    const unsigned nb = (unsigned)(((G + kSmagTX - 1) / kSmagTX) * ((c.nj + kSmagTY - 1) / kSmagTY) * c.nk);
-    hipLaunchKernelGGL(k_smag<R>, dim3(nb), dim3(kSmagTX * kSmagTY), 0, c.stream, g, U, V, h.mch2, h.schs,
-                       P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
-    hipError_t es = hipGetLastError();
-    if (es != hipSuccess) return hip_err(es, "k_smag");
-  }
-  c.n_uv += 1;
+  constexpr bool kSshMain = !std::is_same<R, double>::value;
+  if (!kSshMain) LAUNCH_COLS_ON(c.side, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
   // int_ts
   c.n_w += 1;
   {
@@ -1675,6 +1672,22 @@ std::string step_post_t(Ctx& c) {
     hipError_t et = hipGetLastError();
     if (et != hipSuccess) return hip_err(et, "k_ts_tiles");
   }
+  // int_ssh reads the T, S that int_ts leaves in place (it writes the other
+  // buffer of each pair)
+  if (kSshMain) {
+    LAUNCH_COLS_ON(c.stream, k_ssh<E>, sizeof(E), ge, T, S, h.mch, P<E>(f.alpha), h.ssh, h.acssh);
+    if ((e = fork_side(c, 2)) != hipSuccess) return hip_err(e, "fork");
+  }
+  c.n_ssh += 1;
+  {
+    const int G = c.ni - 2;
+    const unsigned nb = (unsigned)(((G + kSmagTX - 1) / kSmagTX) * ((c.nj + kSmagTY - 1) / kSmagTY) * c.nk);
+    hipLaunchKernelGGL(k_smag<R>, dim3(nb), dim3(kSmagTX * kSmagTY), 0, c.stream, g, U, V, h.mch2, h.schs,
+                       P<R>(f.acu), P<R>(f.acv), P<R>(f.am), c.dtbc);
+    hipError_t es = hipGetLastError();
+    if (es != hipSuccess) return hip_err(es, "k_smag");
+  }
+  c.n_uv += 1;
   c.cur = 1 - c.cur;
   {
     const size_t lds = 2 * (size_t)c.nk * kConvCW * (sizeof(double) + sizeof(E));
